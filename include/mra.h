@@ -1,0 +1,187 @@
+/*
+ * mra.h — C ABI of the MI355X-native nonstationary MRA library (libmra_hip.so).
+ *
+ * Plain pointers and sizes only.  Host arrays passed in are copied at the
+ * call; outputs go to caller-allocated host buffers.  A handle owns all of its
+ * device memory and one HIP stream.  A handle is not thread-safe; distinct
+ * handles are independent.
+ *
+ * Each entry point names the reference interface it replaces
+ * (hhuang90/nonstationary_mra, file:line).
+ */
+#ifndef MRA_H
+#define MRA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ----------------------------------------------------------------- status */
+enum {
+    MRA_OK = 0,
+    MRA_E_ARG = -1,         /* invalid argument / descriptor */
+    MRA_E_HIP = -2,         /* HIP runtime error */
+    MRA_E_NOT_PD = -3,      /* a Cholesky pivot was not positive; see mra_last_error */
+    MRA_E_STATE = -4,       /* call order (e.g. predict without prediction points) */
+    MRA_E_NOMEM = -5,       /* device allocation failed */
+    MRA_E_UNSUPPORTED = -6  /* configuration outside what this build supports */
+};
+
+enum { MRA_METRIC_CHORDAL = 0, MRA_METRIC_GREAT_CIRCLE = 1, MRA_METRIC_PLANAR = 2 };
+enum { MRA_FIELD_CONSTANT = 0, MRA_FIELD_WENDLAND = 1, MRA_FIELD_LINEAR = 2 };
+enum { MRA_PRECISION_FP64 = 0, MRA_PRECISION_MIXED = 1 };
+
+typedef struct mra_handle mra_handle;
+
+/* Flattened MRA structure (the arrays of Partition,
+ * include/class_partition.hpp:14-61).  Region ids follow the reference's heap
+ * order: coarse regions 0..n_coarse-1 (root 0, children of i are J*i+1..J*i+J),
+ * finest regions n_coarse..n_coarse+n_finest-1.  Observations are the knots of
+ * the finest level (OBS_AS_KNOTS_FLAG) and are grouped by finest region. */
+typedef struct mra_tree_desc {
+    int32_t num_levels;      /* M (NUM_LEVELS_M) */
+    int32_t num_partitions;  /* J (NUM_PARTITIONS_J), 2 or 4 */
+    int32_t num_knots;       /* r: knots per coarse region (Partition::nKnots) */
+    int32_t various_j;       /* 1: finest parents come from a parent file */
+    uint64_t n_coarse;       /* regions at levels 0..M-2 */
+    uint64_t n_finest;       /* regions at level M-1 */
+    uint64_t n_obs;          /* observations (sum of finest knot counts) */
+    uint64_t n_pred;         /* prediction locations (0: none) */
+    const double* knot_lon;  /* [n_coarse*r], knot q of region R at R*r+q */
+    const double* knot_lat;
+    const uint64_t* finest_parent; /* [n_finest] coarse id of each finest region's parent */
+    const uint64_t* obs_offsets;   /* [n_finest+1] */
+    const double* obs_lon;         /* [n_obs] grouped by finest region */
+    const double* obs_lat;
+    const double* obs_val;         /* residual values (Partition::knotsResidual) */
+    const uint64_t* pred_offsets;  /* [n_finest+1] or NULL */
+    const double* pred_lon;        /* [n_pred] grouped by finest region */
+    const double* pred_lat;
+} mra_tree_desc;
+
+/* One log-parameter field (class_nonstationary_covariance.hpp:8-49):
+ *   WENDLAND: log f(s) = intercept + sum_i w_i psi(d(s,c_i)/wendland_len),
+ *             psi(d) = (1-d)^6 (35 d^2 + 18 d + 3)/3 for d < 1;
+ *   LINEAR:   log f(s) = intercept + slope_x*lon + slope_y*lat  (planar fields);
+ *   CONSTANT: log f(s) = intercept. */
+typedef struct mra_field_desc {
+    int32_t kind;
+    int32_t reserved;
+    uint64_t n_basis;
+    double intercept;
+    const double* basis_lon;
+    const double* basis_lat;
+    const double* basis_weight;
+    double slope_x, slope_y;
+} mra_field_desc;
+
+/* Covariance model.  nonstationary=1: Paciorek exponential
+ * C = sqrt(s1 s2) (b1 b2/bm^2)^{3/2} exp(-d/bm), bm^2 = (b1^2+b2^2)/2
+ * (src/evaluate_covariance.cpp:155-313); nonstationary=0: sill*exp(-d/range)
+ * (src/evaluate_covariance.cpp:56-132).  Metric: chordal or great-circle on
+ * the unit sphere times 6371 (lon/lat in degrees), or planar (unit 1). */
+typedef struct mra_cov_desc {
+    int32_t nonstationary;
+    int32_t metric;
+    double wendland_len;
+    mra_field_desc sigmasq, beta, tausq;
+} mra_cov_desc;
+
+/* ----------------------------------------------------------- lifecycle */
+/* Upload the structure (replaces the Partition arrays consumed by
+ * Approximation, src/class_approximation.cpp:16-34,410-463). */
+int mra_create(const mra_tree_desc* tree, mra_handle** out);
+void mra_destroy(mra_handle* h);
+/* Message of the last error on this handle (or a global one if h is NULL). */
+const char* mra_last_error(const mra_handle* h);
+
+/* Covariance model (Nonstationary_covariance::load_data,
+ * src/class_nonstationary_covariance.cpp:46-429, plus the global flags
+ * NONSTATIONARY_FLAG / CHORDAL_DISTANCE_FLAG / WENDLAND_LEN). */
+int mra_set_covariance(mra_handle* h, const mra_cov_desc* cov);
+
+/* Approximation(sill, range, nugget, sill_scale, range_scale, nugget_scale)
+ * (src/class_approximation.cpp:396-408). */
+int mra_set_params(mra_handle* h, double sill, double range, double nugget,
+                   double sill_scale, double range_scale, double nugget_scale);
+
+/* MRA_PRECISION_FP64 (default) or MRA_PRECISION_MIXED. */
+int mra_set_precision(mra_handle* h, int precision);
+
+/* Approximation::likelihood() (src/class_approximation.cpp:466-470):
+ * writes D = Approximation::loglikelihood (= -2 loglik - n log(2*3.14159265359)). */
+int mra_likelihood(mra_handle* h, double* D);
+
+/* likelihood() + predict() (src/class_approximation-predict.cpp:15-200) in
+ * one pass: D, and the posterior predictive mean / variance at the
+ * prediction points in the handle's (finest-region) order. */
+int mra_predict(mra_handle* h, double* D, double* mean, double* var);
+
+/* Per-level partial sums of D: [0] finest level, [1..M-1] levels M-2..0. */
+int mra_likelihood_levels(mra_handle* h, double* terms);
+
+/* Draw y ~ N(0, Sigma_MRA) at the observation locations with the current
+ * parameters (no counterpart in the reference: the synthetic fields of the
+ * benchmark configurations are sampled from the MRA prior).  Writes n_obs
+ * values in the handle's observation order and also replaces the handle's
+ * observation values with them when replace != 0. */
+int mra_sample_prior(mra_handle* h, uint64_t seed, double* y, int replace);
+
+/* Replace the observation values (same order as obs_val at create). */
+int mra_set_observations(mra_handle* h, const double* values);
+
+/* Capture one complete likelihood evaluation as a HIP graph and replay it on
+ * later mra_likelihood calls (1), or launch kernels directly (0). */
+int mra_use_graph(mra_handle* h, int enable);
+
+/* Timing of the last evaluation: per-phase milliseconds (HIP events). */
+int mra_last_timings(mra_handle* h, double* ms, int n);
+
+/* ------------------------------------------------ single-kernel entries */
+/* evaluate_nonstationary_cross_covariance / evaluate_cross_covariance
+ * (src/evaluate_covariance.cpp:56-84,155-234): out[i1*n2+i2] = C(s1_i1, s2_i2).
+ * Host pointers; fields and scales from the covariance model. */
+int mra_cross_covariance(const mra_cov_desc* cov, double sill, double range, double nugget,
+                         double sill_scale, double range_scale, double nugget_scale,
+                         const double* lon1, const double* lat1, uint64_t n1,
+                         const double* lon2, const double* lat2, uint64_t n2,
+                         double* out);
+/* evaluate_nonstationary_variance_covariance / evaluate_variance_covariance
+ * (src/evaluate_covariance.cpp:100-132,248-313): nugget on the diagonal. */
+int mra_variance_covariance(const mra_cov_desc* cov, double sill, double range, double nugget,
+                            double sill_scale, double range_scale, double nugget_scale,
+                            const double* lon, const double* lat, uint64_t n, double* out);
+/* Device-resident covariance tile kernel for throughput measurement:
+ * d_out[i*n2+j] for n1 x n2 points whose scaled sqrt-sill, range and
+ * coordinates are already on the device (SoA).  precision selects FP64/FP32
+ * output (d_out is double* or float*).  stream may be NULL. */
+int mra_cov_tile_device(int precision, int metric, const double* d_xyz1, const double* d_sb1, uint64_t n1,
+                        const double* d_xyz2, const double* d_sb2, uint64_t n2, void* d_out, void* stream);
+
+/* ------------------------------------------------------------ partition */
+/* Sort-based replacement of Partition::build_partition_by_program
+ * (src/class_partition.cpp:459-654; same knot grid, half-open boxes and
+ * ancestor-knot skip; O(n log n) instead of the reference's O(n*F) scan).
+ * bounds = {lonMin, lonMax, latMin, latMax} as Data::load_data pads them
+ * (src/class_data.cpp:149-163).  Two-step: sizes first, then fill. */
+int mra_partition_program_sizes(const double* lon, const double* lat, uint64_t n,
+                                const double* plon, const double* plat, uint64_t np,
+                                int M, int J, int r, double offset, const double* bounds,
+                                uint64_t* sizes /* [4]: n_coarse*rk, n_finest, n_obs_kept, n_pred_kept */);
+int mra_partition_program(const double* lon, const double* lat, const double* val, uint64_t n,
+                          const double* plon, const double* plat, uint64_t np,
+                          int M, int J, int r, double offset, const double* bounds,
+                          double* knot_lon, double* knot_lat, uint64_t* finest_parent,
+                          uint64_t* obs_offsets, double* obs_lon, double* obs_lat, double* obs_val,
+                          uint64_t* obs_index, uint64_t* pred_offsets, double* pred_lon,
+                          double* pred_lat, uint64_t* pred_index);
+
+/* Library version string. */
+const char* mra_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MRA_H */

@@ -1,0 +1,285 @@
+"""ctypes binding of libmra_hip.so (the C ABI declared in include/mra.h).
+
+Loading fails loudly when the library has not been built: there is no
+fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libmra_hip.so")
+
+MRA_OK, MRA_E_ARG, MRA_E_HIP, MRA_E_NOT_PD, MRA_E_STATE, MRA_E_NOMEM, MRA_E_UNSUPPORTED = 0, -1, -2, -3, -4, -5, -6
+METRIC = {"chordal": 0, "great_circle": 1, "planar": 2}
+FIELD_CONSTANT, FIELD_WENDLAND, FIELD_LINEAR = 0, 1, 2
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+
+u64p = C.POINTER(C.c_uint64)
+dp = C.POINTER(C.c_double)
+
+# Every symbol include/mra.h declares.
+EXPORTED = ["mra_create", "mra_destroy", "mra_last_error", "mra_set_covariance", "mra_set_params",
+            "mra_set_precision", "mra_likelihood", "mra_predict", "mra_likelihood_levels", "mra_sample_prior",
+            "mra_set_observations", "mra_use_graph", "mra_last_timings", "mra_cross_covariance",
+            "mra_variance_covariance", "mra_cov_tile_device", "mra_partition_program_sizes",
+            "mra_partition_program", "mra_version"]
+
+
+class TreeDesc(C.Structure):
+    _fields_ = [("num_levels", C.c_int32), ("num_partitions", C.c_int32), ("num_knots", C.c_int32),
+                ("various_j", C.c_int32), ("n_coarse", C.c_uint64), ("n_finest", C.c_uint64),
+                ("n_obs", C.c_uint64), ("n_pred", C.c_uint64), ("knot_lon", dp), ("knot_lat", dp),
+                ("finest_parent", u64p), ("obs_offsets", u64p), ("obs_lon", dp), ("obs_lat", dp),
+                ("obs_val", dp), ("pred_offsets", u64p), ("pred_lon", dp), ("pred_lat", dp)]
+
+
+class FieldDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("n_basis", C.c_uint64), ("intercept", C.c_double),
+                ("basis_lon", dp), ("basis_lat", dp), ("basis_weight", dp), ("slope_x", C.c_double),
+                ("slope_y", C.c_double)]
+
+
+class CovDesc(C.Structure):
+    _fields_ = [("nonstationary", C.c_int32), ("metric", C.c_int32), ("wendland_len", C.c_double),
+                ("sigmasq", FieldDesc), ("beta", FieldDesc), ("tausq", FieldDesc)]
+
+
+class MraError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"MRA error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -c 'import __graft_entry__; "
+                               f"__graft_entry__.build()'` first (no CPU fallback exists)")
+        _lib = C.CDLL(LIB_PATH)
+        _lib.mra_last_error.restype = C.c_char_p
+        _lib.mra_last_error.argtypes = [C.c_void_p]
+        _lib.mra_version.restype = C.c_char_p
+        _lib.mra_create.argtypes = [C.POINTER(TreeDesc), C.POINTER(C.c_void_p)]
+        _lib.mra_destroy.argtypes = [C.c_void_p]
+        _lib.mra_set_covariance.argtypes = [C.c_void_p, C.POINTER(CovDesc)]
+        _lib.mra_set_params.argtypes = [C.c_void_p] + [C.c_double] * 6
+        _lib.mra_set_precision.argtypes = [C.c_void_p, C.c_int]
+        _lib.mra_likelihood.argtypes = [C.c_void_p, dp]
+        _lib.mra_likelihood_levels.argtypes = [C.c_void_p, dp]
+        _lib.mra_predict.argtypes = [C.c_void_p, dp, dp, dp]
+        _lib.mra_sample_prior.argtypes = [C.c_void_p, C.c_uint64, dp, C.c_int]
+        _lib.mra_set_observations.argtypes = [C.c_void_p, dp]
+        _lib.mra_use_graph.argtypes = [C.c_void_p, C.c_int]
+        _lib.mra_last_timings.argtypes = [C.c_void_p, dp, C.c_int]
+        _lib.mra_cross_covariance.argtypes = [C.POINTER(CovDesc)] + [C.c_double] * 6 + [dp, dp, C.c_uint64, dp, dp,
+                                                                                       C.c_uint64, dp]
+        _lib.mra_variance_covariance.argtypes = [C.POINTER(CovDesc)] + [C.c_double] * 6 + [dp, dp, C.c_uint64, dp]
+        _lib.mra_cov_tile_device.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
+                                             C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]
+        _lib.mra_partition_program_sizes.argtypes = [dp, dp, C.c_uint64, dp, dp, C.c_uint64, C.c_int, C.c_int,
+                                                     C.c_int, C.c_double, dp, u64p]
+        _lib.mra_partition_program.argtypes = [dp, dp, dp, C.c_uint64, dp, dp, C.c_uint64, C.c_int, C.c_int,
+                                               C.c_int, C.c_double, dp, dp, dp, u64p, u64p, dp, dp, dp, u64p, u64p,
+                                               dp, dp, u64p]
+    return _lib
+
+
+def check(rc, h=None):
+    if rc != MRA_OK:
+        msg = lib().mra_last_error(h)
+        raise MraError(rc, msg.decode() if msg else "")
+
+
+def _d(a):
+    return a.ctypes.data_as(dp)
+
+
+def _u(a):
+    return a.ctypes.data_as(u64p)
+
+
+class Keep:
+    def __init__(self):
+        self.refs = []
+
+    def d(self, x):
+        a = np.ascontiguousarray(x, dtype=np.float64)
+        self.refs.append(a)
+        return _d(a)
+
+    def u(self, x):
+        a = np.ascontiguousarray(x, dtype=np.uint64)
+        self.refs.append(a)
+        return _u(a)
+
+
+def tree_desc(tree, keep, with_pred=True):
+    t = TreeDesc()
+    t.num_levels, t.num_partitions, t.num_knots = tree["M"], tree["J"], tree["r"]
+    t.various_j = int(tree.get("various_j", 0))
+    t.n_coarse, t.n_finest = tree["n_coarse"], tree["n_finest"]
+    t.n_obs = len(tree["obs_lon"])
+    t.knot_lon, t.knot_lat = keep.d(tree["knot_lon"]), keep.d(tree["knot_lat"])
+    t.finest_parent = keep.u(tree["finest_parent"])
+    t.obs_offsets = keep.u(tree["obs_offsets"])
+    t.obs_lon, t.obs_lat, t.obs_val = keep.d(tree["obs_lon"]), keep.d(tree["obs_lat"]), keep.d(tree["obs_val"])
+    if with_pred and tree.get("pred_offsets") is not None and len(tree["pred_lon"]):
+        t.n_pred = len(tree["pred_lon"])
+        t.pred_offsets = keep.u(tree["pred_offsets"])
+        t.pred_lon, t.pred_lat = keep.d(tree["pred_lon"]), keep.d(tree["pred_lat"])
+    return t
+
+
+def _field(f, keep):
+    o = FieldDesc()
+    o.kind = f["kind"]
+    o.intercept = f.get("intercept", 0.0)
+    if f["kind"] == FIELD_WENDLAND:
+        o.n_basis = len(f["lon"])
+        o.basis_lon, o.basis_lat, o.basis_weight = keep.d(f["lon"]), keep.d(f["lat"]), keep.d(f["w"])
+    o.slope_x = f.get("slope_x", 0.0)
+    o.slope_y = f.get("slope_y", 0.0)
+    return o
+
+
+def cov_desc(cov, keep):
+    c = CovDesc()
+    c.nonstationary = int(cov["nonstationary"])
+    c.metric = cov["metric"]
+    c.wendland_len = cov.get("wendland_len", 1.0)
+    c.sigmasq = _field(cov["sigmasq"], keep)
+    c.beta = _field(cov["beta"], keep)
+    c.tausq = _field(cov["tausq"], keep)
+    return c
+
+
+class Handle:
+    """Owns one mra_handle (device-resident MRA structure)."""
+
+    def __init__(self, tree, with_pred=True):
+        keep = Keep()
+        t = tree_desc(tree, keep, with_pred)
+        h = C.c_void_p()
+        check(lib().mra_create(C.byref(t), C.byref(h)))
+        self.h = h
+        self.n_obs = int(t.n_obs)
+        self.n_pred = int(t.n_pred)
+        self.M = tree["M"]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mra_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_covariance(self, cov):
+        keep = Keep()
+        c = cov_desc(cov, keep)
+        check(lib().mra_set_covariance(self.h, C.byref(c)), self.h)
+
+    def set_params(self, sill=0.0, range=1.0, nugget=0.0, sill_scale=1.0, range_scale=1.0, nugget_scale=1.0):
+        check(lib().mra_set_params(self.h, sill, range, nugget, sill_scale, range_scale, nugget_scale), self.h)
+
+    def set_precision(self, p):
+        check(lib().mra_set_precision(self.h, p), self.h)
+
+    def use_graph(self, on=True):
+        check(lib().mra_use_graph(self.h, int(on)), self.h)
+
+    def likelihood(self):
+        D = C.c_double()
+        check(lib().mra_likelihood(self.h, C.byref(D)), self.h)
+        return D.value
+
+    def likelihood_levels(self):
+        out = np.zeros(self.M)
+        check(lib().mra_likelihood_levels(self.h, _d(out)), self.h)
+        return out
+
+    def predict(self):
+        D = C.c_double()
+        mean, var = np.zeros(self.n_pred), np.zeros(self.n_pred)
+        check(lib().mra_predict(self.h, C.byref(D), _d(mean), _d(var)), self.h)
+        return D.value, mean, var
+
+    def sample_prior(self, seed, replace=True):
+        y = np.zeros(self.n_obs)
+        check(lib().mra_sample_prior(self.h, seed, _d(y), int(replace)), self.h)
+        return y
+
+    def set_observations(self, y):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        check(lib().mra_set_observations(self.h, _d(y)), self.h)
+
+    def timings(self):
+        out = np.zeros(8)
+        check(lib().mra_last_timings(self.h, _d(out), 8), self.h)
+        return out
+
+
+def cross_covariance(cov, prm, lon1, lat1, lon2, lat2):
+    keep = Keep()
+    c = cov_desc(cov, keep)
+    n1, n2 = len(lon1), len(lon2)
+    out = np.zeros(n1 * n2)
+    check(lib().mra_cross_covariance(C.byref(c), prm.get("sill", 0.0), prm.get("range", 1.0), prm.get("nugget", 0.0),
+                                     prm.get("sill_scale", 1.0), prm.get("range_scale", 1.0),
+                                     prm.get("nugget_scale", 1.0), keep.d(lon1), keep.d(lat1), n1, keep.d(lon2),
+                                     keep.d(lat2), n2, _d(out)))
+    return out.reshape(n1, n2)
+
+
+def variance_covariance(cov, prm, lon, lat):
+    keep = Keep()
+    c = cov_desc(cov, keep)
+    n = len(lon)
+    out = np.zeros(n * n)
+    check(lib().mra_variance_covariance(C.byref(c), prm.get("sill", 0.0), prm.get("range", 1.0),
+                                        prm.get("nugget", 0.0), prm.get("sill_scale", 1.0),
+                                        prm.get("range_scale", 1.0), prm.get("nugget_scale", 1.0), keep.d(lon),
+                                        keep.d(lat), n, _d(out)))
+    return out.reshape(n, n)
+
+
+def partition_program(lon, lat, val, plon, plat, M, J, r, offset=np.e / 100, bounds=None):
+    """Program-placed partition (host code of the library)."""
+    lon = np.ascontiguousarray(lon, dtype=np.float64)
+    lat = np.ascontiguousarray(lat, dtype=np.float64)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    plon = np.ascontiguousarray(plon if plon is not None else np.zeros(0), dtype=np.float64)
+    plat = np.ascontiguousarray(plat if plat is not None else np.zeros(0), dtype=np.float64)
+    if bounds is None:
+        mnl, mxl, mna, mxa = lon.min(), lon.max(), lat.min(), lat.max()
+        bounds = np.array([mnl, mxl + 1e-6 * (mxl - mnl), mna, mxa + 1e-6 * (mxa - mna)])
+    b = np.ascontiguousarray(bounds, dtype=np.float64)
+    n, npred = len(lon), len(plon)
+    sizes = np.zeros(4, dtype=np.uint64)
+    check(lib().mra_partition_program_sizes(_d(lon), _d(lat), n, _d(plon), _d(plat), npred, M, J, r, offset, _d(b),
+                                            _u(sizes)))
+    nk, nF, no, nq = (int(x) for x in sizes)
+    nC = sum(J ** l for l in range(M - 1))
+    kl, kt = np.zeros(nk), np.zeros(nk)
+    parent = np.zeros(nF, dtype=np.uint64)
+    oo = np.zeros(nF + 1, dtype=np.uint64)
+    po = np.zeros(nF + 1, dtype=np.uint64)
+    ol, ot, ov = np.zeros(max(no, 1)), np.zeros(max(no, 1)), np.zeros(max(no, 1))
+    oi = np.zeros(max(no, 1), dtype=np.uint64)
+    pl, pt = np.zeros(max(nq, 1)), np.zeros(max(nq, 1))
+    pi = np.zeros(max(nq, 1), dtype=np.uint64)
+    check(lib().mra_partition_program(_d(lon), _d(lat), _d(val), n, _d(plon), _d(plat), npred, M, J, r, offset,
+                                      _d(b), _d(kl), _d(kt), _u(parent), _u(oo), _d(ol), _d(ot), _d(ov), _u(oi),
+                                      _u(po), _d(pl), _d(pt), _u(pi)))
+    return dict(M=M, J=J, r=nk // nC, various_j=0, n_coarse=nC, n_finest=nF, knot_lon=kl, knot_lat=kt,
+                finest_parent=parent, obs_offsets=oo, obs_lon=ol[:no], obs_lat=ot[:no], obs_val=ov[:no],
+                obs_index=oi[:no], pred_offsets=po, pred_lon=pl[:nq], pred_lat=pt[:nq], pred_index=pi[:nq], bounds=b)

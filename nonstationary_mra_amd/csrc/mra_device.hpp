@@ -1,0 +1,182 @@
+// Device building blocks for the MRA kernels (gfx950 / CDNA4).
+//
+// Work-group conventions: 256 threads = 4 waves of 64.  Small dense blocks
+// (r x r knot blocks, r <= 64, padded to RP = 16/32/64) live in LDS with a
+// row stride of RP+1 doubles (odd stride: no LDS bank conflicts on column
+// walks).  Dense products use the FP64 matrix core instruction
+// v_mfma_f64_16x16x4_f64 through __builtin_amdgcn_mfma_f64_16x16x4f64:
+//   A fragment (16x4): lane t holds A[t%16][t/16]
+//   B fragment (4x16): lane t holds B[t/16][t%16]
+//   D (16x16):         lane t, element v holds D[4*(t/16)+v][t%16]
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MRA_WG 256
+
+typedef double mra_d4 __attribute__((ext_vector_type(4)));
+typedef float mra_f4 __attribute__((ext_vector_type(4)));
+
+namespace mra {
+
+// ---------------------------------------------------------------- geometry
+// Point sets are SoA on the device.  Coordinates per metric:
+//   chordal: unit-sphere x, y, z;  planar: lon, lat, 0;
+//   great circle: sin(lat), cos(lat), lon (radians).
+// s = sqrt(scaled sill), b = scaled range, v = scaled sill + scaled nugget
+// (the diagonal of a variance-covariance matrix).
+struct Pts {
+    const double *x, *y, *z, *s, *b, *v;
+};
+
+struct CovCtx {
+    int metric;      // 0 chordal, 1 great circle, 2 planar
+    int stationary;  // 1: sill * exp(-d / range)
+    double sill, range;
+};
+
+__device__ __forceinline__ double pdist(const CovCtx& c, double x1, double y1, double z1, double x2, double y2,
+                                        double z2) {
+    if (c.metric == 1) {
+        // src/evaluate_covariance.cpp:9-25
+        double ang = x1 * x2 + y1 * y2 * cos(z1 - z2);
+        ang = fmin(1.0, fmax(-1.0, ang));
+        return 6371.0 * acos(ang);
+    }
+    // src/evaluate_covariance.cpp:28-37 (chordal x 6371) / planar
+    double dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
+    double d = sqrt(dx * dx + dy * dy + dz * dz);
+    return c.metric == 0 ? 6371.0 * d : d;
+}
+
+// Cross-covariance of points i (set P) and j (set Q), no nugget
+// (src/evaluate_covariance.cpp:207-223).
+__device__ __forceinline__ double pcov(const CovCtx& c, const Pts& P, uint64_t i, const Pts& Q, uint64_t j) {
+    double d = pdist(c, P.x[i], P.y[i], P.z[i], Q.x[j], Q.y[j], Q.z[j]);
+    if (c.stationary) return c.sill * exp(-d / c.range);
+    double b1 = P.b[i], b2 = Q.b[j];
+    double bms = (b1 * b1 + b2 * b2) * 0.5;
+    double ratio = (b1 * b2) / bms;
+    return (P.s[i] * Q.s[j]) * (ratio * sqrt(ratio)) * exp(-d / sqrt(bms));
+}
+
+__device__ __forceinline__ float pcovf(const CovCtx& c, const Pts& P, uint64_t i, const Pts& Q, uint64_t j) {
+    float x1 = (float)P.x[i], y1 = (float)P.y[i], z1 = (float)P.z[i];
+    float x2 = (float)Q.x[j], y2 = (float)Q.y[j], z2 = (float)Q.z[j];
+    float d;
+    if (c.metric == 1) {
+        float ang = x1 * x2 + y1 * y2 * cosf(z1 - z2);
+        ang = fminf(1.0f, fmaxf(-1.0f, ang));
+        d = 6371.0f * acosf(ang);
+    } else {
+        float dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
+        d = sqrtf(dx * dx + dy * dy + dz * dz);
+        if (c.metric == 0) d *= 6371.0f;
+    }
+    if (c.stationary) return (float)c.sill * expf(-d / (float)c.range);
+    float b1 = (float)P.b[i], b2 = (float)Q.b[j];
+    float bms = (b1 * b1 + b2 * b2) * 0.5f;
+    float ratio = (b1 * b2) / bms;
+    return ((float)P.s[i] * (float)Q.s[j]) * (ratio * sqrtf(ratio)) * expf(-d / sqrtf(bms));
+}
+
+// ----------------------------------------------------------- MFMA GEMM TN
+// C[i*ldc+j] += alpha * sum_k A(k,i) B(k,j) over the M x N tile (multiples
+// of 16).  A(k,i) = A[k*lda+i] valid for k < Kv, i < Mv (else 0); B likewise
+// with Nv.  K is rounded up to a multiple of 4.  C is LDS (or global) and
+// must be made visible by the caller (__syncthreads before and after).
+__device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int Kv, double alpha, const double* A,
+                                           int64_t lda, int Mv, const double* B, int64_t ldb, int Nv) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, k4 = lane >> 4;
+    const int tM = M >> 4, tN = N >> 4;
+    // 2x2 tile blocks per wave where possible
+    const int bM = (tM + 1) >> 1, bN = (tN + 1) >> 1;
+    for (int blk = wave; blk < bM * bN; blk += 4) {
+        const int ti0 = (blk / bN) * 2, tj0 = (blk % bN) * 2;
+        mra_d4 acc00 = {0, 0, 0, 0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
+        const int ia = ti0 * 16 + r16, ib = ia + 16;
+        const int ja = tj0 * 16 + r16, jb = ja + 16;
+        const bool ha = ia < Mv, hb = (ti0 + 1 < tM) && ib < Mv;
+        const bool hja = ja < Nv, hjb = (tj0 + 1 < tN) && jb < Nv;
+        for (int k0 = 0; k0 < Kv; k0 += 4) {  // uniform across the wave
+            const int k = k0 + k4;
+            const bool kv = k < Kv;
+            double a0 = (kv && ha) ? A[(int64_t)k * lda + ia] : 0.0;
+            double a1 = (kv && hb) ? A[(int64_t)k * lda + ib] : 0.0;
+            double b0 = (kv && hja) ? B[(int64_t)k * ldb + ja] : 0.0;
+            double b1 = (kv && hjb) ? B[(int64_t)k * ldb + jb] : 0.0;
+            acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc00, 0, 0, 0);
+            acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc01, 0, 0, 0);
+            acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
+            acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc11, 0, 0, 0);
+        }
+        // only the valid Mv x Nv part of C is updated
+        const int row = ti0 * 16 + 4 * k4;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const bool r0 = row + v < Mv, r1 = (ti0 + 1 < tM) && (row + 16 + v < Mv);
+            if (r0 && hja) C[(row + v) * ldc + ja] += alpha * acc00[v];
+            if (r0 && hjb) C[(row + v) * ldc + jb] += alpha * acc01[v];
+            if (r1 && hja) C[(row + 16 + v) * ldc + ja] += alpha * acc10[v];
+            if (r1 && hjb) C[(row + 16 + v) * ldc + jb] += alpha * acc11[v];
+        }
+    }
+}
+
+// ------------------------------------------------------------ LDS solvers
+// Lower Cholesky in place of the n x n block (row stride lda); entries with
+// i < j are left alone.  A non-positive pivot sets *bad and is replaced by 1
+// so that the kernel still completes.
+__device__ __forceinline__ void wg_chol(double* A, int lda, int n, int* bad) {
+    for (int k = 0; k < n; k++) {
+        if (threadIdx.x == 0) {
+            double d = A[k * lda + k];
+            if (!(d > 0.0)) {
+                *bad = 1;
+                d = 1.0;
+            }
+            A[k * lda + k] = sqrt(d);
+        }
+        __syncthreads();
+        const double dk = A[k * lda + k];
+        for (int i = k + 1 + threadIdx.x; i < n; i += MRA_WG) A[i * lda + k] /= dk;
+        __syncthreads();
+        const int m = n - k - 1;
+        for (int idx = threadIdx.x; idx < m * m; idx += MRA_WG) {
+            const int i = k + 1 + idx / m, j = k + 1 + idx % m;
+            if (j <= i) A[i * lda + j] -= A[i * lda + k] * A[j * lda + k];
+        }
+        __syncthreads();
+    }
+}
+
+// B <- L^{-1} B for lower L (n x n, row stride ldl), B n x ncol (ldb).
+__device__ __forceinline__ void wg_trsm(const double* L, int ldl, double* B, int ldb, int n, int ncol) {
+    for (int k = 0; k < n; k++) {
+        const double lkk = L[k * ldl + k];
+        for (int c = threadIdx.x; c < ncol; c += MRA_WG) B[k * ldb + c] /= lkk;
+        __syncthreads();
+        const int m = n - k - 1;
+        for (int idx = threadIdx.x; idx < m * ncol; idx += MRA_WG) {
+            const int i = k + 1 + idx / ncol, c = idx % ncol;
+            B[i * ldb + c] -= L[i * ldl + k] * B[k * ldb + c];
+        }
+        __syncthreads();
+    }
+}
+
+// Deterministic work-group sum of one value per thread (tree in LDS).
+__device__ __forceinline__ double wg_sum(double x, double* red) {
+    red[threadIdx.x] = x;
+    __syncthreads();
+    for (int s = MRA_WG / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+}  // namespace mra

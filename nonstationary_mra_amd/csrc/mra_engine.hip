@@ -1,0 +1,823 @@
+// Host engine of libmra_hip.so: the C ABI of include/mra.h over the kernels in
+// mra_kernels.hip.  One handle = one uploaded MRA structure, one HIP stream,
+// all device buffers sized once at create (no allocation per evaluation).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mra.h"
+#include "mra_kernels.hip"
+
+using namespace mra;
+
+static thread_local std::string g_err;
+
+#define HIP_TRY(x)                                                                    \
+    do {                                                                              \
+        hipError_t e_ = (x);                                                          \
+        if (e_ != hipSuccess) {                                                       \
+            set_err(h, std::string(#x) + ": " + hipGetErrorString(e_));               \
+            return MRA_E_HIP;                                                         \
+        }                                                                             \
+    } while (0)
+
+struct mra_handle {
+    hipStream_t stream = nullptr;
+    std::string err;
+    // structure
+    int M = 0, J = 0, r = 0, RP = 0, L = 0, LRP = 0;
+    bool various = false;
+    uint64_t nC = 0, nF = 0, nObs = 0, nPred = 0;
+    std::vector<uint64_t> lvlStart;  // L+1 entries
+    std::vector<uint64_t> hObsOff, hPredOff, hParent, hChS, hChE, hSigOff, hGoff;
+    std::vector<uint8_t> hWorking;
+    std::vector<double> hObsVal;
+    // device inputs
+    std::vector<void*> allocs;
+    double *dKlon = nullptr, *dKlat = nullptr, *dOlon = nullptr, *dOlat = nullptr, *dPlon = nullptr, *dPlat = nullptr;
+    double *dKc[3] = {}, *dOc[3] = {}, *dPc[3] = {};     // coordinates
+    double *dKf[3] = {}, *dOf[3] = {}, *dPf[3] = {};     // unscaled fields
+    double *dKs[3] = {}, *dOs[3] = {}, *dPs[3] = {};     // s, b, v
+    double* dY = nullptr;
+    uint64_t *dParent = nullptr, *dObsOff = nullptr, *dPredOff = nullptr, *dChS = nullptr, *dChE = nullptr,
+             *dSigOff = nullptr, *dGoff = nullptr;
+    uint8_t* dWorking = nullptr;
+    // engine storage
+    double *dGc = nullptr, *dLt = nullptr, *dLiT = nullptr, *dLpt = nullptr, *dLpiT = nullptr, *dP = nullptr,
+           *dV = nullptr, *dGf = nullptr, *dBf = nullptr, *dH = nullptr, *dU = nullptr, *dSig = nullptr,
+           *dDterm = nullptr, *dLev = nullptr, *dD = nullptr;
+    std::vector<double*> dW, dA;
+    int* dBad = nullptr;
+    // work lists
+    uint32_t *dItemsX = nullptr, *dItemsF = nullptr;
+    uint32_t nItemsX = 0, nItemsF = 0;
+    std::vector<uint32_t*> dItemsPost;
+    std::vector<uint32_t> nItemsPost;
+    // prediction
+    uint32_t* dItemsPred = nullptr;
+    std::vector<uint32_t> predChunkStart;  // item index where each chunk starts (+ end)
+    double *dBt = nullptr, *dGh = nullptr, *dGL = nullptr, *dMean = nullptr, *dVar = nullptr;
+    uint64_t* dGlOff = nullptr;
+    // model
+    bool covSet = false, paramsSet = false;
+    int nonstationary = 1, metric = 0;
+    double wlen = 1.0;
+    double sill = 1, range = 1, nugget = 0, ss = 1, rs = 1, ns = 1;
+    int precision = MRA_PRECISION_FP64;
+    std::vector<void*> fieldAllocs;
+    // timing
+    hipEvent_t ev[8] = {};
+    double timings[8] = {};
+    bool useGraph = false;
+};
+
+static void set_err(mra_handle* h, const std::string& s) {
+    if (h) h->err = s;
+    g_err = s;
+}
+
+template <typename T>
+static int dalloc(mra_handle* h, T** p, size_t count, std::vector<void*>& list) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * count);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        set_err(h, "hipMalloc of " + std::to_string(sizeof(T) * count) + " bytes failed");
+        return MRA_E_NOMEM;
+    }
+    list.push_back(*p);
+    return MRA_OK;
+}
+
+template <typename T>
+static int upload(mra_handle* h, T** p, const T* src, size_t count) {
+    int rc = dalloc(h, p, count, h->allocs);
+    if (rc) return rc;
+    if (count && src) HIP_TRY(hipMemcpy(*p, src, sizeof(T) * count, hipMemcpyHostToDevice));
+    return MRA_OK;
+}
+
+#define RC_TRY(x)              \
+    do {                       \
+        int rc_ = (x);         \
+        if (rc_) return rc_;   \
+    } while (0)
+
+static inline unsigned nblk(uint64_t n) { return (unsigned)((n + 255) / 256); }
+
+extern "C" const char* mra_version(void) { return "mra-mi355x 0.1 (gfx950)"; }
+
+extern "C" const char* mra_last_error(const mra_handle* h) { return h ? h->err.c_str() : g_err.c_str(); }
+
+extern "C" void mra_destroy(mra_handle* h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (void* p : h->allocs) (void)hipFree(p);
+    for (void* p : h->fieldAllocs) (void)hipFree(p);
+    for (auto& e : h->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+static int build_structure(mra_handle* h, const mra_tree_desc* t) {
+    h->M = t->num_levels;
+    h->J = t->num_partitions;
+    h->r = t->num_knots;
+    h->L = h->M - 1;
+    h->various = t->various_j != 0;
+    h->nC = t->n_coarse;
+    h->nF = t->n_finest;
+    h->nObs = t->n_obs;
+    h->nPred = t->pred_offsets ? t->n_pred : 0;
+    if (h->M < 2 || h->M > 24 || (h->J != 2 && h->J != 4) || h->r < 1) {
+        set_err(h, "invalid num_levels / num_partitions / num_knots");
+        return MRA_E_ARG;
+    }
+    if (h->r > 64) {
+        set_err(h, "num_knots > 64 is not supported by this build");
+        return MRA_E_UNSUPPORTED;
+    }
+    h->RP = h->r <= 16 ? 16 : (h->r <= 32 ? 32 : 64);
+    h->LRP = h->L * h->RP;
+    h->lvlStart.assign(h->L + 1, 0);
+    uint64_t s = 0, c = 1;
+    for (int l = 0; l < h->L; l++) {
+        h->lvlStart[l] = s;
+        s += c;
+        c *= (uint64_t)h->J;
+    }
+    h->lvlStart[h->L] = s;
+    if (s != h->nC) {
+        set_err(h, "n_coarse does not match num_levels / num_partitions");
+        return MRA_E_ARG;
+    }
+    if (!t->knot_lon || !t->knot_lat || !t->finest_parent || !t->obs_offsets) {
+        set_err(h, "null structure array");
+        return MRA_E_ARG;
+    }
+    h->hObsOff.assign(t->obs_offsets, t->obs_offsets + h->nF + 1);
+    if (h->hObsOff[0] != 0 || h->hObsOff[h->nF] != h->nObs) {
+        set_err(h, "obs_offsets inconsistent with n_obs");
+        return MRA_E_ARG;
+    }
+    for (uint64_t f = 0; f < h->nF; f++)
+        if (h->hObsOff[f + 1] < h->hObsOff[f]) {
+            set_err(h, "obs_offsets not monotone");
+            return MRA_E_ARG;
+        }
+    if (h->nObs && (!t->obs_lon || !t->obs_lat || !t->obs_val)) {
+        set_err(h, "null observation arrays");
+        return MRA_E_ARG;
+    }
+    if (h->nPred) {
+        h->hPredOff.assign(t->pred_offsets, t->pred_offsets + h->nF + 1);
+        if (h->hPredOff[0] != 0 || h->hPredOff[h->nF] != h->nPred || !t->pred_lon || !t->pred_lat) {
+            set_err(h, "pred_offsets inconsistent with n_pred");
+            return MRA_E_ARG;
+        }
+    }
+    h->hParent.assign(t->finest_parent, t->finest_parent + h->nF);
+    // children ranges of level L-1 regions and working flags
+    // (src/class_partition.cpp:746-769, 880-904)
+    const uint64_t s2 = h->lvlStart[h->L - 1], n2 = h->nC - s2;
+    h->hChS.assign(n2, 1);
+    h->hChE.assign(n2, 0);
+    h->hWorking.assign(h->nC, 0);
+    for (uint64_t f = 0; f < h->nF; f++) {
+        const uint64_t p = h->hParent[f];
+        if (p < s2 || p >= h->nC) {
+            set_err(h, "finest_parent outside the second-finest level");
+            return MRA_E_ARG;
+        }
+        const uint64_t k = p - s2;
+        if (h->hChE[k] < h->hChS[k]) {
+            h->hChS[k] = f;
+            h->hChE[k] = f;
+        } else {
+            h->hChS[k] = std::min(h->hChS[k], f);
+            h->hChE[k] = std::max(h->hChE[k], f);
+        }
+        uint64_t a = p;
+        h->hWorking[a] = 1;
+        while (a > 0) {
+            a = (a - 1) / (uint64_t)h->J;
+            h->hWorking[a] = 1;
+        }
+    }
+    // workspace offsets
+    h->hSigOff.assign(h->nF + 1, 0);
+    for (uint64_t f = 0; f < h->nF; f++) {
+        const uint64_t n = h->hObsOff[f + 1] - h->hObsOff[f];
+        h->hSigOff[f + 1] = h->hSigOff[f] + n * n;
+    }
+    h->hGoff.assign(h->L + 1, 0);
+    const uint64_t RP2 = (uint64_t)h->RP * h->RP;
+    for (int l = 0; l < h->L; l++) h->hGoff[l + 1] = h->hGoff[l] + (h->lvlStart[l + 1] - h->lvlStart[l]) * l * RP2;
+    return MRA_OK;
+}
+
+static int alloc_engine(mra_handle* h, const mra_tree_desc* t) {
+    auto& A = h->allocs;
+    const uint64_t RP = h->RP, RP2 = RP * RP, nKp = h->nC * RP;
+    // knots padded to RP per region
+    std::vector<double> kl(nKp, 0.0), kt(nKp, 0.0);
+    for (uint64_t R = 0; R < h->nC; R++)
+        for (int q = 0; q < h->r; q++) {
+            kl[R * RP + q] = t->knot_lon[R * h->r + q];
+            kt[R * RP + q] = t->knot_lat[R * h->r + q];
+        }
+    RC_TRY(upload(h, &h->dKlon, kl.data(), nKp));
+    RC_TRY(upload(h, &h->dKlat, kt.data(), nKp));
+    RC_TRY(upload(h, &h->dOlon, t->obs_lon, h->nObs));
+    RC_TRY(upload(h, &h->dOlat, t->obs_lat, h->nObs));
+    h->hObsVal.assign(t->obs_val, t->obs_val + h->nObs);
+    RC_TRY(upload(h, &h->dY, t->obs_val, h->nObs));
+    if (h->nPred) {
+        RC_TRY(upload(h, &h->dPlon, t->pred_lon, h->nPred));
+        RC_TRY(upload(h, &h->dPlat, t->pred_lat, h->nPred));
+    }
+    for (int k = 0; k < 3; k++) {
+        RC_TRY(dalloc(h, &h->dKc[k], nKp, A));
+        RC_TRY(dalloc(h, &h->dKf[k], nKp, A));
+        RC_TRY(dalloc(h, &h->dKs[k], nKp, A));
+        RC_TRY(dalloc(h, &h->dOc[k], h->nObs, A));
+        RC_TRY(dalloc(h, &h->dOf[k], h->nObs, A));
+        RC_TRY(dalloc(h, &h->dOs[k], h->nObs, A));
+        if (h->nPred) {
+            RC_TRY(dalloc(h, &h->dPc[k], h->nPred, A));
+            RC_TRY(dalloc(h, &h->dPf[k], h->nPred, A));
+            RC_TRY(dalloc(h, &h->dPs[k], h->nPred, A));
+        }
+    }
+    RC_TRY(upload(h, &h->dParent, h->hParent.data(), h->nF));
+    RC_TRY(upload(h, &h->dObsOff, h->hObsOff.data(), h->nF + 1));
+    if (h->nPred) RC_TRY(upload(h, &h->dPredOff, h->hPredOff.data(), h->nF + 1));
+    RC_TRY(upload(h, &h->dChS, h->hChS.data(), h->hChS.size()));
+    RC_TRY(upload(h, &h->dChE, h->hChE.data(), h->hChE.size()));
+    RC_TRY(upload(h, &h->dWorking, h->hWorking.data(), h->nC));
+    RC_TRY(upload(h, &h->dSigOff, h->hSigOff.data(), h->nF + 1));
+    RC_TRY(upload(h, &h->dGoff, h->hGoff.data(), h->L + 1));
+    // engine storage
+    RC_TRY(dalloc(h, &h->dGc, h->hGoff[h->L], A));
+    RC_TRY(dalloc(h, &h->dP, h->hGoff[h->L], A));
+    RC_TRY(dalloc(h, &h->dLt, h->nC * RP2, A));
+    RC_TRY(dalloc(h, &h->dLiT, h->nC * RP2, A));
+    RC_TRY(dalloc(h, &h->dLpt, h->nC * RP2, A));
+    RC_TRY(dalloc(h, &h->dLpiT, h->nC * RP2, A));
+    RC_TRY(dalloc(h, &h->dV, h->nC * RP, A));
+    RC_TRY(dalloc(h, &h->dGf, (uint64_t)h->LRP * h->nObs, A));
+    RC_TRY(dalloc(h, &h->dBf, (uint64_t)h->LRP * h->nObs, A));
+    RC_TRY(dalloc(h, &h->dH, (uint64_t)h->LRP * h->nObs, A));
+    RC_TRY(dalloc(h, &h->dU, h->nObs, A));
+    RC_TRY(dalloc(h, &h->dSig, h->hSigOff[h->nF], A));
+    RC_TRY(dalloc(h, &h->dDterm, h->nC + h->nF, A));
+    RC_TRY(dalloc(h, &h->dLev, h->M, A));
+    RC_TRY(dalloc(h, &h->dD, 1, A));
+    RC_TRY(dalloc(h, &h->dBad, 1, A));
+    h->dW.assign(h->L, nullptr);
+    h->dA.assign(h->L, nullptr);
+    for (int l = 1; l < h->L; l++) {
+        const uint64_t nl = h->lvlStart[l + 1] - h->lvlStart[l];
+        RC_TRY(dalloc(h, &h->dW[l], nl * l * RP, A));
+        RC_TRY(dalloc(h, &h->dA[l], nl * (uint64_t)(l * (l + 1) / 2) * RP2, A));
+    }
+    // work lists
+    std::vector<uint32_t> ix, iF;
+    for (uint64_t f = 0; f < h->nF; f++) {
+        const uint64_t n = h->hObsOff[f + 1] - h->hObsOff[f];
+        if (!n) continue;
+        iF.push_back((uint32_t)f);
+        for (uint64_t c0 = 0; c0 < n; c0 += 64) {
+            ix.push_back((uint32_t)f);
+            ix.push_back((uint32_t)c0);
+        }
+    }
+    h->nItemsX = (uint32_t)(ix.size() / 2);
+    h->nItemsF = (uint32_t)iF.size();
+    RC_TRY(upload(h, &h->dItemsX, ix.data(), ix.size()));
+    RC_TRY(upload(h, &h->dItemsF, iF.data(), iF.size()));
+    h->dItemsPost.assign(h->L, nullptr);
+    h->nItemsPost.assign(h->L, 0);
+    for (int l = 0; l < h->L; l++) {
+        std::vector<uint32_t> it;
+        for (uint64_t R = h->lvlStart[l]; R < h->lvlStart[l + 1]; R++)
+            if (h->hWorking[R]) it.push_back((uint32_t)(R - h->lvlStart[l]));
+        h->nItemsPost[l] = (uint32_t)it.size();
+        RC_TRY(upload(h, &h->dItemsPost[l], it.data(), it.size()));
+    }
+    if (h->nPred) {
+        // prediction items (F, p0, slot) in chunks bounding the scratch
+        const uint32_t CH = 4096;
+        std::vector<uint32_t> ip;
+        std::vector<uint64_t> gl;
+        uint64_t glCur = 0, glMax = 1;
+        uint32_t inChunk = 0;
+        h->predChunkStart.clear();
+        h->predChunkStart.push_back(0);
+        for (uint64_t f = 0; f < h->nF; f++) {
+            const uint64_t p = h->hPredOff[f + 1] - h->hPredOff[f];
+            const uint64_t n = h->hObsOff[f + 1] - h->hObsOff[f];
+            for (uint64_t p0 = 0; p0 < p; p0 += 64) {
+                if (inChunk == CH) {
+                    h->predChunkStart.push_back((uint32_t)(ip.size() / 3));
+                    inChunk = 0;
+                    glCur = 0;
+                }
+                ip.push_back((uint32_t)f);
+                ip.push_back((uint32_t)p0);
+                ip.push_back(inChunk);
+                gl.push_back(glCur);
+                glCur += n * 64;
+                glMax = std::max(glMax, glCur);
+                inChunk++;
+            }
+        }
+        h->predChunkStart.push_back((uint32_t)(ip.size() / 3));
+        // gl offsets indexed by slot within chunk: store per item, kernel reads by slot via chunk base
+        RC_TRY(upload(h, &h->dItemsPred, ip.data(), ip.size()));
+        RC_TRY(upload(h, &h->dGlOff, gl.data(), gl.size()));
+        RC_TRY(dalloc(h, &h->dGh, (uint64_t)CH * h->LRP * 64, A));
+        RC_TRY(dalloc(h, &h->dGL, glMax, A));
+        RC_TRY(dalloc(h, &h->dBt, (uint64_t)h->LRP * h->nPred, A));
+        RC_TRY(dalloc(h, &h->dMean, h->nPred, A));
+        RC_TRY(dalloc(h, &h->dVar, h->nPred, A));
+    }
+    return MRA_OK;
+}
+
+extern "C" int mra_create(const mra_tree_desc* tree, mra_handle** out) {
+    if (!tree || !out) {
+        g_err = "null argument";
+        return MRA_E_ARG;
+    }
+    *out = nullptr;
+    mra_handle* h = new mra_handle();
+    int rc = build_structure(h, tree);
+    if (!rc) {
+        hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            set_err(h, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+            rc = MRA_E_HIP;
+        }
+    }
+    if (!rc) rc = alloc_engine(h, tree);
+    if (!rc)
+        for (auto& e : h->ev)
+            if (hipEventCreate(&e) != hipSuccess) rc = MRA_E_HIP;
+    if (rc) {
+        g_err = h->err;
+        mra_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return MRA_OK;
+}
+
+static int field_dev(mra_handle* h, const mra_field_desc& f, FieldDev& d) {
+    d.kind = f.kind;
+    d.nb = (int)f.n_basis;
+    d.intercept = f.intercept;
+    d.sx = f.slope_x;
+    d.sy = f.slope_y;
+    d.bx = d.by = d.bz = d.blon = d.blat = d.w = nullptr;
+    if (f.kind != MRA_FIELD_CONSTANT && f.kind != MRA_FIELD_WENDLAND && f.kind != MRA_FIELD_LINEAR) {
+        set_err(h, "unknown field kind");
+        return MRA_E_ARG;
+    }
+    if (f.kind == MRA_FIELD_WENDLAND && f.n_basis) {
+        if (!f.basis_lon || !f.basis_lat || !f.basis_weight) {
+            set_err(h, "null basis arrays");
+            return MRA_E_ARG;
+        }
+        const uint64_t nb = f.n_basis;
+        std::vector<double> x(nb), y(nb), z(nb);
+        for (uint64_t i = 0; i < nb; i++) {
+            // src/class_nonstationary_covariance.cpp:373-386
+            double cosLat = std::cos(f.basis_lat[i] * M_PI / 180), cosLon = std::cos(f.basis_lon[i] * M_PI / 180);
+            double sinLat = std::sin(f.basis_lat[i] * M_PI / 180), sinLon = std::sin(f.basis_lon[i] * M_PI / 180);
+            x[i] = cosLat * cosLon;
+            y[i] = cosLat * sinLon;
+            z[i] = sinLat;
+        }
+        double* p[6];
+        const double* src[6] = {x.data(), y.data(), z.data(), f.basis_lon, f.basis_lat, f.basis_weight};
+        for (int k = 0; k < 6; k++) {
+            RC_TRY(dalloc(h, &p[k], nb, h->fieldAllocs));
+            HIP_TRY(hipMemcpy(p[k], src[k], sizeof(double) * nb, hipMemcpyHostToDevice));
+        }
+        d.bx = p[0];
+        d.by = p[1];
+        d.bz = p[2];
+        d.blon = p[3];
+        d.blat = p[4];
+        d.w = p[5];
+    }
+    return MRA_OK;
+}
+
+extern "C" int mra_set_covariance(mra_handle* h, const mra_cov_desc* cov) {
+    if (!h || !cov) return MRA_E_ARG;
+    if (cov->metric < 0 || cov->metric > 2) {
+        set_err(h, "unknown metric");
+        return MRA_E_ARG;
+    }
+    if (cov->nonstationary && cov->wendland_len <= 0 &&
+        (cov->sigmasq.kind == MRA_FIELD_WENDLAND || cov->beta.kind == MRA_FIELD_WENDLAND ||
+         cov->tausq.kind == MRA_FIELD_WENDLAND)) {
+        set_err(h, "wendland_len must be positive");
+        return MRA_E_ARG;
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (void* p : h->fieldAllocs) (void)hipFree(p);
+    h->fieldAllocs.clear();
+    FieldDev f[3];
+    RC_TRY(field_dev(h, cov->sigmasq, f[0]));
+    RC_TRY(field_dev(h, cov->beta, f[1]));
+    RC_TRY(field_dev(h, cov->tausq, f[2]));
+    h->nonstationary = cov->nonstationary;
+    h->metric = cov->metric;
+    h->wlen = cov->wendland_len;
+    struct Set {
+        uint64_t n;
+        double *lon, *lat, **c, **fo;
+    } sets[3] = {{h->nC * h->RP, h->dKlon, h->dKlat, h->dKc, h->dKf},
+                 {h->nObs, h->dOlon, h->dOlat, h->dOc, h->dOf},
+                 {h->nPred, h->dPlon, h->dPlat, h->dPc, h->dPf}};
+    for (auto& s : sets) {
+        if (!s.n) continue;
+        k_coords<<<nblk(s.n), 256, 0, h->stream>>>(s.n, s.lon, s.lat, h->metric, s.c[0], s.c[1], s.c[2]);
+        k_fields<<<nblk(s.n), 256, 0, h->stream>>>(s.n, s.lon, s.lat, h->metric, h->wlen, f[0], f[1], f[2], s.fo[0],
+                                                   s.fo[1], s.fo[2]);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->covSet = true;
+    return MRA_OK;
+}
+
+extern "C" int mra_set_params(mra_handle* h, double sill, double range, double nugget, double sill_scale,
+                              double range_scale, double nugget_scale) {
+    if (!h) return MRA_E_ARG;
+    h->sill = sill;
+    h->range = range;
+    h->nugget = nugget;
+    h->ss = sill_scale;
+    h->rs = range_scale;
+    h->ns = nugget_scale;
+    h->paramsSet = true;
+    return MRA_OK;
+}
+
+extern "C" int mra_set_precision(mra_handle* h, int precision) {
+    if (!h) return MRA_E_ARG;
+    if (precision != MRA_PRECISION_FP64 && precision != MRA_PRECISION_MIXED) {
+        set_err(h, "unknown precision");
+        return MRA_E_ARG;
+    }
+    h->precision = precision;
+    return MRA_OK;
+}
+
+extern "C" int mra_set_observations(mra_handle* h, const double* values) {
+    if (!h || (!values && h->nObs)) return MRA_E_ARG;
+    h->hObsVal.assign(values, values + h->nObs);
+    HIP_TRY(hipMemcpyAsync(h->dY, values, sizeof(double) * h->nObs, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return MRA_OK;
+}
+
+extern "C" int mra_use_graph(mra_handle* h, int enable) {
+    if (!h) return MRA_E_ARG;
+    h->useGraph = enable != 0;
+    return MRA_OK;
+}
+
+extern "C" int mra_last_timings(mra_handle* h, double* ms, int n) {
+    if (!h || !ms) return MRA_E_ARG;
+    for (int i = 0; i < n && i < 8; i++) ms[i] = h->timings[i];
+    return MRA_OK;
+}
+
+static Tree make_tree(mra_handle* h) {
+    Tree T;
+    T.M = h->M;
+    T.J = h->J;
+    T.L = h->L;
+    T.r = h->r;
+    T.RP = h->RP;
+    T.LRP = h->LRP;
+    T.nC = h->nC;
+    T.nF = h->nF;
+    T.parent = h->dParent;
+    T.obs_off = h->dObsOff;
+    T.pred_off = h->dPredOff;
+    T.ch_s = h->dChS;
+    T.ch_e = h->dChE;
+    T.working = h->dWorking;
+    T.sig_off = h->dSigOff;
+    return T;
+}
+
+static CovCtx make_cc(mra_handle* h) {
+    CovCtx c;
+    c.metric = h->metric;
+    c.stationary = h->nonstationary ? 0 : 1;
+    c.sill = h->sill;
+    c.range = h->range;
+    return c;
+}
+
+static Pts pts(double* const* c, double* const* s) { return Pts{c[0], c[1], c[2], s[0], s[1], s[2]}; }
+
+template <int RP>
+static void launch_eval(mra_handle* h, bool predicting) {
+    hipStream_t st = h->stream;
+    const Tree T = make_tree(h);
+    const CovCtx cc = make_cc(h);
+    const Pts K = pts(h->dKc, h->dKs), O = pts(h->dOc, h->dOs);
+    const Pts Pp = predicting ? pts(h->dPc, h->dPs) : Pts{};
+    (void)hipEventRecord(h->ev[0], st);
+    // scaled parameters
+    struct S {
+        uint64_t n;
+        double **f, **s;
+    } sets[3] = {{h->nC * h->RP, h->dKf, h->dKs}, {h->nObs, h->dOf, h->dOs}, {h->nPred, h->dPf, h->dPs}};
+    for (int k = 0; k < (predicting ? 3 : 2); k++)
+        if (sets[k].n)
+            k_scale<<<nblk(sets[k].n), 256, 0, st>>>(sets[k].n, sets[k].f[0], sets[k].f[1], sets[k].f[2],
+                                                     cc.stationary, h->sill, h->range, h->nugget, h->ss, h->rs, h->ns,
+                                                     sets[k].s[0], sets[k].s[1], sets[k].s[2]);
+    k_fill<<<nblk(h->nC + h->nF), 256, 0, st>>>(h->dDterm, h->nC + h->nF, 0.0);
+    k_zero_int<<<1, 64, 0, st>>>(h->dBad, 1);
+    // coarse prior, level by level
+    for (int l = 0; l < h->L; l++) {
+        const uint64_t nl = h->lvlStart[l + 1] - h->lvlStart[l];
+        k_prior_coarse<RP><<<(unsigned)nl, 256, 0, st>>>(T, cc, K, l, h->lvlStart[l], h->dGoff, h->dGc, h->dLt,
+                                                         h->dLiT, h->dBad);
+    }
+    (void)hipEventRecord(h->ev[1], st);
+    if (h->nItemsX)
+        k_finest_cross<RP><<<h->nItemsX, 256, 0, st>>>(T, cc, K, O, h->dItemsX, h->dGoff, h->dGc, h->dLiT, h->dGf,
+                                                       h->dBf);
+    if (h->nItemsF)
+        k_finest_local<RP><<<h->nItemsF, 256, 0, st>>>(T, cc, O, h->dY, h->dItemsF, h->dGf, h->dBf, h->dSig, h->dH,
+                                                       h->dU, h->dDterm, h->dBad);
+    (void)hipEventRecord(h->ev[2], st);
+    PredArgs pa{};
+    if (predicting) {
+        pa.goff = h->dGoff;
+        pa.Gc = h->dGc;
+        pa.LiT = h->dLiT;
+        pa.Gf = h->dGf;
+        pa.Sig = h->dSig;
+        pa.H = h->dH;
+        pa.u = h->dU;
+        pa.Bt = h->dBt;
+        pa.Gh = h->dGh;
+        pa.GL = h->dGL;
+        pa.mean = h->dMean;
+        pa.var = h->dVar;
+        pa.LpiT = h->dLpiT;
+        pa.P = h->dP;
+        pa.v = h->dV;
+        for (size_t c = 0; c + 1 < h->predChunkStart.size(); c++) {
+            const uint32_t b = h->predChunkStart[c], e = h->predChunkStart[c + 1];
+            if (e == b) continue;
+            pa.items = h->dItemsPred + 3 * (uint64_t)b;
+            pa.gl_off = h->dGlOff + b;
+            k_pred_pre<RP><<<e - b, 256, 0, st>>>(T, cc, K, O, Pp, pa);
+        }
+    }
+    (void)hipEventRecord(h->ev[3], st);
+    // posterior, finest-but-one level up to the root
+    for (int l = h->L - 1; l >= 0; l--) {
+        if (!h->nItemsPost[l]) continue;
+        PostArgs a{};
+        a.l = l;
+        a.start_l = h->lvlStart[l];
+        a.items = h->dItemsPost[l];
+        a.goff = h->dGoff;
+        a.Lt = h->dLt;
+        a.Lpt = h->dLpt;
+        a.LpiT = h->dLpiT;
+        a.P = h->dP;
+        a.v = h->dV;
+        a.dterm = h->dDterm;
+        a.H = h->dH;
+        a.u = h->dU;
+        a.wo = l > 0 ? h->dW[l] : nullptr;
+        a.Ao = l > 0 ? h->dA[l] : nullptr;
+        a.bad_flag = h->dBad;
+        if (l == h->L - 1) {
+            k_posterior<RP, true><<<h->nItemsPost[l], 256, 0, st>>>(T, a);
+        } else {
+            a.wc = h->dW[l + 1];
+            a.Ac = h->dA[l + 1];
+            a.start_c = h->lvlStart[l + 1];
+            k_posterior<RP, false><<<h->nItemsPost[l], 256, 0, st>>>(T, a);
+        }
+    }
+    (void)hipEventRecord(h->ev[4], st);
+    if (predicting) {
+        for (size_t c = 0; c + 1 < h->predChunkStart.size(); c++) {
+            const uint32_t b = h->predChunkStart[c], e = h->predChunkStart[c + 1];
+            if (e == b) continue;
+            pa.items = h->dItemsPred + 3 * (uint64_t)b;
+            pa.gl_off = h->dGlOff + b;
+            k_pred_post<RP><<<e - b, 256, 0, st>>>(T, pa);
+        }
+    }
+    (void)hipEventRecord(h->ev[5], st);
+    k_reduce<<<1, 256, 0, st>>>(h->dDterm + h->nC, h->nF, h->dLev);
+    for (int l = h->L - 1; l >= 0; l--)
+        k_reduce<<<1, 256, 0, st>>>(h->dDterm + h->lvlStart[l], h->lvlStart[l + 1] - h->lvlStart[l],
+                                    h->dLev + (h->L - l));
+    k_total<<<1, 64, 0, st>>>(h->dLev, h->M, h->dD);
+    (void)hipEventRecord(h->ev[6], st);
+}
+
+static int run_eval(mra_handle* h, bool predicting) {
+    if (!h->covSet || !h->paramsSet) {
+        set_err(h, "covariance model and parameters must be set before evaluating");
+        return MRA_E_STATE;
+    }
+    switch (h->RP) {
+        case 16: launch_eval<16>(h, predicting); break;
+        case 32: launch_eval<32>(h, predicting); break;
+        default: launch_eval<64>(h, predicting); break;
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (int i = 0; i < 6; i++) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, h->ev[i], h->ev[i + 1]);
+        h->timings[i] = ms;
+    }
+    float tot = 0;
+    (void)hipEventElapsedTime(&tot, h->ev[0], h->ev[6]);
+    h->timings[6] = tot;
+    int bad = 0;
+    HIP_TRY(hipMemcpy(&bad, h->dBad, sizeof(int), hipMemcpyDeviceToHost));
+    if (bad) {
+        set_err(h, "a Cholesky factorisation met a non-positive pivot (matrix not positive definite)");
+        return MRA_E_NOT_PD;
+    }
+    return MRA_OK;
+}
+
+extern "C" int mra_likelihood(mra_handle* h, double* D) {
+    if (!h || !D) return MRA_E_ARG;
+    RC_TRY(run_eval(h, false));
+    HIP_TRY(hipMemcpy(D, h->dD, sizeof(double), hipMemcpyDeviceToHost));
+    return MRA_OK;
+}
+
+extern "C" int mra_likelihood_levels(mra_handle* h, double* terms) {
+    if (!h || !terms) return MRA_E_ARG;
+    RC_TRY(run_eval(h, false));
+    HIP_TRY(hipMemcpy(terms, h->dLev, sizeof(double) * h->M, hipMemcpyDeviceToHost));
+    return MRA_OK;
+}
+
+extern "C" int mra_predict(mra_handle* h, double* D, double* mean, double* var) {
+    if (!h) return MRA_E_ARG;
+    if (!h->nPred) {
+        set_err(h, "the structure has no prediction points");
+        return MRA_E_STATE;
+    }
+    RC_TRY(run_eval(h, true));
+    if (D) HIP_TRY(hipMemcpy(D, h->dD, sizeof(double), hipMemcpyDeviceToHost));
+    if (mean) HIP_TRY(hipMemcpy(mean, h->dMean, sizeof(double) * h->nPred, hipMemcpyDeviceToHost));
+    if (var) HIP_TRY(hipMemcpy(var, h->dVar, sizeof(double) * h->nPred, hipMemcpyDeviceToHost));
+    return MRA_OK;
+}
+
+extern "C" int mra_sample_prior(mra_handle* h, uint64_t seed, double* y, int replace) {
+    (void)seed;
+    (void)y;
+    (void)replace;
+    if (!h) return MRA_E_ARG;
+    set_err(h, "mra_sample_prior is not available yet");
+    return MRA_E_UNSUPPORTED;
+}
+
+// ------------------------------------------------- single-kernel entries
+struct TmpPts {
+    std::vector<void*> list;
+    double *c[3] = {}, *f[3] = {}, *s[3] = {};
+    ~TmpPts() {
+        for (void* p : list) (void)hipFree(p);
+    }
+};
+
+static int tmp_points(mra_handle* h, const mra_cov_desc* cov, double sill, double range, double nugget, double ss,
+                      double rs, double ns, const double* lon, const double* lat, uint64_t n, TmpPts& t) {
+    double *dl, *da;
+    RC_TRY(dalloc(h, &dl, n, t.list));
+    RC_TRY(dalloc(h, &da, n, t.list));
+    HIP_TRY(hipMemcpy(dl, lon, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(da, lat, sizeof(double) * n, hipMemcpyHostToDevice));
+    for (int k = 0; k < 3; k++) {
+        RC_TRY(dalloc(h, &t.c[k], n, t.list));
+        RC_TRY(dalloc(h, &t.f[k], n, t.list));
+        RC_TRY(dalloc(h, &t.s[k], n, t.list));
+    }
+    FieldDev f[3];
+    std::vector<void*> saved;
+    saved.swap(h->fieldAllocs);
+    int rc = field_dev(h, cov->sigmasq, f[0]);
+    if (!rc) rc = field_dev(h, cov->beta, f[1]);
+    if (!rc) rc = field_dev(h, cov->tausq, f[2]);
+    for (void* p : h->fieldAllocs) t.list.push_back(p);
+    h->fieldAllocs.swap(saved);
+    if (rc) return rc;
+    k_coords<<<nblk(n), 256>>>(n, dl, da, cov->metric, t.c[0], t.c[1], t.c[2]);
+    k_fields<<<nblk(n), 256>>>(n, dl, da, cov->metric, cov->wendland_len, f[0], f[1], f[2], t.f[0], t.f[1], t.f[2]);
+    k_scale<<<nblk(n), 256>>>(n, t.f[0], t.f[1], t.f[2], cov->nonstationary ? 0 : 1, sill, range, nugget, ss, rs, ns,
+                              t.s[0], t.s[1], t.s[2]);
+    HIP_TRY(hipGetLastError());
+    return MRA_OK;
+}
+
+extern "C" int mra_cross_covariance(const mra_cov_desc* cov, double sill, double range, double nugget,
+                                    double sill_scale, double range_scale, double nugget_scale, const double* lon1,
+                                    const double* lat1, uint64_t n1, const double* lon2, const double* lat2,
+                                    uint64_t n2, double* out) {
+    mra_handle tmp;
+    mra_handle* h = &tmp;
+    if (!cov || !out || !n1 || !n2 || !lon1 || !lat1 || !lon2 || !lat2) {
+        g_err = "invalid argument";
+        return MRA_E_ARG;
+    }
+    TmpPts a, b;
+    RC_TRY(tmp_points(h, cov, sill, range, nugget, sill_scale, range_scale, nugget_scale, lon1, lat1, n1, a));
+    RC_TRY(tmp_points(h, cov, sill, range, nugget, sill_scale, range_scale, nugget_scale, lon2, lat2, n2, b));
+    double* d;
+    std::vector<void*> l;
+    RC_TRY(dalloc(h, &d, n1 * n2, l));
+    CovCtx cc{cov->metric, cov->nonstationary ? 0 : 1, sill, range};
+    k_cross_cov<<<nblk(n1 * n2), 256>>>(cc, Pts{a.c[0], a.c[1], a.c[2], a.s[0], a.s[1], a.s[2]}, n1,
+                                        Pts{b.c[0], b.c[1], b.c[2], b.s[0], b.s[1], b.s[2]}, n2, d);
+    hipError_t e = hipMemcpy(out, d, sizeof(double) * n1 * n2, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) {
+        g_err = hipGetErrorString(e);
+        return MRA_E_HIP;
+    }
+    return MRA_OK;
+}
+
+extern "C" int mra_variance_covariance(const mra_cov_desc* cov, double sill, double range, double nugget,
+                                       double sill_scale, double range_scale, double nugget_scale, const double* lon,
+                                       const double* lat, uint64_t n, double* out) {
+    mra_handle tmp;
+    mra_handle* h = &tmp;
+    if (!cov || !out || !n || !lon || !lat) {
+        g_err = "invalid argument";
+        return MRA_E_ARG;
+    }
+    TmpPts a;
+    RC_TRY(tmp_points(h, cov, sill, range, nugget, sill_scale, range_scale, nugget_scale, lon, lat, n, a));
+    double* d;
+    std::vector<void*> l;
+    RC_TRY(dalloc(h, &d, n * n, l));
+    CovCtx cc{cov->metric, cov->nonstationary ? 0 : 1, sill, range};
+    k_var_cov<<<nblk(n * n), 256>>>(cc, Pts{a.c[0], a.c[1], a.c[2], a.s[0], a.s[1], a.s[2]}, n, d);
+    hipError_t e = hipMemcpy(out, d, sizeof(double) * n * n, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) {
+        g_err = hipGetErrorString(e);
+        return MRA_E_HIP;
+    }
+    return MRA_OK;
+}
+
+extern "C" int mra_cov_tile_device(int precision, int metric, const double* d_xyz1, const double* d_sb1, uint64_t n1,
+                                   const double* d_xyz2, const double* d_sb2, uint64_t n2, void* d_out,
+                                   void* stream) {
+    if (!d_xyz1 || !d_sb1 || !d_xyz2 || !d_sb2 || !d_out || !n1 || !n2 || metric < 0 || metric > 2) {
+        g_err = "invalid argument";
+        return MRA_E_ARG;
+    }
+    CovCtx cc{metric, 0, 1.0, 1.0};
+    // SoA: xyz1 = [x(n1) | y(n1) | z(n1)], sb1 = [s(n1) | b(n1)]
+    Pts P{d_xyz1, d_xyz1 + n1, d_xyz1 + 2 * n1, d_sb1, d_sb1 + n1, nullptr};
+    Pts Q{d_xyz2, d_xyz2 + n2, d_xyz2 + 2 * n2, d_sb2, d_sb2 + n2, nullptr};
+    dim3 grid((unsigned)((n2 + 63) / 64), (unsigned)((n1 + 63) / 64));
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == MRA_PRECISION_MIXED)
+        k_cov_tile<float><<<grid, 256, 0, st>>>(cc, P, n1, Q, n2, (float*)d_out);
+    else
+        k_cov_tile<double><<<grid, 256, 0, st>>>(cc, P, n1, Q, n2, (double*)d_out);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        g_err = hipGetErrorString(e);
+        return MRA_E_HIP;
+    }
+    return MRA_OK;
+}

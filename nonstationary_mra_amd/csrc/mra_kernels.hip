@@ -1,0 +1,769 @@
+// HIP kernels of the MRA likelihood / prediction path (gfx950).
+//
+// One launch per level and phase, batched over every region of the level
+// (the reference runs an OpenMP loop per level over the same regions:
+// src/class_approximation.cpp:108-159,
+// src/class_approximation-loop-regions-at-finest-level-in-prior.cpp:52-400,
+// src/class_approximation-posterior-inference.cpp:51-219,
+// src/class_approximation-predict.cpp:105-189).
+//
+// Layouts (RP = knots per coarse region padded to a multiple of 16; pad
+// knots have zero covariance and identity on the knot-matrix diagonal):
+//   Gc   coarse G_{R,j}: level-l region block of l*RP*RP, row m (whitened
+//        ancestor knot, stacked over j) x column c (region knot)
+//   Lt   prior Cholesky factor L_R stored column-major (Lt[k*RP+i] = L[i][k])
+//   LiT  L_R^{-1} column-major;   Lpt / LpiT: the posterior factor and inverse
+//   Gf, Bf  finest G_{F,j} / B_{F,j}: region block at L*RP*obs_off[F],
+//        element (m, o) at m*n_F + o
+//   H    L_F^{-1} B_F^T, observation-major rows of L*RP
+//   Sig  L_F (row-major n_F x n_F) at sig_off[F]
+//   msgs w~ (l*RP) and A~ (l(l+1)/2 slices of RP*RP, slice order
+//        (0,0),(0,1)..(0,l-1),(1,1).. as src/class_approximation.cpp:62-73)
+#include "mra_device.hpp"
+
+namespace mra {
+
+struct Tree {
+    int M, J, L, r, RP, LRP;
+    uint64_t nC, nF;
+    const uint64_t* parent;   // [nF] global coarse id
+    const uint64_t* obs_off;  // [nF+1]
+    const uint64_t* pred_off; // [nF+1] or null
+    const uint64_t* ch_s;     // [n level L-1] first finest child (local), ch_e < ch_s: none
+    const uint64_t* ch_e;
+    const uint8_t* working;   // [nC]
+    const uint64_t* sig_off;  // [nF]
+};
+
+__device__ __forceinline__ uint64_t lvl_start(int l, int J) {
+    uint64_t s = 0, c = 1;
+    for (int i = 0; i < l; i++) {
+        s += c;
+        c *= (uint64_t)J;
+    }
+    return s;
+}
+
+// Ancestor at level j of a region at level l (j <= l).
+__device__ __forceinline__ uint64_t ancestor_up(uint64_t R, int l, int j, int J) {
+    for (int i = j; i < l; i++) R = (R - 1) / (uint64_t)J;
+    return R;
+}
+
+// ------------------------------------------------------------- point setup
+__global__ void k_coords(uint64_t n, const double* lon, const double* lat, int metric, double* x, double* y,
+                         double* z) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double lo = lon[i], la = lat[i];
+    if (metric == 0) {
+        // src/class_partition.cpp:507-517
+        double cosLat = cos(la * M_PI / 180), cosLon = cos(lo * M_PI / 180);
+        double sinLat = sin(la * M_PI / 180), sinLon = sin(lo * M_PI / 180);
+        x[i] = cosLat * cosLon;
+        y[i] = cosLat * sinLon;
+        z[i] = sinLat;
+    } else if (metric == 1) {
+        double lar = la * (M_PI / 180);
+        x[i] = sin(lar);
+        y[i] = cos(lar);
+        z[i] = lo * (M_PI / 180);
+    } else {
+        x[i] = lo;
+        y[i] = la;
+        z[i] = 0.0;
+    }
+}
+
+struct FieldDev {
+    int kind;
+    int nb;
+    double intercept, sx, sy;
+    const double *bx, *by, *bz, *blon, *blat, *w;
+};
+
+// exp(log field) at one point (src/class_nonstationary_covariance.cpp:479-573).
+__device__ double field_eval(const FieldDev& f, int metric, double wlen, double lon, double lat, double x, double y,
+                             double z) {
+    if (f.kind == 2) return exp(f.intercept + f.sx * lon + f.sy * lat);
+    double v = f.intercept;
+    if (f.kind == 1) {
+        for (int i = 0; i < f.nb; i++) {
+            double d;
+            if (metric == 0) {
+                double dx = f.bx[i] - x, dy = f.by[i] - y, dz = f.bz[i] - z;
+                d = sqrt(dx * dx + dy * dy + dz * dz);
+            } else if (metric == 1) {
+                double l1 = lon * (M_PI / 180), l2 = f.blon[i] * (M_PI / 180);
+                double a1 = lat * (M_PI / 180), a2 = f.blat[i] * (M_PI / 180);
+                double ang = sin(a1) * sin(a2) + cos(a1) * cos(a2) * cos(l1 - l2);
+                ang = fmin(1.0, fmax(-1.0, ang));
+                d = acos(ang);
+            } else {
+                double dx = f.blon[i] - lon, dy = f.blat[i] - lat;
+                d = sqrt(dx * dx + dy * dy);
+            }
+            d /= wlen;
+            if (d >= 1) continue;
+            double om = 1 - d;
+            double om6 = om * om * om;
+            om6 *= om6;
+            v += om6 * (35 * d * d + 18 * d + 3) * f.w[i] / 3;
+        }
+    }
+    return exp(v);
+}
+
+__global__ void k_fields(uint64_t n, const double* lon, const double* lat, int metric, double wlen, FieldDev f0,
+                         FieldDev f1, FieldDev f2, double* o0, double* o1, double* o2) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double lo = lon[i], la = lat[i];
+    double cosLat = cos(la * M_PI / 180), cosLon = cos(lo * M_PI / 180);
+    double sinLat = sin(la * M_PI / 180), sinLon = sin(lo * M_PI / 180);
+    double x = cosLat * cosLon, y = cosLat * sinLon, z = sinLat;
+    o0[i] = field_eval(f0, metric, wlen, lo, la, x, y, z);
+    o1[i] = field_eval(f1, metric, wlen, lo, la, x, y, z);
+    o2[i] = field_eval(f2, metric, wlen, lo, la, x, y, z);
+}
+
+// Scaled per-point parameters (src/evaluate_covariance.cpp:170-189,258-269).
+__global__ void k_scale(uint64_t n, const double* f0, const double* f1, const double* f2, int stationary, double sill,
+                        double range, double nugget, double ss, double rs, double ns, double* s, double* b,
+                        double* v) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (stationary) {
+        s[i] = sqrt(sill);
+        b[i] = range;
+        v[i] = sill + nugget;
+    } else {
+        double sg = ss * f0[i];
+        s[i] = sqrt(sg);
+        b[i] = rs * f1[i];
+        v[i] = sg + ns * f2[i];
+    }
+}
+
+__global__ void k_zero_int(int* p, uint64_t n) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__global__ void k_fill(double* p, uint64_t n, double val) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = val;
+}
+
+// ------------------------------------------------------ covariance tiles
+// Standalone pairwise covariance tile (throughput entry point).
+template <typename T>
+__global__ __launch_bounds__(256) void k_cov_tile(CovCtx cc, Pts P, uint64_t n1, Pts Q, uint64_t n2, T* out) {
+    // 64 x 64 output tile per work group, each thread 16 outputs, coalesced rows
+    const uint64_t i0 = blockIdx.y * 64ull, j0 = blockIdx.x * 64ull;
+    const int tj = threadIdx.x & 63, ti = threadIdx.x >> 6;
+    const uint64_t j = j0 + tj;
+    if (j >= n2) return;
+    const double x2 = Q.x[j], y2 = Q.y[j], z2 = Q.z[j], s2 = Q.s[j], b2 = Q.b[j];
+    for (int k = 0; k < 16; k++) {
+        const uint64_t i = i0 + ti * 16 + k;
+        if (i >= n1) break;
+        double d = pdist(cc, P.x[i], P.y[i], P.z[i], x2, y2, z2);
+        double c;
+        if (cc.stationary) {
+            c = cc.sill * exp(-d / cc.range);
+        } else {
+            double b1 = P.b[i];
+            double bms = (b1 * b1 + b2 * b2) * 0.5;
+            double ratio = (b1 * b2) / bms;
+            c = (P.s[i] * s2) * (ratio * sqrt(ratio)) * exp(-d / sqrt(bms));
+        }
+        out[i * n2 + j] = (T)c;
+    }
+}
+
+template <>
+__global__ __launch_bounds__(256) void k_cov_tile<float>(CovCtx cc, Pts P, uint64_t n1, Pts Q, uint64_t n2,
+                                                          float* out) {
+    const uint64_t i0 = blockIdx.y * 64ull, j0 = blockIdx.x * 64ull;
+    const int tj = threadIdx.x & 63, ti = threadIdx.x >> 6;
+    const uint64_t j = j0 + tj;
+    if (j >= n2) return;
+    for (int k = 0; k < 16; k++) {
+        const uint64_t i = i0 + ti * 16 + k;
+        if (i >= n1) break;
+        out[i * n2 + j] = pcovf(cc, P, i, Q, j);
+    }
+}
+
+// Variance-covariance of a point set (nugget on the diagonal).
+__global__ void k_var_cov(CovCtx cc, Pts P, uint64_t n, double* out) {
+    uint64_t idx = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (idx >= n * n) return;
+    uint64_t i = idx / n, j = idx % n;
+    out[idx] = (i == j) ? P.v[i] : pcov(cc, P, i, P, j);
+}
+
+__global__ void k_cross_cov(CovCtx cc, Pts P, uint64_t n1, Pts Q, uint64_t n2, double* out) {
+    uint64_t idx = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (idx >= n1 * n2) return;
+    uint64_t i = idx / n2, j = idx % n2;
+    out[idx] = pcov(cc, P, i, Q, j);
+}
+
+// --------------------------------------------------------- coarse prior
+// One work group per region R of level l.  For j < l:
+//   B = C(Q_{a_j}, Q_R) - Gcat_{a_j}^T Gcat_R[0:j]      (MFMA, K = j*RP)
+//   G_{R,j} = L_{a_j}^{-1} B                            (MFMA with L^{-1})
+// then K_R = V(Q_R) - Gcat_R^T Gcat_R, L_R = chol(K_R), L_R^{-1}.
+// (src/class_approximation.cpp:108-159)
+template <int RP>
+__global__ __launch_bounds__(256) void k_prior_coarse(Tree T, CovCtx cc, Pts K, int l, uint64_t start_l,
+                                                      const uint64_t* goff, double* Gc, double* Lt, double* LiT,
+                                                      int* bad_flag) {
+    constexpr int LD = RP + 1;
+    __shared__ double Cm[RP * LD];
+    __shared__ double Wm[RP * LD];
+    __shared__ int bad;
+    const uint64_t R = start_l + blockIdx.x;
+    const int r = T.r;
+    double* GR = Gc + goff[l] + (uint64_t)blockIdx.x * l * RP * RP;
+    if (threadIdx.x == 0) bad = 0;
+    for (int j = 0; j < l; j++) {
+        const uint64_t a = ancestor_up(R, l, j, T.J);
+        for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
+            const int i = idx / RP, c = idx % RP;
+            Cm[i * LD + c] = (i < r && c < r) ? pcov(cc, K, a * RP + i, K, R * RP + c) : 0.0;
+            Wm[i * LD + c] = 0.0;
+        }
+        __syncthreads();
+        if (j > 0) {
+            const double* Ga = Gc + goff[j] + (a - lvl_start(j, T.J)) * (uint64_t)j * RP * RP;
+            wg_gemm_tn(Cm, LD, RP, RP, j * RP, -1.0, Ga, RP, RP, GR, RP, RP);
+            __syncthreads();
+        }
+        wg_gemm_tn(Wm, LD, RP, RP, RP, 1.0, LiT + a * RP * RP, RP, RP, Cm, LD, RP);
+        __syncthreads();
+        double* Gj = GR + (uint64_t)j * RP * RP;
+        for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) Gj[idx] = Wm[(idx / RP) * LD + idx % RP];
+        __syncthreads();
+    }
+    for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
+        const int i = idx / RP, c = idx % RP;
+        double v;
+        if (i < r && c < r)
+            v = (i == c) ? K.v[R * RP + i] : pcov(cc, K, R * RP + i, K, R * RP + c);
+        else
+            v = (i == c) ? 1.0 : 0.0;
+        Cm[i * LD + c] = v;
+        Wm[i * LD + c] = (i == c) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (l > 0) {
+        wg_gemm_tn(Cm, LD, RP, RP, l * RP, -1.0, GR, RP, RP, GR, RP, RP);
+        __syncthreads();
+    }
+    wg_chol(Cm, LD, RP, &bad);
+    wg_trsm(Cm, LD, Wm, LD, RP, RP);
+    for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
+        const int k = idx / RP, i = idx % RP;  // column-major: [k*RP + i] = M[i][k]
+        Lt[R * RP * RP + idx] = (i >= k) ? Cm[i * LD + k] : 0.0;
+        LiT[R * RP * RP + idx] = (i >= k) ? Wm[i * LD + k] : 0.0;
+    }
+    if (threadIdx.x == 0 && bad) atomicOr(bad_flag, 1);
+}
+
+// ---------------------------------------------------------- finest level
+// Observation columns of one finest region, tiles of 64 columns:
+//   B_{F,j} = C(Q_{a_j}, S) - Gcat_{a_j}^T Gcat_F[0:j],  G_{F,j} = L_{a_j}^{-1} B_{F,j}
+// (…finest-level-in-prior.cpp:116-163)
+template <int RP>
+__global__ __launch_bounds__(256) void k_finest_cross(Tree T, CovCtx cc, Pts K, Pts O, const uint32_t* items,
+                                                      const uint64_t* goff, const double* Gc, const double* LiT,
+                                                      double* Gf, double* Bf) {
+    constexpr int NT = 64, LD = NT + 1;
+    __shared__ double Cm[RP * LD];
+    __shared__ double Wm[RP * LD];
+    const uint32_t F = items[2 * blockIdx.x], c0 = items[2 * blockIdx.x + 1];
+    const uint64_t ob = T.obs_off[F];
+    const int nF = (int)(T.obs_off[F + 1] - ob);
+    const int n = min(NT, nF - (int)c0);
+    const int r = T.r, L = T.L;
+    double* GF = Gf + (uint64_t)T.LRP * ob;
+    double* BF = Bf + (uint64_t)T.LRP * ob;
+    for (int j = 0; j < L; j++) {
+        const uint64_t a = ancestor_up(T.parent[F], L - 1, j, T.J);
+        for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
+            const int i = idx / NT, c = idx % NT;
+            Cm[i * LD + c] = (i < r && c < n) ? pcov(cc, K, a * RP + i, O, ob + c0 + c) : 0.0;
+            Wm[i * LD + c] = 0.0;
+        }
+        __syncthreads();
+        if (j > 0) {
+            const double* Ga = Gc + goff[j] + (a - lvl_start(j, T.J)) * (uint64_t)j * RP * RP;
+            wg_gemm_tn(Cm, LD, RP, NT, j * RP, -1.0, Ga, RP, RP, GF + c0, nF, n);
+            __syncthreads();
+        }
+        for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
+            const int i = idx / NT, c = idx % NT;
+            if (c < n) BF[(uint64_t)(j * RP + i) * nF + c0 + c] = Cm[i * LD + c];
+        }
+        wg_gemm_tn(Wm, LD, RP, NT, RP, 1.0, LiT + a * RP * RP, RP, RP, Cm, LD, NT);
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
+            const int i = idx / NT, c = idx % NT;
+            if (c < n) GF[(uint64_t)(j * RP + i) * nF + c0 + c] = Wm[i * LD + c];
+        }
+        __syncthreads();
+    }
+}
+
+// Per finest region: Sigma_F = V(S) - G^T G, L_F = chol, u = L_F^{-1} y,
+// H = L_F^{-1} B^T, d_F = |u|^2 + 2 sum log diag L_F.
+// (…finest-level-in-prior.cpp:165-256)
+template <int RP>
+__global__ __launch_bounds__(256) void k_finest_local(Tree T, CovCtx cc, Pts O, const double* yv,
+                                                      const uint32_t* items, double* Gf, double* Bf, double* Sig,
+                                                      double* H, double* u, double* dterm, int* bad_flag) {
+    constexpr int LDS_LD = 65;
+    __shared__ double Sm[64 * LDS_LD];
+    __shared__ double red[MRA_WG];
+    __shared__ int bad;
+    const uint32_t F = items[blockIdx.x];
+    const uint64_t ob = T.obs_off[F];
+    const int n = (int)(T.obs_off[F + 1] - ob);
+    const int LRP = T.LRP;
+    const bool small = n <= 64;
+    double* Sg = small ? Sm : Sig + T.sig_off[F];
+    const int ld = small ? LDS_LD : n;
+    const double* GF = Gf + (uint64_t)LRP * ob;
+    double* BF = Bf + (uint64_t)LRP * ob;
+    if (threadIdx.x == 0) bad = 0;
+    for (int idx = threadIdx.x; idx < n * n; idx += MRA_WG) {
+        const int i = idx / n, c = idx % n;
+        Sg[i * ld + c] = (i == c) ? O.v[ob + i] : pcov(cc, O, ob + i, O, ob + c);
+    }
+    __syncthreads();
+    for (int ti = 0; ti < n; ti += 64)
+        for (int tj = 0; tj < n; tj += 64) {
+            const int mv = min(64, n - ti), nv = min(64, n - tj);
+            wg_gemm_tn(Sg + ti * ld + tj, ld, (mv + 15) & ~15, (nv + 15) & ~15, LRP, -1.0, GF + ti, n, mv, GF + tj, n,
+                       nv);
+        }
+    __syncthreads();
+    wg_chol(Sg, ld, n, &bad);
+    // u = L_F^{-1} y
+    double* uF = u + ob;
+    for (int i = threadIdx.x; i < n; i += MRA_WG) uF[i] = yv[ob + i];
+    __syncthreads();
+    wg_trsm(Sg, ld, uF, 1, n, 1);
+    double part = 0.0;
+    for (int i = threadIdx.x; i < n; i += MRA_WG) part += uF[i] * uF[i] + 2.0 * log(Sg[i * ld + i]);
+    double dF = wg_sum(part, red);
+    // H rows: solve L_F h = b_m for every row m of B_F (n values each)
+    for (int m = threadIdx.x; m < LRP; m += MRA_WG) {
+        double* b = BF + (uint64_t)m * n;
+        for (int i = 0; i < n; i++) {
+            double s = b[i];
+            for (int k = 0; k < i; k++) s -= Sg[i * ld + k] * b[k];
+            s /= Sg[i * ld + i];
+            b[i] = s;
+            H[(ob + i) * LRP + m] = s;
+        }
+    }
+    // keep L_F for prediction
+    if (small) {
+        double* dst = Sig + T.sig_off[F];
+        for (int idx = threadIdx.x; idx < n * n; idx += MRA_WG) dst[idx] = Sm[(idx / n) * LDS_LD + idx % n];
+    }
+    if (threadIdx.x == 0) {
+        dterm[T.nC + F] = dF;
+        if (bad) atomicOr(bad_flag, 1);
+    }
+}
+
+// -------------------------------------------------------------- posterior
+// One work group per working region R of level l:
+//   A_{jk} = sum over children (from the stacked finest rows H when l = L-1,
+//            else from the children's A~ messages)
+//   L~ = chol(L L^T + A_{ll}), v = L~^{-1} w_l, P_j = L~^{-1} A_{jl}^T,
+//   w~_j = w_j - P_j^T v, A~_{jk} = A_{jk} - P_j^T P_k   (j <= k < l)
+//   d_R = 2 sum log diag L~ - 2 sum log diag L - v^T v
+// (src/class_approximation-posterior-inference.cpp:51-131,150-214;
+//  src/class_approximation.cpp:186-306)
+struct PostArgs {
+    int l;
+    uint64_t start_l;
+    const uint32_t* items;
+    const uint64_t* goff;
+    const double* Lt;
+    double* Lpt;
+    double* LpiT;
+    double* P;
+    double* v;
+    double* dterm;
+    // children messages (level l+1), or the finest rows
+    const double* wc;
+    const double* Ac;
+    uint64_t start_c;
+    const double* H;
+    const double* u;
+    // outputs (level l)
+    double* wo;
+    double* Ao;
+    int* bad_flag;
+};
+
+template <int RP, bool FROM_OBS>
+__device__ __forceinline__ void post_add_A(const Tree& T, const PostArgs& a, uint64_t R, int j, int k, double* C,
+                                           int ld, bool transpose) {
+    // C += A_{jk} (or its transpose)
+    if (FROM_OBS) {
+        const uint64_t loc = R - a.start_l;
+        if (T.ch_e[loc] < T.ch_s[loc]) return;
+        const uint64_t os = T.obs_off[T.ch_s[loc]], oe = T.obs_off[T.ch_e[loc] + 1];
+        const int ns = (int)(oe - os);
+        const double* X = a.H + os * T.LRP;
+        if (!transpose)
+            wg_gemm_tn(C, ld, RP, RP, ns, 1.0, X + j * RP, T.LRP, RP, X + k * RP, T.LRP, RP);
+        else
+            wg_gemm_tn(C, ld, RP, RP, ns, 1.0, X + k * RP, T.LRP, RP, X + j * RP, T.LRP, RP);
+    } else {
+        const int lc = a.l + 1;
+        const int nslc = lc * (lc + 1) / 2;
+        const int sl = j * lc - j * (j - 1) / 2 + (k - j);
+        for (int c = 0; c < T.J; c++) {
+            const uint64_t ch = R * T.J + 1 + c;
+            if (!T.working[ch]) continue;
+            const double* S = a.Ac + ((ch - a.start_c) * nslc + sl) * (uint64_t)RP * RP;
+            for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
+                const int i = idx / RP, q = idx % RP;
+                C[i * ld + q] += transpose ? S[q * RP + i] : S[idx];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <int RP, bool FROM_OBS>
+__device__ __forceinline__ double post_w(const Tree& T, const PostArgs& a, uint64_t R, int j, int q) {
+    double s = 0.0;
+    if (FROM_OBS) {
+        const uint64_t loc = R - a.start_l;
+        if (T.ch_e[loc] < T.ch_s[loc]) return 0.0;
+        const uint64_t os = T.obs_off[T.ch_s[loc]], oe = T.obs_off[T.ch_e[loc] + 1];
+        for (uint64_t o = os; o < oe; o++) s += a.H[o * T.LRP + j * RP + q] * a.u[o];
+    } else {
+        const int lc = a.l + 1;
+        bool first = true;
+        for (int c = 0; c < T.J; c++) {
+            const uint64_t ch = R * T.J + 1 + c;
+            if (!T.working[ch]) continue;
+            const double x = a.wc[(ch - a.start_c) * (uint64_t)lc * RP + j * RP + q];
+            s = first ? x : s + x;
+            first = false;
+        }
+    }
+    return s;
+}
+
+template <int RP, bool FROM_OBS>
+__global__ __launch_bounds__(256) void k_posterior(Tree T, PostArgs a) {
+    constexpr int LD = RP + 1;
+    __shared__ double Km[RP * LD];
+    __shared__ double Wm[RP * LD];
+    __shared__ double vv[RP];
+    __shared__ double wl[RP];
+    __shared__ double red[MRA_WG];
+    __shared__ int bad;
+    const int l = a.l;
+    const uint64_t R = a.start_l + a.items[blockIdx.x];
+    const int r = T.r;
+    if (threadIdx.x == 0) bad = 0;
+    const double* LR = a.Lt + R * RP * RP;  // column-major L
+    for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
+        Km[(idx / RP) * LD + idx % RP] = 0.0;
+        Wm[(idx / RP) * LD + idx % RP] = ((idx / RP) == (idx % RP)) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    // L L^T
+    wg_gemm_tn(Km, LD, RP, RP, RP, 1.0, LR, RP, RP, LR, RP, RP);
+    __syncthreads();
+    post_add_A<RP, FROM_OBS>(T, a, R, l, l, Km, LD, false);
+    __syncthreads();
+    double lp = 0.0;
+    for (int i = threadIdx.x; i < r; i += MRA_WG) lp += log(LR[i * RP + i]);
+    const double logdet_prior = wg_sum(lp, red);
+    wg_chol(Km, LD, RP, &bad);
+    wg_trsm(Km, LD, Wm, LD, RP, RP);  // Wm = L~^{-1}
+    for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
+        const int k = idx / RP, i = idx % RP;
+        a.Lpt[R * RP * RP + idx] = (i >= k) ? Km[i * LD + k] : 0.0;
+        a.LpiT[R * RP * RP + idx] = (i >= k) ? Wm[i * LD + k] : 0.0;
+    }
+    for (int q = threadIdx.x; q < RP; q += MRA_WG) wl[q] = (q < r) ? post_w<RP, FROM_OBS>(T, a, R, l, q) : 0.0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < RP; i += MRA_WG) {
+        double s = 0.0;
+        for (int k = 0; k <= i; k++) s += Wm[i * LD + k] * wl[k];
+        vv[i] = s;
+        a.v[R * RP + i] = s;
+    }
+    __syncthreads();
+    double lq = 0.0;
+    for (int i = threadIdx.x; i < r; i += MRA_WG) lq += 2.0 * log(Km[i * LD + i]) - vv[i] * vv[i];
+    const double post_part = wg_sum(lq, red);
+    if (threadIdx.x == 0) a.dterm[R] = -2.0 * logdet_prior + post_part;
+    if (l > 0) {
+        double* PR = a.P + a.goff[l] + (R - a.start_l) * (uint64_t)l * RP * RP;
+        const double* LpiTR = a.LpiT + R * RP * RP;
+        // P_j = L~^{-1} A_{jl}^T  (Km reused as scratch for A_{jl}^T)
+        for (int j = 0; j < l; j++) {
+            for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) Km[(idx / RP) * LD + idx % RP] = 0.0;
+            __syncthreads();
+            post_add_A<RP, FROM_OBS>(T, a, R, j, l, Km, LD, true);
+            __syncthreads();
+            for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) Wm[(idx / RP) * LD + idx % RP] = 0.0;
+            __syncthreads();
+            wg_gemm_tn(Wm, LD, RP, RP, RP, 1.0, LpiTR, RP, RP, Km, LD, RP);
+            __syncthreads();
+            double* Pj = PR + (uint64_t)j * RP * RP;
+            for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) Pj[idx] = Wm[(idx / RP) * LD + idx % RP];
+            // w~_j = w_j - P_j^T v
+            for (int q = threadIdx.x; q < RP; q += MRA_WG) {
+                double wj = (q < r) ? post_w<RP, FROM_OBS>(T, a, R, j, q) : 0.0;
+                double s = 0.0;
+                for (int m = 0; m < RP; m++) s += Wm[m * LD + q] * vv[m];
+                a.wo[(R - a.start_l) * (uint64_t)l * RP + j * RP + q] = wj - s;
+            }
+            __syncthreads();
+        }
+        // A~_{jk} = A_{jk} - P_j^T P_k
+        const int nsl = l * (l + 1) / 2;
+        for (int j = 0; j < l; j++)
+            for (int k = j; k < l; k++) {
+                for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) Km[(idx / RP) * LD + idx % RP] = 0.0;
+                __syncthreads();
+                post_add_A<RP, FROM_OBS>(T, a, R, j, k, Km, LD, false);
+                __syncthreads();
+                wg_gemm_tn(Km, LD, RP, RP, RP, -1.0, PR + (uint64_t)j * RP * RP, RP, RP, PR + (uint64_t)k * RP * RP,
+                           RP, RP);
+                __syncthreads();
+                const int sl = j * l - j * (j - 1) / 2 + (k - j);
+                double* out = a.Ao + ((R - a.start_l) * nsl + sl) * (uint64_t)RP * RP;
+                for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) out[idx] = Km[(idx / RP) * LD + idx % RP];
+                __syncthreads();
+            }
+    }
+    if (threadIdx.x == 0 && bad) atomicOr(a.bad_flag, 1);
+}
+
+// ------------------------------------------------------------- prediction
+// Pre-pass per (finest region, tile of 64 prediction points)
+// (…finest-level-in-prior.cpp:271-392): G^_j, E_j (into Bt), var0, mean0, and
+// B~_j = E_j - H_j^T G^_L.
+struct PredArgs {
+    const uint32_t* items;  // (F, p0, scratch slot) triples
+    const uint64_t* goff;
+    const double* Gc;
+    const double* LiT;
+    const double* Gf;
+    const double* Sig;
+    const double* H;
+    const double* u;
+    double* Bt;    // per prediction point block: L*RP*pred_off[F], (m, p) at m*p_F + p
+    double* Gh;    // scratch: slot * L*RP*64
+    double* GL;    // scratch: gl_off[slot] (n_F x 64)
+    const uint64_t* gl_off;
+    double* mean;
+    double* var;
+    // posterior
+    const double* LpiT;
+    const double* P;
+    const double* v;
+};
+
+template <int RP>
+__global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts O, Pts Pp, PredArgs a) {
+    constexpr int NT = 64, LD = NT + 1;
+    __shared__ double Cm[RP * LD];
+    __shared__ double Wm[RP * LD];
+    const uint32_t F = a.items[3 * blockIdx.x], p0 = a.items[3 * blockIdx.x + 1], slot = a.items[3 * blockIdx.x + 2];
+    const uint64_t pb = T.pred_off[F];
+    const int pF = (int)(T.pred_off[F + 1] - pb);
+    const int np = min(NT, pF - (int)p0);
+    const uint64_t ob = T.obs_off[F];
+    const int nF = (int)(T.obs_off[F + 1] - ob);
+    const int r = T.r, L = T.L, LRP = T.LRP;
+    double* Gh = a.Gh + (uint64_t)slot * LRP * NT;
+    double* BtF = a.Bt + (uint64_t)LRP * pb;
+    for (int j = 0; j < L; j++) {
+        const uint64_t an = ancestor_up(T.parent[F], L - 1, j, T.J);
+        for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
+            const int i = idx / NT, c = idx % NT;
+            Cm[i * LD + c] = (i < r && c < np) ? pcov(cc, K, an * RP + i, Pp, pb + p0 + c) : 0.0;
+            Wm[i * LD + c] = 0.0;
+        }
+        __syncthreads();
+        if (j > 0) {
+            const double* Ga = a.Gc + a.goff[j] + (an - lvl_start(j, T.J)) * (uint64_t)j * RP * RP;
+            wg_gemm_tn(Cm, LD, RP, NT, j * RP, -1.0, Ga, RP, RP, Gh, NT, NT);
+            __syncthreads();
+        }
+        for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
+            const int i = idx / NT, c = idx % NT;
+            if (c < np) BtF[(uint64_t)(j * RP + i) * pF + p0 + c] = Cm[i * LD + c];
+        }
+        wg_gemm_tn(Wm, LD, RP, NT, RP, 1.0, a.LiT + an * RP * RP, RP, RP, Cm, LD, NT);
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) Gh[(idx / NT) * NT + idx % NT] = Wm[(idx / NT) * LD + idx % NT];
+        __syncthreads();
+    }
+    // var0 and the conditional cross block with the region's observations
+    double var0 = 0.0, mean0 = 0.0;
+    const int c = threadIdx.x;
+    if (c < np) {
+        double s = 0.0;
+        for (int m = 0; m < LRP; m++) s += Gh[m * NT + c] * Gh[m * NT + c];
+        var0 = Pp.v[pb + p0 + c] - s;
+    }
+    if (nF > 0) {
+        double* GL = a.GL + a.gl_off[slot];
+        const double* GF = a.Gf + (uint64_t)LRP * ob;
+        for (int t0 = 0; t0 < nF; t0 += RP) {  // row tiles of RP (the tile height of Cm)
+            const int mv = min(RP, nF - t0);
+            for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
+                const int i = idx / NT, cc2 = idx % NT;
+                Cm[i * LD + cc2] = (i < mv && cc2 < np) ? pcov(cc, O, ob + t0 + i, Pp, pb + p0 + cc2) : 0.0;
+            }
+            __syncthreads();
+            wg_gemm_tn(Cm, LD, (mv + 15) & ~15, NT, LRP, -1.0, GF + t0, nF, mv, Gh, NT, NT);
+            __syncthreads();
+            for (int idx = threadIdx.x; idx < mv * NT; idx += MRA_WG)
+                GL[(uint64_t)(t0 + idx / NT) * NT + idx % NT] = Cm[(idx / NT) * LD + idx % NT];
+            __syncthreads();
+        }
+        // GL <- L_F^{-1} GL, one column per thread
+        const double* LF = a.Sig + T.sig_off[F];
+        if (c < NT) {
+            for (int i = 0; i < nF; i++) {
+                double s = GL[(uint64_t)i * NT + c];
+                for (int k = 0; k < i; k++) s -= LF[(uint64_t)i * nF + k] * GL[(uint64_t)k * NT + c];
+                s /= LF[(uint64_t)i * nF + i];
+                GL[(uint64_t)i * NT + c] = s;
+            }
+            if (c < np) {
+                double sv = 0.0, sm = 0.0;
+                for (int i = 0; i < nF; i++) {
+                    const double g = GL[(uint64_t)i * NT + c];
+                    sv += g * g;
+                    sm += g * a.u[ob + i];
+                }
+                var0 -= sv;
+                mean0 = sm;
+            }
+        }
+        __syncthreads();
+        // B~_j = E_j - H_j^T G^_L
+        for (int j = 0; j < L; j++) {
+            for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
+                const int i = idx / NT, cc2 = idx % NT;
+                Cm[i * LD + cc2] = (cc2 < np) ? BtF[(uint64_t)(j * RP + i) * pF + p0 + cc2] : 0.0;
+            }
+            __syncthreads();
+            wg_gemm_tn(Cm, LD, RP, NT, nF, -1.0, a.H + ob * LRP + j * RP, LRP, RP, GL, NT, NT);
+            __syncthreads();
+            for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
+                const int i = idx / NT, cc2 = idx % NT;
+                if (cc2 < np) BtF[(uint64_t)(j * RP + i) * pF + p0 + cc2] = Cm[i * LD + cc2];
+            }
+            __syncthreads();
+        }
+    }
+    if (c < np) {
+        a.mean[pb + p0 + c] = mean0;
+        a.var[pb + p0 + c] = var0;
+    }
+}
+
+// Top-down posterior prediction (src/class_approximation-predict.cpp:123-188):
+//   Z_j = L~_{a_j}^{-1} (B~_j - sum_{j'>j} P_{a_j',j}^T Z_j'),
+//   mean += Z_j^T v_{a_j}, var += colsumsq(Z_j).
+template <int RP>
+__global__ __launch_bounds__(256) void k_pred_post(Tree T, PredArgs a) {
+    constexpr int NT = 64, LD = NT + 1;
+    __shared__ double Cm[RP * LD];
+    __shared__ double Zm[RP * LD];
+    const uint32_t F = a.items[3 * blockIdx.x], p0 = a.items[3 * blockIdx.x + 1];
+    const uint64_t pb = T.pred_off[F];
+    const int pF = (int)(T.pred_off[F + 1] - pb);
+    const int np = min(NT, pF - (int)p0);
+    const int L = T.L, LRP = T.LRP;
+    double* BtF = a.Bt + (uint64_t)LRP * pb;
+    double macc = 0.0, vacc = 0.0;
+    const int c = threadIdx.x;
+    for (int jj = L - 1; jj >= 0; jj--) {
+        const uint64_t an = ancestor_up(T.parent[F], L - 1, jj, T.J);
+        for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
+            const int i = idx / NT, cc2 = idx % NT;
+            Cm[i * LD + cc2] = (cc2 < np) ? BtF[(uint64_t)(jj * RP + i) * pF + p0 + cc2] : 0.0;
+            Zm[i * LD + cc2] = 0.0;
+        }
+        __syncthreads();
+        wg_gemm_tn(Zm, LD, RP, NT, RP, 1.0, a.LpiT + an * RP * RP, RP, RP, Cm, LD, NT);
+        __syncthreads();
+        if (c < np) {
+            const double* vA = a.v + an * RP;
+            double sm = 0.0, sv = 0.0;
+            for (int m = 0; m < RP; m++) {
+                const double z = Zm[m * LD + c];
+                sm += z * vA[m];
+                sv += z * z;
+            }
+            macc += sm;
+            vacc += sv;
+        }
+        if (jj > 0) {
+            const double* PA = a.P + a.goff[jj] + (an - lvl_start(jj, T.J)) * (uint64_t)jj * RP * RP;
+            for (int jp = 0; jp < jj; jp++) {
+                for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
+                    const int i = idx / NT, cc2 = idx % NT;
+                    Cm[i * LD + cc2] = (cc2 < np) ? BtF[(uint64_t)(jp * RP + i) * pF + p0 + cc2] : 0.0;
+                }
+                __syncthreads();
+                wg_gemm_tn(Cm, LD, RP, NT, RP, -1.0, PA + (uint64_t)jp * RP * RP, RP, RP, Zm, LD, NT);
+                __syncthreads();
+                for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
+                    const int i = idx / NT, cc2 = idx % NT;
+                    if (cc2 < np) BtF[(uint64_t)(jp * RP + i) * pF + p0 + cc2] = Cm[i * LD + cc2];
+                }
+                __syncthreads();
+            }
+        }
+    }
+    if (c < np) {
+        a.mean[pb + p0 + c] += macc;
+        a.var[pb + p0 + c] += vacc;
+    }
+}
+
+// ------------------------------------------------------------- reductions
+// Deterministic sum of x[0..n) into out[0] (one work group, fixed order).
+__global__ __launch_bounds__(256) void k_reduce(const double* x, uint64_t n, double* out) {
+    __shared__ double red[MRA_WG];
+    double s = 0.0;
+    for (uint64_t i = threadIdx.x; i < n; i += MRA_WG) s += x[i];
+    double t = wg_sum(s, red);
+    if (threadIdx.x == 0) *out = t;
+}
+
+// D = finest term + level terms (M-2 .. 0), in that order.
+__global__ void k_total(const double* lev, int M, double* D) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double d = 0.0;
+        for (int i = 0; i < M; i++) d += lev[i];
+        *D = d;
+    }
+}
+
+}  // namespace mra

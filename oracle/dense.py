@@ -127,6 +127,13 @@ def dense_sigma(tree, cov, prm):
             reg[k, idx] = a
             a = (a - 1) // J
     cur = cross_cov(cov, prm, lon, lat, lon, lat)
+    # C(x, x) = sigma^2(x) exactly (the variance-covariance matrices of the
+    # reference set their diagonal directly, src/evaluate_covariance.cpp:307;
+    # an acos-based great-circle distance of a point to itself is not 0)
+    if cov["nonstationary"]:
+        cur[np.diag_indices(N)] = prm["sill_scale"] * field(cov, "sigmasq", lon, lat)
+    else:
+        cur[np.diag_indices(N)] = prm["sill"]
     tau = nugget(cov, prm, lon, lat)
     sig = np.zeros((n + npred, n + npred))
     for k in range(L):

@@ -1,0 +1,151 @@
+"""GPU parity: the HIP FP64 path (through the C ABI) against the CPU oracle.
+
+Tolerances: the pure-FP64 GPU path must match the FP64 reference restatement
+to 1e-10 relative in D = -2 loglik - n log(2 pi); predictive means and
+variances to 1e-9 relative (absolute floor 1e-10).
+"""
+import numpy as np
+import pytest
+
+from nonstationary_mra_amd import capi, synth
+from oracle import oracle
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+TOL_D = 1e-10
+TOL_PRED = 1e-9
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def run_gpu(tree, cov, prm, predict=True):
+    h = capi.Handle(tree, with_pred=predict)
+    try:
+        h.set_covariance(cov)
+        h.set_params(**prm)
+        if predict and h.n_pred:
+            return h.predict()
+        return h.likelihood(), None, None
+    finally:
+        h.close()
+
+
+def check(tree, cov, prm, predict=True):
+    D, mean, var = run_gpu(tree, cov, prm, predict)
+    Do, mo, vo = oracle.evaluate(tree, cov, prm, predict=predict)
+    print(f"D gpu {D!r} oracle {Do!r} rel {rel(D, Do):.3e}", flush=True)
+    assert rel(D, Do) < TOL_D
+    if mo is not None:
+        np.testing.assert_allclose(mean, mo, rtol=TOL_PRED, atol=1e-10)
+        np.testing.assert_allclose(var, vo, rtol=TOL_PRED, atol=1e-10)
+
+
+def test_example_dataset(example):
+    """The reference's own example (J=2, M=5, r=4, user knots, parent file,
+    chordal Wendland fields, finest regions up to 268 observations)."""
+    tree, cov, prm = example
+    check(tree, cov, prm)
+
+
+@pytest.mark.parametrize("M,J,r,metric,stationary,clustered,seed", [
+    (2, 4, 9, "planar", False, False, 1),
+    (3, 4, 16, "planar", False, True, 2),
+    (3, 2, 4, "planar", True, False, 3),
+    (4, 2, 4, "chordal", False, False, 4),
+    (3, 4, 16, "great_circle", False, True, 5),
+    (4, 4, 30, "planar", False, True, 6),
+    (3, 4, 64, "planar", False, False, 7),
+])
+def test_synthetic(M, J, r, metric, stationary, clustered, seed):
+    tree, cov, prm = cases.synthetic_case(n=600, npred=80, M=M, J=J, r=r, seed=seed, metric=metric,
+                                          stationary=stationary, clustered=clustered)
+    check(tree, cov, prm)
+
+
+def test_levels_match_oracle():
+    tree, cov, prm = cases.synthetic_case(n=800, npred=0, M=4, J=4, r=16, seed=21, clustered=True)
+    h = capi.Handle(tree, with_pred=False)
+    h.set_covariance(cov)
+    h.set_params(**prm)
+    lev = h.likelihood_levels()
+    h.close()
+    lo = oracle.evaluate_levels(tree, cov, prm)
+    for a, b in zip(lev, lo):
+        assert abs(a - b) <= 1e-10 * max(abs(b), abs(lo).max() * 1e-3)
+
+
+def test_config0_small_fp64():
+    """n = 4,000 uniform points, exact dense GP draw (log sigma^2 = 0.5x - 0.3y,
+    log beta = log 0.05 + 0.8x, nugget 0.05), M=4, J=4, r=16, 1,000
+    prediction points."""
+    rng = np.random.default_rng(2024)
+    n, npred = 4000, 1000
+    x, y = synth.uniform_points(n, rng)
+    cov = synth.linear_trend_cov()
+    val = synth.dense_gp_draw(cov, x, y, rng)
+    px, py = synth.uniform_points(npred, rng)
+    tree = capi.partition_program(x, y, val, px, py, M=4, J=4, r=16, bounds=synth.UNIT_BOUNDS)
+    check(tree, cov, dict(synth.UNIT_PARAMS))
+
+
+def test_clustered_r64():
+    """Clustered points (uneven finest counts, some > 64), r = 64, J = 4."""
+    rng = np.random.default_rng(77)
+    n, npred = 12000, 600
+    x, y = synth.clustered_points(n, rng)
+    px, py = synth.uniform_points(npred, rng)
+    tree = capi.partition_program(x, y, rng.standard_normal(n), px, py, M=4, J=4, r=64,
+                                  bounds=synth.UNIT_BOUNDS)
+    counts = np.diff(tree["obs_offsets"].astype(np.int64))
+    print("finest counts max", counts.max(), "empty", (counts == 0).sum(), flush=True)
+    assert counts.max() > 64
+    check(tree, synth.linear_trend_cov(), dict(synth.UNIT_PARAMS))
+
+
+def test_covariance_entry_points():
+    tree, cov, prm = cases.synthetic_case(n=70, npred=0, M=2, J=4, r=4, seed=31, metric="chordal")
+    lon, lat = tree["obs_lon"], tree["obs_lat"]
+    C = capi.cross_covariance(cov, prm, lon[:40], lat[:40], lon[40:], lat[40:])
+    Co = oracle.cross_covariance(cov, prm, lon[:40], lat[:40], lon[40:], lat[40:])
+    np.testing.assert_allclose(C, Co, rtol=1e-13)
+    V = capi.variance_covariance(cov, prm, lon, lat)
+    Vo = oracle.variance_covariance(cov, prm, lon, lat)
+    np.testing.assert_allclose(V, Vo, rtol=1e-13)
+
+
+def test_repeated_evaluations_identical():
+    """Device buffers are reused across evaluations; results are bitwise
+    repeatable (deterministic reductions)."""
+    tree, cov, prm = cases.synthetic_case(n=500, npred=50, M=3, J=4, r=16, seed=41, clustered=True)
+    h = capi.Handle(tree)
+    h.set_covariance(cov)
+    h.set_params(**prm)
+    a = h.predict()
+    b = h.predict()
+    c = h.likelihood()
+    h.close()
+    assert a[0] == b[0] == c
+    np.testing.assert_array_equal(a[1], b[1])
+    np.testing.assert_array_equal(a[2], b[2])
+
+
+def test_not_positive_definite_reports_error():
+    """A negative nugget scale makes every variance-covariance diagonal
+    sigma^2 - 2 < 0: the first pivot is negative and the library must report
+    MRA_E_NOT_PD, not return a number."""
+    rng = np.random.default_rng(5)
+    x, y = rng.random(200), rng.random(200)
+    cov = synth.linear_trend_cov(nugget=1.0)
+    prm = dict(synth.UNIT_PARAMS, nugget_scale=-2.0)
+    tree = capi.partition_program(x, y, rng.standard_normal(200), None, None, M=2, J=4, r=4,
+                                  bounds=synth.UNIT_BOUNDS)
+    h = capi.Handle(tree, with_pred=False)
+    h.set_covariance(cov)
+    h.set_params(**prm)
+    with pytest.raises(capi.MraError) as e:
+        h.likelihood()
+    assert e.value.code == capi.MRA_E_NOT_PD
+    h.close()
