@@ -136,7 +136,19 @@ int mra_set_observations(mra_handle* h, const double* values);
  * later mra_likelihood calls (1), or launch kernels directly (0). */
 int mra_use_graph(mra_handle* h, int enable);
 
-/* Timing of the last evaluation: per-phase milliseconds (HIP events). */
+/* Timing of the last evaluation: milliseconds per launch group, measured
+ * with HIP events on the handle's stream, indexed by the MRA_T_* ids. */
+enum {
+    MRA_T_SETUP = 0,          /* parameter scaling, clears */
+    MRA_T_FINEST_CROSS = 1,   /* k_finest_cross */
+    MRA_T_FINEST_LOCAL = 2,   /* k_finest_local */
+    MRA_T_PRED_PRE = 3,
+    MRA_T_PRED_POST = 4,
+    MRA_T_REDUCE = 5,
+    MRA_T_PRIOR0 = 8,         /* + level l: k_prior_coarse of level l (l < 24) */
+    MRA_T_POST0 = 32,         /* + level l: k_posterior of level l */
+    MRA_T_COUNT = 56
+};
 int mra_last_timings(mra_handle* h, double* ms, int n);
 
 /* ------------------------------------------------ single-kernel entries */
@@ -177,6 +189,15 @@ int mra_partition_program(const double* lon, const double* lat, const double* va
                           uint64_t* obs_offsets, double* obs_lon, double* obs_lat, double* obs_val,
                           uint64_t* obs_index, uint64_t* pred_offsets, double* pred_lon,
                           double* pred_lat, uint64_t* pred_index);
+
+/* Self test of the device building blocks (no reference counterpart; test
+ * hook): one work group computes Cout = C0 - A^T B with the FP64 matrix-core
+ * GEMM (A, B: K x 64 row-major; only the Mv x Nv leading part of Cout is
+ * updated), Lout = lower Cholesky of the SPD 64 x 64 matrix S (row-major;
+ * entries above the diagonal keep S), Xout = L^{-1} Bt; *bad = 1 on a
+ * non-positive pivot. */
+int mra_selftest(int K, int Mv, int Nv, const double* A, const double* B, const double* C0,
+                 const double* S, const double* Bt, double* Cout, double* Lout, double* Xout, int* bad);
 
 /* Library version string. */
 const char* mra_version(void);

@@ -15,6 +15,8 @@ MRA_OK, MRA_E_ARG, MRA_E_HIP, MRA_E_NOT_PD, MRA_E_STATE, MRA_E_NOMEM, MRA_E_UNSU
 METRIC = {"chordal": 0, "great_circle": 1, "planar": 2}
 FIELD_CONSTANT, FIELD_WENDLAND, FIELD_LINEAR = 0, 1, 2
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+T_SETUP, T_FINEST_CROSS, T_FINEST_LOCAL, T_PRED_PRE, T_PRED_POST, T_REDUCE, T_PRIOR0, T_POST0, T_COUNT = \
+    0, 1, 2, 3, 4, 5, 8, 32, 56
 
 u64p = C.POINTER(C.c_uint64)
 dp = C.POINTER(C.c_double)
@@ -24,7 +26,7 @@ EXPORTED = ["mra_create", "mra_destroy", "mra_last_error", "mra_set_covariance",
             "mra_set_precision", "mra_likelihood", "mra_predict", "mra_likelihood_levels", "mra_sample_prior",
             "mra_set_observations", "mra_use_graph", "mra_last_timings", "mra_cross_covariance",
             "mra_variance_covariance", "mra_cov_tile_device", "mra_partition_program_sizes",
-            "mra_partition_program", "mra_version"]
+            "mra_partition_program", "mra_version", "mra_selftest"]
 
 
 class TreeDesc(C.Structure):
@@ -87,7 +89,18 @@ def lib():
         _lib.mra_partition_program.argtypes = [dp, dp, dp, C.c_uint64, dp, dp, C.c_uint64, C.c_int, C.c_int,
                                                C.c_int, C.c_double, dp, dp, dp, u64p, u64p, dp, dp, dp, u64p, u64p,
                                                dp, dp, u64p]
+        _lib.mra_selftest.argtypes = [C.c_int, C.c_int, C.c_int] + [dp] * 8 + [C.POINTER(C.c_int)]
     return _lib
+
+
+def selftest(K, Mv, Nv, A, B, C0, S, Bt):
+    """Device building-block self test (see mra_selftest in include/mra.h)."""
+    keep = Keep()
+    Co, Lo, Xo = np.zeros(64 * 64), np.zeros(64 * 64), np.zeros(64 * 64)
+    bad = C.c_int(0)
+    check(lib().mra_selftest(K, Mv, Nv, keep.d(A), keep.d(B), keep.d(C0), keep.d(S), keep.d(Bt), _d(Co), _d(Lo),
+                             _d(Xo), C.byref(bad)))
+    return Co.reshape(64, 64), Lo.reshape(64, 64), Xo.reshape(64, 64), bad.value
 
 
 def check(rc, h=None):
@@ -223,8 +236,9 @@ class Handle:
         check(lib().mra_set_observations(self.h, _d(y)), self.h)
 
     def timings(self):
-        out = np.zeros(8)
-        check(lib().mra_last_timings(self.h, _d(out), 8), self.h)
+        """ms per launch group of the last evaluation (index = MRA_T_* ids)."""
+        out = np.zeros(T_COUNT)
+        check(lib().mra_last_timings(self.h, _d(out), T_COUNT), self.h)
         return out
 
 

@@ -7,7 +7,8 @@
 // v_mfma_f64_16x16x4_f64 through __builtin_amdgcn_mfma_f64_16x16x4f64:
 //   A fragment (16x4): lane t holds A[t%16][t/16]
 //   B fragment (4x16): lane t holds B[t/16][t%16]
-//   D (16x16):         lane t, element v holds D[4*(t/16)+v][t%16]
+//   D (16x16):         lane t, element v holds D[t/16 + 4*v][t%16]
+//   (the D layout was confirmed on gfx950 by tests/test_gpu_blocks.py)
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -111,15 +112,16 @@ __device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int
             acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
             acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc11, 0, 0, 0);
         }
-        // only the valid Mv x Nv part of C is updated
-        const int row = ti0 * 16 + 4 * k4;
+        // only the valid Mv x Nv part of C is updated; element v of lane t
+        // holds row t/16 + 4v of the 16 x 16 tile
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            const bool r0 = row + v < Mv, r1 = (ti0 + 1 < tM) && (row + 16 + v < Mv);
-            if (r0 && hja) C[(row + v) * ldc + ja] += alpha * acc00[v];
-            if (r0 && hjb) C[(row + v) * ldc + jb] += alpha * acc01[v];
-            if (r1 && hja) C[(row + 16 + v) * ldc + ja] += alpha * acc10[v];
-            if (r1 && hjb) C[(row + 16 + v) * ldc + jb] += alpha * acc11[v];
+            const int row = ti0 * 16 + k4 + 4 * v;
+            const bool r0 = row < Mv, r1 = (ti0 + 1 < tM) && (row + 16 < Mv);
+            if (r0 && hja) C[row * ldc + ja] += alpha * acc00[v];
+            if (r0 && hjb) C[row * ldc + jb] += alpha * acc01[v];
+            if (r1 && hja) C[(row + 16) * ldc + ja] += alpha * acc10[v];
+            if (r1 && hjb) C[(row + 16) * ldc + jb] += alpha * acc11[v];
         }
     }
 }

@@ -69,8 +69,12 @@ struct mra_handle {
     int precision = MRA_PRECISION_FP64;
     std::vector<void*> fieldAllocs;
     // timing
-    hipEvent_t ev[8] = {};
-    double timings[8] = {};
+    // one event before every launch group; timings[g] = ms of group g of the
+    // last evaluation, group ids as in mra.h (MRA_T_*)
+    hipEvent_t ev[MRA_T_COUNT + 1] = {};
+    int evGroup[MRA_T_COUNT + 1] = {};
+    int nEv = 0;
+    double timings[MRA_T_COUNT] = {};
     bool useGraph = false;
 };
 
@@ -499,7 +503,7 @@ extern "C" int mra_use_graph(mra_handle* h, int enable) {
 
 extern "C" int mra_last_timings(mra_handle* h, double* ms, int n) {
     if (!h || !ms) return MRA_E_ARG;
-    for (int i = 0; i < n && i < 8; i++) ms[i] = h->timings[i];
+    for (int i = 0; i < n && i < MRA_T_COUNT; i++) ms[i] = h->timings[i];
     return MRA_OK;
 }
 
@@ -541,7 +545,12 @@ static void launch_eval(mra_handle* h, bool predicting) {
     const CovCtx cc = make_cc(h);
     const Pts K = pts(h->dKc, h->dKs), O = pts(h->dOc, h->dOs);
     const Pts Pp = predicting ? pts(h->dPc, h->dPs) : Pts{};
-    (void)hipEventRecord(h->ev[0], st);
+    h->nEv = 0;
+    auto mark = [&](int group) {
+        h->evGroup[h->nEv] = group;
+        (void)hipEventRecord(h->ev[h->nEv++], st);
+    };
+    mark(MRA_T_SETUP);
     // scaled parameters
     struct S {
         uint64_t n;
@@ -557,17 +566,19 @@ static void launch_eval(mra_handle* h, bool predicting) {
     // coarse prior, level by level
     for (int l = 0; l < h->L; l++) {
         const uint64_t nl = h->lvlStart[l + 1] - h->lvlStart[l];
+        mark(MRA_T_PRIOR0 + l);
         k_prior_coarse<RP><<<(unsigned)nl, 256, 0, st>>>(T, cc, K, l, h->lvlStart[l], h->dGoff, h->dGc, h->dLt,
                                                          h->dLiT, h->dBad);
     }
-    (void)hipEventRecord(h->ev[1], st);
+    mark(MRA_T_FINEST_CROSS);
     if (h->nItemsX)
         k_finest_cross<RP><<<h->nItemsX, 256, 0, st>>>(T, cc, K, O, h->dItemsX, h->dGoff, h->dGc, h->dLiT, h->dGf,
                                                        h->dBf);
+    mark(MRA_T_FINEST_LOCAL);
     if (h->nItemsF)
         k_finest_local<RP><<<h->nItemsF, 256, 0, st>>>(T, cc, O, h->dY, h->dItemsF, h->dGf, h->dBf, h->dSig, h->dH,
                                                        h->dU, h->dDterm, h->dBad);
-    (void)hipEventRecord(h->ev[2], st);
+    mark(MRA_T_PRED_PRE);
     PredArgs pa{};
     if (predicting) {
         pa.goff = h->dGoff;
@@ -593,9 +604,9 @@ static void launch_eval(mra_handle* h, bool predicting) {
             k_pred_pre<RP><<<e - b, 256, 0, st>>>(T, cc, K, O, Pp, pa);
         }
     }
-    (void)hipEventRecord(h->ev[3], st);
     // posterior, finest-but-one level up to the root
     for (int l = h->L - 1; l >= 0; l--) {
+        mark(MRA_T_POST0 + l);
         if (!h->nItemsPost[l]) continue;
         PostArgs a{};
         a.l = l;
@@ -622,7 +633,7 @@ static void launch_eval(mra_handle* h, bool predicting) {
             k_posterior<RP, false><<<h->nItemsPost[l], 256, 0, st>>>(T, a);
         }
     }
-    (void)hipEventRecord(h->ev[4], st);
+    mark(MRA_T_PRED_POST);
     if (predicting) {
         for (size_t c = 0; c + 1 < h->predChunkStart.size(); c++) {
             const uint32_t b = h->predChunkStart[c], e = h->predChunkStart[c + 1];
@@ -632,13 +643,13 @@ static void launch_eval(mra_handle* h, bool predicting) {
             k_pred_post<RP><<<e - b, 256, 0, st>>>(T, pa);
         }
     }
-    (void)hipEventRecord(h->ev[5], st);
+    mark(MRA_T_REDUCE);
     k_reduce<<<1, 256, 0, st>>>(h->dDterm + h->nC, h->nF, h->dLev);
     for (int l = h->L - 1; l >= 0; l--)
         k_reduce<<<1, 256, 0, st>>>(h->dDterm + h->lvlStart[l], h->lvlStart[l + 1] - h->lvlStart[l],
                                     h->dLev + (h->L - l));
     k_total<<<1, 64, 0, st>>>(h->dLev, h->M, h->dD);
-    (void)hipEventRecord(h->ev[6], st);
+    mark(MRA_T_COUNT);
 }
 
 static int run_eval(mra_handle* h, bool predicting) {
@@ -653,14 +664,12 @@ static int run_eval(mra_handle* h, bool predicting) {
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(h->stream));
-    for (int i = 0; i < 6; i++) {
+    for (int i = 0; i < MRA_T_COUNT; i++) h->timings[i] = 0.0;
+    for (int i = 0; i + 1 < h->nEv; i++) {
         float ms = 0;
         (void)hipEventElapsedTime(&ms, h->ev[i], h->ev[i + 1]);
-        h->timings[i] = ms;
+        h->timings[h->evGroup[i]] += ms;
     }
-    float tot = 0;
-    (void)hipEventElapsedTime(&tot, h->ev[0], h->ev[6]);
-    h->timings[6] = tot;
     int bad = 0;
     HIP_TRY(hipMemcpy(&bad, h->dBad, sizeof(int), hipMemcpyDeviceToHost));
     if (bad) {
@@ -698,12 +707,38 @@ extern "C" int mra_predict(mra_handle* h, double* D, double* mean, double* var) 
 }
 
 extern "C" int mra_sample_prior(mra_handle* h, uint64_t seed, double* y, int replace) {
-    (void)seed;
-    (void)y;
-    (void)replace;
-    if (!h) return MRA_E_ARG;
-    set_err(h, "mra_sample_prior is not available yet");
-    return MRA_E_UNSUPPORTED;
+    if (!h || (!y && !replace)) return MRA_E_ARG;
+    // the prior factors (G, L_F) of the current parameters
+    int rc = run_eval(h, false);
+    if (rc && rc != MRA_E_NOT_PD) return rc;
+    if (rc) return rc;
+    double *xc = nullptr, *xo = nullptr, *yn = nullptr;
+    std::vector<void*> tmp;
+    auto cleanup = [&]() {
+        for (void* p : tmp) (void)hipFree(p);
+    };
+    const uint64_t nK = h->nC * h->RP;
+    if ((rc = dalloc(h, &xc, nK, tmp)) || (rc = dalloc(h, &xo, h->nObs, tmp)) || (rc = dalloc(h, &yn, h->nObs, tmp))) {
+        cleanup();
+        return rc;
+    }
+    k_normals<<<nblk(nK), 256, 0, h->stream>>>(seed, 0, nK, xc);
+    if (h->nObs) k_normals<<<nblk(h->nObs), 256, 0, h->stream>>>(seed, nK, h->nObs, xo);
+    if (h->nItemsF)
+        k_sample_finest<<<h->nItemsF, 256, 0, h->stream>>>(make_tree(h), h->dItemsF, h->dGf, h->dSig, xc, xo, yn);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    std::vector<double> host(h->nObs);
+    if (e == hipSuccess && h->nObs) e = hipMemcpy(host.data(), yn, sizeof(double) * h->nObs, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && replace && h->nObs) e = hipMemcpy(h->dY, yn, sizeof(double) * h->nObs, hipMemcpyDeviceToDevice);
+    cleanup();
+    if (e != hipSuccess) {
+        set_err(h, std::string("sample_prior: ") + hipGetErrorString(e));
+        return MRA_E_HIP;
+    }
+    if (y) std::memcpy(y, host.data(), sizeof(double) * h->nObs);
+    if (replace) h->hObsVal = host;
+    return MRA_OK;
 }
 
 // ------------------------------------------------- single-kernel entries
@@ -819,5 +854,41 @@ extern "C" int mra_cov_tile_device(int precision, int metric, const double* d_xy
         g_err = hipGetErrorString(e);
         return MRA_E_HIP;
     }
+    return MRA_OK;
+}
+
+extern "C" int mra_selftest(int K, int Mv, int Nv, const double* A, const double* B, const double* C0,
+                            const double* S, const double* Bt, double* Cout, double* Lout, double* Xout, int* bad) {
+    mra_handle tmp;
+    mra_handle* h = &tmp;
+    if (K < 1 || Mv < 1 || Mv > 64 || Nv < 1 || Nv > 64 || !A || !B || !C0 || !S || !Bt || !Cout || !Lout || !Xout ||
+        !bad)
+        return MRA_E_ARG;
+    std::vector<void*> l;
+    double *dA, *dB, *dC0, *dS, *dBt, *dC, *dL, *dX;
+    int* dbad;
+    const size_t m = 64 * 64;
+    RC_TRY(dalloc(h, &dA, (size_t)K * 64, l));
+    RC_TRY(dalloc(h, &dB, (size_t)K * 64, l));
+    RC_TRY(dalloc(h, &dC0, m, l));
+    RC_TRY(dalloc(h, &dS, m, l));
+    RC_TRY(dalloc(h, &dBt, m, l));
+    RC_TRY(dalloc(h, &dC, m, l));
+    RC_TRY(dalloc(h, &dL, m, l));
+    RC_TRY(dalloc(h, &dX, m, l));
+    RC_TRY(dalloc(h, &dbad, 1, l));
+    HIP_TRY(hipMemcpy(dA, A, sizeof(double) * K * 64, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dB, B, sizeof(double) * K * 64, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dC0, C0, sizeof(double) * m, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dS, S, sizeof(double) * m, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dBt, Bt, sizeof(double) * m, hipMemcpyHostToDevice));
+    k_selftest<<<1, 256>>>(dA, dB, K, Mv, Nv, dC0, dS, dBt, dC, dL, dX, dbad);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(Cout, dC, sizeof(double) * m, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(Lout, dL, sizeof(double) * m, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(Xout, dX, sizeof(double) * m, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(bad, dbad, sizeof(int), hipMemcpyDeviceToHost));
+    for (void* p : l) (void)hipFree(p);
     return MRA_OK;
 }

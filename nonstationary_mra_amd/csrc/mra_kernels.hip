@@ -616,7 +616,8 @@ __global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts 
         }
         wg_gemm_tn(Wm, LD, RP, NT, RP, 1.0, a.LiT + an * RP * RP, RP, RP, Cm, LD, NT);
         __syncthreads();
-        for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) Gh[(idx / NT) * NT + idx % NT] = Wm[(idx / NT) * LD + idx % NT];
+        for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG)
+            Gh[(uint64_t)(j * RP + idx / NT) * NT + idx % NT] = Wm[(idx / NT) * LD + idx % NT];
         __syncthreads();
     }
     // var0 and the conditional cross block with the region's observations
@@ -745,6 +746,82 @@ __global__ __launch_bounds__(256) void k_pred_post(Tree T, PredArgs a) {
         a.mean[pb + p0 + c] += macc;
         a.var[pb + p0 + c] += vacc;
     }
+}
+
+// ---------------------------------------------------------- prior sampler
+// Counter-based standard normals: splitmix64 of (seed, index), Box-Muller.
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+__device__ __forceinline__ double std_normal(uint64_t seed, uint64_t idx) {
+    const uint64_t a = splitmix64(seed ^ splitmix64(2 * idx + 1));
+    const uint64_t b = splitmix64(a ^ 0xD1B54A32D192ED03ull);
+    const double u1 = ((a >> 11) + 1.0) * (1.0 / 9007199254740993.0);  // (0, 1]
+    const double u2 = (b >> 11) * (1.0 / 9007199254740992.0);           // [0, 1)
+    return sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+}
+
+__global__ void k_normals(uint64_t seed, uint64_t base, uint64_t n, double* out) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = std_normal(seed, base + i);
+}
+
+// y_F = sum_k G_{F,k}^T xi_{a_k} + L_F xi_F: a draw from N(0, Sigma_MRA), since
+// G_{F,k}^T G_{F',k} = C_k(S_F, Q) K^{-1} C_k(Q, S_F') and L_F L_F^T = Sigma_F.
+// One work group per non-empty finest region, one thread per observation.
+__global__ __launch_bounds__(256) void k_sample_finest(Tree T, const uint32_t* items, const double* Gf,
+                                                       const double* Sig, const double* xi_c, const double* xi_o,
+                                                       double* y) {
+    const uint32_t F = items[blockIdx.x];
+    const uint64_t ob = T.obs_off[F];
+    const int n = (int)(T.obs_off[F + 1] - ob);
+    const int RP = T.RP, L = T.L;
+    const double* GF = Gf + (uint64_t)T.LRP * ob;
+    const double* LF = Sig + T.sig_off[F];
+    for (int o = threadIdx.x; o < n; o += MRA_WG) {
+        double s = 0.0;
+        for (int j = 0; j < L; j++) {
+            const uint64_t a = ancestor_up(T.parent[F], L - 1, j, T.J);
+            for (int m = 0; m < T.r; m++) s += GF[(uint64_t)(j * RP + m) * n + o] * xi_c[a * RP + m];
+        }
+        for (int k = 0; k <= o; k++) s += LF[(uint64_t)o * n + k] * xi_o[ob + k];
+        y[ob + o] = s;
+    }
+}
+
+// -------------------------------------------------------------- self test
+// Exercises the work-group building blocks on fixed inputs (one work group):
+//   Cout = C0 - A^T B (MFMA, K rows, Mv x Nv valid part), Lout = chol(S),
+//   Xout = L^{-1} Bt.   All 64 x 64 row-major except A, B (K x 64).
+__global__ __launch_bounds__(256) void k_selftest(const double* A, const double* B, int K, int Mv, int Nv,
+                                                  const double* C0, const double* S, const double* Bt, double* Cout,
+                                                  double* Lout, double* Xout, int* bad_out) {
+    constexpr int LD = 65;
+    __shared__ double Cm[64 * LD];
+    __shared__ double Wm[64 * LD];
+    __shared__ int bad;
+    if (threadIdx.x == 0) bad = 0;
+    for (int idx = threadIdx.x; idx < 64 * 64; idx += MRA_WG) Cm[(idx / 64) * LD + idx % 64] = C0[idx];
+    __syncthreads();
+    wg_gemm_tn(Cm, LD, 64, 64, K, -1.0, A, 64, Mv, B, 64, Nv);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < 64 * 64; idx += MRA_WG) Cout[idx] = Cm[(idx / 64) * LD + idx % 64];
+    for (int idx = threadIdx.x; idx < 64 * 64; idx += MRA_WG) {
+        Cm[(idx / 64) * LD + idx % 64] = S[idx];
+        Wm[(idx / 64) * LD + idx % 64] = Bt[idx];
+    }
+    __syncthreads();
+    wg_chol(Cm, LD, 64, &bad);
+    wg_trsm(Cm, LD, Wm, LD, 64, 64);
+    for (int idx = threadIdx.x; idx < 64 * 64; idx += MRA_WG) {
+        Lout[idx] = Cm[(idx / 64) * LD + idx % 64];
+        Xout[idx] = Wm[(idx / 64) * LD + idx % 64];
+    }
+    if (threadIdx.x == 0) *bad_out = bad;
 }
 
 // ------------------------------------------------------------- reductions

@@ -1,0 +1,29 @@
+"""GPU unit tests of the work-group building blocks (mra_selftest): the FP64
+matrix-core GEMM lane layout, the LDS Cholesky and triangular solve, against
+numpy on the same inputs."""
+import numpy as np
+import pytest
+
+from nonstationary_mra_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("K,Mv,Nv", [(4, 64, 64), (64, 64, 64), (130, 64, 64), (37, 40, 23), (5, 16, 17)])
+def test_blocks(K, Mv, Nv):
+    rng = np.random.default_rng(K * 100 + Mv)
+    A = rng.standard_normal((K, 64))
+    B = rng.standard_normal((K, 64))
+    C0 = rng.standard_normal((64, 64))
+    X = rng.standard_normal((64, 64))
+    S = X @ X.T / 64 + np.eye(64)
+    Bt = rng.standard_normal((64, 64))
+    Co, Lo, Xo, bad = capi.selftest(K, Mv, Nv, A, B, C0, S, Bt)
+    ref = C0.copy()
+    ref[:Mv, :Nv] -= (A.T @ B)[:Mv, :Nv]
+    print("gemm max err", np.abs(Co - ref).max(), flush=True)
+    np.testing.assert_allclose(Co, ref, rtol=0, atol=1e-12 * K)
+    Lr = np.linalg.cholesky(S)
+    assert bad == 0
+    np.testing.assert_allclose(np.tril(Lo), Lr, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(Xo, np.linalg.solve(Lr, Bt), rtol=0, atol=1e-11)

@@ -149,3 +149,27 @@ def test_not_positive_definite_reports_error():
         h.likelihood()
     assert e.value.code == capi.MRA_E_NOT_PD
     h.close()
+
+
+def test_prior_sampler_matches_dense_covariance():
+    """Draws from mra_sample_prior have covariance Sigma_MRA (dense oracle):
+    every entry of the empirical covariance of N draws lies within 5.5
+    standard errors sqrt((S_ii S_jj + S_ij^2)/N) of Sigma_MRA; the same seed
+    gives the same draw."""
+    from oracle import dense
+    tree, cov, prm = cases.synthetic_case(n=24, npred=0, M=3, J=2, r=4, seed=51)
+    h = capi.Handle(tree, with_pred=False)
+    h.set_covariance(cov)
+    h.set_params(**prm)
+    N = 1500
+    draws = np.stack([h.sample_prior(seed=1000 + s, replace=False) for s in range(N)])
+    again = h.sample_prior(seed=1000, replace=False)
+    h.close()
+    np.testing.assert_array_equal(again, draws[0])
+    sig, n, _ = dense.dense_sigma(tree, cov, prm)
+    emp = draws.T @ draws / N
+    d = np.diag(sig)
+    se = np.sqrt((np.outer(d, d) + sig ** 2) / N)
+    z = np.abs(emp - sig) / se
+    print("max |z|", z.max(), flush=True)
+    assert z.max() < 5.5
