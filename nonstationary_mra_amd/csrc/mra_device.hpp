@@ -100,7 +100,31 @@ __device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int
         const int ja = tj0 * 16 + r16, jb = ja + 16;
         const bool ha = ia < Mv, hb = (ti0 + 1 < tM) && ib < Mv;
         const bool hja = ja < Nv, hjb = (tj0 + 1 < tN) && jb < Nv;
-        for (int k0 = 0; k0 < Kv; k0 += 4) {  // uniform across the wave
+        const bool tM2 = ti0 + 1 < tM, tN2 = tj0 + 1 < tN;  // second tile row / column present
+        // Operand loads of U k-steps are issued together before their MFMAs so
+        // that one memory latency covers U steps (all control flow is
+        // uniform across the wave).
+        constexpr int U = 8;
+        int k0 = 0;
+        for (; k0 + 4 * U <= Kv; k0 += 4 * U) {
+            double a0[U], a1[U], b0[U], b1[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int64_t k = k0 + 4 * u + k4;
+                a0[u] = ha ? A[k * lda + ia] : 0.0;
+                a1[u] = hb ? A[k * lda + ib] : 0.0;
+                b0[u] = hja ? B[k * ldb + ja] : 0.0;
+                b1[u] = hjb ? B[k * ldb + jb] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b0[u], acc00, 0, 0, 0);
+                if (tN2) acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b1[u], acc01, 0, 0, 0);
+                if (tM2) acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b0[u], acc10, 0, 0, 0);
+                if (tM2 && tN2) acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b1[u], acc11, 0, 0, 0);
+            }
+        }
+        for (; k0 < Kv; k0 += 4) {
             const int k = k0 + k4;
             const bool kv = k < Kv;
             double a0 = (kv && ha) ? A[(int64_t)k * lda + ia] : 0.0;
@@ -108,9 +132,9 @@ __device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int
             double b0 = (kv && hja) ? B[(int64_t)k * ldb + ja] : 0.0;
             double b1 = (kv && hjb) ? B[(int64_t)k * ldb + jb] : 0.0;
             acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc00, 0, 0, 0);
-            acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc01, 0, 0, 0);
-            acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
-            acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc11, 0, 0, 0);
+            if (tN2) acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc01, 0, 0, 0);
+            if (tM2) acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
+            if (tM2 && tN2) acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc11, 0, 0, 0);
         }
         // only the valid Mv x Nv part of C is updated; element v of lane t
         // holds row t/16 + 4v of the 16 x 16 tile
@@ -124,6 +148,57 @@ __device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int
             if (r1 && hjb) C[(row + 16) * ldc + jb] += alpha * acc11[v];
         }
     }
+}
+
+// Same product for an output of at most 32 x 32 (one 2x2 tile block), with
+// K split over the 4 waves; the partial tiles are summed in wave order
+// through red (LDS, 3*4*256 doubles), so the result is reproducible.
+__device__ __forceinline__ void wg_gemm_tn_ksplit(double* C, int ldc, int M, int N, int Kv, double alpha,
+                                                  const double* A, int64_t lda, int Mv, const double* B, int64_t ldb,
+                                                  int Nv, double* red) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, k4 = lane >> 4;
+    const int tM = M >> 4, tN = N >> 4;
+    const bool tM2 = tM > 1, tN2 = tN > 1;
+    mra_d4 acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc[q] = mra_d4{0, 0, 0, 0};
+    const int ia = r16, ib = 16 + r16, ja = r16, jb = 16 + r16;
+    const bool ha = ia < Mv, hb = tM2 && ib < Mv, hja = ja < Nv, hjb = tN2 && jb < Nv;
+    // this wave's contiguous K range, a multiple of 4
+    const int kq = ((Kv + 15) / 16) * 4;
+    const int kb = wave * kq, ke = min(Kv, kb + kq);
+    for (int k0 = kb; k0 < ke; k0 += 4) {
+        const int k = k0 + k4;
+        const bool kv = k < ke;
+        const double a0 = (kv && ha) ? A[(int64_t)k * lda + ia] : 0.0;
+        const double a1 = (kv && hb) ? A[(int64_t)k * lda + ib] : 0.0;
+        const double b0 = (kv && hja) ? B[(int64_t)k * ldb + ja] : 0.0;
+        const double b1 = (kv && hjb) ? B[(int64_t)k * ldb + jb] : 0.0;
+        acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0], 0, 0, 0);
+        if (tN2) acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[1], 0, 0, 0);
+        if (tM2) acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[2], 0, 0, 0);
+        if (tM2 && tN2) acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[3], 0, 0, 0);
+    }
+    if (wave > 0) {
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) red[((wave - 1) * 4 + q) * 256 + v * 64 + lane] = acc[q][v];
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                double t = acc[q][v];
+                for (int w = 0; w < 3; w++) t += red[(w * 4 + q) * 256 + v * 64 + lane];
+                const int row = (q >> 1) * 16 + k4 + 4 * v, col = (q & 1) * 16 + r16;
+                if (row < Mv && col < Nv) C[row * ldc + col] += alpha * t;
+            }
+    }
+    __syncthreads();
 }
 
 // ------------------------------------------------------------ LDS solvers

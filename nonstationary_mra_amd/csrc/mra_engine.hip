@@ -34,6 +34,10 @@ struct mra_handle {
     std::vector<uint64_t> lvlStart;  // L+1 entries
     std::vector<uint64_t> hObsOff, hPredOff, hParent, hChS, hChE, hSigOff, hGoff;
     std::vector<uint8_t> hWorking;
+    std::vector<uint64_t> hFSegOs, hSOs, hSParent;
+    std::vector<uint32_t> hFSegN, hSN;
+    uint64_t *dFSegOs = nullptr, *dSOs = nullptr, *dSParent = nullptr;
+    uint32_t *dFSegN = nullptr, *dSN = nullptr;
     std::vector<double> hObsVal;
     // device inputs
     std::vector<void*> allocs;
@@ -213,6 +217,29 @@ static int build_structure(mra_handle* h, const mra_tree_desc* t) {
             h->hWorking[a] = 1;
         }
     }
+    // segments: maximal runs of consecutive finest regions with one parent
+    h->hFSegOs.assign(h->nF, 0);
+    h->hFSegN.assign(h->nF, 0);
+    h->hSOs.clear();
+    h->hSN.clear();
+    h->hSParent.clear();
+    for (uint64_t f = 0; f < h->nF;) {
+        uint64_t e = f + 1;
+        while (e < h->nF && h->hParent[e] == h->hParent[f]) e++;
+        const uint64_t os = h->hObsOff[f], n = h->hObsOff[e] - os;
+        if (n > 0xFFFFFFFFull) {
+            set_err(h, "too many observations below one region");
+            return MRA_E_UNSUPPORTED;
+        }
+        for (uint64_t g = f; g < e; g++) {
+            h->hFSegOs[g] = os;
+            h->hFSegN[g] = (uint32_t)n;
+        }
+        h->hSOs.push_back(os);
+        h->hSN.push_back((uint32_t)n);
+        h->hSParent.push_back(h->hParent[f]);
+        f = e;
+    }
     // workspace offsets
     h->hSigOff.assign(h->nF + 1, 0);
     for (uint64_t f = 0; f < h->nF; f++) {
@@ -265,6 +292,11 @@ static int alloc_engine(mra_handle* h, const mra_tree_desc* t) {
     RC_TRY(upload(h, &h->dChE, h->hChE.data(), h->hChE.size()));
     RC_TRY(upload(h, &h->dWorking, h->hWorking.data(), h->nC));
     RC_TRY(upload(h, &h->dSigOff, h->hSigOff.data(), h->nF + 1));
+    RC_TRY(upload(h, &h->dFSegOs, h->hFSegOs.data(), h->nF));
+    RC_TRY(upload(h, &h->dFSegN, h->hFSegN.data(), h->nF));
+    RC_TRY(upload(h, &h->dSOs, h->hSOs.data(), h->hSOs.size()));
+    RC_TRY(upload(h, &h->dSN, h->hSN.data(), h->hSN.size()));
+    RC_TRY(upload(h, &h->dSParent, h->hSParent.data(), h->hSParent.size()));
     RC_TRY(upload(h, &h->dGoff, h->hGoff.data(), h->L + 1));
     // engine storage
     RC_TRY(dalloc(h, &h->dGc, h->hGoff[h->L], A));
@@ -292,15 +324,13 @@ static int alloc_engine(mra_handle* h, const mra_tree_desc* t) {
     }
     // work lists
     std::vector<uint32_t> ix, iF;
-    for (uint64_t f = 0; f < h->nF; f++) {
-        const uint64_t n = h->hObsOff[f + 1] - h->hObsOff[f];
-        if (!n) continue;
-        iF.push_back((uint32_t)f);
-        for (uint64_t c0 = 0; c0 < n; c0 += 64) {
-            ix.push_back((uint32_t)f);
+    for (uint64_t f = 0; f < h->nF; f++)
+        if (h->hObsOff[f + 1] > h->hObsOff[f]) iF.push_back((uint32_t)f);
+    for (size_t sgi = 0; sgi < h->hSN.size(); sgi++)
+        for (uint64_t c0 = 0; c0 < h->hSN[sgi]; c0 += 64) {
+            ix.push_back((uint32_t)sgi);
             ix.push_back((uint32_t)c0);
         }
-    }
     h->nItemsX = (uint32_t)(ix.size() / 2);
     h->nItemsF = (uint32_t)iF.size();
     RC_TRY(upload(h, &h->dItemsX, ix.data(), ix.size()));
@@ -524,6 +554,11 @@ static Tree make_tree(mra_handle* h) {
     T.ch_e = h->dChE;
     T.working = h->dWorking;
     T.sig_off = h->dSigOff;
+    T.f_seg_os = h->dFSegOs;
+    T.f_seg_n = h->dFSegN;
+    T.s_os = h->dSOs;
+    T.s_n = h->dSN;
+    T.s_parent = h->dSParent;
     return T;
 }
 

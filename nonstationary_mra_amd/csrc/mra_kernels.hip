@@ -13,8 +13,8 @@
 //        ancestor knot, stacked over j) x column c (region knot)
 //   Lt   prior Cholesky factor L_R stored column-major (Lt[k*RP+i] = L[i][k])
 //   LiT  L_R^{-1} column-major;   Lpt / LpiT: the posterior factor and inverse
-//   Gf, Bf  finest G_{F,j} / B_{F,j}: region block at L*RP*obs_off[F],
-//        element (m, o) at m*n_F + o
+//   Gf, Bf  finest G_{F,j} / B_{F,j}: stored per segment (finest regions
+//        sharing a parent), element (m, o) at L*RP*seg_os + m*seg_n + o-seg_os
 //   H    L_F^{-1} B_F^T, observation-major rows of L*RP
 //   Sig  L_F (row-major n_F x n_F) at sig_off[F]
 //   msgs w~ (l*RP) and A~ (l(l+1)/2 slices of RP*RP, slice order
@@ -33,6 +33,14 @@ struct Tree {
     const uint64_t* ch_e;
     const uint8_t* working;   // [nC]
     const uint64_t* sig_off;  // [nF]
+    // segments: maximal runs of finest regions with the same parent; G_F / B_F
+    // of a segment are stored together, element (m, o) at
+    // LRP*seg_os + m*seg_n + (o - seg_os)
+    const uint64_t* f_seg_os;  // [nF] first observation of F's segment
+    const uint32_t* f_seg_n;   // [nF] observations in F's segment
+    const uint64_t* s_os;      // [nSeg] first observation of the segment
+    const uint32_t* s_n;       // [nSeg]
+    const uint64_t* s_parent;  // [nSeg] global coarse id of the shared parent
 };
 
 __device__ __forceinline__ uint64_t lvl_start(int l, int J) {
@@ -274,7 +282,8 @@ __global__ __launch_bounds__(256) void k_prior_coarse(Tree T, CovCtx cc, Pts K, 
 }
 
 // ---------------------------------------------------------- finest level
-// Observation columns of one finest region, tiles of 64 columns:
+// Observation columns of one segment (all observations below one parent,
+// which share every ancestor), tiles of 64 columns:
 //   B_{F,j} = C(Q_{a_j}, S) - Gcat_{a_j}^T Gcat_F[0:j],  G_{F,j} = L_{a_j}^{-1} B_{F,j}
 // (…finest-level-in-prior.cpp:116-163)
 template <int RP>
@@ -284,15 +293,15 @@ __global__ __launch_bounds__(256) void k_finest_cross(Tree T, CovCtx cc, Pts K, 
     constexpr int NT = 64, LD = NT + 1;
     __shared__ double Cm[RP * LD];
     __shared__ double Wm[RP * LD];
-    const uint32_t F = items[2 * blockIdx.x], c0 = items[2 * blockIdx.x + 1];
-    const uint64_t ob = T.obs_off[F];
-    const int nF = (int)(T.obs_off[F + 1] - ob);
+    const uint32_t sg = items[2 * blockIdx.x], c0 = items[2 * blockIdx.x + 1];
+    const uint64_t ob = T.s_os[sg];
+    const int nF = (int)T.s_n[sg];
     const int n = min(NT, nF - (int)c0);
     const int r = T.r, L = T.L;
     double* GF = Gf + (uint64_t)T.LRP * ob;
     double* BF = Bf + (uint64_t)T.LRP * ob;
     for (int j = 0; j < L; j++) {
-        const uint64_t a = ancestor_up(T.parent[F], L - 1, j, T.J);
+        const uint64_t a = ancestor_up(T.s_parent[sg], L - 1, j, T.J);
         for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
             const int i = idx / NT, c = idx % NT;
             Cm[i * LD + c] = (i < r && c < n) ? pcov(cc, K, a * RP + i, O, ob + c0 + c) : 0.0;
@@ -328,6 +337,7 @@ __global__ __launch_bounds__(256) void k_finest_local(Tree T, CovCtx cc, Pts O, 
     constexpr int LDS_LD = 65;
     __shared__ double Sm[64 * LDS_LD];
     __shared__ double red[MRA_WG];
+    __shared__ double kred[3 * 4 * 256];
     __shared__ int bad;
     const uint32_t F = items[blockIdx.x];
     const uint64_t ob = T.obs_off[F];
@@ -336,20 +346,26 @@ __global__ __launch_bounds__(256) void k_finest_local(Tree T, CovCtx cc, Pts O, 
     const bool small = n <= 64;
     double* Sg = small ? Sm : Sig + T.sig_off[F];
     const int ld = small ? LDS_LD : n;
-    const double* GF = Gf + (uint64_t)LRP * ob;
-    double* BF = Bf + (uint64_t)LRP * ob;
+    const uint64_t so = T.f_seg_os[F];
+    const int sn = (int)T.f_seg_n[F];
+    const double* GF = Gf + (uint64_t)LRP * so + (ob - so);
+    double* BF = Bf + (uint64_t)LRP * so + (ob - so);
     if (threadIdx.x == 0) bad = 0;
     for (int idx = threadIdx.x; idx < n * n; idx += MRA_WG) {
         const int i = idx / n, c = idx % n;
         Sg[i * ld + c] = (i == c) ? O.v[ob + i] : pcov(cc, O, ob + i, O, ob + c);
     }
     __syncthreads();
-    for (int ti = 0; ti < n; ti += 64)
-        for (int tj = 0; tj < n; tj += 64) {
-            const int mv = min(64, n - ti), nv = min(64, n - tj);
-            wg_gemm_tn(Sg + ti * ld + tj, ld, (mv + 15) & ~15, (nv + 15) & ~15, LRP, -1.0, GF + ti, n, mv, GF + tj, n,
-                       nv);
-        }
+    if (n <= 32) {
+        wg_gemm_tn_ksplit(Sg, ld, (n + 15) & ~15, (n + 15) & ~15, LRP, -1.0, GF, sn, n, GF, sn, n, kred);
+    } else {
+        for (int ti = 0; ti < n; ti += 64)
+            for (int tj = 0; tj < n; tj += 64) {
+                const int mv = min(64, n - ti), nv = min(64, n - tj);
+                wg_gemm_tn(Sg + ti * ld + tj, ld, (mv + 15) & ~15, (nv + 15) & ~15, LRP, -1.0, GF + ti, sn, mv,
+                           GF + tj, sn, nv);
+            }
+    }
     __syncthreads();
     wg_chol(Sg, ld, n, &bad);
     // u = L_F^{-1} y
@@ -362,7 +378,7 @@ __global__ __launch_bounds__(256) void k_finest_local(Tree T, CovCtx cc, Pts O, 
     double dF = wg_sum(part, red);
     // H rows: solve L_F h = b_m for every row m of B_F (n values each)
     for (int m = threadIdx.x; m < LRP; m += MRA_WG) {
-        double* b = BF + (uint64_t)m * n;
+        double* b = BF + (uint64_t)m * sn;
         for (int i = 0; i < n; i++) {
             double s = b[i];
             for (int k = 0; k < i; k++) s -= Sg[i * ld + k] * b[k];
@@ -630,7 +646,9 @@ __global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts 
     }
     if (nF > 0) {
         double* GL = a.GL + a.gl_off[slot];
-        const double* GF = a.Gf + (uint64_t)LRP * ob;
+        const uint64_t so = T.f_seg_os[F];
+        const int sn = (int)T.f_seg_n[F];
+        const double* GF = a.Gf + (uint64_t)LRP * so + (ob - so);
         for (int t0 = 0; t0 < nF; t0 += RP) {  // row tiles of RP (the tile height of Cm)
             const int mv = min(RP, nF - t0);
             for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
@@ -638,7 +656,7 @@ __global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts 
                 Cm[i * LD + cc2] = (i < mv && cc2 < np) ? pcov(cc, O, ob + t0 + i, Pp, pb + p0 + cc2) : 0.0;
             }
             __syncthreads();
-            wg_gemm_tn(Cm, LD, (mv + 15) & ~15, NT, LRP, -1.0, GF + t0, nF, mv, Gh, NT, NT);
+            wg_gemm_tn(Cm, LD, (mv + 15) & ~15, NT, LRP, -1.0, GF + t0, sn, mv, Gh, NT, NT);
             __syncthreads();
             for (int idx = threadIdx.x; idx < mv * NT; idx += MRA_WG)
                 GL[(uint64_t)(t0 + idx / NT) * NT + idx % NT] = Cm[(idx / NT) * LD + idx % NT];
@@ -780,13 +798,15 @@ __global__ __launch_bounds__(256) void k_sample_finest(Tree T, const uint32_t* i
     const uint64_t ob = T.obs_off[F];
     const int n = (int)(T.obs_off[F + 1] - ob);
     const int RP = T.RP, L = T.L;
-    const double* GF = Gf + (uint64_t)T.LRP * ob;
+    const uint64_t so = T.f_seg_os[F];
+    const int sn = (int)T.f_seg_n[F];
+    const double* GF = Gf + (uint64_t)T.LRP * so + (ob - so);
     const double* LF = Sig + T.sig_off[F];
     for (int o = threadIdx.x; o < n; o += MRA_WG) {
         double s = 0.0;
         for (int j = 0; j < L; j++) {
             const uint64_t a = ancestor_up(T.parent[F], L - 1, j, T.J);
-            for (int m = 0; m < T.r; m++) s += GF[(uint64_t)(j * RP + m) * n + o] * xi_c[a * RP + m];
+            for (int m = 0; m < T.r; m++) s += GF[(uint64_t)(j * RP + m) * sn + o] * xi_c[a * RP + m];
         }
         for (int k = 0; k <= o; k++) s += LF[(uint64_t)o * n + k] * xi_o[ob + k];
         y[ob + o] = s;
