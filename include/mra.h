@@ -136,6 +136,15 @@ int mra_set_observations(mra_handle* h, const double* values);
  * later mra_likelihood calls (1), or launch kernels directly (0). */
 int mra_use_graph(mra_handle* h, int enable);
 
+/* Engine options (no reference counterpart).
+ * MRA_OPT_FACTORED_POSTERIOR: 1 (default) propagates the finest-but-one
+ * level's posterior messages in factored form (A~ = X'^T X', an exact
+ * reformulation by the Woodbury identity); 0 uses the reference's dense
+ * update at every level.  mra_get_option reports whether the factored form is in effect. */
+enum { MRA_OPT_FACTORED_POSTERIOR = 1 };
+int mra_set_option(mra_handle* h, int option, int value);
+int mra_get_option(mra_handle* h, int option, int* value);
+
 /* Timing of the last evaluation: milliseconds per launch group, measured
  * with HIP events on the handle's stream, indexed by the MRA_T_* ids. */
 enum {

@@ -26,7 +26,8 @@ EXPORTED = ["mra_create", "mra_destroy", "mra_last_error", "mra_set_covariance",
             "mra_set_precision", "mra_likelihood", "mra_predict", "mra_likelihood_levels", "mra_sample_prior",
             "mra_set_observations", "mra_use_graph", "mra_last_timings", "mra_cross_covariance",
             "mra_variance_covariance", "mra_cov_tile_device", "mra_partition_program_sizes",
-            "mra_partition_program", "mra_version", "mra_selftest"]
+            "mra_partition_program", "mra_version", "mra_selftest", "mra_set_option", "mra_get_option"]
+OPT_FACTORED_POSTERIOR = 1
 
 
 class TreeDesc(C.Structure):
@@ -89,6 +90,8 @@ def lib():
         _lib.mra_partition_program.argtypes = [dp, dp, dp, C.c_uint64, dp, dp, C.c_uint64, C.c_int, C.c_int,
                                                C.c_int, C.c_double, dp, dp, dp, u64p, u64p, dp, dp, dp, u64p, u64p,
                                                dp, dp, u64p]
+        _lib.mra_set_option.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        _lib.mra_get_option.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
         _lib.mra_selftest.argtypes = [C.c_int, C.c_int, C.c_int] + [dp] * 8 + [C.POINTER(C.c_int)]
     return _lib
 
@@ -206,6 +209,14 @@ class Handle:
 
     def set_precision(self, p):
         check(lib().mra_set_precision(self.h, p), self.h)
+
+    def set_option(self, option, value):
+        check(lib().mra_set_option(self.h, option, int(value)), self.h)
+
+    def get_option(self, option):
+        v = C.c_int()
+        check(lib().mra_get_option(self.h, option, C.byref(v)), self.h)
+        return v.value
 
     def use_graph(self, on=True):
         check(lib().mra_use_graph(self.h, int(on)), self.h)

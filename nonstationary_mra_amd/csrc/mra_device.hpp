@@ -152,7 +152,8 @@ __device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int
 
 // Same product for an output of at most 32 x 32 (one 2x2 tile block), with
 // K split over the 4 waves; the partial tiles are summed in wave order
-// through red (LDS, 3*4*256 doubles), so the result is reproducible.
+// through red (LDS: 3*256 doubles for a 16 x 16 output, else 3*4*256), so
+// the result is reproducible.
 __device__ __forceinline__ void wg_gemm_tn_ksplit(double* C, int ldc, int M, int N, int Kv, double alpha,
                                                   const double* A, int64_t lda, int Mv, const double* B, int64_t ldb,
                                                   int Nv, double* red) {
@@ -168,7 +169,27 @@ __device__ __forceinline__ void wg_gemm_tn_ksplit(double* C, int ldc, int M, int
     // this wave's contiguous K range, a multiple of 4
     const int kq = ((Kv + 15) / 16) * 4;
     const int kb = wave * kq, ke = min(Kv, kb + kq);
-    for (int k0 = kb; k0 < ke; k0 += 4) {
+    constexpr int U = 6;
+    int k0 = kb;
+    for (; k0 + 4 * U <= ke; k0 += 4 * U) {  // batched loads, one latency per U steps
+        double a0[U], a1[U], b0[U], b1[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t k = k0 + 4 * u + k4;
+            a0[u] = ha ? A[k * lda + ia] : 0.0;
+            a1[u] = hb ? A[k * lda + ib] : 0.0;
+            b0[u] = hja ? B[k * ldb + ja] : 0.0;
+            b1[u] = hjb ? B[k * ldb + jb] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b0[u], acc[0], 0, 0, 0);
+            if (tN2) acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b1[u], acc[1], 0, 0, 0);
+            if (tM2) acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b0[u], acc[2], 0, 0, 0);
+            if (tM2 && tN2) acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b1[u], acc[3], 0, 0, 0);
+        }
+    }
+    for (; k0 < ke; k0 += 4) {
         const int k = k0 + k4;
         const bool kv = k < ke;
         const double a0 = (kv && ha) ? A[(int64_t)k * lda + ia] : 0.0;
@@ -180,20 +201,23 @@ __device__ __forceinline__ void wg_gemm_tn_ksplit(double* C, int ldc, int M, int
         if (tM2) acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[2], 0, 0, 0);
         if (tM2 && tN2) acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[3], 0, 0, 0);
     }
+    const int nq = (tM2 || tN2) ? 4 : 1;  // red holds 3*nq*256 doubles
     if (wave > 0) {
 #pragma unroll
         for (int q = 0; q < 4; q++)
+            if (q < nq)
 #pragma unroll
-            for (int v = 0; v < 4; v++) red[((wave - 1) * 4 + q) * 256 + v * 64 + lane] = acc[q][v];
+                for (int v = 0; v < 4; v++) red[((wave - 1) * nq + q) * 256 + v * 64 + lane] = acc[q][v];
     }
     __syncthreads();
     if (wave == 0) {
 #pragma unroll
         for (int q = 0; q < 4; q++)
+            if (q < nq)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 double t = acc[q][v];
-                for (int w = 0; w < 3; w++) t += red[(w * 4 + q) * 256 + v * 64 + lane];
+                for (int w = 0; w < 3; w++) t += red[(w * nq + q) * 256 + v * 64 + lane];
                 const int row = (q >> 1) * 16 + k4 + 4 * v, col = (q & 1) * 16 + r16;
                 if (row < Mv && col < Nv) C[row * ldc + col] += alpha * t;
             }

@@ -38,6 +38,21 @@ struct mra_handle {
     std::vector<uint32_t> hFSegN, hSN;
     uint64_t *dFSegOs = nullptr, *dSOs = nullptr, *dSParent = nullptr;
     uint32_t *dFSegN = nullptr, *dSN = nullptr;
+    // factored posterior of the finest-but-one level
+    bool lowrank = false;
+    int lowrankMode = 1;  // requested: 1 = when possible, 0 = dense everywhere
+    bool denseFull = false;  // full-size dense messages of level L-1 allocated
+    std::vector<uint64_t> hRowS1, hRowE1, hRowS2, hRowE2;  // rows of levels L-1 and L-2
+    uint64_t *dRowS1 = nullptr, *dRowE1 = nullptr, *dRowS2 = nullptr, *dRowE2 = nullptr;
+    double *dXo = nullptr, *dZo = nullptr, *dLrWs = nullptr;
+    uint64_t* dLrWsOff = nullptr;
+    // finest-but-one regions: factored buckets (rows <= 16, 32, 64), all
+    // factored ones, and the larger ones that keep the dense update
+    uint32_t* dItemsLR[3] = {};
+    uint32_t nItemsLR[3] = {};
+    uint32_t *dItemsLRAll = nullptr, *dItemsBig = nullptr;
+    uint32_t nItemsLRAll = 0, nItemsBig = 0;
+    int32_t* dSlot1 = nullptr;  // dense-message slot per finest-but-one region (-1: factored)
     std::vector<double> hObsVal;
     // device inputs
     std::vector<void*> allocs;
@@ -58,6 +73,10 @@ struct mra_handle {
     // work lists
     uint32_t *dItemsX = nullptr, *dItemsF = nullptr;
     uint32_t nItemsX = 0, nItemsF = 0;
+    // finest regions by length bucket: n <= 16, 32, 64, 128, larger
+    static constexpr int NBK = 5;
+    uint32_t* dItemsFB[NBK] = {};
+    uint32_t nItemsFB[NBK] = {};
     std::vector<uint32_t*> dItemsPost;
     std::vector<uint32_t> nItemsPost;
     // prediction
@@ -240,6 +259,37 @@ static int build_structure(mra_handle* h, const mra_tree_desc* t) {
         h->hSParent.push_back(h->hParent[f]);
         f = e;
     }
+    // observation rows below each finest-but-one region, and below each
+    // region one level up (the factored posterior needs the latter contiguous)
+    h->hRowS1.assign(n2, 0);
+    h->hRowE1.assign(n2, 0);
+    for (uint64_t k = 0; k < n2; k++)
+        if (h->hChE[k] >= h->hChS[k]) {
+            h->hRowS1[k] = h->hObsOff[h->hChS[k]];
+            h->hRowE1[k] = h->hObsOff[h->hChE[k] + 1];
+        }
+    h->lowrank = false;
+    if (h->L >= 2) {
+        const uint64_t s3 = h->lvlStart[h->L - 2], n3 = s2 - s3;
+        h->hRowS2.assign(n3, 0);
+        h->hRowE2.assign(n3, 0);
+        bool ok = true;
+        for (uint64_t k = 0; k < n3; k++) {
+            uint64_t lo = UINT64_MAX, hi = 0, cnt = 0;
+            for (int c = 0; c < h->J; c++) {
+                const uint64_t ch = (s3 + k) * h->J + 1 + c - s2;
+                if (h->hRowE1[ch] <= h->hRowS1[ch]) continue;
+                lo = std::min(lo, h->hRowS1[ch]);
+                hi = std::max(hi, h->hRowE1[ch]);
+                cnt += h->hRowE1[ch] - h->hRowS1[ch];
+            }
+            if (cnt == 0) continue;
+            if (hi - lo != cnt) ok = false;
+            h->hRowS2[k] = lo;
+            h->hRowE2[k] = hi;
+        }
+        h->lowrank = ok;
+    }
     // workspace offsets
     h->hSigOff.assign(h->nF + 1, 0);
     for (uint64_t f = 0; f < h->nF; f++) {
@@ -317,10 +367,20 @@ static int alloc_engine(mra_handle* h, const mra_tree_desc* t) {
     RC_TRY(dalloc(h, &h->dBad, 1, A));
     h->dW.assign(h->L, nullptr);
     h->dA.assign(h->L, nullptr);
-    for (int l = 1; l < h->L; l++) {
+    // dense messages (those of level L-1 only when that level is not factored,
+    // allocated by ensure_dense_msgs)
+    for (int l = 1; l < h->L - 1; l++) {
         const uint64_t nl = h->lvlStart[l + 1] - h->lvlStart[l];
         RC_TRY(dalloc(h, &h->dW[l], nl * l * RP, A));
         RC_TRY(dalloc(h, &h->dA[l], nl * (uint64_t)(l * (l + 1) / 2) * RP2, A));
+    }
+    RC_TRY(upload(h, &h->dRowS1, h->hRowS1.data(), h->hRowS1.size()));
+    RC_TRY(upload(h, &h->dRowE1, h->hRowE1.data(), h->hRowE1.size()));
+    if (h->lowrank) {
+        RC_TRY(upload(h, &h->dRowS2, h->hRowS2.data(), h->hRowS2.size()));
+        RC_TRY(upload(h, &h->dRowE2, h->hRowE2.data(), h->hRowE2.size()));
+        RC_TRY(dalloc(h, &h->dXo, (uint64_t)(h->L - 1) * RP * h->nObs, A));
+        RC_TRY(dalloc(h, &h->dZo, h->nObs, A));
     }
     // work lists
     std::vector<uint32_t> ix, iF;
@@ -333,6 +393,17 @@ static int alloc_engine(mra_handle* h, const mra_tree_desc* t) {
         }
     h->nItemsX = (uint32_t)(ix.size() / 2);
     h->nItemsF = (uint32_t)iF.size();
+    {
+        std::vector<uint32_t> b[mra_handle::NBK];
+        for (uint32_t f : iF) {
+            const uint64_t n = h->hObsOff[f + 1] - h->hObsOff[f];
+            b[n <= 16 ? 0 : n <= 32 ? 1 : n <= 64 ? 2 : n <= 128 ? 3 : 4].push_back(f);
+        }
+        for (int k = 0; k < mra_handle::NBK; k++) {
+            h->nItemsFB[k] = (uint32_t)b[k].size();
+            RC_TRY(upload(h, &h->dItemsFB[k], b[k].data(), b[k].size()));
+        }
+    }
     RC_TRY(upload(h, &h->dItemsX, ix.data(), ix.size()));
     RC_TRY(upload(h, &h->dItemsF, iF.data(), iF.size()));
     h->dItemsPost.assign(h->L, nullptr);
@@ -343,6 +414,36 @@ static int alloc_engine(mra_handle* h, const mra_tree_desc* t) {
             if (h->hWorking[R]) it.push_back((uint32_t)(R - h->lvlStart[l]));
         h->nItemsPost[l] = (uint32_t)it.size();
         RC_TRY(upload(h, &h->dItemsPost[l], it.data(), it.size()));
+    }
+    if (h->lowrank) {
+        const int l1 = h->L - 1;
+        const uint64_t n1 = h->lvlStart[l1 + 1] - h->lvlStart[l1];
+        std::vector<uint32_t> b[3], all, big;
+        std::vector<int32_t> slot(n1, -1);
+        for (uint64_t k = 0; k < n1; k++) {
+            if (!h->hWorking[h->lvlStart[l1] + k]) continue;
+            const uint64_t ns = h->hRowE1[k] - h->hRowS1[k];
+            if (ns > 64) {
+                slot[k] = (int32_t)big.size();
+                big.push_back((uint32_t)k);
+            } else {
+                b[ns <= 16 ? 0 : ns <= 32 ? 1 : 2].push_back((uint32_t)k);
+                all.push_back((uint32_t)k);
+            }
+        }
+        for (int q = 0; q < 3; q++) {
+            h->nItemsLR[q] = (uint32_t)b[q].size();
+            RC_TRY(upload(h, &h->dItemsLR[q], b[q].data(), b[q].size()));
+        }
+        h->nItemsLRAll = (uint32_t)all.size();
+        h->nItemsBig = (uint32_t)big.size();
+        RC_TRY(upload(h, &h->dItemsLRAll, all.data(), all.size()));
+        RC_TRY(upload(h, &h->dItemsBig, big.data(), big.size()));
+        RC_TRY(upload(h, &h->dSlot1, slot.data(), slot.size()));
+        // dense messages of the large regions only
+        RC_TRY(dalloc(h, &h->dW[l1], (uint64_t)std::max<uint32_t>(h->nItemsBig, 1) * l1 * RP, A));
+        RC_TRY(dalloc(h, &h->dA[l1],
+                      (uint64_t)std::max<uint32_t>(h->nItemsBig, 1) * (uint64_t)(l1 * (l1 + 1) / 2) * RP2, A));
     }
     if (h->nPred) {
         // prediction items (F, p0, slot) in chunks bounding the scratch
@@ -537,6 +638,38 @@ extern "C" int mra_last_timings(mra_handle* h, double* ms, int n) {
     return MRA_OK;
 }
 
+static bool use_lowrank(const mra_handle* h) { return h->lowrank && h->lowrankMode != 0; }
+
+static int ensure_dense_msgs(mra_handle* h) {
+    const int l = h->L - 1;
+    if (l < 1 || h->denseFull) return MRA_OK;
+    const uint64_t nl = h->lvlStart[l + 1] - h->lvlStart[l], RP = h->RP;
+    RC_TRY(dalloc(h, &h->dW[l], nl * l * RP, h->allocs));
+    RC_TRY(dalloc(h, &h->dA[l], nl * (uint64_t)(l * (l + 1) / 2) * RP * RP, h->allocs));
+    h->denseFull = true;
+    return MRA_OK;
+}
+
+extern "C" int mra_set_option(mra_handle* h, int option, int value) {
+    if (!h) return MRA_E_ARG;
+    if (option == MRA_OPT_FACTORED_POSTERIOR) {
+        h->lowrankMode = value ? 1 : 0;
+        return MRA_OK;
+    }
+    set_err(h, "unknown option");
+    return MRA_E_ARG;
+}
+
+extern "C" int mra_get_option(mra_handle* h, int option, int* value) {
+    if (!h || !value) return MRA_E_ARG;
+    if (option == MRA_OPT_FACTORED_POSTERIOR) {
+        *value = use_lowrank(h) ? 1 : 0;
+        return MRA_OK;
+    }
+    set_err(h, "unknown option");
+    return MRA_E_ARG;
+}
+
 static Tree make_tree(mra_handle* h) {
     Tree T;
     T.M = h->M;
@@ -572,6 +705,27 @@ static CovCtx make_cc(mra_handle* h) {
 }
 
 static Pts pts(double* const* c, double* const* s) { return Pts{c[0], c[1], c[2], s[0], s[1], s[2]}; }
+
+template <int RP>
+static void launch_finest_local(mra_handle* h, const Tree& T, const CovCtx& cc, const Pts& O) {
+    hipStream_t st = h->stream;
+    // larger buckets first (longest work groups start early)
+    if (h->nItemsFB[4])
+        k_finest_local<RP, 0><<<h->nItemsFB[4], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[4], h->dGf, h->dBf,
+                                                              h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
+    if (h->nItemsFB[3])
+        k_finest_local<RP, 128><<<h->nItemsFB[3], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[3], h->dGf, h->dBf,
+                                                                h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
+    if (h->nItemsFB[2])
+        k_finest_local<RP, 64><<<h->nItemsFB[2], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[2], h->dGf, h->dBf,
+                                                               h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
+    if (h->nItemsFB[1])
+        k_finest_local<RP, 32><<<h->nItemsFB[1], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[1], h->dGf, h->dBf,
+                                                               h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
+    if (h->nItemsFB[0])
+        k_finest_local<RP, 16><<<h->nItemsFB[0], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[0], h->dGf, h->dBf,
+                                                               h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
+}
 
 template <int RP>
 static void launch_eval(mra_handle* h, bool predicting) {
@@ -611,8 +765,7 @@ static void launch_eval(mra_handle* h, bool predicting) {
                                                        h->dBf);
     mark(MRA_T_FINEST_LOCAL);
     if (h->nItemsF)
-        k_finest_local<RP><<<h->nItemsF, 256, 0, st>>>(T, cc, O, h->dY, h->dItemsF, h->dGf, h->dBf, h->dSig, h->dH,
-                                                       h->dU, h->dDterm, h->dBad);
+        launch_finest_local<RP>(h, T, cc, O);
     mark(MRA_T_PRED_PRE);
     PredArgs pa{};
     if (predicting) {
@@ -640,6 +793,7 @@ static void launch_eval(mra_handle* h, bool predicting) {
         }
     }
     // posterior, finest-but-one level up to the root
+    const bool lr = use_lowrank(h);
     for (int l = h->L - 1; l >= 0; l--) {
         mark(MRA_T_POST0 + l);
         if (!h->nItemsPost[l]) continue;
@@ -654,12 +808,76 @@ static void launch_eval(mra_handle* h, bool predicting) {
         a.P = h->dP;
         a.v = h->dV;
         a.dterm = h->dDterm;
-        a.H = h->dH;
-        a.u = h->dU;
         a.wo = l > 0 ? h->dW[l] : nullptr;
         a.Ao = l > 0 ? h->dA[l] : nullptr;
+        a.write_msgs = 1;
+        a.write_d = 1;
         a.bad_flag = h->dBad;
         if (l == h->L - 1) {
+            a.X = h->dH;
+            a.z = h->dU;
+            a.ldx = h->LRP;
+            a.row_s = h->dRowS1;
+            a.row_e = h->dRowE1;
+            if (lr) {
+                LowRankArgs b{};
+                b.l = l;
+                b.start_l = h->lvlStart[l];
+                b.row_s = h->dRowS1;
+                b.row_e = h->dRowE1;
+                b.X = h->dH;
+                b.ldx = h->LRP;
+                b.z = h->dU;
+                b.LiT = h->dLiT;
+                b.Xo = h->dXo;
+                b.ldo = (h->L - 1) * h->RP;
+                b.zo = h->dZo;
+                b.dterm = h->dDterm;
+                b.ws = h->dLrWs;
+                b.ws_off = h->dLrWsOff;
+                b.bad_flag = h->dBad;
+                if (h->nItemsBig) {
+                    // large regions: dense update into compact slots, zero factor rows
+                    PostArgs d = a;
+                    d.items = h->dItemsBig;
+                    d.slot = h->dSlot1;
+                    d.Xz = h->dXo;
+                    d.ldxz = (h->L - 1) * h->RP;
+                    d.zz = h->dZo;
+                    k_posterior<RP, true><<<h->nItemsBig, 256, 0, st>>>(T, d);
+                }
+                if (h->nItemsLR[2]) {
+                    b.items = h->dItemsLR[2];
+                    k_post_lowrank<RP, 64><<<h->nItemsLR[2], 256, 0, st>>>(T, b);
+                }
+                if (h->nItemsLR[1]) {
+                    b.items = h->dItemsLR[1];
+                    k_post_lowrank<RP, 32><<<h->nItemsLR[1], 256, 0, st>>>(T, b);
+                }
+                if (h->nItemsLR[0]) {
+                    b.items = h->dItemsLR[0];
+                    k_post_lowrank<RP, 16><<<h->nItemsLR[0], 256, 0, st>>>(T, b);
+                }
+                if (predicting && h->nItemsLRAll) {
+                    // posterior factors, v and P for the top-down prediction
+                    a.write_msgs = 0;
+                    a.write_d = 0;
+                    a.items = h->dItemsLRAll;
+                    k_posterior<RP, true><<<h->nItemsLRAll, 256, 0, st>>>(T, a);
+                }
+            } else {
+                k_posterior<RP, true><<<h->nItemsPost[l], 256, 0, st>>>(T, a);
+            }
+        } else if (l == h->L - 2 && lr) {
+            a.X = h->dXo;
+            a.z = h->dZo;
+            a.ldx = (h->L - 1) * h->RP;
+            a.row_s = h->dRowS2;
+            a.row_e = h->dRowE2;
+            a.cslot = h->dSlot1;
+            a.wc = h->dW[l + 1];
+            a.Ac = h->dA[l + 1];
+            a.start_c = h->lvlStart[l + 1];
             k_posterior<RP, true><<<h->nItemsPost[l], 256, 0, st>>>(T, a);
         } else {
             a.wc = h->dW[l + 1];
@@ -692,6 +910,7 @@ static int run_eval(mra_handle* h, bool predicting) {
         set_err(h, "covariance model and parameters must be set before evaluating");
         return MRA_E_STATE;
     }
+    if (!use_lowrank(h)) RC_TRY(ensure_dense_msgs(h));
     switch (h->RP) {
         case 16: launch_eval<16>(h, predicting); break;
         case 32: launch_eval<32>(h, predicting); break;

@@ -330,22 +330,36 @@ __global__ __launch_bounds__(256) void k_finest_cross(Tree T, CovCtx cc, Pts K, 
 // Per finest region: Sigma_F = V(S) - G^T G, L_F = chol, u = L_F^{-1} y,
 // H = L_F^{-1} B^T, d_F = |u|^2 + 2 sum log diag L_F.
 // (…finest-level-in-prior.cpp:165-256)
-template <int RP>
+// Length-bucketed: NB > 0 handles regions with n_F <= NB with Sigma_F in LDS
+// (NB = 16/32: the GEMM is split over the 4 waves and the triangular solve
+// for H keeps one row of B per thread in registers); NB = 0 handles larger
+// regions with Sigma_F in the global workspace.
+template <int NB>
+struct FinestLds {
+    static constexpr int LD = NB + 1;
+    double S[NB * LD];
+    double kred[NB <= 16 ? 3 * 256 : (NB <= 32 ? 3 * 4 * 256 : 1)];
+};
+template <>
+struct FinestLds<0> {
+    static constexpr int LD = 1;
+    double S[1];
+    double kred[1];
+};
+
+template <int RP, int NB>
 __global__ __launch_bounds__(256) void k_finest_local(Tree T, CovCtx cc, Pts O, const double* yv,
                                                       const uint32_t* items, double* Gf, double* Bf, double* Sig,
                                                       double* H, double* u, double* dterm, int* bad_flag) {
-    constexpr int LDS_LD = 65;
-    __shared__ double Sm[64 * LDS_LD];
+    __shared__ FinestLds<NB> sh;
     __shared__ double red[MRA_WG];
-    __shared__ double kred[3 * 4 * 256];
     __shared__ int bad;
     const uint32_t F = items[blockIdx.x];
     const uint64_t ob = T.obs_off[F];
     const int n = (int)(T.obs_off[F + 1] - ob);
     const int LRP = T.LRP;
-    const bool small = n <= 64;
-    double* Sg = small ? Sm : Sig + T.sig_off[F];
-    const int ld = small ? LDS_LD : n;
+    double* Sg = NB > 0 ? sh.S : Sig + T.sig_off[F];
+    const int ld = NB > 0 ? FinestLds<NB>::LD : n;
     const uint64_t so = T.f_seg_os[F];
     const int sn = (int)T.f_seg_n[F];
     const double* GF = Gf + (uint64_t)LRP * so + (ob - so);
@@ -356,8 +370,8 @@ __global__ __launch_bounds__(256) void k_finest_local(Tree T, CovCtx cc, Pts O, 
         Sg[i * ld + c] = (i == c) ? O.v[ob + i] : pcov(cc, O, ob + i, O, ob + c);
     }
     __syncthreads();
-    if (n <= 32) {
-        wg_gemm_tn_ksplit(Sg, ld, (n + 15) & ~15, (n + 15) & ~15, LRP, -1.0, GF, sn, n, GF, sn, n, kred);
+    if (NB > 0 && NB <= 32) {
+        wg_gemm_tn_ksplit(Sg, ld, (n + 15) & ~15, (n + 15) & ~15, LRP, -1.0, GF, sn, n, GF, sn, n, sh.kred);
     } else {
         for (int ti = 0; ti < n; ti += 64)
             for (int tj = 0; tj < n; tj += 64) {
@@ -375,22 +389,44 @@ __global__ __launch_bounds__(256) void k_finest_local(Tree T, CovCtx cc, Pts O, 
     wg_trsm(Sg, ld, uF, 1, n, 1);
     double part = 0.0;
     for (int i = threadIdx.x; i < n; i += MRA_WG) part += uF[i] * uF[i] + 2.0 * log(Sg[i * ld + i]);
-    double dF = wg_sum(part, red);
-    // H rows: solve L_F h = b_m for every row m of B_F (n values each)
-    for (int m = threadIdx.x; m < LRP; m += MRA_WG) {
-        double* b = BF + (uint64_t)m * sn;
-        for (int i = 0; i < n; i++) {
-            double s = b[i];
-            for (int k = 0; k < i; k++) s -= Sg[i * ld + k] * b[k];
-            s /= Sg[i * ld + i];
-            b[i] = s;
-            H[(ob + i) * LRP + m] = s;
+    const double dF = wg_sum(part, red);
+    // H rows: solve L_F h = b_m for every row m of B_F
+    if (NB > 0 && NB <= 32) {
+        constexpr int NR = NB > 0 ? NB : 1;
+        for (int m = threadIdx.x; m < LRP; m += MRA_WG) {
+            const double* b = BF + (uint64_t)m * sn;
+            double h[NR];
+#pragma unroll
+            for (int i = 0; i < NR; i++) h[i] = (i < n) ? b[i] : 0.0;
+#pragma unroll
+            for (int i = 0; i < NR; i++) {
+                if (i < n) {
+                    double acc = h[i];
+#pragma unroll
+                    for (int k = 0; k < i; k++) acc -= Sg[i * ld + k] * h[k];
+                    h[i] = acc / Sg[i * ld + i];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < NR; i++)
+                if (i < n) H[(ob + i) * LRP + m] = h[i];
+        }
+    } else {
+        for (int m = threadIdx.x; m < LRP; m += MRA_WG) {
+            double* b = BF + (uint64_t)m * sn;
+            for (int i = 0; i < n; i++) {
+                double acc = b[i];
+                for (int k = 0; k < i; k++) acc -= Sg[i * ld + k] * b[k];
+                acc /= Sg[i * ld + i];
+                b[i] = acc;
+                H[(ob + i) * LRP + m] = acc;
+            }
         }
     }
-    // keep L_F for prediction
-    if (small) {
+    // keep L_F for prediction and sampling
+    if (NB > 0) {
         double* dst = Sig + T.sig_off[F];
-        for (int idx = threadIdx.x; idx < n * n; idx += MRA_WG) dst[idx] = Sm[(idx / n) * LDS_LD + idx % n];
+        for (int idx = threadIdx.x; idx < n * n; idx += MRA_WG) dst[idx] = Sg[(idx / n) * ld + idx % n];
     }
     if (threadIdx.x == 0) {
         dterm[T.nC + F] = dF;
@@ -418,15 +454,28 @@ struct PostArgs {
     double* P;
     double* v;
     double* dterm;
-    // children messages (level l+1), or the finest rows
+    // children messages (level l+1) ...
     const double* wc;
     const double* Ac;
     uint64_t start_c;
-    const double* H;
-    const double* u;
+    // ... or stacked rows: A_{jk} = X_j^T X_k, w_j = X_j^T z over the rows
+    // [row_s, row_e) of the region (X = L_F^{-1} B_F^T rows of the finest
+    // regions below, or factor rows X' of the children)
+    const double* X;
+    const double* z;
+    int ldx;
+    const uint64_t* row_s;
+    const uint64_t* row_e;
     // outputs (level l)
     double* wo;
     double* Ao;
+    int write_msgs;  // 0: only L~, L~^{-1}, v, P (prediction after a factored level)
+    int write_d;
+    const int32_t* slot;   // message slot of each region of the level (null: its local index)
+    const int32_t* cslot;  // rows mode: dense-message slot of each child (-1: the child sent rows)
+    double* Xz;            // rows mode of a dense region in a factored level: zero its rows
+    int ldxz;
+    double* zz;
     int* bad_flag;
 };
 
@@ -436,14 +485,32 @@ __device__ __forceinline__ void post_add_A(const Tree& T, const PostArgs& a, uin
     // C += A_{jk} (or its transpose)
     if (FROM_OBS) {
         const uint64_t loc = R - a.start_l;
-        if (T.ch_e[loc] < T.ch_s[loc]) return;
-        const uint64_t os = T.obs_off[T.ch_s[loc]], oe = T.obs_off[T.ch_e[loc] + 1];
+        const uint64_t os = a.row_s[loc], oe = a.row_e[loc];
+        if (oe <= os) return;
         const int ns = (int)(oe - os);
-        const double* X = a.H + os * T.LRP;
+        const double* X = a.X + os * a.ldx;
         if (!transpose)
-            wg_gemm_tn(C, ld, RP, RP, ns, 1.0, X + j * RP, T.LRP, RP, X + k * RP, T.LRP, RP);
+            wg_gemm_tn(C, ld, RP, RP, ns, 1.0, X + j * RP, a.ldx, RP, X + k * RP, a.ldx, RP);
         else
-            wg_gemm_tn(C, ld, RP, RP, ns, 1.0, X + k * RP, T.LRP, RP, X + j * RP, T.LRP, RP);
+            wg_gemm_tn(C, ld, RP, RP, ns, 1.0, X + k * RP, a.ldx, RP, X + j * RP, a.ldx, RP);
+        if (a.cslot) {
+            // children that sent a dense message (their rows are zero)
+            __syncthreads();
+            const int lc = a.l + 1;
+            const int nslc = lc * (lc + 1) / 2;
+            const int sl = j * lc - j * (j - 1) / 2 + (k - j);
+            for (int c = 0; c < T.J; c++) {
+                const uint64_t ch = R * T.J + 1 + c;
+                const int32_t cs = a.cslot[ch - a.start_c];
+                if (cs < 0) continue;
+                const double* S = a.Ac + ((uint64_t)cs * nslc + sl) * (uint64_t)RP * RP;
+                for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
+                    const int i = idx / RP, q = idx % RP;
+                    C[i * ld + q] += transpose ? S[q * RP + i] : S[idx];
+                }
+                __syncthreads();
+            }
+        }
     } else {
         const int lc = a.l + 1;
         const int nslc = lc * (lc + 1) / 2;
@@ -466,9 +533,16 @@ __device__ __forceinline__ double post_w(const Tree& T, const PostArgs& a, uint6
     double s = 0.0;
     if (FROM_OBS) {
         const uint64_t loc = R - a.start_l;
-        if (T.ch_e[loc] < T.ch_s[loc]) return 0.0;
-        const uint64_t os = T.obs_off[T.ch_s[loc]], oe = T.obs_off[T.ch_e[loc] + 1];
-        for (uint64_t o = os; o < oe; o++) s += a.H[o * T.LRP + j * RP + q] * a.u[o];
+        const uint64_t os = a.row_s[loc], oe = a.row_e[loc];
+        for (uint64_t o = os; o < oe; o++) s += a.X[o * a.ldx + j * RP + q] * a.z[o];
+        if (a.cslot) {
+            const int lc = a.l + 1;
+            for (int c = 0; c < T.J; c++) {
+                const uint64_t ch = R * T.J + 1 + c;
+                const int32_t cs = a.cslot[ch - a.start_c];
+                if (cs >= 0) s += a.wc[(uint64_t)cs * lc * RP + j * RP + q];
+            }
+        }
     } else {
         const int lc = a.l + 1;
         bool first = true;
@@ -494,9 +568,16 @@ __global__ __launch_bounds__(256) void k_posterior(Tree T, PostArgs a) {
     __shared__ int bad;
     const int l = a.l;
     const uint64_t R = a.start_l + a.items[blockIdx.x];
+    const uint64_t msg = a.slot ? (uint64_t)a.slot[R - a.start_l] : R - a.start_l;
     const int r = T.r;
     if (threadIdx.x == 0) bad = 0;
     const double* LR = a.Lt + R * RP * RP;  // column-major L
+    if (a.Xz) {
+        // this region sends a dense message: its factor rows contribute nothing
+        const uint64_t os = a.row_s[R - a.start_l], oe = a.row_e[R - a.start_l];
+        for (uint64_t idx = threadIdx.x; idx < (oe - os) * a.ldxz; idx += MRA_WG) a.Xz[os * a.ldxz + idx] = 0.0;
+        for (uint64_t o = os + threadIdx.x; o < oe; o += MRA_WG) a.zz[o] = 0.0;
+    }
     for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
         Km[(idx / RP) * LD + idx % RP] = 0.0;
         Wm[(idx / RP) * LD + idx % RP] = ((idx / RP) == (idx % RP)) ? 1.0 : 0.0;
@@ -529,7 +610,7 @@ __global__ __launch_bounds__(256) void k_posterior(Tree T, PostArgs a) {
     double lq = 0.0;
     for (int i = threadIdx.x; i < r; i += MRA_WG) lq += 2.0 * log(Km[i * LD + i]) - vv[i] * vv[i];
     const double post_part = wg_sum(lq, red);
-    if (threadIdx.x == 0) a.dterm[R] = -2.0 * logdet_prior + post_part;
+    if (threadIdx.x == 0 && a.write_d) a.dterm[R] = -2.0 * logdet_prior + post_part;
     if (l > 0) {
         double* PR = a.P + a.goff[l] + (R - a.start_l) * (uint64_t)l * RP * RP;
         const double* LpiTR = a.LpiT + R * RP * RP;
@@ -546,13 +627,18 @@ __global__ __launch_bounds__(256) void k_posterior(Tree T, PostArgs a) {
             double* Pj = PR + (uint64_t)j * RP * RP;
             for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) Pj[idx] = Wm[(idx / RP) * LD + idx % RP];
             // w~_j = w_j - P_j^T v
-            for (int q = threadIdx.x; q < RP; q += MRA_WG) {
-                double wj = (q < r) ? post_w<RP, FROM_OBS>(T, a, R, j, q) : 0.0;
-                double s = 0.0;
-                for (int m = 0; m < RP; m++) s += Wm[m * LD + q] * vv[m];
-                a.wo[(R - a.start_l) * (uint64_t)l * RP + j * RP + q] = wj - s;
-            }
+            if (a.write_msgs)
+                for (int q = threadIdx.x; q < RP; q += MRA_WG) {
+                    double wj = (q < r) ? post_w<RP, FROM_OBS>(T, a, R, j, q) : 0.0;
+                    double s = 0.0;
+                    for (int m = 0; m < RP; m++) s += Wm[m * LD + q] * vv[m];
+                    a.wo[msg * (uint64_t)l * RP + j * RP + q] = wj - s;
+                }
             __syncthreads();
+        }
+        if (!a.write_msgs) {
+            if (threadIdx.x == 0 && bad) atomicOr(a.bad_flag, 1);
+            return;
         }
         // A~_{jk} = A_{jk} - P_j^T P_k
         const int nsl = l * (l + 1) / 2;
@@ -566,10 +652,141 @@ __global__ __launch_bounds__(256) void k_posterior(Tree T, PostArgs a) {
                            RP, RP);
                 __syncthreads();
                 const int sl = j * l - j * (j - 1) / 2 + (k - j);
-                double* out = a.Ao + ((R - a.start_l) * nsl + sl) * (uint64_t)RP * RP;
+                double* out = a.Ao + (msg * nsl + sl) * (uint64_t)RP * RP;
                 for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) out[idx] = Km[(idx / RP) * LD + idx % RP];
                 __syncthreads();
             }
+    }
+    if (threadIdx.x == 0 && bad) atomicOr(a.bad_flag, 1);
+}
+
+// Factored posterior of a finest-but-one region R (level l = L-1).  The
+// message A~ = X^T S X, w~ = X^T S z of the dense update (rank <= n_s, the
+// observations below R) is emitted as factor rows instead:
+//   W = L_R^{-1} X_l^T,  I + W^T W = Q Q^T (S = Q^{-T} Q^{-1} by Woodbury),
+//   X' = Q^{-1} X[:, 0:l*RP],  z' = Q^{-1} z   =>  A~ = X'^T X', w~ = X'^T z'
+//   d_R = 2 sum log diag Q - z^T z + z'^T z'   (determinant lemma)
+// The parent level reads the rows X' like stacked observation rows.
+// (Replaces the dense update of src/class_approximation.cpp:266-306 and
+// src/class_approximation-posterior-inference.cpp:51-131 for this level.)
+struct LowRankArgs {
+    int l;
+    uint64_t start_l;
+    const uint32_t* items;
+    const uint64_t* row_s;
+    const uint64_t* row_e;
+    const double* X;  // input rows, width (l+1)*RP
+    int ldx;
+    const double* z;
+    const double* LiT;  // prior L^{-1}, column-major
+    double* Xo;         // output rows, width l*RP
+    int ldo;
+    double* zo;
+    double* dterm;
+    double* ws;  // workspace of regions with more than 64 rows
+    const uint64_t* ws_off;
+    int* bad_flag;
+};
+
+template <int RP, int NB>
+__global__ __launch_bounds__(256) void k_post_lowrank(Tree T, LowRankArgs a) {
+    // NB > 0: regions with at most NB rows, buffers in LDS; NB = 0: larger
+    // regions, buffers in the global workspace
+    constexpr int NBL = NB > 0 ? NB : 1;
+    constexpr int LDS_LD = NBL + 1;
+    constexpr int DIM = RP > NBL ? RP : NBL;
+    constexpr int SA = DIM * LDS_LD;
+    constexpr int SB = (DIM * LDS_LD > NBL * 65) ? DIM * LDS_LD : NBL * 65;
+    __shared__ double bufA[NB > 0 ? SA : 1];
+    __shared__ double bufB[NB > 0 ? SB : 1];
+    __shared__ double zbuf[NBL];
+    __shared__ double red[MRA_WG];
+    __shared__ int bad;
+    const uint32_t loc = a.items[blockIdx.x];
+    const uint64_t R = a.start_l + loc;
+    const uint64_t os = a.row_s[loc];
+    const int ns = (int)(a.row_e[loc] - os);
+    const int l = a.l;
+    if (threadIdx.x == 0) bad = 0;
+    if (ns <= 0) {
+        if (threadIdx.x == 0) a.dterm[R] = 0.0;
+        return;
+    }
+    const bool small = NB > 0;
+    const int dim = max(ns, RP);
+    const int ldw = small ? LDS_LD : dim;
+    const int ldo = small ? 65 : dim;  // row stride of the output tile in B
+    double* A = small ? bufA : a.ws + a.ws_off[blockIdx.x];
+    double* B = small ? bufB : A + (uint64_t)dim * ldw;
+    double* zs = small ? zbuf : B + (uint64_t)dim * ldw;
+    const double* Xr = a.X + os * (uint64_t)a.ldx;
+    // 1. A = X_l^T (RP x ns), B = 0, zs = z
+    for (int idx = threadIdx.x; idx < RP * ns; idx += MRA_WG) {
+        const int k = idx % RP, o = idx / RP;
+        A[k * ldw + o] = Xr[(uint64_t)o * a.ldx + l * RP + k];
+        B[k * ldw + o] = 0.0;
+    }
+    for (int o = threadIdx.x; o < ns; o += MRA_WG) zs[o] = a.z[os + o];
+    __syncthreads();
+    // 2. B = W = L^{-1} X_l^T
+    const double* LiTR = a.LiT + R * RP * RP;
+    for (int o0 = 0; o0 < ns; o0 += 64) {
+        const int nv = min(64, ns - o0);
+        wg_gemm_tn(B + o0, ldw, RP, (nv + 15) & ~15, RP, 1.0, LiTR, RP, RP, A + o0, ldw, nv);
+    }
+    __syncthreads();
+    // 3. A = I + W^T W (ns x ns)
+    for (int idx = threadIdx.x; idx < ns * ns; idx += MRA_WG) {
+        const int o = idx / ns, q = idx % ns;
+        A[o * ldw + q] = (o == q) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    for (int ti = 0; ti < ns; ti += 64)
+        for (int tj = 0; tj < ns; tj += 64) {
+            const int mv = min(64, ns - ti), nv = min(64, ns - tj);
+            wg_gemm_tn(A + ti * ldw + tj, ldw, (mv + 15) & ~15, (nv + 15) & ~15, RP, 1.0, B + ti, ldw, mv, B + tj, ldw,
+                       nv);
+        }
+    __syncthreads();
+    // 4. Q = chol(A);  5. z' = Q^{-1} z, d_R
+    wg_chol(A, ldw, ns, &bad);
+    wg_trsm(A, ldw, zs, 1, ns, 1);
+    double part = 0.0;
+    for (int o = threadIdx.x; o < ns; o += MRA_WG) {
+        const double z0 = a.z[os + o];
+        part += 2.0 * log(A[o * ldw + o]) + zs[o] * zs[o] - z0 * z0;
+        a.zo[os + o] = zs[o];
+    }
+    const double d = wg_sum(part, red);
+    if (threadIdx.x == 0) a.dterm[R] = d;
+    // 6. B = Q^{-1};  7. A = Q^{-T} (column-major Q^{-1})
+    for (int idx = threadIdx.x; idx < ns * ns; idx += MRA_WG) {
+        const int o = idx / ns, q = idx % ns;
+        B[o * ldw + q] = (o == q) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    wg_trsm(A, ldw, B, ldw, ns, ns);
+    for (int idx = threadIdx.x; idx < ns * ns; idx += MRA_WG) {
+        const int o = idx / ns, k = idx % ns;
+        A[k * ldw + o] = B[o * ldw + k];
+    }
+    __syncthreads();
+    // 8. X' = Q^{-1} X[:, 0:l*RP], 64 columns at a time (B is the output tile)
+    const int W = l * RP;
+    for (int c0 = 0; c0 < W; c0 += 64) {
+        const int cv = min(64, W - c0);
+        for (int ti = 0; ti < ns; ti += 64) {
+            const int mv = min(64, ns - ti);
+            for (int idx = threadIdx.x; idx < mv * 64; idx += MRA_WG) B[(idx / 64) * ldo + idx % 64] = 0.0;
+            __syncthreads();
+            wg_gemm_tn(B, ldo, (mv + 15) & ~15, 64, ns, 1.0, A + ti, ldw, mv, Xr + c0, a.ldx, cv);
+            __syncthreads();
+            for (int idx = threadIdx.x; idx < mv * 64; idx += MRA_WG) {
+                const int o = idx / 64, m = idx % 64;
+                if (m < cv) a.Xo[(os + ti + o) * (uint64_t)a.ldo + c0 + m] = B[o * ldo + m];
+            }
+            __syncthreads();
+        }
     }
     if (threadIdx.x == 0 && bad) atomicOr(a.bad_flag, 1);
 }

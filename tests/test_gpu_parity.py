@@ -21,11 +21,12 @@ def rel(a, b):
     return abs(a - b) / max(abs(b), 1e-300)
 
 
-def run_gpu(tree, cov, prm, predict=True):
+def run_gpu(tree, cov, prm, predict=True, factored=1):
     h = capi.Handle(tree, with_pred=predict)
     try:
         h.set_covariance(cov)
         h.set_params(**prm)
+        h.set_option(capi.OPT_FACTORED_POSTERIOR, factored)
         if predict and h.n_pred:
             return h.predict()
         return h.likelihood(), None, None
@@ -33,8 +34,8 @@ def run_gpu(tree, cov, prm, predict=True):
         h.close()
 
 
-def check(tree, cov, prm, predict=True):
-    D, mean, var = run_gpu(tree, cov, prm, predict)
+def check(tree, cov, prm, predict=True, factored=1):
+    D, mean, var = run_gpu(tree, cov, prm, predict, factored)
     Do, mo, vo = oracle.evaluate(tree, cov, prm, predict=predict)
     print(f"D gpu {D!r} oracle {Do!r} rel {rel(D, Do):.3e}", flush=True)
     assert rel(D, Do) < TOL_D
@@ -173,3 +174,29 @@ def test_prior_sampler_matches_dense_covariance():
     z = np.abs(emp - sig) / se
     print("max |z|", z.max(), flush=True)
     assert z.max() < 5.5
+
+
+@pytest.mark.parametrize("factored", [0, 1])
+@pytest.mark.parametrize("case", ["example", "clustered64", "j2"])
+def test_dense_and_factored_posterior(case, factored, example):
+    """The reference's dense posterior update and the factored form of the
+    finest-but-one level both match the oracle (the clustered case has
+    regions with more than 64 observations below them)."""
+    if case == "example":
+        tree, cov, prm = example
+    elif case == "clustered64":
+        rng = np.random.default_rng(78)
+        x, y = synth.clustered_points(9000, rng, n_clusters=3, sigma=0.03)
+        px, py = synth.uniform_points(300, rng)
+        tree = capi.partition_program(x, y, rng.standard_normal(9000), px, py, M=4, J=4, r=64,
+                                      bounds=synth.UNIT_BOUNDS)
+        cov, prm = synth.linear_trend_cov(), dict(synth.UNIT_PARAMS)
+    else:
+        tree, cov, prm = cases.synthetic_case(n=900, npred=60, M=5, J=2, r=9, seed=8, clustered=True)
+    h = capi.Handle(tree)
+    h.set_covariance(cov)
+    h.set_params(**prm)
+    h.set_option(capi.OPT_FACTORED_POSTERIOR, factored)
+    assert h.get_option(capi.OPT_FACTORED_POSTERIOR) == factored
+    h.close()
+    check(tree, cov, prm, factored=factored)
