@@ -86,21 +86,28 @@ __device__ __forceinline__ float pcovf(const CovCtx& c, const Pts& P, uint64_t i
 // of 16).  A(k,i) = A[k*lda+i] valid for k < Kv, i < Mv (else 0); B likewise
 // with Nv.  K is rounded up to a multiple of 4.  C is LDS (or global) and
 // must be made visible by the caller (__syncthreads before and after).
+// TA: A is stored transposed, A(k,i) = A[i*lda+k] (an LDS matrix read by
+// columns).  ACC = false stores C = alpha*A^T B instead of accumulating.
+template <bool TA = false, bool ACC = true>
 __device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int Kv, double alpha, const double* A,
                                            int64_t lda, int Mv, const double* B, int64_t ldb, int Nv) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r16 = lane & 15, k4 = lane >> 4;
     const int tM = M >> 4, tN = N >> 4;
-    // 2x2 tile blocks per wave where possible
-    const int bM = (tM + 1) >> 1, bN = (tN + 1) >> 1;
+    // per-wave blocks of bh x bw tiles: 2x2 when that still gives every wave a
+    // block, else 2x1 or 1x1 (all wave-uniform)
+    const int nt = tM * tN;
+    const int bh = (nt >= 16 || (nt >= 8 && tM >= 2)) ? 2 : 1;
+    const int bw = (nt >= 16) ? 2 : 1;
+    const int bM = (tM + bh - 1) / bh, bN = (tN + bw - 1) / bw;
     for (int blk = wave; blk < bM * bN; blk += 4) {
-        const int ti0 = (blk / bN) * 2, tj0 = (blk % bN) * 2;
+        const int ti0 = (blk / bN) * bh, tj0 = (blk % bN) * bw;
         mra_d4 acc00 = {0, 0, 0, 0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
         const int ia = ti0 * 16 + r16, ib = ia + 16;
         const int ja = tj0 * 16 + r16, jb = ja + 16;
-        const bool ha = ia < Mv, hb = (ti0 + 1 < tM) && ib < Mv;
-        const bool hja = ja < Nv, hjb = (tj0 + 1 < tN) && jb < Nv;
-        const bool tM2 = ti0 + 1 < tM, tN2 = tj0 + 1 < tN;  // second tile row / column present
+        const bool tM2 = bh == 2 && ti0 + 1 < tM, tN2 = bw == 2 && tj0 + 1 < tN;  // second tile row / column
+        const bool ha = ia < Mv, hb = tM2 && ib < Mv;
+        const bool hja = ja < Nv, hjb = tN2 && jb < Nv;
         // Operand loads of U k-steps are issued together before their MFMAs so
         // that one memory latency covers U steps (all control flow is
         // uniform across the wave).
@@ -111,8 +118,8 @@ __device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int64_t k = k0 + 4 * u + k4;
-                a0[u] = ha ? A[k * lda + ia] : 0.0;
-                a1[u] = hb ? A[k * lda + ib] : 0.0;
+                a0[u] = ha ? (TA ? A[ia * lda + k] : A[k * lda + ia]) : 0.0;
+                a1[u] = hb ? (TA ? A[ib * lda + k] : A[k * lda + ib]) : 0.0;
                 b0[u] = hja ? B[k * ldb + ja] : 0.0;
                 b1[u] = hjb ? B[k * ldb + jb] : 0.0;
             }
@@ -127,8 +134,8 @@ __device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int
         for (; k0 < Kv; k0 += 4) {
             const int k = k0 + k4;
             const bool kv = k < Kv;
-            double a0 = (kv && ha) ? A[(int64_t)k * lda + ia] : 0.0;
-            double a1 = (kv && hb) ? A[(int64_t)k * lda + ib] : 0.0;
+            double a0 = (kv && ha) ? (TA ? A[ia * lda + k] : A[(int64_t)k * lda + ia]) : 0.0;
+            double a1 = (kv && hb) ? (TA ? A[ib * lda + k] : A[(int64_t)k * lda + ib]) : 0.0;
             double b0 = (kv && hja) ? B[(int64_t)k * ldb + ja] : 0.0;
             double b1 = (kv && hjb) ? B[(int64_t)k * ldb + jb] : 0.0;
             acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc00, 0, 0, 0);
@@ -141,11 +148,20 @@ __device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int row = ti0 * 16 + k4 + 4 * v;
-            const bool r0 = row < Mv, r1 = (ti0 + 1 < tM) && (row + 16 < Mv);
-            if (r0 && hja) C[row * ldc + ja] += alpha * acc00[v];
-            if (r0 && hjb) C[row * ldc + jb] += alpha * acc01[v];
-            if (r1 && hja) C[(row + 16) * ldc + ja] += alpha * acc10[v];
-            if (r1 && hjb) C[(row + 16) * ldc + jb] += alpha * acc11[v];
+            const bool r0 = row < Mv, r1 = tM2 && (row + 16 < Mv);
+            double* c00 = C + (int64_t)row * ldc + ja;
+            double* c10 = C + (int64_t)(row + 16) * ldc + ja;
+            if (ACC) {
+                if (r0 && hja) c00[0] += alpha * acc00[v];
+                if (r0 && hjb) c00[16] += alpha * acc01[v];
+                if (r1 && hja) c10[0] += alpha * acc10[v];
+                if (r1 && hjb) c10[16] += alpha * acc11[v];
+            } else {
+                if (r0 && hja) c00[0] = alpha * acc00[v];
+                if (r0 && hjb) c00[16] = alpha * acc01[v];
+                if (r1 && hja) c10[0] = alpha * acc10[v];
+                if (r1 && hjb) c10[16] = alpha * acc11[v];
+            }
         }
     }
 }
@@ -227,27 +243,59 @@ __device__ __forceinline__ void wg_gemm_tn_ksplit(double* C, int ldc, int M, int
 
 // ------------------------------------------------------------ LDS solvers
 // Lower Cholesky in place of the n x n block (row stride lda); entries with
-// i < j are left alone.  A non-positive pivot sets *bad and is replaced by 1
-// so that the kernel still completes.
+// i < j are left alone.  Right-looking elimination on the unscaled lower
+// triangle, A[i][j] -= A[i][k] A[j][k] / A[k][k] (j > k), which needs one
+// barrier per column, then one pass L[i][k] = A[i][k] / sqrt(A[k][k]).  A
+// non-positive pivot sets *bad and is replaced by 1 so that the kernel
+// still completes.
 __device__ __forceinline__ void wg_chol(double* A, int lda, int n, int* bad) {
-    for (int k = 0; k < n; k++) {
-        if (threadIdx.x == 0) {
-            double d = A[k * lda + k];
-            if (!(d > 0.0)) {
-                *bad = 1;
-                d = 1.0;
-            }
-            A[k * lda + k] = sqrt(d);
-        }
-        __syncthreads();
-        const double dk = A[k * lda + k];
-        for (int i = k + 1 + threadIdx.x; i < n; i += MRA_WG) A[i * lda + k] /= dk;
-        __syncthreads();
+    __syncthreads();
+    for (int k = 0; k < n - 1; k++) {
+        double akk = A[k * lda + k];
+        if (!(akk > 0.0)) akk = 1.0;
+        const double inv = 1.0 / akk;
         const int m = n - k - 1;
         for (int idx = threadIdx.x; idx < m * m; idx += MRA_WG) {
             const int i = k + 1 + idx / m, j = k + 1 + idx % m;
-            if (j <= i) A[i * lda + j] -= A[i * lda + k] * A[j * lda + k];
+            if (j <= i) A[i * lda + j] -= A[i * lda + k] * A[j * lda + k] * inv;
         }
+        __syncthreads();
+    }
+    // scale: pivots are final now
+    for (int idx = threadIdx.x; idx < n * n; idx += MRA_WG) {
+        const int i = idx / n, k = idx % n;
+        if (k > i) continue;
+        double akk = A[k * lda + k];
+        if (!(akk > 0.0)) {
+            if (i == k) *bad = 1;
+            akk = 1.0;
+        }
+        if (i != k) A[i * lda + k] /= sqrt(akk);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += MRA_WG) {
+        const double akk = A[k * lda + k];
+        A[k * lda + k] = (akk > 0.0) ? sqrt(akk) : 1.0;
+    }
+    __syncthreads();
+}
+
+// In-place inverse of the lower-triangular n x n block (row stride lda),
+// column by column from the last: two barriers per column.  tmp: n doubles
+// of scratch.
+__device__ __forceinline__ void wg_trinv(double* A, int lda, int n, double* tmp) {
+    __syncthreads();
+    for (int j = n - 1; j >= 0; j--) {
+        const double dinv = 1.0 / A[j * lda + j];
+        // x_i = sum_{k=j+1..i} Ainv[i][k] A[k][j]  for i > j (old column j)
+        for (int i = j + 1 + threadIdx.x; i < n; i += MRA_WG) {
+            double s = 0.0;
+            for (int k = j + 1; k <= i; k++) s += A[i * lda + k] * A[k * lda + j];
+            tmp[i] = s;
+        }
+        __syncthreads();
+        for (int i = j + 1 + threadIdx.x; i < n; i += MRA_WG) A[i * lda + j] = -tmp[i] * dinv;
+        if (threadIdx.x == 0) A[j * lda + j] = dinv;
         __syncthreads();
     }
 }

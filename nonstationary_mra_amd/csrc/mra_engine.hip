@@ -73,6 +73,8 @@ struct mra_handle {
     // work lists
     uint32_t *dItemsX = nullptr, *dItemsF = nullptr;
     uint32_t nItemsX = 0, nItemsF = 0;
+    uint32_t* dItemsX32 = nullptr;  // segments of at most 32 observations (32-column tiles)
+    uint32_t nItemsX32 = 0;
     // finest regions by length bucket: n <= 16, 32, 64, 128, larger
     static constexpr int NBK = 5;
     uint32_t* dItemsFB[NBK] = {};
@@ -383,14 +385,23 @@ static int alloc_engine(mra_handle* h, const mra_tree_desc* t) {
         RC_TRY(dalloc(h, &h->dZo, h->nObs, A));
     }
     // work lists
-    std::vector<uint32_t> ix, iF;
+    std::vector<uint32_t> ix, ix32, iF;
     for (uint64_t f = 0; f < h->nF; f++)
         if (h->hObsOff[f + 1] > h->hObsOff[f]) iF.push_back((uint32_t)f);
-    for (size_t sgi = 0; sgi < h->hSN.size(); sgi++)
+    for (size_t sgi = 0; sgi < h->hSN.size(); sgi++) {
+        if (h->hSN[sgi] == 0) continue;
+        if (h->hSN[sgi] <= 32) {
+            ix32.push_back((uint32_t)sgi);
+            ix32.push_back(0);
+            continue;
+        }
         for (uint64_t c0 = 0; c0 < h->hSN[sgi]; c0 += 64) {
             ix.push_back((uint32_t)sgi);
             ix.push_back((uint32_t)c0);
         }
+    }
+    h->nItemsX32 = (uint32_t)(ix32.size() / 2);
+    RC_TRY(upload(h, &h->dItemsX32, ix32.data(), ix32.size()));
     h->nItemsX = (uint32_t)(ix.size() / 2);
     h->nItemsF = (uint32_t)iF.size();
     {
@@ -761,8 +772,11 @@ static void launch_eval(mra_handle* h, bool predicting) {
     }
     mark(MRA_T_FINEST_CROSS);
     if (h->nItemsX)
-        k_finest_cross<RP><<<h->nItemsX, 256, 0, st>>>(T, cc, K, O, h->dItemsX, h->dGoff, h->dGc, h->dLiT, h->dGf,
-                                                       h->dBf);
+        k_finest_cross<RP, 64><<<h->nItemsX, 256, 0, st>>>(T, cc, K, O, h->dItemsX, h->dGoff, h->dGc, h->dLiT,
+                                                           h->dGf, h->dBf);
+    if (h->nItemsX32)
+        k_finest_cross<RP, 32><<<h->nItemsX32, 256, 0, st>>>(T, cc, K, O, h->dItemsX32, h->dGoff, h->dGc, h->dLiT,
+                                                             h->dGf, h->dBf);
     mark(MRA_T_FINEST_LOCAL);
     if (h->nItemsF)
         launch_finest_local<RP>(h, T, cc, O);

@@ -13,8 +13,9 @@
 //        ancestor knot, stacked over j) x column c (region knot)
 //   Lt   prior Cholesky factor L_R stored column-major (Lt[k*RP+i] = L[i][k])
 //   LiT  L_R^{-1} column-major;   Lpt / LpiT: the posterior factor and inverse
-//   Gf, Bf  finest G_{F,j} / B_{F,j}: stored per segment (finest regions
-//        sharing a parent), element (m, o) at L*RP*seg_os + m*seg_n + o-seg_os
+//   Gf   finest G_{F,j}: stored per segment (finest regions sharing a
+//        parent), element (m, o) at L*RP*seg_os + m*seg_n + o-seg_os
+//   Bf   finest B_{F,j} observation-major: (m, o) at o*L*RP + m
 //   H    L_F^{-1} B_F^T, observation-major rows of L*RP
 //   Sig  L_F (row-major n_F x n_F) at sig_off[F]
 //   msgs w~ (l*RP) and A~ (l(l+1)/2 slices of RP*RP, slice order
@@ -231,7 +232,7 @@ __global__ __launch_bounds__(256) void k_prior_coarse(Tree T, CovCtx cc, Pts K, 
                                                       int* bad_flag) {
     constexpr int LD = RP + 1;
     __shared__ double Cm[RP * LD];
-    __shared__ double Wm[RP * LD];
+    __shared__ double tmp[RP];
     __shared__ int bad;
     const uint64_t R = start_l + blockIdx.x;
     const int r = T.r;
@@ -242,7 +243,6 @@ __global__ __launch_bounds__(256) void k_prior_coarse(Tree T, CovCtx cc, Pts K, 
         for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
             const int i = idx / RP, c = idx % RP;
             Cm[i * LD + c] = (i < r && c < r) ? pcov(cc, K, a * RP + i, K, R * RP + c) : 0.0;
-            Wm[i * LD + c] = 0.0;
         }
         __syncthreads();
         if (j > 0) {
@@ -250,10 +250,9 @@ __global__ __launch_bounds__(256) void k_prior_coarse(Tree T, CovCtx cc, Pts K, 
             wg_gemm_tn(Cm, LD, RP, RP, j * RP, -1.0, Ga, RP, RP, GR, RP, RP);
             __syncthreads();
         }
-        wg_gemm_tn(Wm, LD, RP, RP, RP, 1.0, LiT + a * RP * RP, RP, RP, Cm, LD, RP);
-        __syncthreads();
-        double* Gj = GR + (uint64_t)j * RP * RP;
-        for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) Gj[idx] = Wm[(idx / RP) * LD + idx % RP];
+        // G_{R,j} = L_a^{-1} B, stored straight to global memory
+        wg_gemm_tn<false, false>(GR + (uint64_t)j * RP * RP, RP, RP, RP, RP, 1.0, LiT + a * RP * RP, RP, RP, Cm, LD,
+                                 RP);
         __syncthreads();
     }
     for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
@@ -264,7 +263,6 @@ __global__ __launch_bounds__(256) void k_prior_coarse(Tree T, CovCtx cc, Pts K, 
         else
             v = (i == c) ? 1.0 : 0.0;
         Cm[i * LD + c] = v;
-        Wm[i * LD + c] = (i == c) ? 1.0 : 0.0;
     }
     __syncthreads();
     if (l > 0) {
@@ -272,11 +270,14 @@ __global__ __launch_bounds__(256) void k_prior_coarse(Tree T, CovCtx cc, Pts K, 
         __syncthreads();
     }
     wg_chol(Cm, LD, RP, &bad);
-    wg_trsm(Cm, LD, Wm, LD, RP, RP);
     for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
-        const int k = idx / RP, i = idx % RP;  // column-major: [k*RP + i] = M[i][k]
+        const int k = idx / RP, i = idx % RP;  // column-major: [k*RP + i] = L[i][k]
         Lt[R * RP * RP + idx] = (i >= k) ? Cm[i * LD + k] : 0.0;
-        LiT[R * RP * RP + idx] = (i >= k) ? Wm[i * LD + k] : 0.0;
+    }
+    wg_trinv(Cm, LD, RP, tmp);
+    for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
+        const int k = idx / RP, i = idx % RP;
+        LiT[R * RP * RP + idx] = (i >= k) ? Cm[i * LD + k] : 0.0;
     }
     if (threadIdx.x == 0 && bad) atomicOr(bad_flag, 1);
 }
@@ -286,26 +287,24 @@ __global__ __launch_bounds__(256) void k_prior_coarse(Tree T, CovCtx cc, Pts K, 
 // which share every ancestor), tiles of 64 columns:
 //   B_{F,j} = C(Q_{a_j}, S) - Gcat_{a_j}^T Gcat_F[0:j],  G_{F,j} = L_{a_j}^{-1} B_{F,j}
 // (…finest-level-in-prior.cpp:116-163)
-template <int RP>
+template <int RP, int NT>
 __global__ __launch_bounds__(256) void k_finest_cross(Tree T, CovCtx cc, Pts K, Pts O, const uint32_t* items,
                                                       const uint64_t* goff, const double* Gc, const double* LiT,
                                                       double* Gf, double* Bf) {
-    constexpr int NT = 64, LD = NT + 1;
+    constexpr int LD = NT + 1;
     __shared__ double Cm[RP * LD];
-    __shared__ double Wm[RP * LD];
     const uint32_t sg = items[2 * blockIdx.x], c0 = items[2 * blockIdx.x + 1];
     const uint64_t ob = T.s_os[sg];
     const int nF = (int)T.s_n[sg];
     const int n = min(NT, nF - (int)c0);
     const int r = T.r, L = T.L;
     double* GF = Gf + (uint64_t)T.LRP * ob;
-    double* BF = Bf + (uint64_t)T.LRP * ob;
+    double* BO = Bf + (uint64_t)T.LRP * ob;
     for (int j = 0; j < L; j++) {
         const uint64_t a = ancestor_up(T.s_parent[sg], L - 1, j, T.J);
         for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
             const int i = idx / NT, c = idx % NT;
             Cm[i * LD + c] = (i < r && c < n) ? pcov(cc, K, a * RP + i, O, ob + c0 + c) : 0.0;
-            Wm[i * LD + c] = 0.0;
         }
         __syncthreads();
         if (j > 0) {
@@ -313,16 +312,14 @@ __global__ __launch_bounds__(256) void k_finest_cross(Tree T, CovCtx cc, Pts K, 
             wg_gemm_tn(Cm, LD, RP, NT, j * RP, -1.0, Ga, RP, RP, GF + c0, nF, n);
             __syncthreads();
         }
+        // B is stored observation-major: row o holds B_{F,0..L-1}(:, o)
         for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
-            const int i = idx / NT, c = idx % NT;
-            if (c < n) BF[(uint64_t)(j * RP + i) * nF + c0 + c] = Cm[i * LD + c];
+            const int i = idx % RP, c = idx / RP;
+            if (c < n) BO[(uint64_t)(c0 + c) * T.LRP + j * RP + i] = Cm[i * LD + c];
         }
-        wg_gemm_tn(Wm, LD, RP, NT, RP, 1.0, LiT + a * RP * RP, RP, RP, Cm, LD, NT);
-        __syncthreads();
-        for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
-            const int i = idx / NT, c = idx % NT;
-            if (c < n) GF[(uint64_t)(j * RP + i) * nF + c0 + c] = Wm[i * LD + c];
-        }
+        // G_{F,j} = L_a^{-1} B, stored straight to global memory
+        wg_gemm_tn<false, false>(GF + (uint64_t)(j * RP) * nF + c0, nF, RP, NT, RP, 1.0, LiT + a * RP * RP, RP, RP,
+                                 Cm, LD, n);
         __syncthreads();
     }
 }
@@ -353,17 +350,18 @@ __global__ __launch_bounds__(256) void k_finest_local(Tree T, CovCtx cc, Pts O, 
                                                       double* H, double* u, double* dterm, int* bad_flag) {
     __shared__ FinestLds<NB> sh;
     __shared__ double red[MRA_WG];
+    __shared__ double tmp[NB > 0 ? NB : 1];
     __shared__ int bad;
     const uint32_t F = items[blockIdx.x];
     const uint64_t ob = T.obs_off[F];
     const int n = (int)(T.obs_off[F + 1] - ob);
     const int LRP = T.LRP;
-    double* Sg = NB > 0 ? sh.S : Sig + T.sig_off[F];
+    double* LF = Sig + T.sig_off[F];  // L_F (row-major) kept for prediction and sampling
+    double* Sg = NB > 0 ? sh.S : LF;
     const int ld = NB > 0 ? FinestLds<NB>::LD : n;
     const uint64_t so = T.f_seg_os[F];
     const int sn = (int)T.f_seg_n[F];
     const double* GF = Gf + (uint64_t)LRP * so + (ob - so);
-    double* BF = Bf + (uint64_t)LRP * so + (ob - so);
     if (threadIdx.x == 0) bad = 0;
     for (int idx = threadIdx.x; idx < n * n; idx += MRA_WG) {
         const int i = idx / n, c = idx % n;
@@ -382,56 +380,55 @@ __global__ __launch_bounds__(256) void k_finest_local(Tree T, CovCtx cc, Pts O, 
     }
     __syncthreads();
     wg_chol(Sg, ld, n, &bad);
-    // u = L_F^{-1} y
-    double* uF = u + ob;
-    for (int i = threadIdx.x; i < n; i += MRA_WG) uF[i] = yv[ob + i];
-    __syncthreads();
-    wg_trsm(Sg, ld, uF, 1, n, 1);
+    // log det part, then keep L_F
     double part = 0.0;
-    for (int i = threadIdx.x; i < n; i += MRA_WG) part += uF[i] * uF[i] + 2.0 * log(Sg[i * ld + i]);
-    const double dF = wg_sum(part, red);
-    // H rows: solve L_F h = b_m for every row m of B_F
-    if (NB > 0 && NB <= 32) {
-        constexpr int NR = NB > 0 ? NB : 1;
-        for (int m = threadIdx.x; m < LRP; m += MRA_WG) {
-            const double* b = BF + (uint64_t)m * sn;
-            double h[NR];
-#pragma unroll
-            for (int i = 0; i < NR; i++) h[i] = (i < n) ? b[i] : 0.0;
-#pragma unroll
-            for (int i = 0; i < NR; i++) {
-                if (i < n) {
-                    double acc = h[i];
-#pragma unroll
-                    for (int k = 0; k < i; k++) acc -= Sg[i * ld + k] * h[k];
-                    h[i] = acc / Sg[i * ld + i];
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < NR; i++)
-                if (i < n) H[(ob + i) * LRP + m] = h[i];
+    for (int i = threadIdx.x; i < n; i += MRA_WG) part += 2.0 * log(Sg[i * ld + i]);
+    if (NB > 0)
+        for (int idx = threadIdx.x; idx < n * n; idx += MRA_WG) LF[idx] = Sg[(idx / n) * ld + idx % n];
+    __syncthreads();
+    if (NB > 0) {
+        // the upper triangle (still Sigma_F) must read as zero in the H product
+        for (int idx = threadIdx.x; idx < n * n; idx += MRA_WG)
+            if (idx % n > idx / n) Sg[(idx / n) * ld + idx % n] = 0.0;
+        wg_trinv(Sg, ld, n, tmp);
+        // u = L_F^{-1} y
+        double* uF = u + ob;
+        for (int i = threadIdx.x; i < n; i += MRA_WG) {
+            double acc = 0.0;
+            for (int k = 0; k <= i; k++) acc += Sg[i * ld + k] * yv[ob + k];
+            uF[i] = acc;
+            part += acc * acc;
         }
+        const double dF = wg_sum(part, red);
+        // H = L_F^{-1} B (observation-major rows), 64 columns at a time
+        const double* BO = Bf + ob * (uint64_t)LRP;
+        for (int c0 = 0; c0 < LRP; c0 += 64) {
+            const int cv = min(64, LRP - c0);
+            wg_gemm_tn<true, false>(H + ob * (uint64_t)LRP + c0, LRP, (n + 15) & ~15, 64, n, 1.0, Sg, ld, n,
+                                    BO + c0, LRP, cv);
+        }
+        if (threadIdx.x == 0) dterm[T.nC + F] = dF;
     } else {
+        // large regions: forward substitution in global memory
+        double* uF = u + ob;
+        for (int i = threadIdx.x; i < n; i += MRA_WG) uF[i] = yv[ob + i];
+        __syncthreads();
+        wg_trsm(Sg, ld, uF, 1, n, 1);
+        for (int i = threadIdx.x; i < n; i += MRA_WG) part += uF[i] * uF[i];
+        const double dF = wg_sum(part, red);
+        double* BO = Bf + ob * (uint64_t)LRP;
         for (int m = threadIdx.x; m < LRP; m += MRA_WG) {
-            double* b = BF + (uint64_t)m * sn;
             for (int i = 0; i < n; i++) {
-                double acc = b[i];
-                for (int k = 0; k < i; k++) acc -= Sg[i * ld + k] * b[k];
+                double acc = BO[(uint64_t)i * LRP + m];
+                for (int k = 0; k < i; k++) acc -= Sg[i * ld + k] * BO[(uint64_t)k * LRP + m];
                 acc /= Sg[i * ld + i];
-                b[i] = acc;
+                BO[(uint64_t)i * LRP + m] = acc;
                 H[(ob + i) * LRP + m] = acc;
             }
         }
+        if (threadIdx.x == 0) dterm[T.nC + F] = dF;
     }
-    // keep L_F for prediction and sampling
-    if (NB > 0) {
-        double* dst = Sig + T.sig_off[F];
-        for (int idx = threadIdx.x; idx < n * n; idx += MRA_WG) dst[idx] = Sg[(idx / n) * ld + idx % n];
-    }
-    if (threadIdx.x == 0) {
-        dterm[T.nC + F] = dF;
-        if (bad) atomicOr(bad_flag, 1);
-    }
+    if (threadIdx.x == 0 && bad) atomicOr(bad_flag, 1);
 }
 
 // -------------------------------------------------------------- posterior
