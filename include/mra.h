@@ -208,6 +208,10 @@ int mra_partition_program(const double* lon, const double* lat, const double* va
 int mra_selftest(int K, int Mv, int Nv, const double* A, const double* B, const double* C0,
                  const double* S, const double* Bt, double* Cout, double* Lout, double* Xout, int* bad);
 
+/* FP32 variant of the GEMM self test: Cout = C0 - A^T B with the FP32
+ * matrix-core GEMM and FP32 accumulation (no reference counterpart). */
+int mra_selftest_f32(int K, int Mv, int Nv, const float* A, const float* B, const float* C0, float* Cout);
+
 /* Library version string. */
 const char* mra_version(void);
 

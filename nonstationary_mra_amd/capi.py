@@ -26,7 +26,7 @@ EXPORTED = ["mra_create", "mra_destroy", "mra_last_error", "mra_set_covariance",
             "mra_set_precision", "mra_likelihood", "mra_predict", "mra_likelihood_levels", "mra_sample_prior",
             "mra_set_observations", "mra_use_graph", "mra_last_timings", "mra_cross_covariance",
             "mra_variance_covariance", "mra_cov_tile_device", "mra_partition_program_sizes",
-            "mra_partition_program", "mra_version", "mra_selftest", "mra_set_option", "mra_get_option"]
+            "mra_partition_program", "mra_version", "mra_selftest", "mra_set_option", "mra_get_option", "mra_selftest_f32"]
 OPT_FACTORED_POSTERIOR = 1
 
 
@@ -92,6 +92,8 @@ def lib():
                                                dp, dp, u64p]
         _lib.mra_set_option.argtypes = [C.c_void_p, C.c_int, C.c_int]
         _lib.mra_get_option.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+        fp = C.POINTER(C.c_float)
+        _lib.mra_selftest_f32.argtypes = [C.c_int, C.c_int, C.c_int, fp, fp, fp, fp]
         _lib.mra_selftest.argtypes = [C.c_int, C.c_int, C.c_int] + [dp] * 8 + [C.POINTER(C.c_int)]
     return _lib
 
@@ -104,6 +106,18 @@ def selftest(K, Mv, Nv, A, B, C0, S, Bt):
     check(lib().mra_selftest(K, Mv, Nv, keep.d(A), keep.d(B), keep.d(C0), keep.d(S), keep.d(Bt), _d(Co), _d(Lo),
                              _d(Xo), C.byref(bad)))
     return Co.reshape(64, 64), Lo.reshape(64, 64), Xo.reshape(64, 64), bad.value
+
+
+def selftest_f32(K, Mv, Nv, A, B, C0):
+    """FP32 GEMM self test (see mra_selftest_f32 in include/mra.h)."""
+    fp = C.POINTER(C.c_float)
+    a = np.ascontiguousarray(A, dtype=np.float32)
+    b = np.ascontiguousarray(B, dtype=np.float32)
+    c0 = np.ascontiguousarray(C0, dtype=np.float32)
+    out = np.zeros(64 * 64, dtype=np.float32)
+    check(lib().mra_selftest_f32(K, Mv, Nv, a.ctypes.data_as(fp), b.ctypes.data_as(fp), c0.ctypes.data_as(fp),
+                                 out.ctypes.data_as(fp)))
+    return out.reshape(64, 64)
 
 
 def check(rc, h=None):

@@ -88,9 +88,9 @@ __device__ __forceinline__ float pcovf(const CovCtx& c, const Pts& P, uint64_t i
 // must be made visible by the caller (__syncthreads before and after).
 // TA: A is stored transposed, A(k,i) = A[i*lda+k] (an LDS matrix read by
 // columns).  ACC = false stores C = alpha*A^T B instead of accumulating.
-template <bool TA = false, bool ACC = true>
-__device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int Kv, double alpha, const double* A,
-                                           int64_t lda, int Mv, const double* B, int64_t ldb, int Nv) {
+template <bool TA = false, bool ACC = true, typename TIA = double, typename TIB = double>
+__device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int Kv, double alpha, const TIA* A,
+                                           int64_t lda, int Mv, const TIB* B, int64_t ldb, int Nv) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r16 = lane & 15, k4 = lane >> 4;
     const int tM = M >> 4, tN = N >> 4;
@@ -118,10 +118,10 @@ __device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int64_t k = k0 + 4 * u + k4;
-                a0[u] = ha ? (TA ? A[ia * lda + k] : A[k * lda + ia]) : 0.0;
-                a1[u] = hb ? (TA ? A[ib * lda + k] : A[k * lda + ib]) : 0.0;
-                b0[u] = hja ? B[k * ldb + ja] : 0.0;
-                b1[u] = hjb ? B[k * ldb + jb] : 0.0;
+                a0[u] = ha ? (double)(TA ? A[ia * lda + k] : A[k * lda + ia]) : 0.0;
+                a1[u] = hb ? (double)(TA ? A[ib * lda + k] : A[k * lda + ib]) : 0.0;
+                b0[u] = hja ? (double)B[k * ldb + ja] : 0.0;
+                b1[u] = hjb ? (double)B[k * ldb + jb] : 0.0;
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
@@ -134,10 +134,10 @@ __device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int
         for (; k0 < Kv; k0 += 4) {
             const int k = k0 + k4;
             const bool kv = k < Kv;
-            double a0 = (kv && ha) ? (TA ? A[ia * lda + k] : A[(int64_t)k * lda + ia]) : 0.0;
-            double a1 = (kv && hb) ? (TA ? A[ib * lda + k] : A[(int64_t)k * lda + ib]) : 0.0;
-            double b0 = (kv && hja) ? B[(int64_t)k * ldb + ja] : 0.0;
-            double b1 = (kv && hjb) ? B[(int64_t)k * ldb + jb] : 0.0;
+            double a0 = (kv && ha) ? (double)(TA ? A[ia * lda + k] : A[(int64_t)k * lda + ia]) : 0.0;
+            double a1 = (kv && hb) ? (double)(TA ? A[ib * lda + k] : A[(int64_t)k * lda + ib]) : 0.0;
+            double b0 = (kv && hja) ? (double)B[(int64_t)k * ldb + ja] : 0.0;
+            double b1 = (kv && hjb) ? (double)B[(int64_t)k * ldb + jb] : 0.0;
             acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc00, 0, 0, 0);
             if (tN2) acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc01, 0, 0, 0);
             if (tM2) acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
@@ -166,12 +166,89 @@ __device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int
     }
 }
 
+// FP32 variant: C (float) += alpha * A^T B with v_mfma_f32_16x16x4_f32 and
+// FP32 accumulation; operands are read as TIA / TIB (double or float) and
+// rounded to float.  Same tiling and validity rules as wg_gemm_tn; the f32 D
+// fragment has element v of lane t at row 4*(t/16) + v (unlike the f64
+// instruction; confirmed on gfx950 by tests/test_gpu_blocks.py).
+template <bool ACC = true, typename TIA = float, typename TIB = float, typename TC = float>
+__device__ __forceinline__ void wg_gemm_tn_f32(TC* C, int ldc, int M, int N, int Kv, float alpha, const TIA* A,
+                                               int64_t lda, int Mv, const TIB* B, int64_t ldb, int Nv) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, k4 = lane >> 4;
+    const int tM = M >> 4, tN = N >> 4;
+    const int nt = tM * tN;
+    const int bh = (nt >= 16 || (nt >= 8 && tM >= 2)) ? 2 : 1;
+    const int bw = (nt >= 16) ? 2 : 1;
+    const int bM = (tM + bh - 1) / bh, bN = (tN + bw - 1) / bw;
+    for (int blk = wave; blk < bM * bN; blk += 4) {
+        const int ti0 = (blk / bN) * bh, tj0 = (blk % bN) * bw;
+        mra_f4 acc00 = {0, 0, 0, 0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
+        const int ia = ti0 * 16 + r16, ib = ia + 16;
+        const int ja = tj0 * 16 + r16, jb = ja + 16;
+        const bool tM2 = bh == 2 && ti0 + 1 < tM, tN2 = bw == 2 && tj0 + 1 < tN;
+        const bool ha = ia < Mv, hb = tM2 && ib < Mv;
+        const bool hja = ja < Nv, hjb = tN2 && jb < Nv;
+        constexpr int U = 8;
+        int k0 = 0;
+        for (; k0 + 4 * U <= Kv; k0 += 4 * U) {
+            float a0[U], a1[U], b0[U], b1[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int64_t k = k0 + 4 * u + k4;
+                a0[u] = ha ? (float)A[k * lda + ia] : 0.0f;
+                a1[u] = hb ? (float)A[k * lda + ib] : 0.0f;
+                b0[u] = hja ? (float)B[k * ldb + ja] : 0.0f;
+                b1[u] = hjb ? (float)B[k * ldb + jb] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                acc00 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[u], b0[u], acc00, 0, 0, 0);
+                if (tN2) acc01 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[u], b1[u], acc01, 0, 0, 0);
+                if (tM2) acc10 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[u], b0[u], acc10, 0, 0, 0);
+                if (tM2 && tN2) acc11 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[u], b1[u], acc11, 0, 0, 0);
+            }
+        }
+        for (; k0 < Kv; k0 += 4) {
+            const int k = k0 + k4;
+            const bool kv = k < Kv;
+            const float a0 = (kv && ha) ? (float)A[(int64_t)k * lda + ia] : 0.0f;
+            const float a1 = (kv && hb) ? (float)A[(int64_t)k * lda + ib] : 0.0f;
+            const float b0 = (kv && hja) ? (float)B[(int64_t)k * ldb + ja] : 0.0f;
+            const float b1 = (kv && hjb) ? (float)B[(int64_t)k * ldb + jb] : 0.0f;
+            acc00 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc00, 0, 0, 0);
+            if (tN2) acc01 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc01, 0, 0, 0);
+            if (tM2) acc10 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc10, 0, 0, 0);
+            if (tM2 && tN2) acc11 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc11, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = ti0 * 16 + 4 * k4 + v;
+            const bool r0 = row < Mv, r1 = tM2 && (row + 16 < Mv);
+            TC* c00 = C + (int64_t)row * ldc + ja;
+            TC* c10 = C + (int64_t)(row + 16) * ldc + ja;
+            if (ACC) {
+                if (r0 && hja) c00[0] += alpha * acc00[v];
+                if (r0 && hjb) c00[16] += alpha * acc01[v];
+                if (r1 && hja) c10[0] += alpha * acc10[v];
+                if (r1 && hjb) c10[16] += alpha * acc11[v];
+            } else {
+                if (r0 && hja) c00[0] = alpha * acc00[v];
+                if (r0 && hjb) c00[16] = alpha * acc01[v];
+                if (r1 && hja) c10[0] = alpha * acc10[v];
+                if (r1 && hjb) c10[16] = alpha * acc11[v];
+            }
+        }
+    }
+}
+
 // Same product for an output of at most 32 x 32 (one 2x2 tile block), with
 // K split over the 4 waves; the partial tiles are summed in wave order
 // through red (LDS: 3*256 doubles for a 16 x 16 output, else 3*4*256), so
 // the result is reproducible.
+template <typename TI = double>
 __device__ __forceinline__ void wg_gemm_tn_ksplit(double* C, int ldc, int M, int N, int Kv, double alpha,
-                                                  const double* A, int64_t lda, int Mv, const double* B, int64_t ldb,
+                                                  const TI* A, int64_t lda, int Mv, const TI* B, int64_t ldb,
                                                   int Nv, double* red) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r16 = lane & 15, k4 = lane >> 4;
@@ -192,10 +269,10 @@ __device__ __forceinline__ void wg_gemm_tn_ksplit(double* C, int ldc, int M, int
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int64_t k = k0 + 4 * u + k4;
-            a0[u] = ha ? A[k * lda + ia] : 0.0;
-            a1[u] = hb ? A[k * lda + ib] : 0.0;
-            b0[u] = hja ? B[k * ldb + ja] : 0.0;
-            b1[u] = hjb ? B[k * ldb + jb] : 0.0;
+            a0[u] = ha ? (double)A[k * lda + ia] : 0.0;
+            a1[u] = hb ? (double)A[k * lda + ib] : 0.0;
+            b0[u] = hja ? (double)B[k * ldb + ja] : 0.0;
+            b1[u] = hjb ? (double)B[k * ldb + jb] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
@@ -208,10 +285,10 @@ __device__ __forceinline__ void wg_gemm_tn_ksplit(double* C, int ldc, int M, int
     for (; k0 < ke; k0 += 4) {
         const int k = k0 + k4;
         const bool kv = k < ke;
-        const double a0 = (kv && ha) ? A[(int64_t)k * lda + ia] : 0.0;
-        const double a1 = (kv && hb) ? A[(int64_t)k * lda + ib] : 0.0;
-        const double b0 = (kv && hja) ? B[(int64_t)k * ldb + ja] : 0.0;
-        const double b1 = (kv && hjb) ? B[(int64_t)k * ldb + jb] : 0.0;
+        const double a0 = (kv && ha) ? (double)A[(int64_t)k * lda + ia] : 0.0;
+        const double a1 = (kv && hb) ? (double)A[(int64_t)k * lda + ib] : 0.0;
+        const double b0 = (kv && hja) ? (double)B[(int64_t)k * ldb + ja] : 0.0;
+        const double b1 = (kv && hjb) ? (double)B[(int64_t)k * ldb + jb] : 0.0;
         acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0], 0, 0, 0);
         if (tN2) acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[1], 0, 0, 0);
         if (tM2) acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[2], 0, 0, 0);

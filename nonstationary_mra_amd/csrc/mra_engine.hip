@@ -717,25 +717,38 @@ static CovCtx make_cc(mra_handle* h) {
 
 static Pts pts(double* const* c, double* const* s) { return Pts{c[0], c[1], c[2], s[0], s[1], s[2]}; }
 
-template <int RP>
+template <int RP, typename TG>
 static void launch_finest_local(mra_handle* h, const Tree& T, const CovCtx& cc, const Pts& O) {
     hipStream_t st = h->stream;
+    TG* Gf = (TG*)h->dGf;
+    TG* Bf = (TG*)h->dBf;
     // larger buckets first (longest work groups start early)
     if (h->nItemsFB[4])
-        k_finest_local<RP, 0><<<h->nItemsFB[4], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[4], h->dGf, h->dBf,
-                                                              h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
+        k_finest_local<RP, 0, TG><<<h->nItemsFB[4], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[4], Gf, Bf, h->dSig,
+                                                                  h->dH, h->dU, h->dDterm, h->dBad);
     if (h->nItemsFB[3])
-        k_finest_local<RP, 128><<<h->nItemsFB[3], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[3], h->dGf, h->dBf,
-                                                                h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
+        k_finest_local<RP, 128, TG><<<h->nItemsFB[3], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[3], Gf, Bf, h->dSig,
+                                                                    h->dH, h->dU, h->dDterm, h->dBad);
     if (h->nItemsFB[2])
-        k_finest_local<RP, 64><<<h->nItemsFB[2], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[2], h->dGf, h->dBf,
-                                                               h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
+        k_finest_local<RP, 64, TG><<<h->nItemsFB[2], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[2], Gf, Bf, h->dSig,
+                                                                   h->dH, h->dU, h->dDterm, h->dBad);
     if (h->nItemsFB[1])
-        k_finest_local<RP, 32><<<h->nItemsFB[1], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[1], h->dGf, h->dBf,
-                                                               h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
+        k_finest_local<RP, 32, TG><<<h->nItemsFB[1], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[1], Gf, Bf, h->dSig,
+                                                                   h->dH, h->dU, h->dDterm, h->dBad);
     if (h->nItemsFB[0])
-        k_finest_local<RP, 16><<<h->nItemsFB[0], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[0], h->dGf, h->dBf,
-                                                               h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
+        k_finest_local<RP, 16, TG><<<h->nItemsFB[0], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[0], Gf, Bf, h->dSig,
+                                                                   h->dH, h->dU, h->dDterm, h->dBad);
+}
+
+template <int RP, typename TG>
+static void launch_finest_cross(mra_handle* h, const Tree& T, const CovCtx& cc, const Pts& K, const Pts& O) {
+    hipStream_t st = h->stream;
+    if (h->nItemsX)
+        k_finest_cross<RP, 64, TG><<<h->nItemsX, 256, 0, st>>>(T, cc, K, O, h->dItemsX, h->dGoff, h->dGc, h->dLiT,
+                                                               (TG*)h->dGf, (TG*)h->dBf);
+    if (h->nItemsX32)
+        k_finest_cross<RP, 32, TG><<<h->nItemsX32, 256, 0, st>>>(T, cc, K, O, h->dItemsX32, h->dGoff, h->dGc,
+                                                                 h->dLiT, (TG*)h->dGf, (TG*)h->dBf);
 }
 
 template <int RP>
@@ -771,15 +784,19 @@ static void launch_eval(mra_handle* h, bool predicting) {
                                                          h->dLiT, h->dBad);
     }
     mark(MRA_T_FINEST_CROSS);
-    if (h->nItemsX)
-        k_finest_cross<RP, 64><<<h->nItemsX, 256, 0, st>>>(T, cc, K, O, h->dItemsX, h->dGoff, h->dGc, h->dLiT,
-                                                           h->dGf, h->dBf);
-    if (h->nItemsX32)
-        k_finest_cross<RP, 32><<<h->nItemsX32, 256, 0, st>>>(T, cc, K, O, h->dItemsX32, h->dGoff, h->dGc, h->dLiT,
-                                                             h->dGf, h->dBf);
+    const bool mixed = h->precision == MRA_PRECISION_MIXED;
+    if (mixed)
+        launch_finest_cross<RP, float>(h, T, cc, K, O);
+    else
+        launch_finest_cross<RP, double>(h, T, cc, K, O);
     mark(MRA_T_FINEST_LOCAL);
     if (h->nItemsF)
-        launch_finest_local<RP>(h, T, cc, O);
+    {
+        if (mixed)
+            launch_finest_local<RP, float>(h, T, cc, O);
+        else
+            launch_finest_local<RP, double>(h, T, cc, O);
+    }
     mark(MRA_T_PRED_PRE);
     PredArgs pa{};
     if (predicting) {
@@ -803,7 +820,10 @@ static void launch_eval(mra_handle* h, bool predicting) {
             if (e == b) continue;
             pa.items = h->dItemsPred + 3 * (uint64_t)b;
             pa.gl_off = h->dGlOff + b;
-            k_pred_pre<RP><<<e - b, 256, 0, st>>>(T, cc, K, O, Pp, pa);
+            if (mixed)
+                k_pred_pre<RP, float><<<e - b, 256, 0, st>>>(T, cc, K, O, Pp, pa);
+            else
+                k_pred_pre<RP, double><<<e - b, 256, 0, st>>>(T, cc, K, O, Pp, pa);
         }
     }
     // posterior, finest-but-one level up to the root
@@ -993,7 +1013,14 @@ extern "C" int mra_sample_prior(mra_handle* h, uint64_t seed, double* y, int rep
     k_normals<<<nblk(nK), 256, 0, h->stream>>>(seed, 0, nK, xc);
     if (h->nObs) k_normals<<<nblk(h->nObs), 256, 0, h->stream>>>(seed, nK, h->nObs, xo);
     if (h->nItemsF)
-        k_sample_finest<<<h->nItemsF, 256, 0, h->stream>>>(make_tree(h), h->dItemsF, h->dGf, h->dSig, xc, xo, yn);
+    {
+        if (h->precision == MRA_PRECISION_MIXED)
+            k_sample_finest<float><<<h->nItemsF, 256, 0, h->stream>>>(make_tree(h), h->dItemsF, (const float*)h->dGf,
+                                                                      h->dSig, xc, xo, yn);
+        else
+            k_sample_finest<double><<<h->nItemsF, 256, 0, h->stream>>>(make_tree(h), h->dItemsF, h->dGf, h->dSig, xc,
+                                                                       xo, yn);
+    }
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     std::vector<double> host(h->nObs);
@@ -1157,6 +1184,28 @@ extern "C" int mra_selftest(int K, int Mv, int Nv, const double* A, const double
     HIP_TRY(hipMemcpy(Lout, dL, sizeof(double) * m, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(Xout, dX, sizeof(double) * m, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(bad, dbad, sizeof(int), hipMemcpyDeviceToHost));
+    for (void* p : l) (void)hipFree(p);
+    return MRA_OK;
+}
+
+extern "C" int mra_selftest_f32(int K, int Mv, int Nv, const float* A, const float* B, const float* C0,
+                                float* Cout) {
+    mra_handle tmp;
+    mra_handle* h = &tmp;
+    if (K < 1 || Mv < 1 || Mv > 64 || Nv < 1 || Nv > 64 || !A || !B || !C0 || !Cout) return MRA_E_ARG;
+    std::vector<void*> l;
+    float *dA, *dB, *dC0, *dC;
+    RC_TRY(dalloc(h, &dA, (size_t)K * 64, l));
+    RC_TRY(dalloc(h, &dB, (size_t)K * 64, l));
+    RC_TRY(dalloc(h, &dC0, 64 * 64, l));
+    RC_TRY(dalloc(h, &dC, 64 * 64, l));
+    HIP_TRY(hipMemcpy(dA, A, sizeof(float) * K * 64, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dB, B, sizeof(float) * K * 64, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dC0, C0, sizeof(float) * 64 * 64, hipMemcpyHostToDevice));
+    k_selftest_f32<<<1, 256>>>(dA, dB, K, Mv, Nv, dC0, dC);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(Cout, dC, sizeof(float) * 64 * 64, hipMemcpyDeviceToHost));
     for (void* p : l) (void)hipFree(p);
     return MRA_OK;
 }

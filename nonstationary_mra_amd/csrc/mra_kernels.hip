@@ -287,39 +287,56 @@ __global__ __launch_bounds__(256) void k_prior_coarse(Tree T, CovCtx cc, Pts K, 
 // which share every ancestor), tiles of 64 columns:
 //   B_{F,j} = C(Q_{a_j}, S) - Gcat_{a_j}^T Gcat_F[0:j],  G_{F,j} = L_{a_j}^{-1} B_{F,j}
 // (…finest-level-in-prior.cpp:116-163)
-template <int RP, int NT>
-__global__ __launch_bounds__(256) void k_finest_cross(Tree T, CovCtx cc, Pts K, Pts O, const uint32_t* items,
+// T = double: FP64 throughout.  T = float (mixed precision): the covariance
+// tile, both products and the stored G, B are FP32 with FP32 accumulation.
+template <int RP, int NT, typename T>
+__global__ __launch_bounds__(256) void k_finest_cross(Tree T_, CovCtx cc, Pts K, Pts O, const uint32_t* items,
                                                       const uint64_t* goff, const double* Gc, const double* LiT,
-                                                      double* Gf, double* Bf) {
+                                                      T* Gf, T* Bf) {
+    const Tree& T0 = T_;
     constexpr int LD = NT + 1;
-    __shared__ double Cm[RP * LD];
+    __shared__ T Cm[RP * LD];
     const uint32_t sg = items[2 * blockIdx.x], c0 = items[2 * blockIdx.x + 1];
-    const uint64_t ob = T.s_os[sg];
-    const int nF = (int)T.s_n[sg];
+    const uint64_t ob = T0.s_os[sg];
+    const int nF = (int)T0.s_n[sg];
     const int n = min(NT, nF - (int)c0);
-    const int r = T.r, L = T.L;
-    double* GF = Gf + (uint64_t)T.LRP * ob;
-    double* BO = Bf + (uint64_t)T.LRP * ob;
+    const int r = T0.r, L = T0.L;
+    T* GF = Gf + (uint64_t)T0.LRP * ob;
+    T* BO = Bf + (uint64_t)T0.LRP * ob;
     for (int j = 0; j < L; j++) {
-        const uint64_t a = ancestor_up(T.s_parent[sg], L - 1, j, T.J);
+        const uint64_t a = ancestor_up(T0.s_parent[sg], L - 1, j, T0.J);
         for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
             const int i = idx / NT, c = idx % NT;
-            Cm[i * LD + c] = (i < r && c < n) ? pcov(cc, K, a * RP + i, O, ob + c0 + c) : 0.0;
+            T v = 0;
+            if (i < r && c < n) {
+                if constexpr (sizeof(T) == 4)
+                    v = pcovf(cc, K, a * RP + i, O, ob + c0 + c);
+                else
+                    v = pcov(cc, K, a * RP + i, O, ob + c0 + c);
+            }
+            Cm[i * LD + c] = v;
         }
         __syncthreads();
         if (j > 0) {
-            const double* Ga = Gc + goff[j] + (a - lvl_start(j, T.J)) * (uint64_t)j * RP * RP;
-            wg_gemm_tn(Cm, LD, RP, NT, j * RP, -1.0, Ga, RP, RP, GF + c0, nF, n);
+            const double* Ga = Gc + goff[j] + (a - lvl_start(j, T0.J)) * (uint64_t)j * RP * RP;
+            if constexpr (sizeof(T) == 4)
+                wg_gemm_tn_f32(Cm, LD, RP, NT, j * RP, -1.0f, Ga, RP, RP, GF + c0, nF, n);
+            else
+                wg_gemm_tn(Cm, LD, RP, NT, j * RP, -1.0, Ga, RP, RP, GF + c0, nF, n);
             __syncthreads();
         }
         // B is stored observation-major: row o holds B_{F,0..L-1}(:, o)
         for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
             const int i = idx % RP, c = idx / RP;
-            if (c < n) BO[(uint64_t)(c0 + c) * T.LRP + j * RP + i] = Cm[i * LD + c];
+            if (c < n) BO[(uint64_t)(c0 + c) * T0.LRP + j * RP + i] = Cm[i * LD + c];
         }
         // G_{F,j} = L_a^{-1} B, stored straight to global memory
-        wg_gemm_tn<false, false>(GF + (uint64_t)(j * RP) * nF + c0, nF, RP, NT, RP, 1.0, LiT + a * RP * RP, RP, RP,
-                                 Cm, LD, n);
+        if constexpr (sizeof(T) == 4)
+            wg_gemm_tn_f32<false>(GF + (uint64_t)(j * RP) * nF + c0, nF, RP, NT, RP, 1.0f, LiT + a * RP * RP, RP, RP,
+                                  (const float*)Cm, LD, n);
+        else
+            wg_gemm_tn<false, false>(GF + (uint64_t)(j * RP) * nF + c0, nF, RP, NT, RP, 1.0, LiT + a * RP * RP, RP,
+                                     RP, (const double*)Cm, LD, n);
         __syncthreads();
     }
 }
@@ -344,9 +361,9 @@ struct FinestLds<0> {
     double kred[1];
 };
 
-template <int RP, int NB>
+template <int RP, int NB, typename TG>
 __global__ __launch_bounds__(256) void k_finest_local(Tree T, CovCtx cc, Pts O, const double* yv,
-                                                      const uint32_t* items, double* Gf, double* Bf, double* Sig,
+                                                      const uint32_t* items, TG* Gf, TG* Bf, double* Sig,
                                                       double* H, double* u, double* dterm, int* bad_flag) {
     __shared__ FinestLds<NB> sh;
     __shared__ double red[MRA_WG];
@@ -361,11 +378,16 @@ __global__ __launch_bounds__(256) void k_finest_local(Tree T, CovCtx cc, Pts O, 
     const int ld = NB > 0 ? FinestLds<NB>::LD : n;
     const uint64_t so = T.f_seg_os[F];
     const int sn = (int)T.f_seg_n[F];
-    const double* GF = Gf + (uint64_t)LRP * so + (ob - so);
+    const TG* GF = Gf + (uint64_t)LRP * so + (ob - so);
     if (threadIdx.x == 0) bad = 0;
     for (int idx = threadIdx.x; idx < n * n; idx += MRA_WG) {
         const int i = idx / n, c = idx % n;
-        Sg[i * ld + c] = (i == c) ? O.v[ob + i] : pcov(cc, O, ob + i, O, ob + c);
+        double v;
+        if constexpr (sizeof(TG) == 4)
+            v = (i == c) ? O.v[ob + i] : (double)pcovf(cc, O, ob + i, O, ob + c);
+        else
+            v = (i == c) ? O.v[ob + i] : pcov(cc, O, ob + i, O, ob + c);
+        Sg[i * ld + c] = v;
     }
     __syncthreads();
     if (NB > 0 && NB <= 32) {
@@ -401,7 +423,7 @@ __global__ __launch_bounds__(256) void k_finest_local(Tree T, CovCtx cc, Pts O, 
         }
         const double dF = wg_sum(part, red);
         // H = L_F^{-1} B (observation-major rows), 64 columns at a time
-        const double* BO = Bf + ob * (uint64_t)LRP;
+        const TG* BO = Bf + ob * (uint64_t)LRP;
         for (int c0 = 0; c0 < LRP; c0 += 64) {
             const int cv = min(64, LRP - c0);
             wg_gemm_tn<true, false>(H + ob * (uint64_t)LRP + c0, LRP, (n + 15) & ~15, 64, n, 1.0, Sg, ld, n,
@@ -416,14 +438,14 @@ __global__ __launch_bounds__(256) void k_finest_local(Tree T, CovCtx cc, Pts O, 
         wg_trsm(Sg, ld, uF, 1, n, 1);
         for (int i = threadIdx.x; i < n; i += MRA_WG) part += uF[i] * uF[i];
         const double dF = wg_sum(part, red);
-        double* BO = Bf + ob * (uint64_t)LRP;
+        const TG* BO = Bf + ob * (uint64_t)LRP;
+        double* HF = H + ob * (uint64_t)LRP;
         for (int m = threadIdx.x; m < LRP; m += MRA_WG) {
             for (int i = 0; i < n; i++) {
-                double acc = BO[(uint64_t)i * LRP + m];
-                for (int k = 0; k < i; k++) acc -= Sg[i * ld + k] * BO[(uint64_t)k * LRP + m];
+                double acc = (double)BO[(uint64_t)i * LRP + m];
+                for (int k = 0; k < i; k++) acc -= Sg[i * ld + k] * HF[(uint64_t)k * LRP + m];
                 acc /= Sg[i * ld + i];
-                BO[(uint64_t)i * LRP + m] = acc;
-                H[(ob + i) * LRP + m] = acc;
+                HF[(uint64_t)i * LRP + m] = acc;
             }
         }
         if (threadIdx.x == 0) dterm[T.nC + F] = dF;
@@ -797,7 +819,7 @@ struct PredArgs {
     const uint64_t* goff;
     const double* Gc;
     const double* LiT;
-    const double* Gf;
+    const void* Gf;  // double or float (mixed precision)
     const double* Sig;
     const double* H;
     const double* u;
@@ -813,7 +835,7 @@ struct PredArgs {
     const double* v;
 };
 
-template <int RP>
+template <int RP, typename TG>
 __global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts O, Pts Pp, PredArgs a) {
     constexpr int NT = 64, LD = NT + 1;
     __shared__ double Cm[RP * LD];
@@ -862,7 +884,7 @@ __global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts 
         double* GL = a.GL + a.gl_off[slot];
         const uint64_t so = T.f_seg_os[F];
         const int sn = (int)T.f_seg_n[F];
-        const double* GF = a.Gf + (uint64_t)LRP * so + (ob - so);
+        const TG* GF = (const TG*)a.Gf + (uint64_t)LRP * so + (ob - so);
         for (int t0 = 0; t0 < nF; t0 += RP) {  // row tiles of RP (the tile height of Cm)
             const int mv = min(RP, nF - t0);
             for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
@@ -1005,7 +1027,8 @@ __global__ void k_normals(uint64_t seed, uint64_t base, uint64_t n, double* out)
 // y_F = sum_k G_{F,k}^T xi_{a_k} + L_F xi_F: a draw from N(0, Sigma_MRA), since
 // G_{F,k}^T G_{F',k} = C_k(S_F, Q) K^{-1} C_k(Q, S_F') and L_F L_F^T = Sigma_F.
 // One work group per non-empty finest region, one thread per observation.
-__global__ __launch_bounds__(256) void k_sample_finest(Tree T, const uint32_t* items, const double* Gf,
+template <typename TG>
+__global__ __launch_bounds__(256) void k_sample_finest(Tree T, const uint32_t* items, const TG* Gf,
                                                        const double* Sig, const double* xi_c, const double* xi_o,
                                                        double* y) {
     const uint32_t F = items[blockIdx.x];
@@ -1014,7 +1037,7 @@ __global__ __launch_bounds__(256) void k_sample_finest(Tree T, const uint32_t* i
     const int RP = T.RP, L = T.L;
     const uint64_t so = T.f_seg_os[F];
     const int sn = (int)T.f_seg_n[F];
-    const double* GF = Gf + (uint64_t)T.LRP * so + (ob - so);
+    const TG* GF = Gf + (uint64_t)T.LRP * so + (ob - so);
     const double* LF = Sig + T.sig_off[F];
     for (int o = threadIdx.x; o < n; o += MRA_WG) {
         double s = 0.0;
@@ -1056,6 +1079,17 @@ __global__ __launch_bounds__(256) void k_selftest(const double* A, const double*
         Xout[idx] = Wm[(idx / 64) * LD + idx % 64];
     }
     if (threadIdx.x == 0) *bad_out = bad;
+}
+
+__global__ __launch_bounds__(256) void k_selftest_f32(const float* A, const float* B, int K, int Mv, int Nv,
+                                                      const float* C0, float* Cout) {
+    constexpr int LD = 65;
+    __shared__ float Cm[64 * LD];
+    for (int idx = threadIdx.x; idx < 64 * 64; idx += MRA_WG) Cm[(idx / 64) * LD + idx % 64] = C0[idx];
+    __syncthreads();
+    wg_gemm_tn_f32(Cm, LD, 64, 64, K, -1.0f, A, 64, Mv, B, 64, Nv);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < 64 * 64; idx += MRA_WG) Cout[idx] = Cm[(idx / 64) * LD + idx % 64];
 }
 
 // ------------------------------------------------------------- reductions

@@ -27,3 +27,19 @@ def test_blocks(K, Mv, Nv):
     assert bad == 0
     np.testing.assert_allclose(np.tril(Lo), Lr, rtol=0, atol=1e-13)
     np.testing.assert_allclose(Xo, np.linalg.solve(Lr, Bt), rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("K,Mv,Nv", [(4, 64, 64), (64, 64, 64), (130, 64, 64), (37, 40, 23)])
+def test_gemm_f32(K, Mv, Nv):
+    """FP32 matrix-core GEMM with FP32 accumulation: agrees with the float64
+    product to the FP32 error bound K * 2^-24 * max|A||B| (plus rounding)."""
+    rng = np.random.default_rng(K + 7 * Mv)
+    A = rng.standard_normal((K, 64)).astype(np.float32)
+    B = rng.standard_normal((K, 64)).astype(np.float32)
+    C0 = rng.standard_normal((64, 64)).astype(np.float32)
+    Co = capi.selftest_f32(K, Mv, Nv, A, B, C0)
+    ref = C0.astype(np.float64)
+    ref[:Mv, :Nv] -= (A.astype(np.float64).T @ B.astype(np.float64))[:Mv, :Nv]
+    bound = (K + 2) * 2.0 ** -24 * (np.abs(A).T @ np.abs(B) + np.abs(C0)).max()
+    print("f32 gemm max err", np.abs(Co - ref).max(), "bound", bound, flush=True)
+    assert np.abs(Co - ref).max() <= bound

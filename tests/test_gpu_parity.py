@@ -200,3 +200,46 @@ def test_dense_and_factored_posterior(case, factored, example):
     assert h.get_option(capi.OPT_FACTORED_POSTERIOR) == factored
     h.close()
     check(tree, cov, prm, factored=factored)
+
+
+TOL_MIXED = 1e-6   # |delta loglik| / n for the mixed-precision path
+
+
+@pytest.mark.parametrize("case", ["clustered64", "example"])
+def test_mixed_precision_loglik(case, example):
+    """Mixed precision (FP32 covariance evaluation, FP32 MFMA finest
+    conditioning, FP32 storage of G_F/B_F; FP64 Sigma_F assembly, Cholesky,
+    log-determinants and everything above the finest level) stays within
+    |delta loglik| / n <= 1e-6 of the FP64 oracle, and predictions within
+    1e-4 relative."""
+    if case == "example":
+        tree, cov, prm = example
+    else:
+        rng = np.random.default_rng(79)
+        x, y = synth.clustered_points(12000, rng)
+        px, py = synth.uniform_points(400, rng)
+        tree = capi.partition_program(x, y, np.zeros(12000), px, py, M=4, J=4, r=64,
+                                      bounds=synth.UNIT_BOUNDS)
+        cov, prm = synth.linear_trend_cov(), dict(synth.UNIT_PARAMS)
+        h = capi.Handle(tree, with_pred=False)
+        h.set_covariance(cov)
+        h.set_params(**prm)
+        tree["obs_val"] = h.sample_prior(seed=7, replace=False)
+        h.close()
+    n = len(tree["obs_lon"])
+    Do, mo, vo = oracle.evaluate(tree, cov, prm, predict=True)
+    h = capi.Handle(tree)
+    try:
+        h.set_covariance(cov)
+        h.set_params(**prm)
+        h.set_precision(capi.PRECISION_MIXED)
+        D, mean, var = h.predict()
+        Dl = h.likelihood()
+    finally:
+        h.close()
+    err = abs(D - Do) / (2 * n)
+    print(f"mixed D {D!r} oracle {Do!r} |dl|/n {err:.3e}", flush=True)
+    assert err <= TOL_MIXED
+    assert Dl == D
+    np.testing.assert_allclose(mean, mo, rtol=1e-4, atol=1e-4 * np.abs(mo).max())
+    np.testing.assert_allclose(var, vo, rtol=1e-4, atol=1e-6)
