@@ -1,15 +1,23 @@
 """Benchmark: MRA log-likelihood evaluations per second on one MI355X.
 
-Workload (the metric's configuration): n = 2,000,000 points on [0,1]^2, 60%
-uniform + 40% in 20 Gaussian clusters (sigma 0.02); nonstationary exponential
-kernel with log sigma^2 = 0.5x - 0.3y, log beta = log 0.05 + 0.8x, nugget
-0.05 (planar metric); M = 10 levels, J = 4, r = 64 knots; observations drawn
-from the MRA prior itself (mra_sample_prior).  One step = one full
-log-likelihood evaluation (coarse prior, finest level, posterior, reduction)
-with the inputs resident in HBM.
+Workload: n = 2,000,000 points on [0,1]^2, 60% uniform + 40% in 20 Gaussian
+clusters (sigma 0.02); nonstationary exponential kernel with
+log sigma^2 = 0.5x - 0.3y, log beta = log 0.05 + 0.8x, nugget 0.05 (planar
+metric); M = 10 levels, J = 4, r = 64 knots; observations drawn from the MRA
+prior itself (mra_sample_prior, FP64); 200,000 uniform prediction points.
+One step = one full log-likelihood evaluation (coarse prior, finest level,
+posterior, reduction) with every input resident in HBM.
 
-Prints one JSON line (see the keys below).  --gpus N > 1 is refused: this
-build runs on one GPU.
+The headline `value` is the mixed-precision path (FP32 covariance and
+finest-level observation GEMMs, FP64 knot Cholesky / conditioning /
+log-determinants) replayed as one captured HIP graph per evaluation.  The
+same line reports the pure-FP64 GPU path, |loglik_mixed - loglik_fp64| / n,
+the prediction error of the mixed path against FP64 at the prediction
+points, per-launch-group times (HIP events on the evaluation stream, from
+an eager pass of the same number of steps), the roofline of the dominant
+kernel, and the CPU oracle timed on a bounded sample.
+
+Prints one JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
 """
 import argparse
 import json
@@ -24,16 +32,22 @@ sys.path.insert(0, ROOT)
 
 from nonstationary_mra_amd import capi, synth, workmodel  # noqa: E402
 
-PEAK_FP64 = 78.6e12   # MI355X dense FP64 (matrix = vector) FLOP/s
-PEAK_HBM = 8.0e12     # bytes/s
+PEAK = {"f64": 78.6e12, "f32": 157.3e12}   # MI355X dense matrix FLOP/s (FP64 matrix = vector)
+PEAK_HBM = 8.0e12                         # bytes/s
 SEED_LOC, SEED_FIELD = 20260101, 20260102
+METRIC = "MRA loglik evals/sec at n=2M,M=10,r=64: mixed vs FP64 GPU, |Δℓ|/n≤1e-6"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
-def make_workload(n, M, J, r, seed=SEED_LOC):
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_workload(n, npred, M, J, r, seed=SEED_LOC):
     rng = np.random.default_rng(seed)
     x, y = synth.clustered_points(n, rng)
-    tree = capi.partition_program(x, y, np.zeros(n), None, None, M=M, J=J, r=r, bounds=synth.UNIT_BOUNDS)
-    return tree
+    px, py = synth.uniform_points(npred, rng) if npred else (None, None)
+    return capi.partition_program(x, y, np.zeros(n), px, py, M=M, J=J, r=r, bounds=synth.UNIT_BOUNDS)
 
 
 def group_names(M):
@@ -48,31 +62,81 @@ KERNEL_OF = {"prior": "k_prior_coarse", "post": "k_posterior", "finest_cross": "
              "finest_local": "k_finest_local"}
 
 
-def cpu_baseline(M, J, r, n, budget_s=30.0):
+def kernel_of(gname):
+    return KERNEL_OF[gname.split("_l")[0]] if gname.startswith(("prior_l", "post_l")) else KERNEL_OF[gname]
+
+
+def timed(h, steps, warmup):
+    """(evals/s, ms/step, D) over `steps` evaluations after `warmup`."""
+    D = None
+    for _ in range(warmup):
+        D = h.likelihood()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        D = h.likelihood()
+    el = time.perf_counter() - t1
+    return steps / el, 1e3 * el / steps, D
+
+
+def eager_groups(h, steps):
+    """Average per-launch-group milliseconds over `steps` eager evaluations
+    (HIP events recorded on the evaluation stream around every group)."""
+    h.use_graph(False)
+    tim = np.zeros(capi.T_COUNT)
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        h.likelihood()
+        tim += h.timings()
+    return tim / steps, 1e3 * (time.perf_counter() - t1) / steps
+
+
+def cpu_baseline(h_unused, M, J, r, n, budget_s=30.0):
     """The FP64 oracle (a restatement of the reference's CPU algorithm, OpenMP
     over regions, MKL kernels when libmkl_rt is present) on a bounded sample:
     the same generator at n/16 points and M-2 levels (same points per finest
-    region), scaled to the full workload by the ratio of algorithmic flops."""
+    region), scaled to the full workload by the ratio of algorithmic flops.
+    The FP64 GPU path is evaluated on the same sample for its error against
+    the oracle."""
     from oracle import oracle
     mkl = oracle.lib().orc_set_blas(1)
     ns, Ms = n // 16, M - 2
-    tree = make_workload(ns, Ms, J, r, seed=SEED_LOC + 1)
+    tree = make_workload(ns, 0, Ms, J, r, seed=SEED_LOC + 1)
     cov = synth.linear_trend_cov()
     prm = dict(synth.UNIT_PARAMS)
     tree["obs_val"] = np.random.default_rng(3).standard_normal(len(tree["obs_lon"]))
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     times = []
     t_start = time.time()
+    Do = None
     while True:
         t0 = time.perf_counter()
-        oracle.evaluate(tree, cov, prm, predict=False, nthreads=threads)
+        Do, _, _ = oracle.evaluate(tree, cov, prm, predict=False, nthreads=threads)
         times.append(time.perf_counter() - t0)
         if len(times) >= 3 or time.time() - t_start > budget_s:
             break
     t = float(np.median(times))
     cs = np.diff(tree["obs_offsets"].astype(np.int64))
     w_sample = workmodel.total(Ms, J, r, cs)
-    return t, w_sample, threads, mkl, f"n={ns} M={Ms} J={J} r={r} (same generator), {len(times)} runs, median"
+    g = capi.Handle(tree, with_pred=False)
+    try:
+        g.set_covariance(cov)
+        g.set_params(**prm)
+        Dg = g.likelihood()
+    finally:
+        g.close()
+    return dict(t=t, w=w_sample, threads=threads, mkl=mkl, n_runs=len(times),
+                sample=f"n={ns} M={Ms} J={J} r={r} (same generator)", rel=abs(Dg - Do) / abs(Do))
+
+
+def load_traffic(launch):
+    """HBM bytes per launch of `launch` from the committed PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        return t["per_launch_bytes"].get(launch)
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def main():
@@ -81,81 +145,99 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=2_000_000)
+    ap.add_argument("--npred", type=int, default=200_000)
     ap.add_argument("--M", type=int, default=10)
     ap.add_argument("--r", type=int, default=64)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
+    ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--graph", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
     J = 4
+    graph = not args.no_graph
     t0 = time.time()
-    tree = make_workload(args.n, args.M, J, args.r)
+    tree = make_workload(args.n, args.npred, args.M, J, args.r)
     counts = np.diff(tree["obs_offsets"].astype(np.int64))
-    print(f"[bench] partition {time.time() - t0:.1f}s n_obs={counts.sum()} max_nF={counts.max()} "
-          f"empty={(counts == 0).sum()}", file=sys.stderr, flush=True)
-    h = capi.Handle(tree, with_pred=False)
+    n = int(counts.sum())
+    log(f"partition {time.time() - t0:.1f}s n_obs={n} max_nF={counts.max()} empty={(counts == 0).sum()}")
+    h = capi.Handle(tree, with_pred=args.npred > 0)
     h.set_covariance(synth.linear_trend_cov())
     h.set_params(**synth.UNIT_PARAMS)
     h.sample_prior(SEED_FIELD, replace=True)
-    if args.precision == "mixed":
-        h.set_precision(capi.PRECISION_MIXED)
-    if args.graph:
-        h.use_graph(True)
-    print(f"[bench] setup {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        D = h.likelihood()
-    tim = np.zeros(capi.T_COUNT)
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
-        D = h.likelihood()
-        tim += h.timings()
-    elapsed = time.perf_counter() - t1
-    tim /= args.steps
-    ms_per_step = 1e3 * elapsed / args.steps
-    value = args.steps / elapsed
-    n = int(counts.sum())
-    loglik = -0.5 * (D + n * np.log(2 * 3.14159265359))
-    print(f"[bench] {ms_per_step:.2f} ms/eval D={D:.10e}", file=sys.stderr, flush=True)
-    # roofline of the dominant kernel (largest average launch time)
+    log(f"setup {time.time() - t0:.1f}s")
+    prec = capi.PRECISION_MIXED if args.precision == "mixed" else capi.PRECISION_FP64
+    # headline: the requested precision, graph replay
+    h.set_precision(prec)
+    h.use_graph(graph)
+    value, ms_per_step, D = timed(h, args.steps, args.warmup)
+    log(f"{args.precision}{' graph' if graph else ''}: {ms_per_step:.2f} ms/eval D={D:.10e}")
+    # per-group times of the same evaluation, eager
+    tim, eager_ms = eager_groups(h, args.steps)
     names = group_names(args.M)
     flops = workmodel.groups(args.M, J, args.r, counts)
     gid = max(names, key=lambda g: tim[g])
     gname = names[gid]
+    fp32_group = args.precision == "mixed" and gname == "finest_cross"
+    peak = PEAK["f32" if fp32_group else "f64"]
     achieved = flops[gname] / (tim[gid] * 1e-3)
     per_group = {names[g]: round(float(tim[g]), 3) for g in sorted(names) if tim[g] > 0.01}
-    kern = KERNEL_OF[gname.split("_l")[0]] if gname.startswith(("prior_l", "post_l")) else KERNEL_OF[gname]
-    roof = {"bound": "mfma", "kernel": kern, "launch": gname, "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
-            "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None,
-            "flops_per_launch": flops[gname], "avg_launch_ms": float(tim[gid])}
+    roof = {"bound": "mfma", "kernel": kernel_of(gname), "launch": gname, "achieved": achieved / 1e12,
+            "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": load_traffic(gname),
+            "flops_per_launch": flops[gname], "avg_launch_ms": float(tim[gid]),
+            "arith": "f32" if fp32_group else "f64"}
     w_total = workmodel.total(args.M, J, args.r, counts)
     out = {
-        "metric": "MRA loglik evals/sec at n=2M,M=10,r=64",
+        "metric": METRIC,
         "value": value, "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic (clustered points, field drawn from the MRA prior)",
         "config": {"workload": f"n={args.n} clustered, M={args.M}, J={J}, r={args.r}, "
-                               f"nonstationary linear-trend Matern-1/2, planar", "n": n, "M": args.M, "J": J,
-                   "r": args.r, "precision": args.precision, "graph": bool(args.graph),
-                   "max_obs_per_finest_region": int(counts.max())},
-        "loglik": loglik, "D": D,
+                               f"nonstationary linear-trend Matern-1/2, planar, {args.npred} prediction points",
+                   "n": n, "n_pred": args.npred, "M": args.M, "J": J, "r": args.r, "precision": args.precision,
+                   "graph": graph, "max_obs_per_finest_region": int(counts.max()),
+                   "seeds": {"locations": SEED_LOC, "field": SEED_FIELD}},
+        "loglik": -0.5 * (D + n * np.log(2 * 3.14159265359)), "D": D,
+        "eager_ms_per_step": eager_ms,
         "whole_eval": {"flops": w_total, "tflops": w_total / (ms_per_step * 1e-3) / 1e12,
-                       "frac_fp64_peak": w_total / (ms_per_step * 1e-3) / PEAK_FP64},
+                       "frac_fp64_peak": w_total / (ms_per_step * 1e-3) / PEAK["f64"]},
         "launch_ms": per_group,
         "roofline": roof,
     }
+    if args.precision == "mixed" and not args.no_fp64:
+        h.set_precision(capi.PRECISION_FP64)
+        h.use_graph(graph)
+        v64, ms64, D64 = timed(h, args.steps, args.warmup)
+        log(f"fp64: {ms64:.2f} ms/eval D={D64:.10e}")
+        out["fp64_gpu"] = {"value": v64, "ms_per_step": ms64, "D": D64}
+        out["mixed_vs_fp64"] = {"speedup": value / v64, "abs_dloglik_per_n": abs(D - D64) / (2 * n)}
+        if args.npred:
+            t1 = time.perf_counter()
+            _, m64, s64 = h.predict()
+            tp64 = time.perf_counter() - t1
+            h.set_precision(capi.PRECISION_MIXED)
+            h.predict()
+            t1 = time.perf_counter()
+            _, mm, sm = h.predict()
+            tpm = time.perf_counter() - t1
+            out["prediction"] = {"n_pred": len(mm), "ms_mixed": 1e3 * tpm, "ms_fp64": 1e3 * tp64,
+                                 "mean_max_abs_err_rel_to_max": float(np.abs(mm - m64).max() / np.abs(m64).max()),
+                                 "var_max_rel_err": float(np.max(np.abs(sm - s64) / np.abs(s64)))}
     h.close()
     if not args.no_cpu_baseline:
-        t_cpu, w_sample, threads, mkl, sample = cpu_baseline(args.M, J, args.r, args.n)
-        cpu_evals = (1.0 / t_cpu) * (w_sample / w_total)
-        out["cpu_baseline"] = {"value": cpu_evals, "unit": "evals/s", "cores": threads, "kind": "port",
-                               "sample": sample + f"; {t_cpu:.2f}s per sample eval, "
-                                                  f"{w_sample:.3e} of {w_total:.3e} flops; "
-                                                  f"blas={'mkl' if mkl else 'oracle loops'}"}
+        cb = cpu_baseline(None, args.M, J, args.r, args.n)
+        cpu_evals = (1.0 / cb["t"]) * (cb["w"] / w_total)
+        out["cpu_baseline"] = {"value": cpu_evals, "unit": "evals/s", "cores": cb["threads"], "kind": "port",
+                               "sample": cb["sample"] + f", {cb['n_runs']} runs, median {cb['t']:.2f}s per sample "
+                                         f"eval, {cb['w']:.3e} of {w_total:.3e} flops, scaled by flops; "
+                                         f"blas={'mkl' if cb['mkl'] else 'oracle loops'}"}
         out["gpu_vs_cpu"] = value / cpu_evals
+        if "fp64_gpu" in out:
+            out["fp64_gpu"]["vs_cpu"] = out["fp64_gpu"]["value"] / cpu_evals
+        out["fp64_gpu_vs_oracle_rel_on_sample"] = cb["rel"]
     print(json.dumps(out), flush=True)
 
 

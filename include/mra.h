@@ -132,8 +132,12 @@ int mra_sample_prior(mra_handle* h, uint64_t seed, double* y, int replace);
 /* Replace the observation values (same order as obs_val at create). */
 int mra_set_observations(mra_handle* h, const double* values);
 
-/* Capture one complete likelihood evaluation as a HIP graph and replay it on
- * later mra_likelihood calls (1), or launch kernels directly (0). */
+/* Capture one complete evaluation (likelihood, or likelihood + prediction)
+ * as a HIP graph and replay it on later calls (1), or launch kernels
+ * directly (0).  A graph is captured per (likelihood | predict, precision)
+ * on first use; parameter changes (mra_set_params) are picked up without
+ * re-capture; mra_set_covariance and mra_set_option discard the graphs.
+ * With graphs, mra_last_timings reports the whole replay as MRA_T_GRAPH. */
 int mra_use_graph(mra_handle* h, int enable);
 
 /* Engine options (no reference counterpart).
@@ -154,6 +158,7 @@ enum {
     MRA_T_PRED_PRE = 3,
     MRA_T_PRED_POST = 4,
     MRA_T_REDUCE = 5,
+    MRA_T_GRAPH = 6,          /* one replay of the captured evaluation */
     MRA_T_PRIOR0 = 8,         /* + level l: k_prior_coarse of level l (l < 24) */
     MRA_T_POST0 = 32,         /* + level l: k_posterior of level l */
     MRA_T_COUNT = 56

@@ -15,8 +15,8 @@ MRA_OK, MRA_E_ARG, MRA_E_HIP, MRA_E_NOT_PD, MRA_E_STATE, MRA_E_NOMEM, MRA_E_UNSU
 METRIC = {"chordal": 0, "great_circle": 1, "planar": 2}
 FIELD_CONSTANT, FIELD_WENDLAND, FIELD_LINEAR = 0, 1, 2
 PRECISION_FP64, PRECISION_MIXED = 0, 1
-T_SETUP, T_FINEST_CROSS, T_FINEST_LOCAL, T_PRED_PRE, T_PRED_POST, T_REDUCE, T_PRIOR0, T_POST0, T_COUNT = \
-    0, 1, 2, 3, 4, 5, 8, 32, 56
+T_SETUP, T_FINEST_CROSS, T_FINEST_LOCAL, T_PRED_PRE, T_PRED_POST, T_REDUCE, T_GRAPH, T_PRIOR0, T_POST0, T_COUNT = \
+    0, 1, 2, 3, 4, 5, 6, 8, 32, 56
 
 u64p = C.POINTER(C.c_uint64)
 dp = C.POINTER(C.c_double)

@@ -101,7 +101,18 @@ struct mra_handle {
     int nEv = 0;
     double timings[MRA_T_COUNT] = {};
     bool useGraph = false;
+    // parameters as read by the kernels (pinned host copy, device copy)
+    double* hPrm = nullptr;
+    double* dPrm = nullptr;
+    // captured evaluations, keyed by (predicting, precision); a stationary
+    // model bakes sill/range into the launches, so those are remembered
+    static constexpr int NGRAPH = 4;
+    hipGraph_t graph[NGRAPH] = {};
+    hipGraphExec_t gexec[NGRAPH] = {};
+    double graphSill[NGRAPH] = {}, graphRange[NGRAPH] = {};
 };
+
+static void graph_invalidate(mra_handle* h);
 
 static void set_err(mra_handle* h, const std::string& s) {
     if (h) h->err = s;
@@ -145,6 +156,11 @@ extern "C" const char* mra_last_error(const mra_handle* h) { return h ? h->err.c
 extern "C" void mra_destroy(mra_handle* h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (int g = 0; g < mra_handle::NGRAPH; g++) {
+        if (h->gexec[g]) (void)hipGraphExecDestroy(h->gexec[g]);
+        if (h->graph[g]) (void)hipGraphDestroy(h->graph[g]);
+    }
+    if (h->hPrm) (void)hipHostFree(h->hPrm);
     for (void* p : h->allocs) (void)hipFree(p);
     for (void* p : h->fieldAllocs) (void)hipFree(p);
     for (auto& e : h->ev)
@@ -367,6 +383,8 @@ static int alloc_engine(mra_handle* h, const mra_tree_desc* t) {
     RC_TRY(dalloc(h, &h->dLev, h->M, A));
     RC_TRY(dalloc(h, &h->dD, 1, A));
     RC_TRY(dalloc(h, &h->dBad, 1, A));
+    RC_TRY(dalloc(h, &h->dPrm, 8, A));
+    HIP_TRY(hipHostMalloc((void**)&h->hPrm, sizeof(double) * 8, hipHostMallocDefault));
     h->dW.assign(h->L, nullptr);
     h->dA.assign(h->L, nullptr);
     // dense messages (those of level L-1 only when that level is not factored,
@@ -603,6 +621,7 @@ extern "C" int mra_set_covariance(mra_handle* h, const mra_cov_desc* cov) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->covSet = true;
+    graph_invalidate(h);
     return MRA_OK;
 }
 
@@ -665,6 +684,7 @@ extern "C" int mra_set_option(mra_handle* h, int option, int value) {
     if (!h) return MRA_E_ARG;
     if (option == MRA_OPT_FACTORED_POSTERIOR) {
         h->lowrankMode = value ? 1 : 0;
+        graph_invalidate(h);
         return MRA_OK;
     }
     set_err(h, "unknown option");
@@ -752,14 +772,17 @@ static void launch_finest_cross(mra_handle* h, const Tree& T, const CovCtx& cc, 
 }
 
 template <int RP>
-static void launch_eval(mra_handle* h, bool predicting) {
+static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
     hipStream_t st = h->stream;
     const Tree T = make_tree(h);
     const CovCtx cc = make_cc(h);
     const Pts K = pts(h->dKc, h->dKs), O = pts(h->dOc, h->dOs);
     const Pts Pp = predicting ? pts(h->dPc, h->dPs) : Pts{};
     h->nEv = 0;
+    // per-group events only in eager mode (a captured graph is timed as a whole)
+    const bool timed = !capturing;
     auto mark = [&](int group) {
+        if (!timed) return;
         h->evGroup[h->nEv] = group;
         (void)hipEventRecord(h->ev[h->nEv++], st);
     };
@@ -772,8 +795,8 @@ static void launch_eval(mra_handle* h, bool predicting) {
     for (int k = 0; k < (predicting ? 3 : 2); k++)
         if (sets[k].n)
             k_scale<<<nblk(sets[k].n), 256, 0, st>>>(sets[k].n, sets[k].f[0], sets[k].f[1], sets[k].f[2],
-                                                     cc.stationary, h->sill, h->range, h->nugget, h->ss, h->rs, h->ns,
-                                                     sets[k].s[0], sets[k].s[1], sets[k].s[2]);
+                                                     cc.stationary, h->dPrm, sets[k].s[0], sets[k].s[1],
+                                                     sets[k].s[2]);
     k_fill<<<nblk(h->nC + h->nF), 256, 0, st>>>(h->dDterm, h->nC + h->nF, 0.0);
     k_zero_int<<<1, 64, 0, st>>>(h->dBad, 1);
     // coarse prior, level by level
@@ -939,16 +962,78 @@ static void launch_eval(mra_handle* h, bool predicting) {
     mark(MRA_T_COUNT);
 }
 
+static void graph_invalidate(mra_handle* h) {
+    for (int g = 0; g < mra_handle::NGRAPH; g++) {
+        if (h->gexec[g]) (void)hipGraphExecDestroy(h->gexec[g]);
+        if (h->graph[g]) (void)hipGraphDestroy(h->graph[g]);
+        h->gexec[g] = nullptr;
+        h->graph[g] = nullptr;
+    }
+}
+
+static void launch_eval_rp(mra_handle* h, bool predicting, bool capturing) {
+    switch (h->RP) {
+        case 16: launch_eval<16>(h, predicting, capturing); break;
+        case 32: launch_eval<32>(h, predicting, capturing); break;
+        default: launch_eval<64>(h, predicting, capturing); break;
+    }
+}
+
+// One evaluation as a captured HIP graph: every launch of launch_eval is
+// recorded once per (predicting, precision) and replayed.  The graph holds
+// kernel nodes only (zeroing is done by fill kernels); parameters reach the
+// kernels through dPrm, written before each replay.
+static int run_graph(mra_handle* h, bool predicting) {
+    const int g = (predicting ? 2 : 0) + (h->precision == MRA_PRECISION_MIXED ? 1 : 0);
+    if (h->gexec[g] && !h->nonstationary && (h->graphSill[g] != h->sill || h->graphRange[g] != h->range)) {
+        (void)hipGraphExecDestroy(h->gexec[g]);
+        (void)hipGraphDestroy(h->graph[g]);
+        h->gexec[g] = nullptr;
+        h->graph[g] = nullptr;
+    }
+    if (!h->gexec[g]) {
+        HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        launch_eval_rp(h, predicting, true);
+        hipGraph_t gr = nullptr;
+        hipError_t e = hipStreamEndCapture(h->stream, &gr);
+        if (e != hipSuccess) {
+            if (gr) (void)hipGraphDestroy(gr);
+            set_err(h, std::string("graph capture failed: ") + hipGetErrorString(e));
+            return MRA_E_HIP;
+        }
+        hipGraphExec_t ex = nullptr;
+        e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(gr);
+            set_err(h, std::string("graph instantiation failed: ") + hipGetErrorString(e));
+            return MRA_E_HIP;
+        }
+        h->graph[g] = gr;
+        h->gexec[g] = ex;
+        h->graphSill[g] = h->sill;
+        h->graphRange[g] = h->range;
+    }
+    HIP_TRY(hipEventRecord(h->ev[0], h->stream));
+    HIP_TRY(hipGraphLaunch(h->gexec[g], h->stream));
+    HIP_TRY(hipEventRecord(h->ev[1], h->stream));
+    h->evGroup[0] = MRA_T_GRAPH;
+    h->nEv = 2;
+    return MRA_OK;
+}
+
 static int run_eval(mra_handle* h, bool predicting) {
     if (!h->covSet || !h->paramsSet) {
         set_err(h, "covariance model and parameters must be set before evaluating");
         return MRA_E_STATE;
     }
     if (!use_lowrank(h)) RC_TRY(ensure_dense_msgs(h));
-    switch (h->RP) {
-        case 16: launch_eval<16>(h, predicting); break;
-        case 32: launch_eval<32>(h, predicting); break;
-        default: launch_eval<64>(h, predicting); break;
+    const double prm[6] = {h->sill, h->range, h->nugget, h->ss, h->rs, h->ns};
+    for (int i = 0; i < 6; i++) h->hPrm[i] = prm[i];
+    HIP_TRY(hipMemcpyAsync(h->dPrm, h->hPrm, sizeof(prm), hipMemcpyHostToDevice, h->stream));
+    if (h->useGraph) {
+        RC_TRY(run_graph(h, predicting));
+    } else {
+        launch_eval_rp(h, predicting, false);
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(h->stream));
@@ -1068,8 +1153,11 @@ static int tmp_points(mra_handle* h, const mra_cov_desc* cov, double sill, doubl
     if (rc) return rc;
     k_coords<<<nblk(n), 256>>>(n, dl, da, cov->metric, t.c[0], t.c[1], t.c[2]);
     k_fields<<<nblk(n), 256>>>(n, dl, da, cov->metric, cov->wendland_len, f[0], f[1], f[2], t.f[0], t.f[1], t.f[2]);
-    k_scale<<<nblk(n), 256>>>(n, t.f[0], t.f[1], t.f[2], cov->nonstationary ? 0 : 1, sill, range, nugget, ss, rs, ns,
-                              t.s[0], t.s[1], t.s[2]);
+    double* dp;
+    RC_TRY(dalloc(h, &dp, 6, t.list));
+    const double prm[6] = {sill, range, nugget, ss, rs, ns};
+    HIP_TRY(hipMemcpy(dp, prm, sizeof(prm), hipMemcpyHostToDevice));
+    k_scale<<<nblk(n), 256>>>(n, t.f[0], t.f[1], t.f[2], cov->nonstationary ? 0 : 1, dp, t.s[0], t.s[1], t.s[2]);
     HIP_TRY(hipGetLastError());
     return MRA_OK;
 }

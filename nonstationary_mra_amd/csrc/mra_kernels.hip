@@ -137,20 +137,22 @@ __global__ void k_fields(uint64_t n, const double* lon, const double* lat, int m
 }
 
 // Scaled per-point parameters (src/evaluate_covariance.cpp:170-189,258-269).
-__global__ void k_scale(uint64_t n, const double* f0, const double* f1, const double* f2, int stationary, double sill,
-                        double range, double nugget, double ss, double rs, double ns, double* s, double* b,
-                        double* v) {
+// prm = {sill, range, nugget, sill_scale, range_scale, nugget_scale}, read
+// from device memory so that a captured evaluation graph follows parameter
+// updates without re-capture
+__global__ void k_scale(uint64_t n, const double* f0, const double* f1, const double* f2, int stationary,
+                        const double* __restrict__ prm, double* s, double* b, double* v) {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (stationary) {
-        s[i] = sqrt(sill);
-        b[i] = range;
-        v[i] = sill + nugget;
+        s[i] = sqrt(prm[0]);
+        b[i] = prm[1];
+        v[i] = prm[0] + prm[2];
     } else {
-        double sg = ss * f0[i];
+        double sg = prm[3] * f0[i];
         s[i] = sqrt(sg);
-        b[i] = rs * f1[i];
-        v[i] = sg + ns * f2[i];
+        b[i] = prm[4] * f1[i];
+        v[i] = sg + prm[5] * f2[i];
     }
 }
 

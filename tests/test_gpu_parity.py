@@ -243,3 +243,40 @@ def test_mixed_precision_loglik(case, example):
     assert Dl == D
     np.testing.assert_allclose(mean, mo, rtol=1e-4, atol=1e-4 * np.abs(mo).max())
     np.testing.assert_allclose(var, vo, rtol=1e-4, atol=1e-6)
+
+
+def test_graph_replay_matches_eager():
+    """A captured evaluation replayed many times, with eager evaluations,
+    parameter changes and precision changes in between, gives bitwise the
+    eager results (same kernels, deterministic reductions)."""
+    tree, cov, prm = cases.synthetic_case(n=3000, npred=200, M=5, J=4, r=16, seed=61, clustered=True)
+    prm2 = dict(prm, sill_scale=prm["sill_scale"] * 1.3, range_scale=prm["range_scale"] * 0.7,
+                nugget_scale=prm["nugget_scale"] * 1.5)
+    eager = capi.Handle(tree)
+    graph = capi.Handle(tree)
+    try:
+        for h in (eager, graph):
+            h.set_covariance(cov)
+        graph.use_graph(True)
+        for rep in range(3):
+            for p in (prm, prm2):
+                for precision in (capi.PRECISION_FP64, capi.PRECISION_MIXED):
+                    for h in (eager, graph):
+                        h.set_params(**p)
+                        h.set_precision(precision)
+                    a = eager.likelihood()
+                    b = graph.likelihood()
+                    assert a == b, (rep, precision, a, b)
+                    pa = eager.predict()
+                    pb = graph.predict()
+                    assert pa[0] == pb[0]
+                    np.testing.assert_array_equal(pa[1], pb[1])
+                    np.testing.assert_array_equal(pa[2], pb[2])
+            assert graph.timings()[capi.T_GRAPH] > 0
+        Do, _, _ = oracle.evaluate(tree, cov, prm2, predict=False)
+        graph.set_params(**prm2)
+        graph.set_precision(capi.PRECISION_FP64)
+        assert rel(graph.likelihood(), Do) < TOL_D
+    finally:
+        eager.close()
+        graph.close()
