@@ -61,6 +61,106 @@ __device__ __forceinline__ double pcov(const CovCtx& c, const Pts& P, uint64_t i
     return (P.s[i] * Q.s[j]) * (ratio * sqrt(ratio)) * exp(-d / sqrt(bms));
 }
 
+// ---------------------------------------------------- covariance tiles
+// 1/sqrt(x) for x > 0: hardware estimate refined by two Newton steps
+// y <- y + y/2 (1 - x y^2) (each step squares the relative error).
+__device__ __forceinline__ double rsqrt_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    double e = __builtin_fma(-x * y, y, 1.0);
+    y = __builtin_fma(0.5 * y, e, y);
+    e = __builtin_fma(-x * y, y, 1.0);
+    return __builtin_fma(0.5 * y, e, y);
+}
+
+// Point data of one tile side: coordinates, q = s b^1.5 and b.  With
+// q_i = s_i b_i^1.5 and t = 1/sqrt(bms), bms = (b_i^2 + b_j^2)/2, the
+// nonstationary covariance s_i s_j (b_i b_j / bms)^1.5 exp(-d / sqrt(bms))
+// (src/evaluate_covariance.cpp:207-223) is q_i q_j t^3 exp(-d t).
+struct PtV {
+    double x, y, z, q, b;
+};
+
+__device__ __forceinline__ PtV load_pt(const CovCtx& c, const Pts& P, uint64_t i) {
+    PtV p;
+    p.x = P.x[i];
+    p.y = P.y[i];
+    p.z = P.z[i];
+    p.b = P.b[i];
+    p.q = c.stationary ? 0.0 : P.s[i] * (p.b * sqrt(p.b));
+    return p;
+}
+
+__device__ __forceinline__ double pcov_pt(const CovCtx& c, const PtV& a, const PtV& b) {
+    double d;
+    if (c.metric == 1) {
+        d = pdist(c, a.x, a.y, a.z, b.x, b.y, b.z);
+    } else {
+        const double dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        d = d2 > 0.0 ? d2 * rsqrt_nr(d2) : 0.0;
+        if (c.metric == 0) d *= 6371.0;
+    }
+    if (c.stationary) return c.sill * exp(-d / c.range);
+    const double t = rsqrt_nr((a.b * a.b + b.b * b.b) * 0.5);
+    return (a.q * b.q) * (t * t * t) * exp(-d * t);
+}
+
+// FP32 evaluation of the same form (mixed precision): inputs rounded to
+// float, hardware rsqrt (about 1 ulp) and expf.
+__device__ __forceinline__ float pcov_ptf(const CovCtx& c, const PtV& a, const PtV& b) {
+    float d;
+    if (c.metric == 1) {
+        d = (float)pdist(c, a.x, a.y, a.z, b.x, b.y, b.z);
+    } else {
+        const float dx = (float)b.x - (float)a.x, dy = (float)b.y - (float)a.y, dz = (float)b.z - (float)a.z;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        d = d2 > 0.0f ? d2 * __builtin_amdgcn_rsqf(d2) : 0.0f;
+        if (c.metric == 0) d *= 6371.0f;
+    }
+    if (c.stationary) return (float)c.sill * expf(-d / (float)c.range);
+    const float ab = (float)a.b, bb = (float)b.b;
+    const float t = __builtin_amdgcn_rsqf((ab * ab + bb * bb) * 0.5f);
+    return ((float)a.q * (float)b.q) * (t * t * t) * expf(-d * t);
+}
+
+// Fill the RP x NC tile C[i*LD + c] (i < RP, c < NC) with the covariance of
+// row points P[p0 + i] (valid i < nr) and column points Q[q0 + c] (valid
+// c < nc), zero outside.  diagv: rows and columns are the same point set and
+// the diagonal takes the variance v = sigma^2 + tau^2 instead.  Row data are
+// staged in `stage` (LDS, 5 * RP doubles); the caller synchronises after.
+template <int RP, int NC, typename T>
+__device__ __forceinline__ void cov_tile(T* C, int LD, const CovCtx& cc, const Pts& P, uint64_t p0, int nr,
+                                         const Pts& Q, uint64_t q0, int nc, bool diagv, double* stage) {
+    static_assert(MRA_WG % NC == 0, "tile width must divide the work-group size");
+    for (int i = threadIdx.x; i < RP; i += MRA_WG) {
+        if (i < nr) {
+            const PtV p = load_pt(cc, P, p0 + i);
+            stage[i] = p.x;
+            stage[RP + i] = p.y;
+            stage[2 * RP + i] = p.z;
+            stage[3 * RP + i] = p.q;
+            stage[4 * RP + i] = p.b;
+        }
+    }
+    const int c = threadIdx.x % NC;
+    PtV qc = {0, 0, 0, 0, 1};
+    if (c < nc) qc = load_pt(cc, Q, q0 + c);
+    __syncthreads();
+    for (int i = threadIdx.x / NC; i < RP; i += MRA_WG / NC) {
+        T v = T(0);
+        if (diagv && i == c) {
+            v = i < nr ? (T)P.v[p0 + i] : T(1);  // padding rows: identity
+        } else if (i < nr && c < nc) {
+            const PtV pr = {stage[i], stage[RP + i], stage[2 * RP + i], stage[3 * RP + i], stage[4 * RP + i]};
+            if constexpr (sizeof(T) == 4)
+                v = pcov_ptf(cc, pr, qc);
+            else
+                v = pcov_pt(cc, pr, qc);
+        }
+        C[i * LD + c] = v;
+    }
+}
+
 __device__ __forceinline__ float pcovf(const CovCtx& c, const Pts& P, uint64_t i, const Pts& Q, uint64_t j) {
     float x1 = (float)P.x[i], y1 = (float)P.y[i], z1 = (float)P.z[i];
     float x2 = (float)Q.x[j], y2 = (float)Q.y[j], z2 = (float)Q.z[j];
@@ -113,22 +213,55 @@ __device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int
         // uniform across the wave).
         constexpr int U = 8;
         int k0 = 0;
+        const bool full = !TA && ti0 * 16 + (tM2 ? 32 : 16) <= Mv && tj0 * 16 + (tN2 ? 32 : 16) <= Nv;
+        if (full) {
+            // every operand in range: wave-uniform row pointers plus 32-bit
+            // per-lane byte offsets, no predication
+            const uint32_t oa0 = (uint32_t)((k4 * lda + ia) * sizeof(TIA)), oa1 = oa0 + 16 * sizeof(TIA);
+            const uint32_t ob0 = (uint32_t)((k4 * ldb + ja) * sizeof(TIB)), ob1 = ob0 + 16 * sizeof(TIB);
+            for (; k0 + 4 * U <= Kv; k0 += 4 * U) {
+                TIA a0[U], a1[U];
+                TIB b0[U], b1[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const char* Ak = (const char*)(A + (int64_t)(k0 + 4 * u) * lda);
+                    const char* Bk = (const char*)(B + (int64_t)(k0 + 4 * u) * ldb);
+                    a0[u] = *(const TIA*)(Ak + oa0);
+                    if (tM2) a1[u] = *(const TIA*)(Ak + oa1);
+                    b0[u] = *(const TIB*)(Bk + ob0);
+                    if (tN2) b1[u] = *(const TIB*)(Bk + ob1);
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const double xa0 = (double)a0[u], xb0 = (double)b0[u];
+                    acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(xa0, xb0, acc00, 0, 0, 0);
+                    if (tN2) acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(xa0, (double)b1[u], acc01, 0, 0, 0);
+                    if (tM2) acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a1[u], xb0, acc10, 0, 0, 0);
+                    if (tM2 && tN2)
+                        acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a1[u], (double)b1[u], acc11, 0, 0, 0);
+                }
+            }
+        }
         for (; k0 + 4 * U <= Kv; k0 += 4 * U) {
-            double a0[U], a1[U], b0[U], b1[U];
+            // raw operands (input type) first, widened at use, so that a
+            // conversion does not wait on its load before the next load issues
+            TIA a0[U], a1[U];
+            TIB b0[U], b1[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int64_t k = k0 + 4 * u + k4;
-                a0[u] = ha ? (double)(TA ? A[ia * lda + k] : A[k * lda + ia]) : 0.0;
-                a1[u] = hb ? (double)(TA ? A[ib * lda + k] : A[k * lda + ib]) : 0.0;
-                b0[u] = hja ? (double)B[k * ldb + ja] : 0.0;
-                b1[u] = hjb ? (double)B[k * ldb + jb] : 0.0;
+                a0[u] = ha ? (TA ? A[ia * lda + k] : A[k * lda + ia]) : TIA(0);
+                a1[u] = hb ? (TA ? A[ib * lda + k] : A[k * lda + ib]) : TIA(0);
+                b0[u] = hja ? B[k * ldb + ja] : TIB(0);
+                b1[u] = hjb ? B[k * ldb + jb] : TIB(0);
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b0[u], acc00, 0, 0, 0);
-                if (tN2) acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b1[u], acc01, 0, 0, 0);
-                if (tM2) acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b0[u], acc10, 0, 0, 0);
-                if (tM2 && tN2) acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b1[u], acc11, 0, 0, 0);
+                const double xa0 = (double)a0[u], xa1 = (double)a1[u], xb0 = (double)b0[u], xb1 = (double)b1[u];
+                acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(xa0, xb0, acc00, 0, 0, 0);
+                if (tN2) acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(xa0, xb1, acc01, 0, 0, 0);
+                if (tM2) acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(xa1, xb0, acc10, 0, 0, 0);
+                if (tM2 && tN2) acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(xa1, xb1, acc11, 0, 0, 0);
             }
         }
         for (; k0 < Kv; k0 += 4) {
@@ -192,21 +325,23 @@ __device__ __forceinline__ void wg_gemm_tn_f32(TC* C, int ldc, int M, int N, int
         constexpr int U = 8;
         int k0 = 0;
         for (; k0 + 4 * U <= Kv; k0 += 4 * U) {
-            float a0[U], a1[U], b0[U], b1[U];
+            TIA a0[U], a1[U];
+            TIB b0[U], b1[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int64_t k = k0 + 4 * u + k4;
-                a0[u] = ha ? (float)A[k * lda + ia] : 0.0f;
-                a1[u] = hb ? (float)A[k * lda + ib] : 0.0f;
-                b0[u] = hja ? (float)B[k * ldb + ja] : 0.0f;
-                b1[u] = hjb ? (float)B[k * ldb + jb] : 0.0f;
+                a0[u] = ha ? A[k * lda + ia] : TIA(0);
+                a1[u] = hb ? A[k * lda + ib] : TIA(0);
+                b0[u] = hja ? B[k * ldb + ja] : TIB(0);
+                b1[u] = hjb ? B[k * ldb + jb] : TIB(0);
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                acc00 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[u], b0[u], acc00, 0, 0, 0);
-                if (tN2) acc01 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[u], b1[u], acc01, 0, 0, 0);
-                if (tM2) acc10 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[u], b0[u], acc10, 0, 0, 0);
-                if (tM2 && tN2) acc11 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[u], b1[u], acc11, 0, 0, 0);
+                const float xa0 = (float)a0[u], xa1 = (float)a1[u], xb0 = (float)b0[u], xb1 = (float)b1[u];
+                acc00 = __builtin_amdgcn_mfma_f32_16x16x4f32(xa0, xb0, acc00, 0, 0, 0);
+                if (tN2) acc01 = __builtin_amdgcn_mfma_f32_16x16x4f32(xa0, xb1, acc01, 0, 0, 0);
+                if (tM2) acc10 = __builtin_amdgcn_mfma_f32_16x16x4f32(xa1, xb0, acc10, 0, 0, 0);
+                if (tM2 && tN2) acc11 = __builtin_amdgcn_mfma_f32_16x16x4f32(xa1, xb1, acc11, 0, 0, 0);
             }
         }
         for (; k0 < Kv; k0 += 4) {
@@ -265,21 +400,22 @@ __device__ __forceinline__ void wg_gemm_tn_ksplit(double* C, int ldc, int M, int
     constexpr int U = 6;
     int k0 = kb;
     for (; k0 + 4 * U <= ke; k0 += 4 * U) {  // batched loads, one latency per U steps
-        double a0[U], a1[U], b0[U], b1[U];
+        TI a0[U], a1[U], b0[U], b1[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int64_t k = k0 + 4 * u + k4;
-            a0[u] = ha ? (double)A[k * lda + ia] : 0.0;
-            a1[u] = hb ? (double)A[k * lda + ib] : 0.0;
-            b0[u] = hja ? (double)B[k * ldb + ja] : 0.0;
-            b1[u] = hjb ? (double)B[k * ldb + jb] : 0.0;
+            a0[u] = ha ? A[k * lda + ia] : TI(0);
+            a1[u] = hb ? A[k * lda + ib] : TI(0);
+            b0[u] = hja ? B[k * ldb + ja] : TI(0);
+            b1[u] = hjb ? B[k * ldb + jb] : TI(0);
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b0[u], acc[0], 0, 0, 0);
-            if (tN2) acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b1[u], acc[1], 0, 0, 0);
-            if (tM2) acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b0[u], acc[2], 0, 0, 0);
-            if (tM2 && tN2) acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b1[u], acc[3], 0, 0, 0);
+            const double xa0 = (double)a0[u], xa1 = (double)a1[u], xb0 = (double)b0[u], xb1 = (double)b1[u];
+            acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa0, xb0, acc[0], 0, 0, 0);
+            if (tN2) acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa0, xb1, acc[1], 0, 0, 0);
+            if (tM2) acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa1, xb0, acc[2], 0, 0, 0);
+            if (tM2 && tN2) acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa1, xb1, acc[3], 0, 0, 0);
         }
     }
     for (; k0 < ke; k0 += 4) {

@@ -235,6 +235,7 @@ __global__ __launch_bounds__(256) void k_prior_coarse(Tree T, CovCtx cc, Pts K, 
     constexpr int LD = RP + 1;
     __shared__ double Cm[RP * LD];
     __shared__ double tmp[RP];
+    __shared__ double stage[5 * RP];
     __shared__ int bad;
     const uint64_t R = start_l + blockIdx.x;
     const int r = T.r;
@@ -242,10 +243,7 @@ __global__ __launch_bounds__(256) void k_prior_coarse(Tree T, CovCtx cc, Pts K, 
     if (threadIdx.x == 0) bad = 0;
     for (int j = 0; j < l; j++) {
         const uint64_t a = ancestor_up(R, l, j, T.J);
-        for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
-            const int i = idx / RP, c = idx % RP;
-            Cm[i * LD + c] = (i < r && c < r) ? pcov(cc, K, a * RP + i, K, R * RP + c) : 0.0;
-        }
+        cov_tile<RP, RP>(Cm, LD, cc, K, a * RP, r, K, R * RP, r, false, stage);
         __syncthreads();
         if (j > 0) {
             const double* Ga = Gc + goff[j] + (a - lvl_start(j, T.J)) * (uint64_t)j * RP * RP;
@@ -257,15 +255,7 @@ __global__ __launch_bounds__(256) void k_prior_coarse(Tree T, CovCtx cc, Pts K, 
                                  RP);
         __syncthreads();
     }
-    for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
-        const int i = idx / RP, c = idx % RP;
-        double v;
-        if (i < r && c < r)
-            v = (i == c) ? K.v[R * RP + i] : pcov(cc, K, R * RP + i, K, R * RP + c);
-        else
-            v = (i == c) ? 1.0 : 0.0;
-        Cm[i * LD + c] = v;
-    }
+    cov_tile<RP, RP>(Cm, LD, cc, K, R * RP, r, K, R * RP, r, true, stage);
     __syncthreads();
     if (l > 0) {
         wg_gemm_tn(Cm, LD, RP, RP, l * RP, -1.0, GR, RP, RP, GR, RP, RP);
@@ -298,6 +288,7 @@ __global__ __launch_bounds__(256) void k_finest_cross(Tree T_, CovCtx cc, Pts K,
     const Tree& T0 = T_;
     constexpr int LD = NT + 1;
     __shared__ T Cm[RP * LD];
+    __shared__ double stage[5 * RP];
     const uint32_t sg = items[2 * blockIdx.x], c0 = items[2 * blockIdx.x + 1];
     const uint64_t ob = T0.s_os[sg];
     const int nF = (int)T0.s_n[sg];
@@ -307,17 +298,7 @@ __global__ __launch_bounds__(256) void k_finest_cross(Tree T_, CovCtx cc, Pts K,
     T* BO = Bf + (uint64_t)T0.LRP * ob;
     for (int j = 0; j < L; j++) {
         const uint64_t a = ancestor_up(T0.s_parent[sg], L - 1, j, T0.J);
-        for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
-            const int i = idx / NT, c = idx % NT;
-            T v = 0;
-            if (i < r && c < n) {
-                if constexpr (sizeof(T) == 4)
-                    v = pcovf(cc, K, a * RP + i, O, ob + c0 + c);
-                else
-                    v = pcov(cc, K, a * RP + i, O, ob + c0 + c);
-            }
-            Cm[i * LD + c] = v;
-        }
+        cov_tile<RP, NT, T>(Cm, LD, cc, K, a * RP, r, O, ob + c0, n, false, stage);
         __syncthreads();
         if (j > 0) {
             const double* Ga = Gc + goff[j] + (a - lvl_start(j, T0.J)) * (uint64_t)j * RP * RP;
@@ -723,7 +704,8 @@ __global__ __launch_bounds__(256) void k_post_lowrank(Tree T, LowRankArgs a) {
     __shared__ double zbuf[NBL];
     __shared__ double red[MRA_WG];
     __shared__ int bad;
-    const uint32_t loc = a.items[blockIdx.x];
+    const uint32_t bid = blockIdx.x;
+    const uint32_t loc = a.items[bid];
     const uint64_t R = a.start_l + loc;
     const uint64_t os = a.row_s[loc];
     const int ns = (int)(a.row_e[loc] - os);
@@ -737,7 +719,7 @@ __global__ __launch_bounds__(256) void k_post_lowrank(Tree T, LowRankArgs a) {
     const int dim = max(ns, RP);
     const int ldw = small ? LDS_LD : dim;
     const int ldo = small ? 65 : dim;  // row stride of the output tile in B
-    double* A = small ? bufA : a.ws + a.ws_off[blockIdx.x];
+    double* A = small ? bufA : a.ws + a.ws_off[bid];
     double* B = small ? bufB : A + (uint64_t)dim * ldw;
     double* zs = small ? zbuf : B + (uint64_t)dim * ldw;
     const double* Xr = a.X + os * (uint64_t)a.ldx;
