@@ -216,6 +216,10 @@ int mra_selftest(int K, int Mv, int Nv, const double* A, const double* B, const 
 /* FP32 variant of the GEMM self test: Cout = C0 - A^T B with the FP32
  * matrix-core GEMM and FP32 accumulation (no reference counterpart). */
 int mra_selftest_f32(int K, int Mv, int Nv, const float* A, const float* B, const float* C0, float* Cout);
+/* Blocked LDS Cholesky / triangular inverse on one n x n SPD matrix S
+ * (n = 16, 32, 48, 64 or 128; row-major): Lout = lower Cholesky factor,
+ * Xout = L^{-1} (lower, zero strict upper), *bad = 1 on a non-positive pivot. */
+int mra_selftest_blk(int n, const double* S, double* Lout, double* Xout, int* bad);
 
 /* Library version string. */
 const char* mra_version(void);

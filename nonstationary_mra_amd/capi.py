@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "libmra_hip.so")
+# MRA_LIBRARY selects another build of the same library (same C ABI)
+LIB_PATH = os.environ.get("MRA_LIBRARY") or os.path.join(HERE, "_build", "libmra_hip.so")
 
 MRA_OK, MRA_E_ARG, MRA_E_HIP, MRA_E_NOT_PD, MRA_E_STATE, MRA_E_NOMEM, MRA_E_UNSUPPORTED = 0, -1, -2, -3, -4, -5, -6
 METRIC = {"chordal": 0, "great_circle": 1, "planar": 2}
@@ -26,7 +27,7 @@ EXPORTED = ["mra_create", "mra_destroy", "mra_last_error", "mra_set_covariance",
             "mra_set_precision", "mra_likelihood", "mra_predict", "mra_likelihood_levels", "mra_sample_prior",
             "mra_set_observations", "mra_use_graph", "mra_last_timings", "mra_cross_covariance",
             "mra_variance_covariance", "mra_cov_tile_device", "mra_partition_program_sizes",
-            "mra_partition_program", "mra_version", "mra_selftest", "mra_set_option", "mra_get_option", "mra_selftest_f32"]
+            "mra_partition_program", "mra_version", "mra_selftest", "mra_set_option", "mra_get_option", "mra_selftest_f32", "mra_selftest_blk"]
 OPT_FACTORED_POSTERIOR = 1
 
 
@@ -95,6 +96,7 @@ def lib():
         fp = C.POINTER(C.c_float)
         _lib.mra_selftest_f32.argtypes = [C.c_int, C.c_int, C.c_int, fp, fp, fp, fp]
         _lib.mra_selftest.argtypes = [C.c_int, C.c_int, C.c_int] + [dp] * 8 + [C.POINTER(C.c_int)]
+        _lib.mra_selftest_blk.argtypes = [C.c_int, dp, dp, dp, C.POINTER(C.c_int)]
     return _lib
 
 
@@ -106,6 +108,16 @@ def selftest(K, Mv, Nv, A, B, C0, S, Bt):
     check(lib().mra_selftest(K, Mv, Nv, keep.d(A), keep.d(B), keep.d(C0), keep.d(S), keep.d(Bt), _d(Co), _d(Lo),
                              _d(Xo), C.byref(bad)))
     return Co.reshape(64, 64), Lo.reshape(64, 64), Xo.reshape(64, 64), bad.value
+
+
+def selftest_blk(S):
+    """Blocked Cholesky / inverse self test (see mra_selftest_blk in include/mra.h)."""
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    n = S.shape[0]
+    Lo, Xo = np.zeros((n, n)), np.zeros((n, n))
+    bad = C.c_int(0)
+    check(lib().mra_selftest_blk(n, _d(S), _d(Lo), _d(Xo), C.byref(bad)))
+    return Lo, Xo, bad.value
 
 
 def selftest_f32(K, Mv, Nv, A, B, C0):

@@ -188,9 +188,15 @@ __device__ __forceinline__ float pcovf(const CovCtx& c, const Pts& P, uint64_t i
 // must be made visible by the caller (__syncthreads before and after).
 // TA: A is stored transposed, A(k,i) = A[i*lda+k] (an LDS matrix read by
 // columns).  ACC = false stores C = alpha*A^T B instead of accumulating.
-template <bool TA = false, bool ACC = true, typename TIA = double, typename TIB = double>
+// TILES: GEMM_ANY checks per wave block whether every operand is in range
+// and then takes the predication-free path; GEMM_FULL asserts that
+// Mv >= M and Nv >= N (only that path is compiled); GEMM_PART compiles only
+// the predicated path (smaller register footprint).
+enum { GEMM_ANY = 0, GEMM_FULL = 1, GEMM_PART = 2 };
+template <bool TA = false, bool ACC = true, typename TIA = double, typename TIB = double, int TILES = GEMM_PART>
 __device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int Kv, double alpha, const TIA* A,
                                            int64_t lda, int Mv, const TIB* B, int64_t ldb, int Nv) {
+    static_assert(!(TA && TILES == GEMM_FULL), "the full-tile path reads A untransposed");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r16 = lane & 15, k4 = lane >> 4;
     const int tM = M >> 4, tN = N >> 4;
@@ -213,8 +219,9 @@ __device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int
         // uniform across the wave).
         constexpr int U = 8;
         int k0 = 0;
-        const bool full = !TA && ti0 * 16 + (tM2 ? 32 : 16) <= Mv && tj0 * 16 + (tN2 ? 32 : 16) <= Nv;
-        if (full) {
+        const bool full = TILES == GEMM_FULL ||
+                          (TILES == GEMM_ANY && !TA && ti0 * 16 + (tM2 ? 32 : 16) <= Mv && tj0 * 16 + (tN2 ? 32 : 16) <= Nv);
+        if (TILES != GEMM_PART && full) {
             // every operand in range: wave-uniform row pointers plus 32-bit
             // per-lane byte offsets, no predication
             const uint32_t oa0 = (uint32_t)((k4 * lda + ia) * sizeof(TIA)), oa1 = oa0 + 16 * sizeof(TIA);
@@ -242,13 +249,14 @@ __device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int
                 }
             }
         }
-        for (; k0 + 4 * U <= Kv; k0 += 4 * U) {
+        constexpr int UP = 4;
+        for (; TILES != GEMM_FULL && k0 + 4 * UP <= Kv; k0 += 4 * UP) {
             // raw operands (input type) first, widened at use, so that a
             // conversion does not wait on its load before the next load issues
-            TIA a0[U], a1[U];
-            TIB b0[U], b1[U];
+            TIA a0[UP], a1[UP];
+            TIB b0[UP], b1[UP];
 #pragma unroll
-            for (int u = 0; u < U; u++) {
+            for (int u = 0; u < UP; u++) {
                 const int64_t k = k0 + 4 * u + k4;
                 a0[u] = ha ? (TA ? A[ia * lda + k] : A[k * lda + ia]) : TIA(0);
                 a1[u] = hb ? (TA ? A[ib * lda + k] : A[k * lda + ib]) : TIA(0);
@@ -256,7 +264,7 @@ __device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int
                 b1[u] = hjb ? B[k * ldb + jb] : TIB(0);
             }
 #pragma unroll
-            for (int u = 0; u < U; u++) {
+            for (int u = 0; u < UP; u++) {
                 const double xa0 = (double)a0[u], xa1 = (double)a1[u], xb0 = (double)b0[u], xb1 = (double)b1[u];
                 acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(xa0, xb0, acc00, 0, 0, 0);
                 if (tN2) acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(xa0, xb1, acc01, 0, 0, 0);
@@ -489,6 +497,225 @@ __device__ __forceinline__ void wg_chol(double* A, int lda, int n, int* bad) {
     for (int k = threadIdx.x; k < n; k += MRA_WG) {
         const double akk = A[k * lda + k];
         A[k * lda + k] = (akk > 0.0) ? sqrt(akk) : 1.0;
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------ blocked LDS factorisations
+// N x N (N a multiple of 16, at most 128) lower Cholesky and triangular
+// inverse on 16 x 16 blocks.  One wave factors each diagonal block and
+// inverts it (wave-synchronous, no work-group barrier); panel solves, trailing
+// updates and the off-diagonal blocks of the inverse are 16 x 16 x 16 MFMA
+// products spread over the 4 waves.  The 16 x 16 inverse diagonal blocks
+// D_b = L_bb^{-1} are kept in the strictly upper blocks (b, b+1) and
+// (0, N/16-1) of the matrix itself when N >= 48 (those entries are free),
+// else in the caller's buffer dbuf (N/16 blocks of 256 doubles).
+// Barriers: 3 per block column for the factorisation, 2 per block row for
+// the inverse.
+
+// compiler-only ordering of LDS accesses made by one wave (LDS executes a
+// wave's instructions in order)
+__device__ __forceinline__ void wave_lds_fence() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
+
+template <int N>
+struct BlkLayout {
+    static constexpr int NB = N / 16;
+    static constexpr bool UPPER = NB >= 3;
+    __device__ static double* dptr(double* A, int lda, double* dbuf, int b) {
+        if (UPPER) return b < NB - 1 ? A + b * 16 * lda + (b + 1) * 16 : A + (NB - 1) * 16;
+        return dbuf + b * 256;
+    }
+    __device__ static int ldd(int lda) { return UPPER ? lda : 16; }
+};
+
+// 16 x 16 x 16 product on one wave: acc[v] (row k4 + 4v, column r16) +=
+// sum_k X(row, k) Y(k, column) with X(m, k) = X[m*ldx + k] and
+// Y(k, n) = TY ? Y[n*ldy + k] : Y[k*ldy + n].
+template <bool TY>
+__device__ __forceinline__ mra_d4 mfma16(mra_d4 acc, const double* X, int ldx, const double* Y, int ldy) {
+    const int lane = threadIdx.x & 63, r16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const int k = 4 * s + k4;
+        const double a = X[r16 * ldx + k];
+        const double b = TY ? Y[r16 * ldy + k] : Y[k * ldy + r16];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// One wave: factor the 16 x 16 diagonal block P in place (lower triangle)
+// and write its inverse (lower, zeros above) to Dp (row stride ldd).
+__device__ __forceinline__ void wave_chol16(double* P, int lda, double* Dp, int ldd, int* bad) {
+    const int lane = threadIdx.x & 63, i = lane & 15, cg = lane >> 4;
+    // unscaled right-looking elimination of the lower triangle:
+    // A[i][j] -= A[i][k] A[j][k] / A[k][k]  (k < j <= i)
+    for (int k = 0; k < 15; k++) {
+        wave_lds_fence();
+        double akk = P[k * lda + k];
+        if (!(akk > 0.0)) akk = 1.0;
+        const double inv = 1.0 / akk;
+        const double aik = P[i * lda + k];
+        double upd[4];
+        bool act[4];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const int j = cg + 4 * m;
+            act[m] = j > k && j <= i;
+            upd[m] = act[m] ? P[i * lda + j] - aik * P[j * lda + k] * inv : 0.0;
+        }
+        wave_lds_fence();
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+            if (act[m]) P[i * lda + cg + 4 * m] = upd[m];
+    }
+    wave_lds_fence();
+    // pivots are final: L[i][j] = A[i][j] / sqrt(A[j][j]), L[i][i] = sqrt(A[i][i])
+    double val[4], piv[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const int j = cg + 4 * m;
+        val[m] = j <= i ? P[i * lda + j] : 0.0;
+        piv[m] = P[j * lda + j];
+    }
+    wave_lds_fence();
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const int j = cg + 4 * m;
+        double pj = piv[m];
+        if (!(pj > 0.0)) {
+            if (j == i) *bad = 1;
+            pj = 1.0;
+        }
+        if (j < i) P[i * lda + j] = val[m] / sqrt(pj);
+        if (j == i) P[i * lda + i] = sqrt(pj);
+    }
+    wave_lds_fence();
+    // inverse by forward substitution, one column per lane (lanes 0..15);
+    // the partial column lives in Dp (rolled loops keep register use low)
+    if (lane < 16) {
+        const int c = lane;
+#pragma unroll 1
+        for (int ii = 0; ii < 16; ii++) {
+            double sacc = ii == c ? 1.0 : 0.0;
+#pragma unroll 1
+            for (int kk = 0; kk < ii; kk++) sacc -= P[ii * lda + kk] * Dp[kk * ldd + c];
+            Dp[ii * ldd + c] = sacc / P[ii * lda + ii];
+        }
+    }
+    wave_lds_fence();
+}
+
+template <int N>
+__device__ __forceinline__ void wg_chol_blk(double* A, int lda, double* dbuf, int* bad) {
+    using BL = BlkLayout<N>;
+    constexpr int NB = BL::NB;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, k4 = lane >> 4;
+    const int ldd = BL::ldd(lda);
+    __syncthreads();
+    for (int kb = 0; kb < NB; kb++) {
+        double* Pkk = A + kb * 16 * (lda + 1);
+        double* Dk = BL::dptr(A, lda, dbuf, kb);
+        if (wave == 0) wave_chol16(Pkk, lda, Dk, ldd, bad);
+        __syncthreads();
+        if (kb == NB - 1) break;
+        // panel: L_{ib,kb} = A_{ib,kb} D_kb^T
+        for (int ib = kb + 1 + wave; ib < NB; ib += 4) {
+            double* Pib = A + ib * 16 * lda + kb * 16;
+            mra_d4 acc = {0, 0, 0, 0};
+            acc = mfma16<true>(acc, Pib, lda, Dk, ldd);
+            wave_lds_fence();
+#pragma unroll
+            for (int v = 0; v < 4; v++) Pib[(k4 + 4 * v) * lda + r16] = acc[v];
+        }
+        __syncthreads();
+        // trailing update A_{ib,jb} -= L_{ib,kb} L_{jb,kb}^T, kb < jb <= ib
+        const int m = NB - 1 - kb;
+        for (int q = wave; q < m * (m + 1) / 2; q += 4) {
+            int ib = 0, rem = q;
+            while (rem > ib) {
+                rem -= ib + 1;
+                ib++;
+            }
+            const int jb = rem;  // 0 <= jb <= ib (relative)
+            const double* Li = A + (kb + 1 + ib) * 16 * lda + kb * 16;
+            const double* Lj = A + (kb + 1 + jb) * 16 * lda + kb * 16;
+            mra_d4 acc = {0, 0, 0, 0};
+            acc = mfma16<true>(acc, Li, lda, Lj, lda);
+            double* Cb = A + (kb + 1 + ib) * 16 * lda + (kb + 1 + jb) * 16;
+#pragma unroll
+            for (int v = 0; v < 4; v++) Cb[(k4 + 4 * v) * lda + r16] -= acc[v];
+        }
+        __syncthreads();
+    }
+}
+
+// A <- L^{-1} in place after wg_chol_blk<N> (same dbuf); the strictly upper
+// part is set to zero.
+template <int N>
+__device__ __forceinline__ void wg_trinv_blk(double* A, int lda, double* dbuf) {
+    using BL = BlkLayout<N>;
+    constexpr int NB = BL::NB;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, k4 = lane >> 4;
+    const int ldd = BL::ldd(lda);
+    __syncthreads();
+    for (int i = 1; i < NB; i++) {
+        // T_ij = sum_{k=j}^{i-1} L_ik X_kj (X_jj = D_j; X_kj, k > j, final),
+        // at most two blocks j per wave
+        mra_d4 acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const int j = wave + 4 * t;
+            if (j < i)
+                for (int k = j; k < i; k++) {
+                    const double* Lik = A + i * 16 * lda + k * 16;
+                    const double* Xkj = k == j ? BL::dptr(A, lda, dbuf, j) : A + k * 16 * lda + j * 16;
+                    acc[t] = mfma16<false>(acc[t], Lik, lda, Xkj, k == j ? ldd : lda);
+                }
+        }
+        __syncthreads();  // every L_ij of block row i has been read
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const int j = wave + 4 * t;
+            if (j < i) {
+                double* Cb = A + i * 16 * lda + j * 16;
+#pragma unroll
+                for (int v = 0; v < 4; v++) Cb[(k4 + 4 * v) * lda + r16] = acc[t][v];
+                wave_lds_fence();
+                // X_ij = -D_i T_ij
+                mra_d4 x = {0, 0, 0, 0};
+                x = mfma16<false>(x, BL::dptr(A, lda, dbuf, i), ldd, Cb, lda);
+                wave_lds_fence();
+#pragma unroll
+                for (int v = 0; v < 4; v++) Cb[(k4 + 4 * v) * lda + r16] = -x[v];
+            }
+        }
+        __syncthreads();
+    }
+    // diagonal blocks <- D_b (read every D_b before any upper block is zeroed)
+    constexpr int PER = (NB * 256 + MRA_WG - 1) / MRA_WG;
+    double dv[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int idx = threadIdx.x + q * MRA_WG;
+        dv[q] = 0.0;
+        if (idx < NB * 256) {
+            const int b = idx >> 8, e = idx & 255;
+            dv[q] = BL::dptr(A, lda, dbuf, b)[(e >> 4) * ldd + (e & 15)];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int idx = threadIdx.x + q * MRA_WG;
+        if (idx < NB * 256) {
+            const int b = idx >> 8, e = idx & 255, ii = e >> 4, jj = e & 15;
+            A[(b * 16 + ii) * lda + b * 16 + jj] = jj <= ii ? dv[q] : 0.0;
+        }
+    }
+    for (int idx = threadIdx.x; idx < N * N; idx += MRA_WG) {
+        const int ii = idx / N, jj = idx % N;
+        if ((jj >> 4) > (ii >> 4)) A[ii * lda + jj] = 0.0;
     }
     __syncthreads();
 }

@@ -1297,3 +1297,37 @@ extern "C" int mra_selftest_f32(int K, int Mv, int Nv, const float* A, const flo
     for (void* p : l) (void)hipFree(p);
     return MRA_OK;
 }
+
+extern "C" int mra_selftest_blk(int n, const double* S, double* Lout, double* Xout, int* bad) {
+    mra_handle tmp;
+    mra_handle* h = &tmp;
+    if (!S || !Lout || !Xout || !bad || (n != 16 && n != 32 && n != 48 && n != 64 && n != 128)) return MRA_E_ARG;
+    std::vector<void*> l;
+    double *dS, *dL, *dX;
+    int* dbad;
+    const size_t m = (size_t)n * n;
+    struct Free {
+        std::vector<void*>& l;
+        ~Free() {
+            for (void* p : l) (void)hipFree(p);
+        }
+    } fr{l};
+    RC_TRY(dalloc(h, &dS, m, l));
+    RC_TRY(dalloc(h, &dL, m, l));
+    RC_TRY(dalloc(h, &dX, m, l));
+    RC_TRY(dalloc(h, &dbad, 1, l));
+    HIP_TRY(hipMemcpy(dS, S, sizeof(double) * m, hipMemcpyHostToDevice));
+    switch (n) {
+        case 16: k_selftest_blk<16><<<1, 256>>>(dS, dL, dX, dbad); break;
+        case 32: k_selftest_blk<32><<<1, 256>>>(dS, dL, dX, dbad); break;
+        case 48: k_selftest_blk<48><<<1, 256>>>(dS, dL, dX, dbad); break;
+        case 64: k_selftest_blk<64><<<1, 256>>>(dS, dL, dX, dbad); break;
+        default: k_selftest_blk<128><<<1, 256>>>(dS, dL, dX, dbad); break;
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(Lout, dL, sizeof(double) * m, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(Xout, dX, sizeof(double) * m, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(bad, dbad, sizeof(int), hipMemcpyDeviceToHost));
+    return MRA_OK;
+}

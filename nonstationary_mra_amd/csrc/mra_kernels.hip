@@ -234,8 +234,8 @@ __global__ __launch_bounds__(256) void k_prior_coarse(Tree T, CovCtx cc, Pts K, 
                                                       int* bad_flag) {
     constexpr int LD = RP + 1;
     __shared__ double Cm[RP * LD];
-    __shared__ double tmp[RP];
     __shared__ double stage[5 * RP];
+    __shared__ double dbuf[RP >= 48 ? 1 : RP * 16];
     __shared__ int bad;
     const uint64_t R = start_l + blockIdx.x;
     const int r = T.r;
@@ -247,26 +247,26 @@ __global__ __launch_bounds__(256) void k_prior_coarse(Tree T, CovCtx cc, Pts K, 
         __syncthreads();
         if (j > 0) {
             const double* Ga = Gc + goff[j] + (a - lvl_start(j, T.J)) * (uint64_t)j * RP * RP;
-            wg_gemm_tn(Cm, LD, RP, RP, j * RP, -1.0, Ga, RP, RP, GR, RP, RP);
+            wg_gemm_tn<false, true, double, double, GEMM_FULL>(Cm, LD, RP, RP, j * RP, -1.0, Ga, RP, RP, GR, RP, RP);
             __syncthreads();
         }
         // G_{R,j} = L_a^{-1} B, stored straight to global memory
-        wg_gemm_tn<false, false>(GR + (uint64_t)j * RP * RP, RP, RP, RP, RP, 1.0, LiT + a * RP * RP, RP, RP, Cm, LD,
-                                 RP);
+        wg_gemm_tn<false, false, double, double, GEMM_FULL>(GR + (uint64_t)j * RP * RP, RP, RP, RP, RP, 1.0,
+                                                            LiT + a * RP * RP, RP, RP, Cm, LD, RP);
         __syncthreads();
     }
     cov_tile<RP, RP>(Cm, LD, cc, K, R * RP, r, K, R * RP, r, true, stage);
     __syncthreads();
     if (l > 0) {
-        wg_gemm_tn(Cm, LD, RP, RP, l * RP, -1.0, GR, RP, RP, GR, RP, RP);
+        wg_gemm_tn<false, true, double, double, GEMM_FULL>(Cm, LD, RP, RP, l * RP, -1.0, GR, RP, RP, GR, RP, RP);
         __syncthreads();
     }
-    wg_chol(Cm, LD, RP, &bad);
+    wg_chol_blk<RP>(Cm, LD, dbuf, &bad);
     for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
         const int k = idx / RP, i = idx % RP;  // column-major: [k*RP + i] = L[i][k]
         Lt[R * RP * RP + idx] = (i >= k) ? Cm[i * LD + k] : 0.0;
     }
-    wg_trinv(Cm, LD, RP, tmp);
+    wg_trinv_blk<RP>(Cm, LD, dbuf);
     for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
         const int k = idx / RP, i = idx % RP;
         LiT[R * RP * RP + idx] = (i >= k) ? Cm[i * LD + k] : 0.0;
@@ -305,7 +305,8 @@ __global__ __launch_bounds__(256) void k_finest_cross(Tree T_, CovCtx cc, Pts K,
             if constexpr (sizeof(T) == 4)
                 wg_gemm_tn_f32(Cm, LD, RP, NT, j * RP, -1.0f, Ga, RP, RP, GF + c0, nF, n);
             else
-                wg_gemm_tn(Cm, LD, RP, NT, j * RP, -1.0, Ga, RP, RP, GF + c0, nF, n);
+                wg_gemm_tn<false, true, double, double, GEMM_ANY>(Cm, LD, RP, NT, j * RP, -1.0, Ga, RP, RP, GF + c0,
+                                                                  nF, n);
             __syncthreads();
         }
         // B is stored observation-major: row o holds B_{F,0..L-1}(:, o)
@@ -318,8 +319,8 @@ __global__ __launch_bounds__(256) void k_finest_cross(Tree T_, CovCtx cc, Pts K,
             wg_gemm_tn_f32<false>(GF + (uint64_t)(j * RP) * nF + c0, nF, RP, NT, RP, 1.0f, LiT + a * RP * RP, RP, RP,
                                   (const float*)Cm, LD, n);
         else
-            wg_gemm_tn<false, false>(GF + (uint64_t)(j * RP) * nF + c0, nF, RP, NT, RP, 1.0, LiT + a * RP * RP, RP,
-                                     RP, (const double*)Cm, LD, n);
+            wg_gemm_tn<false, false, double, double, GEMM_ANY>(GF + (uint64_t)(j * RP) * nF + c0, nF, RP, NT, RP, 1.0,
+                                                               LiT + a * RP * RP, RP, RP, (const double*)Cm, LD, n);
         __syncthreads();
     }
 }
@@ -492,9 +493,9 @@ __device__ __forceinline__ void post_add_A(const Tree& T, const PostArgs& a, uin
         const int ns = (int)(oe - os);
         const double* X = a.X + os * a.ldx;
         if (!transpose)
-            wg_gemm_tn(C, ld, RP, RP, ns, 1.0, X + j * RP, a.ldx, RP, X + k * RP, a.ldx, RP);
+            wg_gemm_tn<false, true, double, double, GEMM_FULL>(C, ld, RP, RP, ns, 1.0, X + j * RP, a.ldx, RP, X + k * RP, a.ldx, RP);
         else
-            wg_gemm_tn(C, ld, RP, RP, ns, 1.0, X + k * RP, a.ldx, RP, X + j * RP, a.ldx, RP);
+            wg_gemm_tn<false, true, double, double, GEMM_FULL>(C, ld, RP, RP, ns, 1.0, X + k * RP, a.ldx, RP, X + j * RP, a.ldx, RP);
         if (a.cslot) {
             // children that sent a dense message (their rows are zero)
             __syncthreads();
@@ -567,6 +568,7 @@ __global__ __launch_bounds__(256) void k_posterior(Tree T, PostArgs a) {
     __shared__ double vv[RP];
     __shared__ double wl[RP];
     __shared__ double red[MRA_WG];
+    __shared__ double dbuf[RP >= 48 ? 1 : RP * 16];
     __shared__ int bad;
     const int l = a.l;
     const uint64_t R = a.start_l + a.items[blockIdx.x];
@@ -580,37 +582,38 @@ __global__ __launch_bounds__(256) void k_posterior(Tree T, PostArgs a) {
         for (uint64_t idx = threadIdx.x; idx < (oe - os) * a.ldxz; idx += MRA_WG) a.Xz[os * a.ldxz + idx] = 0.0;
         for (uint64_t o = os + threadIdx.x; o < oe; o += MRA_WG) a.zz[o] = 0.0;
     }
-    for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
-        Km[(idx / RP) * LD + idx % RP] = 0.0;
-        Wm[(idx / RP) * LD + idx % RP] = ((idx / RP) == (idx % RP)) ? 1.0 : 0.0;
-    }
+    for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) Km[(idx / RP) * LD + idx % RP] = 0.0;
     __syncthreads();
     // L L^T
-    wg_gemm_tn(Km, LD, RP, RP, RP, 1.0, LR, RP, RP, LR, RP, RP);
+    wg_gemm_tn<false, true, double, double, GEMM_FULL>(Km, LD, RP, RP, RP, 1.0, LR, RP, RP, LR, RP, RP);
     __syncthreads();
     post_add_A<RP, FROM_OBS>(T, a, R, l, l, Km, LD, false);
     __syncthreads();
     double lp = 0.0;
     for (int i = threadIdx.x; i < r; i += MRA_WG) lp += log(LR[i * RP + i]);
     const double logdet_prior = wg_sum(lp, red);
-    wg_chol(Km, LD, RP, &bad);
-    wg_trsm(Km, LD, Wm, LD, RP, RP);  // Wm = L~^{-1}
+    wg_chol_blk<RP>(Km, LD, dbuf, &bad);
+    double lq = 0.0;
+    for (int i = threadIdx.x; i < r; i += MRA_WG) lq += 2.0 * log(Km[i * LD + i]);
     for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
         const int k = idx / RP, i = idx % RP;
         a.Lpt[R * RP * RP + idx] = (i >= k) ? Km[i * LD + k] : 0.0;
-        a.LpiT[R * RP * RP + idx] = (i >= k) ? Wm[i * LD + k] : 0.0;
+    }
+    wg_trinv_blk<RP>(Km, LD, dbuf);  // Km = L~^{-1}
+    for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
+        const int k = idx / RP, i = idx % RP;
+        a.LpiT[R * RP * RP + idx] = (i >= k) ? Km[i * LD + k] : 0.0;
     }
     for (int q = threadIdx.x; q < RP; q += MRA_WG) wl[q] = (q < r) ? post_w<RP, FROM_OBS>(T, a, R, l, q) : 0.0;
     __syncthreads();
     for (int i = threadIdx.x; i < RP; i += MRA_WG) {
         double s = 0.0;
-        for (int k = 0; k <= i; k++) s += Wm[i * LD + k] * wl[k];
+        for (int k = 0; k <= i; k++) s += Km[i * LD + k] * wl[k];
         vv[i] = s;
         a.v[R * RP + i] = s;
     }
     __syncthreads();
-    double lq = 0.0;
-    for (int i = threadIdx.x; i < r; i += MRA_WG) lq += 2.0 * log(Km[i * LD + i]) - vv[i] * vv[i];
+    for (int i = threadIdx.x; i < r; i += MRA_WG) lq -= vv[i] * vv[i];
     const double post_part = wg_sum(lq, red);
     if (threadIdx.x == 0 && a.write_d) a.dterm[R] = -2.0 * logdet_prior + post_part;
     if (l > 0) {
@@ -624,7 +627,7 @@ __global__ __launch_bounds__(256) void k_posterior(Tree T, PostArgs a) {
             __syncthreads();
             for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) Wm[(idx / RP) * LD + idx % RP] = 0.0;
             __syncthreads();
-            wg_gemm_tn(Wm, LD, RP, RP, RP, 1.0, LpiTR, RP, RP, Km, LD, RP);
+            wg_gemm_tn<false, true, double, double, GEMM_FULL>(Wm, LD, RP, RP, RP, 1.0, LpiTR, RP, RP, Km, LD, RP);
             __syncthreads();
             double* Pj = PR + (uint64_t)j * RP * RP;
             for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) Pj[idx] = Wm[(idx / RP) * LD + idx % RP];
@@ -1062,6 +1065,27 @@ __global__ __launch_bounds__(256) void k_selftest(const double* A, const double*
         Lout[idx] = Cm[(idx / 64) * LD + idx % 64];
         Xout[idx] = Wm[(idx / 64) * LD + idx % 64];
     }
+    if (threadIdx.x == 0) *bad_out = bad;
+}
+
+// Blocked Cholesky / triangular inverse on one N x N matrix S (row-major):
+// Lout = lower factor (strict upper reported as zero), Xout = L^{-1} as left
+// in place (its strict upper part must be zero).
+template <int N>
+__global__ __launch_bounds__(256) void k_selftest_blk(const double* S, double* Lout, double* Xout, int* bad_out) {
+    constexpr int LD = N + 1;
+    __shared__ double Cm[N * LD];
+    __shared__ double dbuf[N >= 48 ? 1 : N * 16];
+    __shared__ int bad;
+    if (threadIdx.x == 0) bad = 0;
+    for (int idx = threadIdx.x; idx < N * N; idx += MRA_WG) Cm[(idx / N) * LD + idx % N] = S[idx];
+    wg_chol_blk<N>(Cm, LD, dbuf, &bad);
+    for (int idx = threadIdx.x; idx < N * N; idx += MRA_WG) {
+        const int i = idx / N, k = idx % N;
+        Lout[idx] = k <= i ? Cm[i * LD + k] : 0.0;
+    }
+    wg_trinv_blk<N>(Cm, LD, dbuf);
+    for (int idx = threadIdx.x; idx < N * N; idx += MRA_WG) Xout[idx] = Cm[(idx / N) * LD + idx % N];
     if (threadIdx.x == 0) *bad_out = bad;
 }
 

@@ -43,3 +43,24 @@ def test_gemm_f32(K, Mv, Nv):
     bound = (K + 2) * 2.0 ** -24 * (np.abs(A).T @ np.abs(B) + np.abs(C0)).max()
     print("f32 gemm max err", np.abs(Co - ref).max(), "bound", bound, flush=True)
     assert np.abs(Co - ref).max() <= bound
+
+
+@pytest.mark.parametrize("n", [16, 32, 48, 64, 128])
+def test_blocked_cholesky_and_inverse(n):
+    """Blocked LDS Cholesky (16 x 16 blocks, MFMA panel and trailing updates)
+    and in-place triangular inverse against numpy; the strict upper part of
+    the inverse is zero; an indefinite matrix raises the pivot flag."""
+    rng = np.random.default_rng(n)
+    X = rng.standard_normal((n, n))
+    S = X @ X.T / n + 0.5 * np.eye(n)
+    Lo, Xo, bad = capi.selftest_blk(S)
+    Lr = np.linalg.cholesky(S)
+    assert bad == 0
+    print("blk chol err", np.abs(Lo - Lr).max(), "inv err", np.abs(Xo - np.linalg.inv(Lr)).max(), flush=True)
+    np.testing.assert_allclose(Lo, Lr, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(Xo, np.linalg.inv(Lr), rtol=0, atol=1e-10 * np.abs(np.linalg.inv(Lr)).max())
+    assert np.all(np.triu(Xo, 1) == 0.0)
+    S2 = S.copy()
+    S2[n - 3, n - 3] = -1.0
+    _, _, bad2 = capi.selftest_blk(S2)
+    assert bad2 == 1
