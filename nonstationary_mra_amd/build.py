@@ -16,8 +16,10 @@ ROOT = os.path.dirname(HERE)
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MRA_OFFLOAD_ARCH", "gfx950")
+# -amdgpu-mfma-vgpr-form keeps MFMA accumulators in VGPRs (no AGPR copies
+# around every matrix instruction in loops whose accumulators are carried)
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
-             "-packed-fp32-ops", "-I", os.path.join(ROOT, "include")]
+             "-packed-fp32-ops", "-mllvm", "-amdgpu-mfma-vgpr-form", "-I", os.path.join(ROOT, "include")]
 HIP_SOURCES = ["mra_engine.hip"]
 CPP_SOURCES = ["mra_partition.cpp"]
 

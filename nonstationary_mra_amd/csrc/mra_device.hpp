@@ -16,6 +16,9 @@
 #define MRA_WG 256
 
 typedef double mra_d4 __attribute__((ext_vector_type(4)));
+
+// wave index within the work group, as a wave-uniform (scalar) value
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
 typedef float mra_f4 __attribute__((ext_vector_type(4)));
 
 namespace mra {
@@ -184,20 +187,141 @@ __device__ __forceinline__ float pcovf(const CovCtx& c, const Pts& P, uint64_t i
 // ----------------------------------------------------------- MFMA GEMM TN
 // C[i*ldc+j] += alpha * sum_k A(k,i) B(k,j) over the M x N tile (multiples
 // of 16).  A(k,i) = A[k*lda+i] valid for k < Kv, i < Mv (else 0); B likewise
-// with Nv.  K is rounded up to a multiple of 4.  C is LDS (or global) and
-// must be made visible by the caller (__syncthreads before and after).
-// TA: A is stored transposed, A(k,i) = A[i*lda+k] (an LDS matrix read by
-// columns).  ACC = false stores C = alpha*A^T B instead of accumulating.
+// with Nv (Mv, Nv >= 1).  K is rounded up to a multiple of 4.  C is LDS (or
+// global) and must be made visible by the caller (__syncthreads before and
+// after).  TA: A is stored transposed, A(k,i) = A[i*lda+k] (an LDS matrix
+// read by columns).  ACC = false stores C = alpha*A^T B instead of
+// accumulating.
 // TILES: GEMM_ANY checks per wave block whether every operand is in range
-// and then takes the predication-free path; GEMM_FULL asserts that
-// Mv >= M and Nv >= N (only that path is compiled); GEMM_PART compiles only
-// the predicated path (smaller register footprint).
+// and then takes the unpredicated path; GEMM_FULL asserts that Mv >= M and
+// Nv >= N (only that path is compiled); GEMM_PART compiles only the
+// predicated path.
+// Each wave owns blocks of up to 2 x 2 tiles of 16 x 16.  The k loop is
+// instantiated per block shape (no branches inside): operand loads of U
+// k-steps are issued together, at wave-uniform row addresses plus 32-bit
+// per-lane offsets; out-of-range rows are read at a clamped valid row and
+// replaced by zero with a select.
 enum { GEMM_ANY = 0, GEMM_FULL = 1, GEMM_PART = 2 };
-template <bool TA = false, bool ACC = true, typename TIA = double, typename TIB = double, int TILES = GEMM_PART>
-__device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int Kv, double alpha, const TIA* A,
-                                           int64_t lda, int Mv, const TIB* B, int64_t ldb, int Nv) {
+
+template <typename TC>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef mra_d4 V;
+    __device__ static V op(double a, double b, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    // D element v of lane t is row t/16 + 4v (column t%16)
+    __device__ static int row(int k4, int v) { return k4 + 4 * v; }
+};
+template <>
+struct Mfma<float> {
+    typedef mra_f4 V;
+    __device__ static V op(float a, float b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    // D element v of lane t is row 4*(t/16) + v (column t%16)
+    __device__ static int row(int k4, int v) { return 4 * k4 + v; }
+};
+
+// Operands of one wave block: lane offsets (bytes) of the two row tiles of
+// A and the two column tiles of B at k = k4, and the per-k strides.
+template <bool TA, typename TIA, typename TIB>
+struct BlockOps {
+    const char* A;
+    const char* B;
+    int64_t sa, sb;         // bytes per k step
+    uint32_t oa0, oa1, ob0, ob1;
+    bool ha, hb, hja, hjb;  // rows / columns in range
+    __device__ BlockOps(const TIA* A_, int64_t lda, const TIB* B_, int64_t ldb, int ia, int ib, int ja, int jb,
+                        bool ha_, bool hb_, bool hja_, bool hjb_, int k4)
+        : A((const char*)A_), B((const char*)B_), ha(ha_), hb(hb_), hja(hja_), hjb(hjb_) {
+        const int iac = ha ? ia : 0, ibc = hb ? ib : 0, jac = hja ? ja : 0, jbc = hjb ? jb : 0;
+        sa = (TA ? 1 : lda) * (int64_t)sizeof(TIA);
+        sb = ldb * (int64_t)sizeof(TIB);
+        oa0 = (uint32_t)((TA ? iac * lda + k4 : k4 * lda + iac) * sizeof(TIA));
+        oa1 = (uint32_t)((TA ? ibc * lda + k4 : k4 * lda + ibc) * sizeof(TIA));
+        ob0 = (uint32_t)((k4 * ldb + jac) * sizeof(TIB));
+        ob1 = (uint32_t)((k4 * ldb + jbc) * sizeof(TIB));
+    }
+};
+
+template <typename TC, bool M2, bool N2, bool PRED, int U, bool TA, typename TIA, typename TIB>
+__device__ __forceinline__ void block_kloop(typename Mfma<TC>::V (&acc)[4], const BlockOps<TA, TIA, TIB>& o, int kb,
+                                            int ke, int k4) {
+    int k0 = kb;
+    for (; k0 + 4 * U <= ke; k0 += 4 * U) {
+        TIA a0[U], a1[U];
+        TIB b0[U], b1[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const char* Ak = o.A + (int64_t)(k0 + 4 * u) * o.sa;
+            const char* Bk = o.B + (int64_t)(k0 + 4 * u) * o.sb;
+            a0[u] = *(const TIA*)(Ak + o.oa0);
+            if (M2) a1[u] = *(const TIA*)(Ak + o.oa1);
+            b0[u] = *(const TIB*)(Bk + o.ob0);
+            if (N2) b1[u] = *(const TIB*)(Bk + o.ob1);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const TC xa0 = (!PRED || o.ha) ? (TC)a0[u] : TC(0);
+            const TC xb0 = (!PRED || o.hja) ? (TC)b0[u] : TC(0);
+            acc[0] = Mfma<TC>::op(xa0, xb0, acc[0]);
+            if (N2) {
+                const TC xb1 = (!PRED || o.hjb) ? (TC)b1[u] : TC(0);
+                acc[1] = Mfma<TC>::op(xa0, xb1, acc[1]);
+            }
+            if (M2) {
+                const TC xa1 = (!PRED || o.hb) ? (TC)a1[u] : TC(0);
+                acc[2] = Mfma<TC>::op(xa1, xb0, acc[2]);
+                if (N2) {
+                    const TC xb1 = (!PRED || o.hjb) ? (TC)b1[u] : TC(0);
+                    acc[3] = Mfma<TC>::op(xa1, xb1, acc[3]);
+                }
+            }
+        }
+    }
+    // remaining k steps: k clamped into range, zero beyond ke
+    for (; k0 < ke; k0 += 4) {
+        const int k = k0 + k4;
+        const bool kv = k < ke;
+        const int64_t dk = (int64_t)((kv ? k : ke - 1) - k4);  // offset from the lane's k4 row
+        const char* Ak = o.A + dk * o.sa;
+        const char* Bk = o.B + dk * o.sb;
+        const TC xa0 = (kv && o.ha) ? (TC) * (const TIA*)(Ak + o.oa0) : TC(0);
+        const TC xb0 = (kv && o.hja) ? (TC) * (const TIB*)(Bk + o.ob0) : TC(0);
+        acc[0] = Mfma<TC>::op(xa0, xb0, acc[0]);
+        if (N2) {
+            const TC xb1 = (kv && o.hjb) ? (TC) * (const TIB*)(Bk + o.ob1) : TC(0);
+            acc[1] = Mfma<TC>::op(xa0, xb1, acc[1]);
+        }
+        if (M2) {
+            const TC xa1 = (kv && o.hb) ? (TC) * (const TIA*)(Ak + o.oa1) : TC(0);
+            acc[2] = Mfma<TC>::op(xa1, xb0, acc[2]);
+            if (N2) {
+                const TC xb1 = (kv && o.hjb) ? (TC) * (const TIB*)(Bk + o.ob1) : TC(0);
+                acc[3] = Mfma<TC>::op(xa1, xb1, acc[3]);
+            }
+        }
+    }
+}
+
+// dispatch on the block shape (wave-uniform)
+template <typename TC, bool PRED, int U, bool TA, typename TIA, typename TIB>
+__device__ __forceinline__ void block_kloop_any(typename Mfma<TC>::V (&acc)[4], const BlockOps<TA, TIA, TIB>& o,
+                                                bool m2, bool n2, int kb, int ke, int k4) {
+    if (m2 && n2)
+        block_kloop<TC, true, true, PRED, U>(acc, o, kb, ke, k4);
+    else if (m2)
+        block_kloop<TC, true, false, PRED, U>(acc, o, kb, ke, k4);
+    else if (n2)
+        block_kloop<TC, false, true, PRED, U>(acc, o, kb, ke, k4);
+    else
+        block_kloop<TC, false, false, PRED, U>(acc, o, kb, ke, k4);
+}
+
+template <typename TC, bool TA, bool ACC, typename TIA, typename TIB, int TILES, typename TO>
+__device__ __forceinline__ void wg_gemm_tn_impl(TO* C, int ldc, int M, int N, int Kv, TC alpha, const TIA* A,
+                                                int64_t lda, int Mv, const TIB* B, int64_t ldb, int Nv) {
     static_assert(!(TA && TILES == GEMM_FULL), "the full-tile path reads A untransposed");
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    typedef typename Mfma<TC>::V V;
+    const int lane = threadIdx.x & 63, wave = wave_id();
     const int r16 = lane & 15, k4 = lane >> 4;
     const int tM = M >> 4, tN = N >> 4;
     // per-wave blocks of bh x bw tiles: 2x2 when that still gives every wave a
@@ -208,181 +332,55 @@ __device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int
     const int bM = (tM + bh - 1) / bh, bN = (tN + bw - 1) / bw;
     for (int blk = wave; blk < bM * bN; blk += 4) {
         const int ti0 = (blk / bN) * bh, tj0 = (blk % bN) * bw;
-        mra_d4 acc00 = {0, 0, 0, 0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
-        const int ia = ti0 * 16 + r16, ib = ia + 16;
-        const int ja = tj0 * 16 + r16, jb = ja + 16;
-        const bool tM2 = bh == 2 && ti0 + 1 < tM, tN2 = bw == 2 && tj0 + 1 < tN;  // second tile row / column
-        const bool ha = ia < Mv, hb = tM2 && ib < Mv;
-        const bool hja = ja < Nv, hjb = tN2 && jb < Nv;
-        // Operand loads of U k-steps are issued together before their MFMAs so
-        // that one memory latency covers U steps (all control flow is
-        // uniform across the wave).
-        constexpr int U = 8;
-        int k0 = 0;
-        const bool full = TILES == GEMM_FULL ||
-                          (TILES == GEMM_ANY && !TA && ti0 * 16 + (tM2 ? 32 : 16) <= Mv && tj0 * 16 + (tN2 ? 32 : 16) <= Nv);
-        if (TILES != GEMM_PART && full) {
-            // every operand in range: wave-uniform row pointers plus 32-bit
-            // per-lane byte offsets, no predication
-            const uint32_t oa0 = (uint32_t)((k4 * lda + ia) * sizeof(TIA)), oa1 = oa0 + 16 * sizeof(TIA);
-            const uint32_t ob0 = (uint32_t)((k4 * ldb + ja) * sizeof(TIB)), ob1 = ob0 + 16 * sizeof(TIB);
-            for (; k0 + 4 * U <= Kv; k0 += 4 * U) {
-                TIA a0[U], a1[U];
-                TIB b0[U], b1[U];
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    const char* Ak = (const char*)(A + (int64_t)(k0 + 4 * u) * lda);
-                    const char* Bk = (const char*)(B + (int64_t)(k0 + 4 * u) * ldb);
-                    a0[u] = *(const TIA*)(Ak + oa0);
-                    if (tM2) a1[u] = *(const TIA*)(Ak + oa1);
-                    b0[u] = *(const TIB*)(Bk + ob0);
-                    if (tN2) b1[u] = *(const TIB*)(Bk + ob1);
-                }
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    const double xa0 = (double)a0[u], xb0 = (double)b0[u];
-                    acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(xa0, xb0, acc00, 0, 0, 0);
-                    if (tN2) acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(xa0, (double)b1[u], acc01, 0, 0, 0);
-                    if (tM2) acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a1[u], xb0, acc10, 0, 0, 0);
-                    if (tM2 && tN2)
-                        acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a1[u], (double)b1[u], acc11, 0, 0, 0);
-                }
-            }
-        }
-        constexpr int UP = 4;
-        for (; TILES != GEMM_FULL && k0 + 4 * UP <= Kv; k0 += 4 * UP) {
-            // raw operands (input type) first, widened at use, so that a
-            // conversion does not wait on its load before the next load issues
-            TIA a0[UP], a1[UP];
-            TIB b0[UP], b1[UP];
-#pragma unroll
-            for (int u = 0; u < UP; u++) {
-                const int64_t k = k0 + 4 * u + k4;
-                a0[u] = ha ? (TA ? A[ia * lda + k] : A[k * lda + ia]) : TIA(0);
-                a1[u] = hb ? (TA ? A[ib * lda + k] : A[k * lda + ib]) : TIA(0);
-                b0[u] = hja ? B[k * ldb + ja] : TIB(0);
-                b1[u] = hjb ? B[k * ldb + jb] : TIB(0);
-            }
-#pragma unroll
-            for (int u = 0; u < UP; u++) {
-                const double xa0 = (double)a0[u], xa1 = (double)a1[u], xb0 = (double)b0[u], xb1 = (double)b1[u];
-                acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(xa0, xb0, acc00, 0, 0, 0);
-                if (tN2) acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(xa0, xb1, acc01, 0, 0, 0);
-                if (tM2) acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(xa1, xb0, acc10, 0, 0, 0);
-                if (tM2 && tN2) acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(xa1, xb1, acc11, 0, 0, 0);
-            }
-        }
-        for (; k0 < Kv; k0 += 4) {
-            const int k = k0 + k4;
-            const bool kv = k < Kv;
-            double a0 = (kv && ha) ? (double)(TA ? A[ia * lda + k] : A[(int64_t)k * lda + ia]) : 0.0;
-            double a1 = (kv && hb) ? (double)(TA ? A[ib * lda + k] : A[(int64_t)k * lda + ib]) : 0.0;
-            double b0 = (kv && hja) ? (double)B[(int64_t)k * ldb + ja] : 0.0;
-            double b1 = (kv && hjb) ? (double)B[(int64_t)k * ldb + jb] : 0.0;
-            acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc00, 0, 0, 0);
-            if (tN2) acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc01, 0, 0, 0);
-            if (tM2) acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
-            if (tM2 && tN2) acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc11, 0, 0, 0);
-        }
-        // only the valid Mv x Nv part of C is updated; element v of lane t
-        // holds row t/16 + 4v of the 16 x 16 tile
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int row = ti0 * 16 + k4 + 4 * v;
-            const bool r0 = row < Mv, r1 = tM2 && (row + 16 < Mv);
-            double* c00 = C + (int64_t)row * ldc + ja;
-            double* c10 = C + (int64_t)(row + 16) * ldc + ja;
-            if (ACC) {
-                if (r0 && hja) c00[0] += alpha * acc00[v];
-                if (r0 && hjb) c00[16] += alpha * acc01[v];
-                if (r1 && hja) c10[0] += alpha * acc10[v];
-                if (r1 && hjb) c10[16] += alpha * acc11[v];
-            } else {
-                if (r0 && hja) c00[0] = alpha * acc00[v];
-                if (r0 && hjb) c00[16] = alpha * acc01[v];
-                if (r1 && hja) c10[0] = alpha * acc10[v];
-                if (r1 && hjb) c10[16] = alpha * acc11[v];
-            }
-        }
-    }
-}
-
-// FP32 variant: C (float) += alpha * A^T B with v_mfma_f32_16x16x4_f32 and
-// FP32 accumulation; operands are read as TIA / TIB (double or float) and
-// rounded to float.  Same tiling and validity rules as wg_gemm_tn; the f32 D
-// fragment has element v of lane t at row 4*(t/16) + v (unlike the f64
-// instruction; confirmed on gfx950 by tests/test_gpu_blocks.py).
-template <bool ACC = true, typename TIA = float, typename TIB = float, typename TC = float>
-__device__ __forceinline__ void wg_gemm_tn_f32(TC* C, int ldc, int M, int N, int Kv, float alpha, const TIA* A,
-                                               int64_t lda, int Mv, const TIB* B, int64_t ldb, int Nv) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int r16 = lane & 15, k4 = lane >> 4;
-    const int tM = M >> 4, tN = N >> 4;
-    const int nt = tM * tN;
-    const int bh = (nt >= 16 || (nt >= 8 && tM >= 2)) ? 2 : 1;
-    const int bw = (nt >= 16) ? 2 : 1;
-    const int bM = (tM + bh - 1) / bh, bN = (tN + bw - 1) / bw;
-    for (int blk = wave; blk < bM * bN; blk += 4) {
-        const int ti0 = (blk / bN) * bh, tj0 = (blk % bN) * bw;
-        mra_f4 acc00 = {0, 0, 0, 0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
         const int ia = ti0 * 16 + r16, ib = ia + 16;
         const int ja = tj0 * 16 + r16, jb = ja + 16;
         const bool tM2 = bh == 2 && ti0 + 1 < tM, tN2 = bw == 2 && tj0 + 1 < tN;
         const bool ha = ia < Mv, hb = tM2 && ib < Mv;
         const bool hja = ja < Nv, hjb = tN2 && jb < Nv;
-        constexpr int U = 8;
-        int k0 = 0;
-        for (; k0 + 4 * U <= Kv; k0 += 4 * U) {
-            TIA a0[U], a1[U];
-            TIB b0[U], b1[U];
+        const BlockOps<TA, TIA, TIB> o(A, lda, B, ldb, ia, ib, ja, jb, ha, hb, hja, hjb, k4);
+        V acc[4];
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int64_t k = k0 + 4 * u + k4;
-                a0[u] = ha ? A[k * lda + ia] : TIA(0);
-                a1[u] = hb ? A[k * lda + ib] : TIA(0);
-                b0[u] = hja ? B[k * ldb + ja] : TIB(0);
-                b1[u] = hjb ? B[k * ldb + jb] : TIB(0);
-            }
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                const float xa0 = (float)a0[u], xa1 = (float)a1[u], xb0 = (float)b0[u], xb1 = (float)b1[u];
-                acc00 = __builtin_amdgcn_mfma_f32_16x16x4f32(xa0, xb0, acc00, 0, 0, 0);
-                if (tN2) acc01 = __builtin_amdgcn_mfma_f32_16x16x4f32(xa0, xb1, acc01, 0, 0, 0);
-                if (tM2) acc10 = __builtin_amdgcn_mfma_f32_16x16x4f32(xa1, xb0, acc10, 0, 0, 0);
-                if (tM2 && tN2) acc11 = __builtin_amdgcn_mfma_f32_16x16x4f32(xa1, xb1, acc11, 0, 0, 0);
-            }
-        }
-        for (; k0 < Kv; k0 += 4) {
-            const int k = k0 + k4;
-            const bool kv = k < Kv;
-            const float a0 = (kv && ha) ? (float)A[(int64_t)k * lda + ia] : 0.0f;
-            const float a1 = (kv && hb) ? (float)A[(int64_t)k * lda + ib] : 0.0f;
-            const float b0 = (kv && hja) ? (float)B[(int64_t)k * ldb + ja] : 0.0f;
-            const float b1 = (kv && hjb) ? (float)B[(int64_t)k * ldb + jb] : 0.0f;
-            acc00 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc00, 0, 0, 0);
-            if (tN2) acc01 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc01, 0, 0, 0);
-            if (tM2) acc10 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc10, 0, 0, 0);
-            if (tM2 && tN2) acc11 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc11, 0, 0, 0);
-        }
+        for (int q = 0; q < 4; q++) acc[q] = V{0, 0, 0, 0};
+        const bool full = TILES == GEMM_FULL ||
+                          (TILES == GEMM_ANY && ti0 * 16 + (tM2 ? 32 : 16) <= Mv && tj0 * 16 + (tN2 ? 32 : 16) <= Nv);
+        if (TILES != GEMM_PART && full)
+            block_kloop_any<TC, false, 4>(acc, o, tM2, tN2, 0, Kv, k4);
+        else if (TILES != GEMM_FULL)
+            block_kloop_any<TC, true, 4>(acc, o, tM2, tN2, 0, Kv, k4);
+        // only the valid Mv x Nv part of C is updated
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            const int row = ti0 * 16 + 4 * k4 + v;
+            const int row = ti0 * 16 + Mfma<TC>::row(k4, v);
             const bool r0 = row < Mv, r1 = tM2 && (row + 16 < Mv);
-            TC* c00 = C + (int64_t)row * ldc + ja;
-            TC* c10 = C + (int64_t)(row + 16) * ldc + ja;
+            TO* c00 = C + (int64_t)row * ldc + ja;
+            TO* c10 = C + (int64_t)(row + 16) * ldc + ja;
             if (ACC) {
-                if (r0 && hja) c00[0] += alpha * acc00[v];
-                if (r0 && hjb) c00[16] += alpha * acc01[v];
-                if (r1 && hja) c10[0] += alpha * acc10[v];
-                if (r1 && hjb) c10[16] += alpha * acc11[v];
+                if (r0 && hja) c00[0] += alpha * acc[0][v];
+                if (r0 && hjb) c00[16] += alpha * acc[1][v];
+                if (r1 && hja) c10[0] += alpha * acc[2][v];
+                if (r1 && hjb) c10[16] += alpha * acc[3][v];
             } else {
-                if (r0 && hja) c00[0] = alpha * acc00[v];
-                if (r0 && hjb) c00[16] = alpha * acc01[v];
-                if (r1 && hja) c10[0] = alpha * acc10[v];
-                if (r1 && hjb) c10[16] = alpha * acc11[v];
+                if (r0 && hja) c00[0] = alpha * acc[0][v];
+                if (r0 && hjb) c00[16] = alpha * acc[1][v];
+                if (r1 && hja) c10[0] = alpha * acc[2][v];
+                if (r1 && hjb) c10[16] = alpha * acc[3][v];
             }
         }
     }
+}
+
+// FP64 MFMA, FP64 accumulation; operands read as TIA / TIB and widened
+template <bool TA = false, bool ACC = true, typename TIA = double, typename TIB = double, int TILES = GEMM_PART>
+__device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int Kv, double alpha, const TIA* A,
+                                           int64_t lda, int Mv, const TIB* B, int64_t ldb, int Nv) {
+    wg_gemm_tn_impl<double, TA, ACC, TIA, TIB, TILES>(C, ldc, M, N, Kv, alpha, A, lda, Mv, B, ldb, Nv);
+}
+
+// FP32 MFMA with FP32 accumulation (operands rounded to float); C of type TC
+template <bool ACC = true, typename TIA = float, typename TIB = float, typename TC = float>
+__device__ __forceinline__ void wg_gemm_tn_f32(TC* C, int ldc, int M, int N, int Kv, float alpha, const TIA* A,
+                                               int64_t lda, int Mv, const TIB* B, int64_t ldb, int Nv) {
+    wg_gemm_tn_impl<float, false, ACC, TIA, TIB, GEMM_ANY>(C, ldc, M, N, Kv, alpha, A, lda, Mv, B, ldb, Nv);
 }
 
 // Same product for an output of at most 32 x 32 (one 2x2 tile block), with
@@ -393,7 +391,7 @@ template <typename TI = double>
 __device__ __forceinline__ void wg_gemm_tn_ksplit(double* C, int ldc, int M, int N, int Kv, double alpha,
                                                   const TI* A, int64_t lda, int Mv, const TI* B, int64_t ldb,
                                                   int Nv, double* red) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = wave_id();
     const int r16 = lane & 15, k4 = lane >> 4;
     const int tM = M >> 4, tN = N >> 4;
     const bool tM2 = tM > 1, tN2 = tN > 1;
@@ -402,42 +400,11 @@ __device__ __forceinline__ void wg_gemm_tn_ksplit(double* C, int ldc, int M, int
     for (int q = 0; q < 4; q++) acc[q] = mra_d4{0, 0, 0, 0};
     const int ia = r16, ib = 16 + r16, ja = r16, jb = 16 + r16;
     const bool ha = ia < Mv, hb = tM2 && ib < Mv, hja = ja < Nv, hjb = tN2 && jb < Nv;
+    const BlockOps<false, TI, TI> o(A, lda, B, ldb, ia, ib, ja, jb, ha, hb, hja, hjb, k4);
     // this wave's contiguous K range, a multiple of 4
     const int kq = ((Kv + 15) / 16) * 4;
-    const int kb = wave * kq, ke = min(Kv, kb + kq);
-    constexpr int U = 6;
-    int k0 = kb;
-    for (; k0 + 4 * U <= ke; k0 += 4 * U) {  // batched loads, one latency per U steps
-        TI a0[U], a1[U], b0[U], b1[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int64_t k = k0 + 4 * u + k4;
-            a0[u] = ha ? A[k * lda + ia] : TI(0);
-            a1[u] = hb ? A[k * lda + ib] : TI(0);
-            b0[u] = hja ? B[k * ldb + ja] : TI(0);
-            b1[u] = hjb ? B[k * ldb + jb] : TI(0);
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const double xa0 = (double)a0[u], xa1 = (double)a1[u], xb0 = (double)b0[u], xb1 = (double)b1[u];
-            acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa0, xb0, acc[0], 0, 0, 0);
-            if (tN2) acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa0, xb1, acc[1], 0, 0, 0);
-            if (tM2) acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa1, xb0, acc[2], 0, 0, 0);
-            if (tM2 && tN2) acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa1, xb1, acc[3], 0, 0, 0);
-        }
-    }
-    for (; k0 < ke; k0 += 4) {
-        const int k = k0 + k4;
-        const bool kv = k < ke;
-        const double a0 = (kv && ha) ? (double)A[(int64_t)k * lda + ia] : 0.0;
-        const double a1 = (kv && hb) ? (double)A[(int64_t)k * lda + ib] : 0.0;
-        const double b0 = (kv && hja) ? (double)B[(int64_t)k * ldb + ja] : 0.0;
-        const double b1 = (kv && hjb) ? (double)B[(int64_t)k * ldb + jb] : 0.0;
-        acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0], 0, 0, 0);
-        if (tN2) acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[1], 0, 0, 0);
-        if (tM2) acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[2], 0, 0, 0);
-        if (tM2 && tN2) acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[3], 0, 0, 0);
-    }
+    const int kb = min(Kv, wave * kq), ke = min(Kv, kb + kq);
+    block_kloop_any<double, true, 6>(acc, o, tM2, tN2, kb, ke, k4);
     const int nq = (tM2 || tN2) ? 4 : 1;  // red holds 3*nq*256 doubles
     if (wave > 0) {
 #pragma unroll
@@ -452,12 +419,12 @@ __device__ __forceinline__ void wg_gemm_tn_ksplit(double* C, int ldc, int M, int
         for (int q = 0; q < 4; q++)
             if (q < nq)
 #pragma unroll
-            for (int v = 0; v < 4; v++) {
-                double t = acc[q][v];
-                for (int w = 0; w < 3; w++) t += red[(w * nq + q) * 256 + v * 64 + lane];
-                const int row = (q >> 1) * 16 + k4 + 4 * v, col = (q & 1) * 16 + r16;
-                if (row < Mv && col < Nv) C[row * ldc + col] += alpha * t;
-            }
+                for (int v = 0; v < 4; v++) {
+                    double t = acc[q][v];
+                    for (int w = 0; w < 3; w++) t += red[(w * nq + q) * 256 + v * 64 + lane];
+                    const int row = (q >> 1) * 16 + k4 + 4 * v, col = (q & 1) * 16 + r16;
+                    if (row < Mv && col < Nv) C[row * ldc + col] += alpha * t;
+                }
     }
     __syncthreads();
 }
@@ -517,15 +484,15 @@ __device__ __forceinline__ void wg_chol(double* A, int lda, int n, int* bad) {
 // wave's instructions in order)
 __device__ __forceinline__ void wave_lds_fence() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
 
-template <int N>
+// layout of the inverse diagonal blocks for nb = N/16 blocks
 struct BlkLayout {
-    static constexpr int NB = N / 16;
-    static constexpr bool UPPER = NB >= 3;
-    __device__ static double* dptr(double* A, int lda, double* dbuf, int b) {
-        if (UPPER) return b < NB - 1 ? A + b * 16 * lda + (b + 1) * 16 : A + (NB - 1) * 16;
+    int nb;
+    __device__ bool upper() const { return nb >= 3; }
+    __device__ double* dptr(double* A, int lda, double* dbuf, int b) const {
+        if (upper()) return b < nb - 1 ? A + b * 16 * lda + (b + 1) * 16 : A + (nb - 1) * 16;
         return dbuf + b * 256;
     }
-    __device__ static int ldd(int lda) { return UPPER ? lda : 16; }
+    __device__ int ldd(int lda) const { return upper() ? lda : 16; }
 };
 
 // 16 x 16 x 16 product on one wave: acc[v] (row k4 + 4v, column r16) +=
@@ -606,21 +573,20 @@ __device__ __forceinline__ void wave_chol16(double* P, int lda, double* Dp, int 
     wave_lds_fence();
 }
 
-template <int N>
-__device__ __forceinline__ void wg_chol_blk(double* A, int lda, double* dbuf, int* bad) {
-    using BL = BlkLayout<N>;
-    constexpr int NB = BL::NB;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, k4 = lane >> 4;
-    const int ldd = BL::ldd(lda);
+// Cholesky of the leading 16*nb x 16*nb block (nb <= 8).
+__device__ __forceinline__ void wg_chol_blk(double* A, int lda, int nb, double* dbuf, int* bad) {
+    const BlkLayout BL{nb};
+    const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, k4 = lane >> 4;
+    const int ldd = BL.ldd(lda);
     __syncthreads();
-    for (int kb = 0; kb < NB; kb++) {
+    for (int kb = 0; kb < nb; kb++) {
         double* Pkk = A + kb * 16 * (lda + 1);
-        double* Dk = BL::dptr(A, lda, dbuf, kb);
+        double* Dk = BL.dptr(A, lda, dbuf, kb);
         if (wave == 0) wave_chol16(Pkk, lda, Dk, ldd, bad);
         __syncthreads();
-        if (kb == NB - 1) break;
+        if (kb == nb - 1) break;
         // panel: L_{ib,kb} = A_{ib,kb} D_kb^T
-        for (int ib = kb + 1 + wave; ib < NB; ib += 4) {
+        for (int ib = kb + 1 + wave; ib < nb; ib += 4) {
             double* Pib = A + ib * 16 * lda + kb * 16;
             mra_d4 acc = {0, 0, 0, 0};
             acc = mfma16<true>(acc, Pib, lda, Dk, ldd);
@@ -630,7 +596,7 @@ __device__ __forceinline__ void wg_chol_blk(double* A, int lda, double* dbuf, in
         }
         __syncthreads();
         // trailing update A_{ib,jb} -= L_{ib,kb} L_{jb,kb}^T, kb < jb <= ib
-        const int m = NB - 1 - kb;
+        const int m = nb - 1 - kb;
         for (int q = wave; q < m * (m + 1) / 2; q += 4) {
             int ib = 0, rem = q;
             while (rem > ib) {
@@ -650,16 +616,15 @@ __device__ __forceinline__ void wg_chol_blk(double* A, int lda, double* dbuf, in
     }
 }
 
-// A <- L^{-1} in place after wg_chol_blk<N> (same dbuf); the strictly upper
-// part is set to zero.
-template <int N>
-__device__ __forceinline__ void wg_trinv_blk(double* A, int lda, double* dbuf) {
-    using BL = BlkLayout<N>;
-    constexpr int NB = BL::NB;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, k4 = lane >> 4;
-    const int ldd = BL::ldd(lda);
+// A <- L^{-1} in place after wg_chol_blk (same nb, dbuf); the strictly
+// upper part of the leading 16*nb block is set to zero.
+__device__ __forceinline__ void wg_trinv_blk(double* A, int lda, int nb, double* dbuf) {
+    const BlkLayout BL{nb};
+    const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, k4 = lane >> 4;
+    const int ldd = BL.ldd(lda);
+    const int N = 16 * nb;
     __syncthreads();
-    for (int i = 1; i < NB; i++) {
+    for (int i = 1; i < nb; i++) {
         // T_ij = sum_{k=j}^{i-1} L_ik X_kj (X_jj = D_j; X_kj, k > j, final),
         // at most two blocks j per wave
         mra_d4 acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
@@ -669,7 +634,7 @@ __device__ __forceinline__ void wg_trinv_blk(double* A, int lda, double* dbuf) {
             if (j < i)
                 for (int k = j; k < i; k++) {
                     const double* Lik = A + i * 16 * lda + k * 16;
-                    const double* Xkj = k == j ? BL::dptr(A, lda, dbuf, j) : A + k * 16 * lda + j * 16;
+                    const double* Xkj = k == j ? BL.dptr(A, lda, dbuf, j) : A + k * 16 * lda + j * 16;
                     acc[t] = mfma16<false>(acc[t], Lik, lda, Xkj, k == j ? ldd : lda);
                 }
         }
@@ -684,7 +649,7 @@ __device__ __forceinline__ void wg_trinv_blk(double* A, int lda, double* dbuf) {
                 wave_lds_fence();
                 // X_ij = -D_i T_ij
                 mra_d4 x = {0, 0, 0, 0};
-                x = mfma16<false>(x, BL::dptr(A, lda, dbuf, i), ldd, Cb, lda);
+                x = mfma16<false>(x, BL.dptr(A, lda, dbuf, i), ldd, Cb, lda);
                 wave_lds_fence();
 #pragma unroll
                 for (int v = 0; v < 4; v++) Cb[(k4 + 4 * v) * lda + r16] = -x[v];
@@ -692,23 +657,22 @@ __device__ __forceinline__ void wg_trinv_blk(double* A, int lda, double* dbuf) {
         }
         __syncthreads();
     }
-    // diagonal blocks <- D_b (read every D_b before any upper block is zeroed)
-    constexpr int PER = (NB * 256 + MRA_WG - 1) / MRA_WG;
-    double dv[PER];
+    // diagonal blocks <- D_b (every D_b is read before any upper block is zeroed)
+    double dv[8];
 #pragma unroll
-    for (int q = 0; q < PER; q++) {
+    for (int q = 0; q < 8; q++) {
         const int idx = threadIdx.x + q * MRA_WG;
         dv[q] = 0.0;
-        if (idx < NB * 256) {
+        if (idx < nb * 256) {
             const int b = idx >> 8, e = idx & 255;
-            dv[q] = BL::dptr(A, lda, dbuf, b)[(e >> 4) * ldd + (e & 15)];
+            dv[q] = BL.dptr(A, lda, dbuf, b)[(e >> 4) * ldd + (e & 15)];
         }
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < PER; q++) {
+    for (int q = 0; q < 8; q++) {
         const int idx = threadIdx.x + q * MRA_WG;
-        if (idx < NB * 256) {
+        if (idx < nb * 256) {
             const int b = idx >> 8, e = idx & 255, ii = e >> 4, jj = e & 15;
             A[(b * 16 + ii) * lda + b * 16 + jj] = jj <= ii ? dv[q] : 0.0;
         }

@@ -261,12 +261,12 @@ __global__ __launch_bounds__(256) void k_prior_coarse(Tree T, CovCtx cc, Pts K, 
         wg_gemm_tn<false, true, double, double, GEMM_FULL>(Cm, LD, RP, RP, l * RP, -1.0, GR, RP, RP, GR, RP, RP);
         __syncthreads();
     }
-    wg_chol_blk<RP>(Cm, LD, dbuf, &bad);
+    wg_chol_blk(Cm, LD, RP / 16, dbuf, &bad);
     for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
         const int k = idx / RP, i = idx % RP;  // column-major: [k*RP + i] = L[i][k]
         Lt[R * RP * RP + idx] = (i >= k) ? Cm[i * LD + k] : 0.0;
     }
-    wg_trinv_blk<RP>(Cm, LD, dbuf);
+    wg_trinv_blk(Cm, LD, RP / 16, dbuf);
     for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
         const int k = idx / RP, i = idx % RP;
         LiT[R * RP * RP + idx] = (i >= k) ? Cm[i * LD + k] : 0.0;
@@ -337,12 +337,14 @@ struct FinestLds {
     static constexpr int LD = NB + 1;
     double S[NB * LD];
     double kred[NB <= 16 ? 3 * 256 : (NB <= 32 ? 3 * 4 * 256 : 1)];
+    double D[NB <= 32 ? NB * 16 : 1];  // inverse diagonal blocks (fewer than 3 blocks)
 };
 template <>
 struct FinestLds<0> {
     static constexpr int LD = 1;
     double S[1];
     double kred[1];
+    double D[1];
 };
 
 template <int RP, int NB, typename TG>
@@ -351,7 +353,6 @@ __global__ __launch_bounds__(256) void k_finest_local(Tree T, CovCtx cc, Pts O, 
                                                       double* H, double* u, double* dterm, int* bad_flag) {
     __shared__ FinestLds<NB> sh;
     __shared__ double red[MRA_WG];
-    __shared__ double tmp[NB > 0 ? NB : 1];
     __shared__ int bad;
     const uint32_t F = items[blockIdx.x];
     const uint64_t ob = T.obs_off[F];
@@ -385,18 +386,30 @@ __global__ __launch_bounds__(256) void k_finest_local(Tree T, CovCtx cc, Pts O, 
             }
     }
     __syncthreads();
-    wg_chol(Sg, ld, n, &bad);
-    // log det part, then keep L_F
+    const int nbk = (n + 15) >> 4;
+    if (NB > 0) {
+        // blocked factorisation of the 16-padded block: identity padding
+        const int N16 = nbk * 16;
+        for (int idx = threadIdx.x; idx < N16 * N16; idx += MRA_WG) {
+            const int i = idx / N16, c = idx % N16;
+            if (i >= n || c >= n) Sg[i * ld + c] = (i == c) ? 1.0 : 0.0;
+        }
+        wg_chol_blk(Sg, ld, nbk, sh.D, &bad);
+    } else {
+        wg_chol(Sg, ld, n, &bad);
+    }
+    // log det part, then keep L_F (lower triangle; zeros above)
     double part = 0.0;
     for (int i = threadIdx.x; i < n; i += MRA_WG) part += 2.0 * log(Sg[i * ld + i]);
     if (NB > 0)
-        for (int idx = threadIdx.x; idx < n * n; idx += MRA_WG) LF[idx] = Sg[(idx / n) * ld + idx % n];
+        for (int idx = threadIdx.x; idx < n * n; idx += MRA_WG) {
+            const int i = idx / n, c = idx % n;
+            LF[idx] = c <= i ? Sg[i * ld + c] : 0.0;
+        }
     __syncthreads();
     if (NB > 0) {
-        // the upper triangle (still Sigma_F) must read as zero in the H product
-        for (int idx = threadIdx.x; idx < n * n; idx += MRA_WG)
-            if (idx % n > idx / n) Sg[(idx / n) * ld + idx % n] = 0.0;
-        wg_trinv(Sg, ld, n, tmp);
+        // L_F^{-1} in place (zero strict upper part, read by the H product)
+        wg_trinv_blk(Sg, ld, nbk, sh.D);
         // u = L_F^{-1} y
         double* uF = u + ob;
         for (int i = threadIdx.x; i < n; i += MRA_WG) {
@@ -592,14 +605,14 @@ __global__ __launch_bounds__(256) void k_posterior(Tree T, PostArgs a) {
     double lp = 0.0;
     for (int i = threadIdx.x; i < r; i += MRA_WG) lp += log(LR[i * RP + i]);
     const double logdet_prior = wg_sum(lp, red);
-    wg_chol_blk<RP>(Km, LD, dbuf, &bad);
+    wg_chol_blk(Km, LD, RP / 16, dbuf, &bad);
     double lq = 0.0;
     for (int i = threadIdx.x; i < r; i += MRA_WG) lq += 2.0 * log(Km[i * LD + i]);
     for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
         const int k = idx / RP, i = idx % RP;
         a.Lpt[R * RP * RP + idx] = (i >= k) ? Km[i * LD + k] : 0.0;
     }
-    wg_trinv_blk<RP>(Km, LD, dbuf);  // Km = L~^{-1}
+    wg_trinv_blk(Km, LD, RP / 16, dbuf);  // Km = L~^{-1}
     for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
         const int k = idx / RP, i = idx % RP;
         a.LpiT[R * RP * RP + idx] = (i >= k) ? Km[i * LD + k] : 0.0;
@@ -1079,12 +1092,12 @@ __global__ __launch_bounds__(256) void k_selftest_blk(const double* S, double* L
     __shared__ int bad;
     if (threadIdx.x == 0) bad = 0;
     for (int idx = threadIdx.x; idx < N * N; idx += MRA_WG) Cm[(idx / N) * LD + idx % N] = S[idx];
-    wg_chol_blk<N>(Cm, LD, dbuf, &bad);
+    wg_chol_blk(Cm, LD, N / 16, dbuf, &bad);
     for (int idx = threadIdx.x; idx < N * N; idx += MRA_WG) {
         const int i = idx / N, k = idx % N;
         Lout[idx] = k <= i ? Cm[i * LD + k] : 0.0;
     }
-    wg_trinv_blk<N>(Cm, LD, dbuf);
+    wg_trinv_blk(Cm, LD, N / 16, dbuf);
     for (int idx = threadIdx.x; idx < N * N; idx += MRA_WG) Xout[idx] = Cm[(idx / N) * LD + idx % N];
     if (threadIdx.x == 0) *bad_out = bad;
 }
