@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(BUILD, "libmra_hip.so")
+SHIM = os.path.join(BUILD, "shim_example")
 ROOT = os.path.dirname(HERE)
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -43,6 +44,9 @@ def build(verbose=True):
     tmp = LIB + ".tmp"
     run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    # the reference-style C++ driver over include/mra_approximation.hpp
+    run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), os.path.join(HERE, "examples",
+         "shim_example.cpp"), "-o", SHIM, "-L", BUILD, "-lmra_hip", "-Wl,-rpath,$ORIGIN"])
     return LIB
 
 
