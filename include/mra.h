@@ -186,6 +186,33 @@ int mra_variance_covariance(const mra_cov_desc* cov, double sill, double range, 
 int mra_cov_tile_device(int precision, int metric, const double* d_xyz1, const double* d_sb1, uint64_t n1,
                         const double* d_xyz2, const double* d_sb2, uint64_t n2, void* d_out, void* stream);
 
+/* ------------------------------------------------------------ optimizer */
+/* Positive status: the minimiser stopped at its evaluation limit (the
+ * reference's dlib::find_min_bobyqa throws bobyqa_failure there). */
+enum { MRA_W_MAX_EVALS = 1 };
+typedef double (*mra_objective_fn)(const double* x, void* ctx);
+/* Bound-constrained derivative-free minimisation of fn over lo <= x <= hi
+ * (src/optimization.cpp:80,169: dlib::find_min_bobyqa with npt = 2n+1):
+ * quadratic interpolation models with least-Frobenius-norm updates, trust
+ * region with bounds, resolution rho from rho_begin down to rho_end.  x is
+ * the start on entry and the best point on return.  npt: 0 or 2n+1.
+ * Requires hi[i] - lo[i] >= 2 rho_begin and max_evals >= 2n+2.  Returns
+ * MRA_OK, MRA_W_MAX_EVALS or a negative error. */
+int mra_bobyqa(int n, int npt, mra_objective_fn fn, void* ctx, double* x, const double* lo, const double* hi,
+               double rho_begin, double rho_end, int max_evals, double* f_best, int* n_evals);
+
+/* Maximum-likelihood fit of the six parameters of the planar linear-trend
+ * nonstationary model, theta = (a_s, bx_s, by_s, a_b, bx_b, log_tau2):
+ *   log sigma^2 = a_s + bx_s x + by_s y,  log beta = a_b + bx_b x,
+ *   tau^2 = exp(log_tau2),
+ * minimising D on the handle's observations (objective_nonstationary,
+ * src/optimization.cpp:107-156, extended from three scales to six field
+ * parameters).  theta: start on entry, best on return; the handle is left
+ * at the best parameters.  metric: MRA_METRIC_*. */
+enum { MRA_TREND_NPARAM = 6 };
+int mra_fit_linear_trend(mra_handle* h, int metric, double* theta, const double* lo, const double* hi,
+                         double rho_begin, double rho_end, int max_evals, double* D_best, int* n_evals);
+
 /* ------------------------------------------------------------ partition */
 /* Sort-based replacement of Partition::build_partition_by_program
  * (src/class_partition.cpp:459-654; same knot grid, half-open boxes and

@@ -14,6 +14,7 @@ LIB_PATH = os.environ.get("MRA_LIBRARY") or os.path.join(HERE, "_build", "libmra
 
 MRA_OK, MRA_E_ARG, MRA_E_HIP, MRA_E_NOT_PD, MRA_E_STATE, MRA_E_NOMEM, MRA_E_UNSUPPORTED = 0, -1, -2, -3, -4, -5, -6
 METRIC = {"chordal": 0, "great_circle": 1, "planar": 2}
+METRIC_PLANAR = 2
 FIELD_CONSTANT, FIELD_WENDLAND, FIELD_LINEAR = 0, 1, 2
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 T_SETUP, T_FINEST_CROSS, T_FINEST_LOCAL, T_PRED_PRE, T_PRED_POST, T_REDUCE, T_GRAPH, T_PRIOR0, T_POST0, T_COUNT = \
@@ -27,7 +28,8 @@ EXPORTED = ["mra_create", "mra_destroy", "mra_last_error", "mra_set_covariance",
             "mra_set_precision", "mra_likelihood", "mra_predict", "mra_likelihood_levels", "mra_sample_prior",
             "mra_set_observations", "mra_use_graph", "mra_last_timings", "mra_cross_covariance",
             "mra_variance_covariance", "mra_cov_tile_device", "mra_partition_program_sizes",
-            "mra_partition_program", "mra_version", "mra_selftest", "mra_set_option", "mra_get_option", "mra_selftest_f32", "mra_selftest_blk"]
+            "mra_partition_program", "mra_version", "mra_selftest", "mra_set_option", "mra_get_option", "mra_selftest_f32", "mra_selftest_blk",
+            "mra_bobyqa", "mra_fit_linear_trend"]
 OPT_FACTORED_POSTERIOR = 1
 
 
@@ -97,6 +99,10 @@ def lib():
         _lib.mra_selftest_f32.argtypes = [C.c_int, C.c_int, C.c_int, fp, fp, fp, fp]
         _lib.mra_selftest.argtypes = [C.c_int, C.c_int, C.c_int] + [dp] * 8 + [C.POINTER(C.c_int)]
         _lib.mra_selftest_blk.argtypes = [C.c_int, dp, dp, dp, C.POINTER(C.c_int)]
+        _lib.mra_bobyqa.argtypes = [C.c_int, C.c_int, OBJECTIVE, C.c_void_p, dp, dp, dp, C.c_double, C.c_double,
+                                    C.c_int, dp, C.POINTER(C.c_int)]
+        _lib.mra_fit_linear_trend.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, C.c_double, C.c_double, C.c_int, dp,
+                                              C.POINTER(C.c_int)]
     return _lib
 
 
@@ -108,6 +114,28 @@ def selftest(K, Mv, Nv, A, B, C0, S, Bt):
     check(lib().mra_selftest(K, Mv, Nv, keep.d(A), keep.d(B), keep.d(C0), keep.d(S), keep.d(Bt), _d(Co), _d(Lo),
                              _d(Xo), C.byref(bad)))
     return Co.reshape(64, 64), Lo.reshape(64, 64), Xo.reshape(64, 64), bad.value
+
+
+OBJECTIVE = C.CFUNCTYPE(C.c_double, C.POINTER(C.c_double), C.c_void_p)
+MRA_W_MAX_EVALS = 1
+TREND_NPARAM = 6
+
+
+def bobyqa(fun, x0, lo, hi, rho_begin, rho_end, max_evals):
+    """mra_bobyqa on a Python objective fun(x) -> float.  Returns
+    (x_best, f_best, n_evals, status) with status MRA_OK or MRA_W_MAX_EVALS."""
+    n = len(x0)
+    x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+    lo = np.ascontiguousarray(lo, dtype=np.float64)
+    hi = np.ascontiguousarray(hi, dtype=np.float64)
+    cb = OBJECTIVE(lambda p, ctx: float(fun(np.ctypeslib.as_array(p, shape=(n,)).copy())))
+    fb = C.c_double(0.0)
+    ne = C.c_int(0)
+    rc = lib().mra_bobyqa(n, 0, cb, None, _d(x), _d(lo), _d(hi), rho_begin, rho_end, max_evals, C.byref(fb),
+                          C.byref(ne))
+    if rc < 0:
+        check(rc)
+    return x, fb.value, ne.value, rc
 
 
 def selftest_blk(S):
@@ -213,6 +241,20 @@ class Handle:
         self.n_obs = int(t.n_obs)
         self.n_pred = int(t.n_pred)
         self.M = tree["M"]
+
+    def fit_linear_trend(self, theta0, lo, hi, rho_begin=0.1, rho_end=1e-6, max_evals=400, metric=METRIC_PLANAR):
+        """Maximum-likelihood fit of the six linear-trend parameters
+        (mra_fit_linear_trend).  Returns (theta, D, n_evals, status)."""
+        th = np.ascontiguousarray(theta0, dtype=np.float64).copy()
+        lo = np.ascontiguousarray(lo, dtype=np.float64)
+        hi = np.ascontiguousarray(hi, dtype=np.float64)
+        D = C.c_double(0.0)
+        ne = C.c_int(0)
+        rc = lib().mra_fit_linear_trend(self.h, metric, _d(th), _d(lo), _d(hi), rho_begin, rho_end, max_evals,
+                                        C.byref(D), C.byref(ne))
+        if rc < 0:
+            check(rc, self.h)
+        return th, D.value, ne.value, rc
 
     def close(self):
         if getattr(self, "h", None):
