@@ -213,6 +213,34 @@ enum { MRA_TREND_NPARAM = 6 };
 int mra_fit_linear_trend(mra_handle* h, int metric, double* theta, const double* lo, const double* hi,
                          double rho_begin, double rho_end, int max_evals, double* D_best, int* n_evals);
 
+/* ---------------------------------------------------- learned warm start */
+/* (No reference counterpart: the reference starts its optimizer from a
+ * fixed guess, src/optimization.cpp:160-169.)
+ * Features of the handle's current observation values (planar metric):
+ * log empirical semivariogram in 4 quadrants of the observations' bounding
+ * box x 16 lag bins (edges hmax (k/16)^2, hmax = half the box's larger side),
+ * from up to 1024 points per quadrant; features[q*16 + b]. */
+int mra_variogram_features(mra_handle* h, float* features);
+
+/* MLP 64 -> 256 -> 256 -> 6 with GELU (exact, erf form), FP32 parameters
+ * trained by Adam with the library's own forward/backward kernels.  Flat
+ * parameter order: W1 (256x64), b1, W2 (256x256), b2, W3 (6x256), b3; W row-major
+ * (out x in).  Batches of 1 to 1024 rows; X row-major batch x 64, targets /
+ * outputs batch x 6.  Loss: mean squared error over batch and outputs. */
+typedef struct mra_mlp mra_mlp;
+int mra_mlp_create(uint64_t seed, mra_mlp** out);
+void mra_mlp_destroy(mra_mlp* m);
+const char* mra_mlp_last_error(const mra_mlp* m);
+int mra_mlp_param_count(void);
+int mra_mlp_set_params(mra_mlp* m, const float* params);  /* also resets Adam state */
+int mra_mlp_get_params(mra_mlp* m, float* params);
+/* bf16 != 0: weights and activations rounded to bfloat16, FP32 accumulation */
+int mra_mlp_forward(mra_mlp* m, const float* X, int batch, float* Y, int bf16);
+/* loss and its gradient with respect to the flat parameters (grad may be NULL) */
+int mra_mlp_loss_grad(mra_mlp* m, const float* X, const float* T, int batch, float* loss, float* grad);
+/* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8); loss before the step */
+int mra_mlp_train_step(mra_mlp* m, const float* X, const float* T, int batch, float lr, float* loss);
+
 /* ------------------------------------------------------------ partition */
 /* Sort-based replacement of Partition::build_partition_by_program
  * (src/class_partition.cpp:459-654; same knot grid, half-open boxes and

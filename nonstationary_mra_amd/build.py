@@ -21,7 +21,7 @@ ARCH = os.environ.get("MRA_OFFLOAD_ARCH", "gfx950")
 # around every matrix instruction in loops whose accumulators are carried)
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-mllvm", "-amdgpu-mfma-vgpr-form", "-I", os.path.join(ROOT, "include")]
-HIP_SOURCES = ["mra_engine.hip"]
+HIP_SOURCES = ["mra_engine.hip", "mra_mlp.hip"]
 CPP_SOURCES = ["mra_partition.cpp", "mra_optimize.cpp"]
 
 

@@ -29,7 +29,9 @@ EXPORTED = ["mra_create", "mra_destroy", "mra_last_error", "mra_set_covariance",
             "mra_set_observations", "mra_use_graph", "mra_last_timings", "mra_cross_covariance",
             "mra_variance_covariance", "mra_cov_tile_device", "mra_partition_program_sizes",
             "mra_partition_program", "mra_version", "mra_selftest", "mra_set_option", "mra_get_option", "mra_selftest_f32", "mra_selftest_blk",
-            "mra_bobyqa", "mra_fit_linear_trend"]
+            "mra_bobyqa", "mra_fit_linear_trend", "mra_variogram_features", "mra_mlp_create", "mra_mlp_destroy",
+            "mra_mlp_last_error", "mra_mlp_param_count", "mra_mlp_set_params", "mra_mlp_get_params",
+            "mra_mlp_forward", "mra_mlp_loss_grad", "mra_mlp_train_step"]
 OPT_FACTORED_POSTERIOR = 1
 
 
@@ -103,6 +105,18 @@ def lib():
                                     C.c_int, dp, C.POINTER(C.c_int)]
         _lib.mra_fit_linear_trend.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, C.c_double, C.c_double, C.c_int, dp,
                                               C.POINTER(C.c_int)]
+        fp_ = C.POINTER(C.c_float)
+        _lib.mra_variogram_features.argtypes = [C.c_void_p, fp_]
+        _lib.mra_mlp_create.argtypes = [C.c_uint64, C.POINTER(C.c_void_p)]
+        _lib.mra_mlp_destroy.argtypes = [C.c_void_p]
+        _lib.mra_mlp_destroy.restype = None
+        _lib.mra_mlp_last_error.argtypes = [C.c_void_p]
+        _lib.mra_mlp_last_error.restype = C.c_char_p
+        _lib.mra_mlp_set_params.argtypes = [C.c_void_p, fp_]
+        _lib.mra_mlp_get_params.argtypes = [C.c_void_p, fp_]
+        _lib.mra_mlp_forward.argtypes = [C.c_void_p, fp_, C.c_int, fp_, C.c_int]
+        _lib.mra_mlp_loss_grad.argtypes = [C.c_void_p, fp_, fp_, C.c_int, fp_, fp_]
+        _lib.mra_mlp_train_step.argtypes = [C.c_void_p, fp_, fp_, C.c_int, C.c_float, fp_]
     return _lib
 
 
@@ -136,6 +150,70 @@ def bobyqa(fun, x0, lo, hi, rho_begin, rho_end, max_evals):
     if rc < 0:
         check(rc)
     return x, fb.value, ne.value, rc
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Mlp:
+    """The warm-start network (mra_mlp_*): 64 -> 256 -> 256 -> 6, GELU."""
+
+    NIN, NOUT = 64, 6
+
+    def __init__(self, seed=0):
+        h = C.c_void_p()
+        rc = lib().mra_mlp_create(seed, C.byref(h))
+        if rc:
+            raise MraError(rc, lib().mra_mlp_last_error(None).decode())
+        self.h = h
+        self.nparam = lib().mra_mlp_param_count()
+
+    def _check(self, rc):
+        if rc:
+            raise MraError(rc, lib().mra_mlp_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mra_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.zeros(self.nparam, dtype=np.float32)
+        self._check(lib().mra_mlp_get_params(self.h, _f(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparam
+        self._check(lib().mra_mlp_set_params(self.h, _f(p)))
+
+    def forward(self, X, bf16=False):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((X.shape[0], self.NOUT), dtype=np.float32)
+        self._check(lib().mra_mlp_forward(self.h, _f(X), X.shape[0], _f(Y), int(bf16)))
+        return Y
+
+    def loss_grad(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.zeros(self.nparam, dtype=np.float32)
+        loss = C.c_float(0)
+        self._check(lib().mra_mlp_loss_grad(self.h, _f(X), _f(T), X.shape[0], C.byref(loss), _f(g)))
+        return loss.value, g
+
+    def train_step(self, X, T, lr):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        loss = C.c_float(0)
+        self._check(lib().mra_mlp_train_step(self.h, _f(X), _f(T), X.shape[0], lr, C.byref(loss)))
+        return loss.value
 
 
 def selftest_blk(S):
@@ -255,6 +333,13 @@ class Handle:
         if rc < 0:
             check(rc, self.h)
         return th, D.value, ne.value, rc
+
+    def variogram_features(self):
+        """64 log-semivariogram features of the current observation values
+        (mra_variogram_features; planar metric)."""
+        f = np.zeros(64, dtype=np.float32)
+        check(lib().mra_variogram_features(self.h, _f(f)), self.h)
+        return f
 
     def close(self):
         if getattr(self, "h", None):

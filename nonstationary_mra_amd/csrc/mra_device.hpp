@@ -222,7 +222,8 @@ struct Mfma<float> {
 
 // Operands of one wave block: lane offsets (bytes) of the two row tiles of
 // A and the two column tiles of B at k = k4, and the per-k strides.
-template <bool TA, typename TIA, typename TIB>
+// TB: B stored transposed, B(k,j) = B[j*ldb+k].
+template <bool TA, typename TIA, typename TIB, bool TB = false>
 struct BlockOps {
     const char* A;
     const char* B;
@@ -234,17 +235,17 @@ struct BlockOps {
         : A((const char*)A_), B((const char*)B_), ha(ha_), hb(hb_), hja(hja_), hjb(hjb_) {
         const int iac = ha ? ia : 0, ibc = hb ? ib : 0, jac = hja ? ja : 0, jbc = hjb ? jb : 0;
         sa = (TA ? 1 : lda) * (int64_t)sizeof(TIA);
-        sb = ldb * (int64_t)sizeof(TIB);
+        sb = (TB ? 1 : ldb) * (int64_t)sizeof(TIB);
         oa0 = (uint32_t)((TA ? iac * lda + k4 : k4 * lda + iac) * sizeof(TIA));
         oa1 = (uint32_t)((TA ? ibc * lda + k4 : k4 * lda + ibc) * sizeof(TIA));
-        ob0 = (uint32_t)((k4 * ldb + jac) * sizeof(TIB));
-        ob1 = (uint32_t)((k4 * ldb + jbc) * sizeof(TIB));
+        ob0 = (uint32_t)((TB ? jac * ldb + k4 : k4 * ldb + jac) * sizeof(TIB));
+        ob1 = (uint32_t)((TB ? jbc * ldb + k4 : k4 * ldb + jbc) * sizeof(TIB));
     }
 };
 
-template <typename TC, bool M2, bool N2, bool PRED, int U, bool TA, typename TIA, typename TIB>
-__device__ __forceinline__ void block_kloop(typename Mfma<TC>::V (&acc)[4], const BlockOps<TA, TIA, TIB>& o, int kb,
-                                            int ke, int k4) {
+template <typename TC, bool M2, bool N2, bool PRED, int U, bool TA, typename TIA, typename TIB, bool TB>
+__device__ __forceinline__ void block_kloop(typename Mfma<TC>::V (&acc)[4], const BlockOps<TA, TIA, TIB, TB>& o,
+                                            int kb, int ke, int k4) {
     int k0 = kb;
     for (; k0 + 4 * U <= ke; k0 += 4 * U) {
         TIA a0[U], a1[U];
@@ -303,8 +304,8 @@ __device__ __forceinline__ void block_kloop(typename Mfma<TC>::V (&acc)[4], cons
 }
 
 // dispatch on the block shape (wave-uniform)
-template <typename TC, bool PRED, int U, bool TA, typename TIA, typename TIB>
-__device__ __forceinline__ void block_kloop_any(typename Mfma<TC>::V (&acc)[4], const BlockOps<TA, TIA, TIB>& o,
+template <typename TC, bool PRED, int U, bool TA, typename TIA, typename TIB, bool TB>
+__device__ __forceinline__ void block_kloop_any(typename Mfma<TC>::V (&acc)[4], const BlockOps<TA, TIA, TIB, TB>& o,
                                                 bool m2, bool n2, int kb, int ke, int k4) {
     if (m2 && n2)
         block_kloop<TC, true, true, PRED, U>(acc, o, kb, ke, k4);
@@ -316,7 +317,7 @@ __device__ __forceinline__ void block_kloop_any(typename Mfma<TC>::V (&acc)[4], 
         block_kloop<TC, false, false, PRED, U>(acc, o, kb, ke, k4);
 }
 
-template <typename TC, bool TA, bool ACC, typename TIA, typename TIB, int TILES, typename TO>
+template <typename TC, bool TA, bool ACC, typename TIA, typename TIB, int TILES, typename TO, bool TB = false>
 __device__ __forceinline__ void wg_gemm_tn_impl(TO* C, int ldc, int M, int N, int Kv, TC alpha, const TIA* A,
                                                 int64_t lda, int Mv, const TIB* B, int64_t ldb, int Nv) {
     static_assert(!(TA && TILES == GEMM_FULL), "the full-tile path reads A untransposed");
@@ -337,7 +338,7 @@ __device__ __forceinline__ void wg_gemm_tn_impl(TO* C, int ldc, int M, int N, in
         const bool tM2 = bh == 2 && ti0 + 1 < tM, tN2 = bw == 2 && tj0 + 1 < tN;
         const bool ha = ia < Mv, hb = tM2 && ib < Mv;
         const bool hja = ja < Nv, hjb = tN2 && jb < Nv;
-        const BlockOps<TA, TIA, TIB> o(A, lda, B, ldb, ia, ib, ja, jb, ha, hb, hja, hjb, k4);
+        const BlockOps<TA, TIA, TIB, TB> o(A, lda, B, ldb, ia, ib, ja, jb, ha, hb, hja, hjb, k4);
         V acc[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) acc[q] = V{0, 0, 0, 0};

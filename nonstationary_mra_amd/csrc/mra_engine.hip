@@ -54,6 +54,12 @@ struct mra_handle {
     uint32_t nItemsLRAll = 0, nItemsBig = 0;
     int32_t* dSlot1 = nullptr;  // dense-message slot per finest-but-one region (-1: factored)
     std::vector<double> hObsVal;
+    // variogram features: sampled observation indices per quadrant
+    uint32_t* dVgIdx = nullptr;
+    uint32_t* dVgCnt = nullptr;
+    double* dVgHmax = nullptr;
+    unsigned long long* dVgAcc = nullptr;
+    bool vgReady = false;
     // device inputs
     std::vector<void*> allocs;
     double *dKlon = nullptr, *dKlat = nullptr, *dOlon = nullptr, *dOlat = nullptr, *dPlon = nullptr, *dPlat = nullptr;
@@ -335,6 +341,35 @@ static int alloc_engine(mra_handle* h, const mra_tree_desc* t) {
     RC_TRY(upload(h, &h->dOlon, t->obs_lon, h->nObs));
     RC_TRY(upload(h, &h->dOlat, t->obs_lat, h->nObs));
     h->hObsVal.assign(t->obs_val, t->obs_val + h->nObs);
+    {
+        // quadrants of the observations' bounding box; up to VG_S points per
+        // quadrant, evenly strided in storage order
+        std::vector<uint32_t> idx(4 * VG_S, 0), cnt(4, 0);
+        std::vector<double> hm(4, 0.0);
+        if (h->nObs) {
+            double x0 = t->obs_lon[0], x1 = x0, y0 = t->obs_lat[0], y1 = y0;
+            for (uint64_t i = 0; i < h->nObs; i++) {
+                x0 = std::min(x0, t->obs_lon[i]);
+                x1 = std::max(x1, t->obs_lon[i]);
+                y0 = std::min(y0, t->obs_lat[i]);
+                y1 = std::max(y1, t->obs_lat[i]);
+            }
+            const double xm = 0.5 * (x0 + x1), ym = 0.5 * (y0 + y1);
+            std::vector<std::vector<uint32_t>> qs(4);
+            for (uint64_t i = 0; i < h->nObs; i++)
+                qs[(t->obs_lon[i] >= xm ? 1 : 0) + (t->obs_lat[i] >= ym ? 2 : 0)].push_back((uint32_t)i);
+            for (int q = 0; q < 4; q++) {
+                const uint64_t nq = qs[q].size(), S = std::min<uint64_t>(nq, VG_S);
+                for (uint64_t k = 0; k < S; k++) idx[q * VG_S + k] = qs[q][k * nq / S];
+                cnt[q] = (uint32_t)S;
+                hm[q] = std::max(0.5 * (x1 - x0), 0.5 * (y1 - y0));
+            }
+        }
+        RC_TRY(upload(h, &h->dVgIdx, idx.data(), idx.size()));
+        RC_TRY(upload(h, &h->dVgCnt, cnt.data(), cnt.size()));
+        RC_TRY(upload(h, &h->dVgHmax, hm.data(), hm.size()));
+        RC_TRY(dalloc(h, &h->dVgAcc, 4 * VG_BINS * 2, h->allocs));
+    }
     RC_TRY(upload(h, &h->dY, t->obs_val, h->nObs));
     if (h->nPred) {
         RC_TRY(upload(h, &h->dPlon, t->pred_lon, h->nPred));
@@ -1329,5 +1364,45 @@ extern "C" int mra_selftest_blk(int n, const double* S, double* Lout, double* Xo
     HIP_TRY(hipMemcpy(Lout, dL, sizeof(double) * m, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(Xout, dX, sizeof(double) * m, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(bad, dbad, sizeof(int), hipMemcpyDeviceToHost));
+    return MRA_OK;
+}
+
+extern "C" int mra_variogram_features(mra_handle* h, float* features) {
+    if (!h || !features) return MRA_E_ARG;
+    if (h->metric != MRA_METRIC_PLANAR) {
+        set_err(h, "variogram features are defined for the planar metric");
+        return MRA_E_UNSUPPORTED;
+    }
+    k_fill_u64<<<1, 128, 0, h->stream>>>(h->dVgAcc, 4 * VG_BINS * 2);
+    dim3 grid(64, 4);
+    k_variogram<<<grid, 256, 0, h->stream>>>(h->dVgIdx, h->dVgCnt, h->dVgHmax, h->dOlon, h->dOlat, h->dY, h->dVgAcc);
+    HIP_TRY(hipGetLastError());
+    unsigned long long acc[4 * VG_BINS * 2];
+    HIP_TRY(hipMemcpyAsync(acc, h->dVgAcc, sizeof(acc), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (int q = 0; q < 4; q++) {
+        double g[VG_BINS];
+        bool have[VG_BINS];
+        for (int b = 0; b < VG_BINS; b++) {
+            const unsigned long long c = acc[(q * VG_BINS + b) * 2 + 1];
+            have[b] = c > 0;
+            g[b] = have[b] ? (double)acc[(q * VG_BINS + b) * 2] / VG_SCALE / (2.0 * (double)c) : 0.0;
+        }
+        // empty bins take the nearest non-empty bin of the quadrant
+        for (int b = 0; b < VG_BINS; b++) {
+            double v = 1e-12;
+            for (int r = 0; r < VG_BINS; r++) {
+                if (b - r >= 0 && have[b - r]) {
+                    v = g[b - r];
+                    break;
+                }
+                if (b + r < VG_BINS && have[b + r]) {
+                    v = g[b + r];
+                    break;
+                }
+            }
+            features[q * VG_BINS + b] = (float)std::log(std::max(v, 1e-12));
+        }
+    }
     return MRA_OK;
 }

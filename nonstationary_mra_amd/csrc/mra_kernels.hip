@@ -156,6 +156,11 @@ __global__ void k_scale(uint64_t n, const double* f0, const double* f1, const do
     }
 }
 
+__global__ void k_fill_u64(unsigned long long* p, uint64_t n) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
 __global__ void k_zero_int(int* p, uint64_t n) {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
@@ -1079,6 +1084,49 @@ __global__ __launch_bounds__(256) void k_selftest(const double* A, const double*
         Xout[idx] = Wm[(idx / 64) * LD + idx % 64];
     }
     if (threadIdx.x == 0) *bad_out = bad;
+}
+
+// -------------------------------------------------- variogram features
+// Empirical semivariogram of the observations in 4 quadrants x 16 lag bins
+// (features of the warm-start network).  Pairs of the (at most VG_S) sampled
+// points of a quadrant; lag d falls in bin floor(16 sqrt(d / hmax)).  Sums of
+// (y_i - y_j)^2 are accumulated in fixed point (2^-24 units) with integer
+// atomics, so the result does not depend on the order of the additions.
+constexpr int VG_S = 1024, VG_BINS = 16;
+constexpr double VG_SCALE = 16777216.0;
+
+__global__ __launch_bounds__(256) void k_variogram(const uint32_t* idx, const uint32_t* cnt, const double* hmax,
+                                                   const double* x, const double* y, const double* val,
+                                                   unsigned long long* acc) {
+    __shared__ unsigned long long s_sum[VG_BINS];
+    __shared__ unsigned int s_cnt[VG_BINS];
+    const int q = blockIdx.y;
+    const int S = (int)cnt[q];
+    if (threadIdx.x < VG_BINS) {
+        s_sum[threadIdx.x] = 0ull;
+        s_cnt[threadIdx.x] = 0u;
+    }
+    __syncthreads();
+    const uint32_t* id = idx + q * VG_S;
+    const double hm = hmax[q];
+    const int64_t np = (int64_t)S * S;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(p / S), j = (int)(p % S);
+        if (j <= i) continue;
+        const uint32_t a = id[i], b = id[j];
+        const double dx = x[a] - x[b], dy = y[a] - y[b];
+        const double d = sqrt(dx * dx + dy * dy);
+        if (d >= hm) continue;
+        const int bin = min(VG_BINS - 1, (int)(VG_BINS * sqrt(d / hm)));
+        const double e = val[a] - val[b];
+        atomicAdd(&s_sum[bin], (unsigned long long)(e * e * VG_SCALE + 0.5));
+        atomicAdd(&s_cnt[bin], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < VG_BINS) {
+        atomicAdd(&acc[(q * VG_BINS + threadIdx.x) * 2], s_sum[threadIdx.x]);
+        atomicAdd(&acc[(q * VG_BINS + threadIdx.x) * 2 + 1], (unsigned long long)s_cnt[threadIdx.x]);
+    }
 }
 
 // Blocked Cholesky / triangular inverse on one N x N matrix S (row-major):

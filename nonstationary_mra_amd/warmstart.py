@@ -1,0 +1,215 @@
+"""Learned warm start for the six-parameter maximum-likelihood fit.
+
+The parameters are theta = (a_s, bx_s, by_s, a_b, bx_b, log_tau2) of the
+planar linear-trend model (mra_fit_linear_trend):
+    log sigma^2 = a_s + bx_s x + by_s y,  log beta = a_b + bx_b x,  tau^2 = exp(log_tau2).
+
+Training fields are drawn from the MRA prior itself (mra_sample_prior) at
+fixed locations, with theta drawn uniformly from PRIOR_LO..PRIOR_HI; the
+features are the 64 log-semivariogram values of mra_variogram_features; the
+network (mra_mlp_*, 64 -> 256 -> 256 -> 6, GELU) is trained on the GPU by the
+library's own kernels on standardised features and targets.  At inference
+the predicted theta (clipped into the optimizer's bounds) replaces the
+default start of the optimizer.
+
+Command line (one GPU):
+    python -m nonstationary_mra_amd.warmstart train --fields 2048 --n 50000 --M 8 --r 32 --out FILE.npz
+    python -m nonstationary_mra_amd.warmstart evaluate --model FILE.npz --fields 64 --n 50000 --M 8 --r 32
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import capi, synth
+
+NTHETA = 6
+NAMES = ("a_s", "bx_s", "by_s", "a_b", "bx_b", "log_tau2")
+# sampling ranges of the training fields (log range in [log 0.02, log 0.2],
+# range and variance slopes in [-1, 1], nugget in [0.01, 0.2])
+PRIOR_LO = np.array([-0.5, -1.0, -0.5, np.log(0.02), -1.0, np.log(0.01)])
+PRIOR_HI = np.array([0.5, 1.0, 0.5, np.log(0.2), 1.0, np.log(0.2)])
+# optimizer bounds and the plain default start
+BOUND_LO = np.array([-3.0, -3.0, -3.0, np.log(0.005), -3.0, np.log(1e-3)])
+BOUND_HI = np.array([3.0, 3.0, 3.0, np.log(1.0), 3.0, np.log(1.0)])
+DEFAULT_START = np.array([0.0, 0.0, 0.0, np.log(0.05), 0.0, np.log(0.05)])
+RHO_BEGIN, RHO_END, MAX_EVALS = 0.3, 1e-4, 800
+# initial trust radius from the network's start: of the order of its
+# held-out prediction error in the standardised parameters, so the optimizer
+# does not spend its first evaluations re-exploring the whole box
+RHO_BEGIN_WARM = 0.05
+
+
+def trend_cov(theta):
+    a_s, bx_s, by_s, a_b, bx_b, lt = (float(v) for v in theta)
+    return dict(nonstationary=1, metric=synth.METRIC["planar"], wendland_len=1.0,
+                sigmasq=dict(kind=synth.FIELD_LINEAR, intercept=a_s, slope_x=bx_s, slope_y=by_s),
+                beta=dict(kind=synth.FIELD_LINEAR, intercept=a_b, slope_x=bx_b, slope_y=0.0),
+                tausq=dict(kind=synth.FIELD_CONSTANT, intercept=lt))
+
+
+def make_locations(n, M, r, seed, J=4):
+    rng = np.random.default_rng(seed)
+    x, y = synth.uniform_points(n, rng)
+    return capi.partition_program(x, y, np.zeros(n), None, None, M=M, J=J, r=r, bounds=synth.UNIT_BOUNDS)
+
+
+def draw_theta(rng, k):
+    return PRIOR_LO + (PRIOR_HI - PRIOR_LO) * rng.random((k, NTHETA))
+
+
+def simulate(handle, theta, seed):
+    """Draw a field from the MRA prior with parameters theta into the handle's
+    observations; returns its features."""
+    handle.set_covariance(trend_cov(theta))
+    handle.set_params(**synth.UNIT_PARAMS)
+    handle.sample_prior(seed, replace=True)
+    return handle.variogram_features()
+
+
+def training_set(n_fields, n, M, r, seed=2026):
+    tree = make_locations(n, M, r, seed)
+    h = capi.Handle(tree, with_pred=False)
+    rng = np.random.default_rng(seed + 1)
+    th = draw_theta(rng, n_fields)
+    X = np.zeros((n_fields, 64), dtype=np.float32)
+    try:
+        for i in range(n_fields):
+            X[i] = simulate(h, th[i], seed * 1000 + i)
+    finally:
+        h.close()
+    return X, th
+
+
+class WarmStart:
+    """Trained network plus the standardisation of features and targets."""
+
+    def __init__(self, params, fmean, fstd, tmean, tstd):
+        self.params = np.asarray(params, dtype=np.float32)
+        self.fmean, self.fstd = np.asarray(fmean, np.float32), np.asarray(fstd, np.float32)
+        self.tmean, self.tstd = np.asarray(tmean, np.float64), np.asarray(tstd, np.float64)
+        self.mlp = capi.Mlp(0)
+        self.mlp.set_params(self.params)
+
+    def predict(self, features, bf16=True):
+        Xs = (np.atleast_2d(features) - self.fmean) / self.fstd
+        Y = self.mlp.forward(Xs.astype(np.float32), bf16=bf16).astype(np.float64)
+        th = Y * self.tstd + self.tmean
+        return np.clip(th, BOUND_LO + 1e-6, BOUND_HI - 1e-6)
+
+    def save(self, path):
+        np.savez(path, params=self.params, fmean=self.fmean, fstd=self.fstd, tmean=self.tmean, tstd=self.tstd)
+
+    @staticmethod
+    def load(path):
+        z = np.load(path, allow_pickle=False)
+        return WarmStart(z["params"], z["fmean"], z["fstd"], z["tmean"], z["tstd"])
+
+
+def train(X, theta, epochs=300, batch=256, lr=1e-3, seed=0, val_frac=0.1, log=None):
+    """Adam on standardised features / targets; returns (WarmStart, history)."""
+    rng = np.random.default_rng(seed)
+    n = X.shape[0]
+    perm = rng.permutation(n)
+    nv = max(1, int(round(val_frac * n))) if n > 10 else 0
+    vi, ti = perm[:nv], perm[nv:]
+    fmean, fstd = X[ti].mean(0), X[ti].std(0) + 1e-6
+    tmean, tstd = theta[ti].mean(0), theta[ti].std(0) + 1e-12
+    Xs = ((X - fmean) / fstd).astype(np.float32)
+    Ts = ((theta - tmean) / tstd).astype(np.float32)
+    m = capi.Mlp(seed)
+    hist = []
+    for ep in range(epochs):
+        order = rng.permutation(ti)
+        tot = 0.0
+        for b0 in range(0, len(order), batch):
+            bi = order[b0:b0 + batch]
+            tot += m.train_step(Xs[bi], Ts[bi], lr) * len(bi)
+        tl = tot / len(ti)
+        vl = float(np.mean((m.forward(Xs[vi]) - Ts[vi]) ** 2)) if nv else float("nan")
+        hist.append((tl, vl))
+        if log and (ep % 25 == 0 or ep == epochs - 1):
+            log(f"epoch {ep} train {tl:.4f} val {vl:.4f}")
+    ws = WarmStart(m.get_params(), fmean, fstd, tmean, tstd)
+    m.close()
+    return ws, hist
+
+
+def fit(handle, theta0, max_evals=MAX_EVALS, rho_begin=RHO_BEGIN):
+    """The optimizer from theta0; returns (theta, D, n_evals, status)."""
+    return handle.fit_linear_trend(theta0, BOUND_LO, BOUND_HI, rho_begin, RHO_END, max_evals)
+
+
+def compare(ws, n_fields, n, M, r, seed=77, log=None, max_evals=MAX_EVALS):
+    """Held-out fields: optimizer evaluations and final D from the network's
+    start vs the default start."""
+    tree = make_locations(n, M, r, seed)
+    h = capi.Handle(tree, with_pred=False)
+    rng = np.random.default_rng(seed + 1)
+    th = draw_theta(rng, n_fields)
+    rows = []
+    try:
+        for i in range(n_fields):
+            # the fits change only the covariance model, never the observations
+            f = simulate(h, th[i], seed * 1000 + i)
+            t0 = ws.predict(f)[0] if ws is not None else None
+            _, Dd, ned, sd = fit(h, DEFAULT_START, max_evals)
+            row = dict(field=i, theta=th[i].tolist(), D_default=Dd, evals_default=ned, status_default=sd)
+            if t0 is not None:
+                _, Dw, new, sw = fit(h, t0, max_evals, RHO_BEGIN_WARM)
+                row.update(D_warm=Dw, evals_warm=new, status_warm=sw, theta0=t0.tolist())
+            rows.append(row)
+            if log:
+                log(json.dumps({k: (round(v, 4) if isinstance(v, float) else v) for k, v in row.items()
+                                if k not in ("theta", "theta0")}))
+    finally:
+        h.close()
+    return rows
+
+
+def summarize(rows):
+    ed = np.array([r["evals_default"] for r in rows], dtype=float)
+    out = dict(fields=len(rows), mean_evals_default=float(ed.mean()))
+    if "evals_warm" in rows[0]:
+        ew = np.array([r["evals_warm"] for r in rows], dtype=float)
+        dD = np.array([r["D_warm"] - r["D_default"] for r in rows])
+        out.update(mean_evals_warm=float(ew.mean()), evals_ratio=float(ed.mean() / ew.mean()),
+                   median_evals_ratio=float(np.median(ed / ew)),
+                   mean_D_warm_minus_default=float(dD.mean()), max_D_warm_minus_default=float(dD.max()))
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("cmd", choices=["train", "evaluate"])
+    ap.add_argument("--fields", type=int, default=2048)
+    ap.add_argument("--n", type=int, default=50000)
+    ap.add_argument("--M", type=int, default=8)
+    ap.add_argument("--r", type=int, default=32)
+    ap.add_argument("--epochs", type=int, default=300)
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--model", default=None)
+    ap.add_argument("--max-evals", type=int, default=MAX_EVALS)
+    a = ap.parse_args(argv)
+    log = lambda s: print(f"[warmstart] {s}", file=sys.stderr, flush=True)
+    if a.cmd == "train":
+        t0 = time.time()
+        X, th = training_set(a.fields, a.n, a.M, a.r)
+        log(f"{a.fields} fields simulated in {time.time() - t0:.1f}s")
+        ws, hist = train(X, th, epochs=a.epochs, log=log)
+        log(f"trained in {time.time() - t0:.1f}s; final train {hist[-1][0]:.4f} val {hist[-1][1]:.4f}")
+        if a.out:
+            ws.save(a.out)
+        print(json.dumps(dict(fields=a.fields, n=a.n, M=a.M, r=a.r, epochs=a.epochs, train_loss=hist[-1][0],
+                              val_loss=hist[-1][1], seconds=time.time() - t0)), flush=True)
+    else:
+        ws = WarmStart.load(a.model) if a.model else None
+        rows = compare(ws, a.fields, a.n, a.M, a.r, log=log, max_evals=a.max_evals)
+        print(json.dumps(summarize(rows)), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,34 @@
+"""CPU checks of the warm-start oracle (oracle/mlp.py): its analytic
+gradient against central finite differences of its loss, and the feature
+definition on fields with known semivariograms."""
+import numpy as np
+
+from oracle import mlp as om
+
+
+def test_oracle_gradient_matches_finite_differences():
+    rng = np.random.default_rng(4)
+    p = rng.standard_normal(om.NPARAM) * 0.05
+    X = rng.standard_normal((9, om.NIN))
+    T = rng.standard_normal((9, om.NOUT))
+    _, g = om.loss_grad(p, X, T)
+    idx = np.concatenate([rng.choice(om.NPARAM, 60, replace=False), [om.NPARAM - 1, 0]])
+    h = 1e-6
+    for i in idx:
+        e = np.zeros(om.NPARAM)
+        e[i] = h
+        fd = (om.loss(p + e, X, T) - om.loss(p - e, X, T)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-7 + 1e-6 * abs(g[i]), (i, fd, g[i])
+
+
+def test_feature_definition_constant_and_scaled_fields():
+    rng = np.random.default_rng(5)
+    x, y = rng.random(3000), rng.random(3000)
+    f0 = om.variogram_features(x, y, np.full(3000, 2.5))
+    assert np.all(f0 == np.log(1e-12))  # constant field: zero semivariance
+    v = rng.standard_normal(3000)
+    f1 = om.variogram_features(x, y, v)
+    f2 = om.variogram_features(x, y, 3.0 * v)
+    np.testing.assert_allclose(f2 - f1, np.log(9.0), atol=1e-12)  # gamma scales with the square
+    # white noise of variance 1: semivariance 1 at every lag (sample average)
+    assert abs(np.exp(f1).mean() - 1.0) < 0.05
