@@ -213,6 +213,24 @@ enum { MRA_TREND_NPARAM = 6 };
 int mra_fit_linear_trend(mra_handle* h, int metric, double* theta, const double* lo, const double* hi,
                          double rho_begin, double rho_end, int max_evals, double* D_best, int* n_evals);
 
+/* ------------------------------------------------------------------ batch */
+/* Many independent fields (one handle each, any sizes) evaluated as one
+ * launch sequence (no reference counterpart; the reference runs one data set
+ * per MPI job): handle i's evaluation is issued on batch stream i mod
+ * `streams` (1..8), as a graph replay (graph = 1) or direct launches, and the
+ * host waits once for all.  The handles must stay alive while the batch
+ * exists and must not be used concurrently from elsewhere. */
+typedef struct mra_batch mra_batch;
+int mra_batch_create(mra_handle* const* handles, int n, mra_batch** out);
+void mra_batch_destroy(mra_batch* b);
+int mra_batch_set(mra_batch* b, int streams, int graph);
+int mra_batch_get(mra_batch* b, int* streams, int* graph);
+/* D[i] of handle i; status[i] = MRA_OK or MRA_E_NOT_PD (status may be NULL). */
+int mra_batch_likelihood(mra_batch* b, double* D, int* status);
+/* Times one batch evaluation for streams in {1,2,4,8} x graph in {0,1} and
+ * keeps the fastest setting (returned; best_ms = its wall time). */
+int mra_batch_autotune(mra_batch* b, int* streams, int* graph, double* best_ms);
+
 /* ---------------------------------------------------- learned warm start */
 /* (No reference counterpart: the reference starts its optimizer from a
  * fixed guess, src/optimization.cpp:160-169.)

@@ -31,7 +31,8 @@ EXPORTED = ["mra_create", "mra_destroy", "mra_last_error", "mra_set_covariance",
             "mra_partition_program", "mra_version", "mra_selftest", "mra_set_option", "mra_get_option", "mra_selftest_f32", "mra_selftest_blk",
             "mra_bobyqa", "mra_fit_linear_trend", "mra_variogram_features", "mra_mlp_create", "mra_mlp_destroy",
             "mra_mlp_last_error", "mra_mlp_param_count", "mra_mlp_set_params", "mra_mlp_get_params",
-            "mra_mlp_forward", "mra_mlp_loss_grad", "mra_mlp_train_step"]
+            "mra_mlp_forward", "mra_mlp_loss_grad", "mra_mlp_train_step", "mra_batch_create", "mra_batch_destroy",
+            "mra_batch_set", "mra_batch_get", "mra_batch_likelihood", "mra_batch_autotune"]
 OPT_FACTORED_POSTERIOR = 1
 
 
@@ -105,6 +106,13 @@ def lib():
                                     C.c_int, dp, C.POINTER(C.c_int)]
         _lib.mra_fit_linear_trend.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, C.c_double, C.c_double, C.c_int, dp,
                                               C.POINTER(C.c_int)]
+        _lib.mra_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]
+        _lib.mra_batch_destroy.argtypes = [C.c_void_p]
+        _lib.mra_batch_destroy.restype = None
+        _lib.mra_batch_set.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        _lib.mra_batch_get.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        _lib.mra_batch_likelihood.argtypes = [C.c_void_p, dp, C.POINTER(C.c_int)]
+        _lib.mra_batch_autotune.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), dp]
         fp_ = C.POINTER(C.c_float)
         _lib.mra_variogram_features.argtypes = [C.c_void_p, fp_]
         _lib.mra_mlp_create.argtypes = [C.c_uint64, C.POINTER(C.c_void_p)]
@@ -152,6 +160,48 @@ def bobyqa(fun, x0, lo, hi, rho_begin, rho_end, max_evals):
     return x, fb.value, ne.value, rc
 
 
+class Batch:
+    """Independent handles evaluated as one launch sequence (mra_batch_*)."""
+
+    def __init__(self, handles):
+        self.handles = list(handles)  # keeps the handles alive
+        arr = (C.c_void_p * len(self.handles))(*[h.h for h in self.handles])
+        b = C.c_void_p()
+        check(lib().mra_batch_create(arr, len(self.handles), C.byref(b)))
+        self.b = b
+
+    def set(self, streams, graph):
+        check(lib().mra_batch_set(self.b, streams, int(graph)))
+
+    def get(self):
+        s, g = C.c_int(0), C.c_int(0)
+        check(lib().mra_batch_get(self.b, C.byref(s), C.byref(g)))
+        return s.value, g.value
+
+    def likelihood(self):
+        n = len(self.handles)
+        D = np.zeros(n)
+        st = (C.c_int * n)()
+        check(lib().mra_batch_likelihood(self.b, _d(D), st))
+        return D, np.array(st[:], dtype=np.int32)
+
+    def autotune(self):
+        s, g, ms = C.c_int(0), C.c_int(0), C.c_double(0)
+        check(lib().mra_batch_autotune(self.b, C.byref(s), C.byref(g), C.byref(ms)))
+        return s.value, g.value, ms.value
+
+    def close(self):
+        if getattr(self, "b", None):
+            lib().mra_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _f(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
 
@@ -194,10 +244,17 @@ class Mlp:
         assert p.size == self.nparam
         self._check(lib().mra_mlp_set_params(self.h, _f(p)))
 
+    MAXB = 1024
+
     def forward(self, X, bf16=False):
+        """Outputs for every row of X (in chunks of at most 1024 rows)."""
         X = np.ascontiguousarray(X, dtype=np.float32)
         Y = np.zeros((X.shape[0], self.NOUT), dtype=np.float32)
-        self._check(lib().mra_mlp_forward(self.h, _f(X), X.shape[0], _f(Y), int(bf16)))
+        for b0 in range(0, X.shape[0], self.MAXB):
+            xb = np.ascontiguousarray(X[b0:b0 + self.MAXB])
+            yb = np.zeros((xb.shape[0], self.NOUT), dtype=np.float32)
+            self._check(lib().mra_mlp_forward(self.h, _f(xb), xb.shape[0], _f(yb), int(bf16)))
+            Y[b0:b0 + xb.shape[0]] = yb
         return Y
 
     def loss_grad(self, X, T):

@@ -370,6 +370,47 @@ __device__ __forceinline__ void wg_gemm_tn_impl(TO* C, int ldc, int M, int N, in
     }
 }
 
+// C = al1 * A1^T B1 + al2 * A2^T B2 stored to C (M x N, all tiles in range:
+// Mv >= M, Nv >= N), both products accumulated in registers and written
+// once: no LDS, no barrier.  FP64.
+__device__ __forceinline__ void wg_gemm2_store(double* C, int ldc, int M, int N, int K1, double al1, const double* A1,
+                                               int64_t lda1, const double* B1, int64_t ldb1, int K2, double al2,
+                                               const double* A2, int64_t lda2, const double* B2, int64_t ldb2) {
+    typedef Mfma<double>::V V;
+    const int lane = threadIdx.x & 63, wave = wave_id();
+    const int r16 = lane & 15, k4 = lane >> 4;
+    const int tM = M >> 4, tN = N >> 4;
+    const int nt = tM * tN;
+    const int bh = (nt >= 16 || (nt >= 8 && tM >= 2)) ? 2 : 1;
+    const int bw = (nt >= 16) ? 2 : 1;
+    const int bM = (tM + bh - 1) / bh, bN = (tN + bw - 1) / bw;
+    for (int blk = wave; blk < bM * bN; blk += 4) {
+        const int ti0 = (blk / bN) * bh, tj0 = (blk % bN) * bw;
+        const int ia = ti0 * 16 + r16, ib = ia + 16;
+        const int ja = tj0 * 16 + r16, jb = ja + 16;
+        const bool tM2 = bh == 2 && ti0 + 1 < tM, tN2 = bw == 2 && tj0 + 1 < tN;
+        V acc1[4], acc2[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc1[q] = acc2[q] = V{0, 0, 0, 0};
+        const BlockOps<false, double, double> o1(A1, lda1, B1, ldb1, ia, ib, ja, jb, true, tM2, true, tN2, k4);
+        block_kloop_any<double, false, 4>(acc1, o1, tM2, tN2, 0, K1, k4);
+        const BlockOps<false, double, double> o2(A2, lda2, B2, ldb2, ia, ib, ja, jb, true, tM2, true, tN2, k4);
+        block_kloop_any<double, false, 4>(acc2, o2, tM2, tN2, 0, K2, k4);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = ti0 * 16 + k4 + 4 * v;
+            double* c0 = C + (int64_t)row * ldc + ja;
+            c0[0] = al1 * acc1[0][v] + al2 * acc2[0][v];
+            if (tN2) c0[16] = al1 * acc1[1][v] + al2 * acc2[1][v];
+            if (tM2) {
+                double* c1 = c0 + (int64_t)16 * ldc;
+                c1[0] = al1 * acc1[2][v] + al2 * acc2[2][v];
+                if (tN2) c1[16] = al1 * acc1[3][v] + al2 * acc2[3][v];
+            }
+        }
+    }
+}
+
 // FP64 MFMA, FP64 accumulation; operands read as TIA / TIB and widened
 template <bool TA = false, bool ACC = true, typename TIA = double, typename TIB = double, int TILES = GEMM_PART>
 __device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int Kv, double alpha, const TIA* A,

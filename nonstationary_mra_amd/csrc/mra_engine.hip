@@ -2,6 +2,7 @@
 // mra_kernels.hip.  One handle = one uploaded MRA structure, one HIP stream,
 // all device buffers sized once at create (no allocation per evaluation).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1404,5 +1405,155 @@ extern "C" int mra_variogram_features(mra_handle* h, float* features) {
             features[q * VG_BINS + b] = (float)std::log(std::max(v, 1e-12));
         }
     }
+    return MRA_OK;
+}
+
+// ------------------------------------------------------------------ batch
+// Independent handles (fields) evaluated as one launch sequence: every
+// handle's evaluation is issued on one of `streams` batch streams (handle i
+// on stream i mod streams), graph replay or direct launches, with no host
+// synchronisation until all are issued.  mra_batch_autotune times the
+// candidate settings on the actual batch and keeps the fastest.
+struct mra_batch {
+    std::vector<mra_handle*> h;
+    std::vector<hipStream_t> st;
+    int streams = 4;
+    int graph = 1;
+    double* hOut = nullptr;  // pinned: D and bad flag per handle
+    std::string err;
+};
+
+extern "C" void mra_batch_destroy(mra_batch* b) {
+    if (!b) return;
+    for (hipStream_t s : b->st)
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    if (b->hOut) (void)hipHostFree(b->hOut);
+    delete b;
+}
+
+static constexpr int MRA_BATCH_MAX_STREAMS = 8;
+
+extern "C" int mra_batch_create(mra_handle* const* handles, int n, mra_batch** out) {
+    if (!handles || n < 1 || !out) return MRA_E_ARG;
+    *out = nullptr;
+    mra_batch* b = new mra_batch();
+    for (int i = 0; i < n; i++) {
+        if (!handles[i]) {
+            delete b;
+            return MRA_E_ARG;
+        }
+        b->h.push_back(handles[i]);
+    }
+    b->st.assign(MRA_BATCH_MAX_STREAMS, nullptr);
+    for (auto& s : b->st)
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+            g_err = "hipStreamCreate failed";
+            mra_batch_destroy(b);
+            return MRA_E_HIP;
+        }
+    if (hipHostMalloc((void**)&b->hOut, sizeof(double) * 2 * n, hipHostMallocDefault) != hipSuccess) {
+        g_err = "hipHostMalloc failed";
+        mra_batch_destroy(b);
+        return MRA_E_HIP;
+    }
+    *out = b;
+    return MRA_OK;
+}
+
+extern "C" int mra_batch_set(mra_batch* b, int streams, int graph) {
+    if (!b || streams < 1 || streams > MRA_BATCH_MAX_STREAMS) return MRA_E_ARG;
+    b->streams = streams;
+    b->graph = graph != 0;
+    return MRA_OK;
+}
+
+extern "C" int mra_batch_get(mra_batch* b, int* streams, int* graph) {
+    if (!b) return MRA_E_ARG;
+    if (streams) *streams = b->streams;
+    if (graph) *graph = b->graph;
+    return MRA_OK;
+}
+
+// D[i] = D of handle i; status[i] = MRA_OK or MRA_E_NOT_PD (may be NULL).
+// Returns the first error that is not a non-positive pivot.
+extern "C" int mra_batch_likelihood(mra_batch* b, double* D, int* status) {
+    if (!b || !D) return MRA_E_ARG;
+    const int n = (int)b->h.size();
+    for (int i = 0; i < n; i++) {
+        mra_handle* h = b->h[i];
+        if (!h->covSet || !h->paramsSet) {
+            b->err = "covariance model and parameters must be set on every handle";
+            return MRA_E_STATE;
+        }
+        if (!use_lowrank(h)) RC_TRY(ensure_dense_msgs(h));
+    }
+    for (int i = 0; i < n; i++) {
+        mra_handle* h = b->h[i];
+        hipStream_t own = h->stream;
+        hipStream_t s = b->st[i % b->streams];
+        const double prm[6] = {h->sill, h->range, h->nugget, h->ss, h->rs, h->ns};
+        for (int k = 0; k < 6; k++) h->hPrm[k] = prm[k];
+        if (b->graph) {
+            // capture (first use) happens on the handle's own stream
+            const int g = h->precision == MRA_PRECISION_MIXED ? 1 : 0;
+            if (!h->gexec[g]) {
+                HIP_TRY(hipMemcpyAsync(h->dPrm, h->hPrm, sizeof(prm), hipMemcpyHostToDevice, own));
+                RC_TRY(run_graph(h, false));
+                HIP_TRY(hipStreamSynchronize(own));
+            }
+            HIP_TRY(hipMemcpyAsync(h->dPrm, h->hPrm, sizeof(prm), hipMemcpyHostToDevice, s));
+            HIP_TRY(hipGraphLaunch(h->gexec[g], s));
+        } else {
+            HIP_TRY(hipMemcpyAsync(h->dPrm, h->hPrm, sizeof(prm), hipMemcpyHostToDevice, s));
+            h->stream = s;
+            launch_eval_rp(h, false, false);
+            h->stream = own;
+        }
+        HIP_TRY(hipMemcpyAsync(b->hOut + 2 * i, h->dD, sizeof(double), hipMemcpyDeviceToHost, s));
+        int* hb = (int*)(b->hOut + 2 * i + 1);
+        HIP_TRY(hipMemcpyAsync(hb, h->dBad, sizeof(int), hipMemcpyDeviceToHost, s));
+    }
+    {
+        mra_handle* const h = nullptr;  // errors below are batch-wide (mra_last_error(NULL))
+        HIP_TRY(hipGetLastError());
+        for (int k = 0; k < b->streams; k++) HIP_TRY(hipStreamSynchronize(b->st[k]));
+    }
+    for (int i = 0; i < n; i++) {
+        D[i] = b->hOut[2 * i];
+        const int bad = *(const int*)(b->hOut + 2 * i + 1);
+        if (status) status[i] = bad ? MRA_E_NOT_PD : MRA_OK;
+        b->h[i]->nEv = 0;
+    }
+    return MRA_OK;
+}
+
+extern "C" int mra_batch_autotune(mra_batch* b, int* streams, int* graph, double* best_ms) {
+    if (!b) return MRA_E_ARG;
+    const int n = (int)b->h.size();
+    std::vector<double> D(n);
+    const int cand_s[] = {1, 2, 4, 8};
+    double best = 1e300;
+    int bs = b->streams, bg = b->graph;
+    for (int g = 0; g <= 1; g++)
+        for (int cs : cand_s) {
+            b->streams = cs;
+            b->graph = g;
+            int rc = mra_batch_likelihood(b, D.data(), nullptr);  // warm-up (and capture)
+            if (rc) return rc;
+            auto t0 = std::chrono::steady_clock::now();
+            rc = mra_batch_likelihood(b, D.data(), nullptr);
+            const double ms =
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (rc) return rc;
+            if (ms < best) best = ms, bs = cs, bg = g;
+        }
+    b->streams = bs;
+    b->graph = bg;
+    if (streams) *streams = bs;
+    if (graph) *graph = bg;
+    if (best_ms) *best_ms = best;
     return MRA_OK;
 }

@@ -665,7 +665,41 @@ __global__ __launch_bounds__(256) void k_posterior(Tree T, PostArgs a) {
         }
         // A~_{jk} = A_{jk} - P_j^T P_k
         const int nsl = l * (l + 1) / 2;
-        for (int j = 0; j < l; j++)
+        if (FROM_OBS) {
+            // A_{jk} = X_j^T X_k over the stacked rows: both products in
+            // registers, stored straight to the message (no LDS, no barrier
+            // between blocks); children that sent dense messages are added after
+            const uint64_t loc = R - a.start_l;
+            const uint64_t os = a.row_s[loc], oe = a.row_e[loc];
+            const int ns = oe > os ? (int)(oe - os) : 0;
+            const double* X = a.X + os * a.ldx;
+            for (int j = 0; j < l; j++)
+                for (int k = j; k < l; k++) {
+                    const int sl = j * l - j * (j - 1) / 2 + (k - j);
+                    double* out = a.Ao + (msg * nsl + sl) * (uint64_t)RP * RP;
+                    wg_gemm2_store(out, RP, RP, RP, ns, 1.0, X + j * RP, a.ldx, X + k * RP, a.ldx, RP, -1.0,
+                                   PR + (uint64_t)j * RP * RP, RP, PR + (uint64_t)k * RP * RP, RP);
+                }
+            if (a.cslot) {
+                __syncthreads();
+                const int lc = l + 1;
+                const int nslc = lc * (lc + 1) / 2;
+                for (int c = 0; c < T.J; c++) {
+                    const uint64_t ch = R * T.J + 1 + c;
+                    const int32_t cs = a.cslot[ch - a.start_c];
+                    if (cs < 0) continue;
+                    for (int j = 0; j < l; j++)
+                        for (int k = j; k < l; k++) {
+                            const int sl = j * l - j * (j - 1) / 2 + (k - j);
+                            const int slc = j * lc - j * (j - 1) / 2 + (k - j);
+                            double* out = a.Ao + (msg * nsl + sl) * (uint64_t)RP * RP;
+                            const double* S = a.Ac + ((uint64_t)cs * nslc + slc) * (uint64_t)RP * RP;
+                            for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) out[idx] += S[idx];
+                        }
+                }
+            }
+        }
+        for (int j = 0; j < (FROM_OBS ? 0 : l); j++)
             for (int k = j; k < l; k++) {
                 for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) Km[(idx / RP) * LD + idx % RP] = 0.0;
                 __syncthreads();

@@ -122,6 +122,7 @@ def train(X, theta, epochs=300, batch=256, lr=1e-3, seed=0, val_frac=0.1, log=No
     Ts = ((theta - tmean) / tstd).astype(np.float32)
     m = capi.Mlp(seed)
     hist = []
+    best = (float("inf"), None)  # early stopping on the held-out loss
     for ep in range(epochs):
         order = rng.permutation(ti)
         tot = 0.0
@@ -131,9 +132,11 @@ def train(X, theta, epochs=300, batch=256, lr=1e-3, seed=0, val_frac=0.1, log=No
         tl = tot / len(ti)
         vl = float(np.mean((m.forward(Xs[vi]) - Ts[vi]) ** 2)) if nv else float("nan")
         hist.append((tl, vl))
+        if nv and vl < best[0]:
+            best = (vl, m.get_params())
         if log and (ep % 25 == 0 or ep == epochs - 1):
             log(f"epoch {ep} train {tl:.4f} val {vl:.4f}")
-    ws = WarmStart(m.get_params(), fmean, fstd, tmean, tstd)
+    ws = WarmStart(best[1] if best[1] is not None else m.get_params(), fmean, fstd, tmean, tstd)
     m.close()
     return ws, hist
 
@@ -143,7 +146,7 @@ def fit(handle, theta0, max_evals=MAX_EVALS, rho_begin=RHO_BEGIN):
     return handle.fit_linear_trend(theta0, BOUND_LO, BOUND_HI, rho_begin, RHO_END, max_evals)
 
 
-def compare(ws, n_fields, n, M, r, seed=77, log=None, max_evals=MAX_EVALS):
+def compare(ws, n_fields, n, M, r, seed=77, log=None, max_evals=MAX_EVALS, rho_warm=RHO_BEGIN_WARM):
     """Held-out fields: optimizer evaluations and final D from the network's
     start vs the default start."""
     tree = make_locations(n, M, r, seed)
@@ -159,7 +162,7 @@ def compare(ws, n_fields, n, M, r, seed=77, log=None, max_evals=MAX_EVALS):
             _, Dd, ned, sd = fit(h, DEFAULT_START, max_evals)
             row = dict(field=i, theta=th[i].tolist(), D_default=Dd, evals_default=ned, status_default=sd)
             if t0 is not None:
-                _, Dw, new, sw = fit(h, t0, max_evals, RHO_BEGIN_WARM)
+                _, Dw, new, sw = fit(h, t0, max_evals, rho_warm)
                 row.update(D_warm=Dw, evals_warm=new, status_warm=sw, theta0=t0.tolist())
             rows.append(row)
             if log:
@@ -193,21 +196,24 @@ def main(argv=None):
     ap.add_argument("--out", default=None)
     ap.add_argument("--model", default=None)
     ap.add_argument("--max-evals", type=int, default=MAX_EVALS)
+    ap.add_argument("--rho-warm", type=float, default=RHO_BEGIN_WARM)
+    ap.add_argument("--seed", type=int, default=2026)
     a = ap.parse_args(argv)
     log = lambda s: print(f"[warmstart] {s}", file=sys.stderr, flush=True)
     if a.cmd == "train":
         t0 = time.time()
-        X, th = training_set(a.fields, a.n, a.M, a.r)
+        X, th = training_set(a.fields, a.n, a.M, a.r, seed=a.seed)
         log(f"{a.fields} fields simulated in {time.time() - t0:.1f}s")
         ws, hist = train(X, th, epochs=a.epochs, log=log)
         log(f"trained in {time.time() - t0:.1f}s; final train {hist[-1][0]:.4f} val {hist[-1][1]:.4f}")
         if a.out:
             ws.save(a.out)
         print(json.dumps(dict(fields=a.fields, n=a.n, M=a.M, r=a.r, epochs=a.epochs, train_loss=hist[-1][0],
-                              val_loss=hist[-1][1], seconds=time.time() - t0)), flush=True)
+                              val_loss=hist[-1][1], best_val_loss=min(v for _, v in hist),
+                              seconds=time.time() - t0)), flush=True)
     else:
         ws = WarmStart.load(a.model) if a.model else None
-        rows = compare(ws, a.fields, a.n, a.M, a.r, log=log, max_evals=a.max_evals)
+        rows = compare(ws, a.fields, a.n, a.M, a.r, log=log, max_evals=a.max_evals, rho_warm=a.rho_warm)
         print(json.dumps(summarize(rows)), flush=True)
 
 
