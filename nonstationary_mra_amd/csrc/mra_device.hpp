@@ -317,9 +317,11 @@ __device__ __forceinline__ void block_kloop_any(typename Mfma<TC>::V (&acc)[4], 
         block_kloop<TC, false, false, PRED, U>(acc, o, kb, ke, k4);
 }
 
-template <typename TC, bool TA, bool ACC, typename TIA, typename TIB, int TILES, typename TO, bool TB = false>
+template <typename TC, bool TA, bool ACC, typename TIA, typename TIB, int TILES, typename TO, bool TB = false,
+          bool DUAL = false>
 __device__ __forceinline__ void wg_gemm_tn_impl(TO* C, int ldc, int M, int N, int Kv, TC alpha, const TIA* A,
-                                                int64_t lda, int Mv, const TIB* B, int64_t ldb, int Nv) {
+                                                int64_t lda, int Mv, const TIB* B, int64_t ldb, int Nv,
+                                                float* C32 = nullptr) {
     static_assert(!(TA && TILES == GEMM_FULL), "the full-tile path reads A untransposed");
     typedef typename Mfma<TC>::V V;
     const int lane = threadIdx.x & 63, wave = wave_id();
@@ -365,6 +367,15 @@ __device__ __forceinline__ void wg_gemm_tn_impl(TO* C, int ldc, int M, int N, in
                 if (r0 && hjb) c00[16] = alpha * acc[1][v];
                 if (r1 && hja) c10[0] = alpha * acc[2][v];
                 if (r1 && hjb) c10[16] = alpha * acc[3][v];
+                if (DUAL) {
+                    // the same values rounded to FP32 (second output, same layout)
+                    float* d00 = C32 + (int64_t)row * ldc + ja;
+                    float* d10 = C32 + (int64_t)(row + 16) * ldc + ja;
+                    if (r0 && hja) d00[0] = (float)(alpha * acc[0][v]);
+                    if (r0 && hjb) d00[16] = (float)(alpha * acc[1][v]);
+                    if (r1 && hja) d10[0] = (float)(alpha * acc[2][v]);
+                    if (r1 && hjb) d10[16] = (float)(alpha * acc[3][v]);
+                }
             }
         }
     }
@@ -412,10 +423,13 @@ __device__ __forceinline__ void wg_gemm2_store(double* C, int ldc, int M, int N,
 }
 
 // FP64 MFMA, FP64 accumulation; operands read as TIA / TIB and widened
-template <bool TA = false, bool ACC = true, typename TIA = double, typename TIB = double, int TILES = GEMM_PART>
+template <bool TA = false, bool ACC = true, typename TIA = double, typename TIB = double, int TILES = GEMM_PART,
+          bool DUAL = false>
 __device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int Kv, double alpha, const TIA* A,
-                                           int64_t lda, int Mv, const TIB* B, int64_t ldb, int Nv) {
-    wg_gemm_tn_impl<double, TA, ACC, TIA, TIB, TILES>(C, ldc, M, N, Kv, alpha, A, lda, Mv, B, ldb, Nv);
+                                           int64_t lda, int Mv, const TIB* B, int64_t ldb, int Nv,
+                                           float* C32 = nullptr) {
+    wg_gemm_tn_impl<double, TA, ACC, TIA, TIB, TILES, double, false, DUAL>(C, ldc, M, N, Kv, alpha, A, lda, Mv, B,
+                                                                          ldb, Nv, C32);
 }
 
 // FP32 MFMA with FP32 accumulation (operands rounded to float); C of type TC

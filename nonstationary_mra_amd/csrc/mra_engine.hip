@@ -72,6 +72,7 @@ struct mra_handle {
              *dSigOff = nullptr, *dGoff = nullptr;
     uint8_t* dWorking = nullptr;
     // engine storage
+    float* dGc32 = nullptr;  // FP32 copy of Gc (mixed precision, allocated on first use)
     double *dGc = nullptr, *dLt = nullptr, *dLiT = nullptr, *dLpt = nullptr, *dLpiT = nullptr, *dP = nullptr,
            *dV = nullptr, *dGf = nullptr, *dBf = nullptr, *dH = nullptr, *dU = nullptr, *dSig = nullptr,
            *dDterm = nullptr, *dLev = nullptr, *dD = nullptr;
@@ -801,10 +802,10 @@ static void launch_finest_cross(mra_handle* h, const Tree& T, const CovCtx& cc, 
     hipStream_t st = h->stream;
     if (h->nItemsX)
         k_finest_cross<RP, 64, TG><<<h->nItemsX, 256, 0, st>>>(T, cc, K, O, h->dItemsX, h->dGoff, h->dGc, h->dLiT,
-                                                               (TG*)h->dGf, (TG*)h->dBf);
+                                                               (TG*)h->dGf, (TG*)h->dBf, h->dGc32);
     if (h->nItemsX32)
         k_finest_cross<RP, 32, TG><<<h->nItemsX32, 256, 0, st>>>(T, cc, K, O, h->dItemsX32, h->dGoff, h->dGc,
-                                                                 h->dLiT, (TG*)h->dGf, (TG*)h->dBf);
+                                                                 h->dLiT, (TG*)h->dGf, (TG*)h->dBf, h->dGc32);
 }
 
 template <int RP>
@@ -835,12 +836,17 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
                                                      sets[k].s[2]);
     k_fill<<<nblk(h->nC + h->nF), 256, 0, st>>>(h->dDterm, h->nC + h->nF, 0.0);
     k_zero_int<<<1, 64, 0, st>>>(h->dBad, 1);
-    // coarse prior, level by level
+    // coarse prior, level by level (mixed precision: also the FP32 copy of G)
+    float* gc32 = h->precision == MRA_PRECISION_MIXED ? h->dGc32 : nullptr;
     for (int l = 0; l < h->L; l++) {
         const uint64_t nl = h->lvlStart[l + 1] - h->lvlStart[l];
         mark(MRA_T_PRIOR0 + l);
-        k_prior_coarse<RP><<<(unsigned)nl, 256, 0, st>>>(T, cc, K, l, h->lvlStart[l], h->dGoff, h->dGc, h->dLt,
-                                                         h->dLiT, h->dBad);
+        if (gc32)
+            k_prior_coarse<RP, true><<<(unsigned)nl, 256, 0, st>>>(T, cc, K, l, h->lvlStart[l], h->dGoff, h->dGc,
+                                                                   h->dLt, h->dLiT, h->dBad, gc32);
+        else
+            k_prior_coarse<RP, false><<<(unsigned)nl, 256, 0, st>>>(T, cc, K, l, h->lvlStart[l], h->dGoff, h->dGc,
+                                                                    h->dLt, h->dLiT, h->dBad, nullptr);
     }
     mark(MRA_T_FINEST_CROSS);
     const bool mixed = h->precision == MRA_PRECISION_MIXED;
@@ -998,6 +1004,16 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
     mark(MRA_T_COUNT);
 }
 
+// FP32 copy of the coarse G blocks, read by the mixed-precision finest-level
+// kernels (allocated at the first mixed-precision evaluation)
+static int ensure_mixed_buffers(mra_handle* h) {
+    if (h->precision != MRA_PRECISION_MIXED || h->dGc32) return MRA_OK;
+    float* p = nullptr;
+    RC_TRY(dalloc(h, &p, h->hGoff[h->L], h->allocs));
+    h->dGc32 = p;
+    return MRA_OK;
+}
+
 static void graph_invalidate(mra_handle* h) {
     for (int g = 0; g < mra_handle::NGRAPH; g++) {
         if (h->gexec[g]) (void)hipGraphExecDestroy(h->gexec[g]);
@@ -1063,6 +1079,7 @@ static int run_eval(mra_handle* h, bool predicting) {
         return MRA_E_STATE;
     }
     if (!use_lowrank(h)) RC_TRY(ensure_dense_msgs(h));
+    RC_TRY(ensure_mixed_buffers(h));
     const double prm[6] = {h->sill, h->range, h->nugget, h->ss, h->rs, h->ns};
     for (int i = 0; i < 6; i++) h->hPrm[i] = prm[i];
     HIP_TRY(hipMemcpyAsync(h->dPrm, h->hPrm, sizeof(prm), hipMemcpyHostToDevice, h->stream));
@@ -1489,6 +1506,7 @@ extern "C" int mra_batch_likelihood(mra_batch* b, double* D, int* status) {
             return MRA_E_STATE;
         }
         if (!use_lowrank(h)) RC_TRY(ensure_dense_msgs(h));
+        RC_TRY(ensure_mixed_buffers(h));
     }
     for (int i = 0; i < n; i++) {
         mra_handle* h = b->h[i];

@@ -233,10 +233,10 @@ __global__ void k_cross_cov(CovCtx cc, Pts P, uint64_t n1, Pts Q, uint64_t n2, d
 //   G_{R,j} = L_{a_j}^{-1} B                            (MFMA with L^{-1})
 // then K_R = V(Q_R) - Gcat_R^T Gcat_R, L_R = chol(K_R), L_R^{-1}.
 // (src/class_approximation.cpp:108-159)
-template <int RP>
-__global__ __launch_bounds__(256) void k_prior_coarse(Tree T, CovCtx cc, Pts K, int l, uint64_t start_l,
+template <int RP, bool COPY32>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_prior_coarse(Tree T, CovCtx cc, Pts K, int l, uint64_t start_l,
                                                       const uint64_t* goff, double* Gc, double* Lt, double* LiT,
-                                                      int* bad_flag) {
+                                                      int* bad_flag, float* Gc32) {
     constexpr int LD = RP + 1;
     __shared__ double Cm[RP * LD];
     __shared__ double stage[5 * RP];
@@ -256,8 +256,11 @@ __global__ __launch_bounds__(256) void k_prior_coarse(Tree T, CovCtx cc, Pts K, 
             __syncthreads();
         }
         // G_{R,j} = L_a^{-1} B, stored straight to global memory
-        wg_gemm_tn<false, false, double, double, GEMM_FULL>(GR + (uint64_t)j * RP * RP, RP, RP, RP, RP, 1.0,
-                                                            LiT + a * RP * RP, RP, RP, Cm, LD, RP);
+        // (mixed precision: also an FP32 copy for the finest-level products)
+        const uint64_t o = goff[l] + (uint64_t)blockIdx.x * l * RP * RP + (uint64_t)j * RP * RP;
+        wg_gemm_tn<false, false, double, double, GEMM_FULL, COPY32>(GR + (uint64_t)j * RP * RP, RP, RP, RP, RP, 1.0,
+                                                                    LiT + a * RP * RP, RP, RP, Cm, LD, RP,
+                                                                    COPY32 ? Gc32 + o : nullptr);
         __syncthreads();
     }
     cov_tile<RP, RP>(Cm, LD, cc, K, R * RP, r, K, R * RP, r, true, stage);
@@ -289,7 +292,7 @@ __global__ __launch_bounds__(256) void k_prior_coarse(Tree T, CovCtx cc, Pts K, 
 template <int RP, int NT, typename T>
 __global__ __launch_bounds__(256) void k_finest_cross(Tree T_, CovCtx cc, Pts K, Pts O, const uint32_t* items,
                                                       const uint64_t* goff, const double* Gc, const double* LiT,
-                                                      T* Gf, T* Bf) {
+                                                      T* Gf, T* Bf, const float* Gc32) {
     const Tree& T0 = T_;
     constexpr int LD = NT + 1;
     __shared__ T Cm[RP * LD];
@@ -306,9 +309,10 @@ __global__ __launch_bounds__(256) void k_finest_cross(Tree T_, CovCtx cc, Pts K,
         cov_tile<RP, NT, T>(Cm, LD, cc, K, a * RP, r, O, ob + c0, n, false, stage);
         __syncthreads();
         if (j > 0) {
-            const double* Ga = Gc + goff[j] + (a - lvl_start(j, T0.J)) * (uint64_t)j * RP * RP;
+            const uint64_t go = goff[j] + (a - lvl_start(j, T0.J)) * (uint64_t)j * RP * RP;
+            const double* Ga = Gc + go;
             if constexpr (sizeof(T) == 4)
-                wg_gemm_tn_f32(Cm, LD, RP, NT, j * RP, -1.0f, Ga, RP, RP, GF + c0, nF, n);
+                wg_gemm_tn_f32(Cm, LD, RP, NT, j * RP, -1.0f, Gc32 + go, RP, RP, GF + c0, nF, n);
             else
                 wg_gemm_tn<false, true, double, double, GEMM_ANY>(Cm, LD, RP, NT, j * RP, -1.0, Ga, RP, RP, GF + c0,
                                                                   nF, n);
