@@ -586,7 +586,6 @@ template <int RP, bool FROM_OBS>
 __global__ __launch_bounds__(256) void k_posterior(Tree T, PostArgs a) {
     constexpr int LD = RP + 1;
     __shared__ double Km[RP * LD];
-    __shared__ double Wm[RP * LD];
     __shared__ double vv[RP];
     __shared__ double wl[RP];
     __shared__ double red[MRA_WG];
@@ -647,18 +646,16 @@ __global__ __launch_bounds__(256) void k_posterior(Tree T, PostArgs a) {
             __syncthreads();
             post_add_A<RP, FROM_OBS>(T, a, R, j, l, Km, LD, true);
             __syncthreads();
-            for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) Wm[(idx / RP) * LD + idx % RP] = 0.0;
-            __syncthreads();
-            wg_gemm_tn<false, true, double, double, GEMM_FULL>(Wm, LD, RP, RP, RP, 1.0, LpiTR, RP, RP, Km, LD, RP);
-            __syncthreads();
+            // P_j stored straight to global memory (row-major RP x RP)
             double* Pj = PR + (uint64_t)j * RP * RP;
-            for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) Pj[idx] = Wm[(idx / RP) * LD + idx % RP];
+            wg_gemm_tn<false, false, double, double, GEMM_FULL>(Pj, RP, RP, RP, RP, 1.0, LpiTR, RP, RP, Km, LD, RP);
+            __syncthreads();
             // w~_j = w_j - P_j^T v
             if (a.write_msgs)
                 for (int q = threadIdx.x; q < RP; q += MRA_WG) {
                     double wj = (q < r) ? post_w<RP, FROM_OBS>(T, a, R, j, q) : 0.0;
                     double s = 0.0;
-                    for (int m = 0; m < RP; m++) s += Wm[m * LD + q] * vv[m];
+                    for (int m = 0; m < RP; m++) s += Pj[m * RP + q] * vv[m];
                     a.wo[msg * (uint64_t)l * RP + j * RP + q] = wj - s;
                 }
             __syncthreads();
