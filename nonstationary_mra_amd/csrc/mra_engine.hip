@@ -45,12 +45,12 @@ struct mra_handle {
     bool denseFull = false;  // full-size dense messages of level L-1 allocated
     std::vector<uint64_t> hRowS1, hRowE1, hRowS2, hRowE2;  // rows of levels L-1 and L-2
     uint64_t *dRowS1 = nullptr, *dRowE1 = nullptr, *dRowS2 = nullptr, *dRowE2 = nullptr;
-    double *dXo = nullptr, *dZo = nullptr, *dLrWs = nullptr;
-    uint64_t* dLrWsOff = nullptr;
-    // finest-but-one regions: factored buckets (rows <= 16, 32, 64), all
+    double *dXo = nullptr, *dZo = nullptr, *dLrWs = nullptr;  // factor rows, z', W workspace
+    // finest-but-one regions: factored buckets (rows <= 16, 32, 64, 128), all
     // factored ones, and the larger ones that keep the dense update
-    uint32_t* dItemsLR[3] = {};
-    uint32_t nItemsLR[3] = {};
+    static constexpr int NLR = 4;
+    uint32_t* dItemsLR[NLR] = {};
+    uint32_t nItemsLR[NLR] = {};
     uint32_t *dItemsLRAll = nullptr, *dItemsBig = nullptr;
     uint32_t nItemsLRAll = 0, nItemsBig = 0;
     int32_t* dSlot1 = nullptr;  // dense-message slot per finest-but-one region (-1: factored)
@@ -437,6 +437,7 @@ static int alloc_engine(mra_handle* h, const mra_tree_desc* t) {
         RC_TRY(upload(h, &h->dRowS2, h->hRowS2.data(), h->hRowS2.size()));
         RC_TRY(upload(h, &h->dRowE2, h->hRowE2.data(), h->hRowE2.size()));
         RC_TRY(dalloc(h, &h->dXo, (uint64_t)(h->L - 1) * RP * h->nObs, A));
+        RC_TRY(dalloc(h, &h->dLrWs, (uint64_t)RP * h->nObs, A));
         RC_TRY(dalloc(h, &h->dZo, h->nObs, A));
     }
     // work lists
@@ -484,20 +485,20 @@ static int alloc_engine(mra_handle* h, const mra_tree_desc* t) {
     if (h->lowrank) {
         const int l1 = h->L - 1;
         const uint64_t n1 = h->lvlStart[l1 + 1] - h->lvlStart[l1];
-        std::vector<uint32_t> b[3], all, big;
+        std::vector<uint32_t> b[mra_handle::NLR], all, big;
         std::vector<int32_t> slot(n1, -1);
         for (uint64_t k = 0; k < n1; k++) {
             if (!h->hWorking[h->lvlStart[l1] + k]) continue;
             const uint64_t ns = h->hRowE1[k] - h->hRowS1[k];
-            if (ns > 64) {
+            if (ns > 128) {
                 slot[k] = (int32_t)big.size();
                 big.push_back((uint32_t)k);
             } else {
-                b[ns <= 16 ? 0 : ns <= 32 ? 1 : 2].push_back((uint32_t)k);
+                b[ns <= 16 ? 0 : ns <= 32 ? 1 : ns <= 64 ? 2 : 3].push_back((uint32_t)k);
                 all.push_back((uint32_t)k);
             }
         }
-        for (int q = 0; q < 3; q++) {
+        for (int q = 0; q < mra_handle::NLR; q++) {
             h->nItemsLR[q] = (uint32_t)b[q].size();
             RC_TRY(upload(h, &h->dItemsLR[q], b[q].data(), b[q].size()));
         }
@@ -933,7 +934,6 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
                 b.zo = h->dZo;
                 b.dterm = h->dDterm;
                 b.ws = h->dLrWs;
-                b.ws_off = h->dLrWsOff;
                 b.bad_flag = h->dBad;
                 if (h->nItemsBig) {
                     // large regions: dense update into compact slots, zero factor rows
@@ -944,6 +944,10 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
                     d.ldxz = (h->L - 1) * h->RP;
                     d.zz = h->dZo;
                     k_posterior<RP, true><<<h->nItemsBig, 256, 0, st>>>(T, d);
+                }
+                if (h->nItemsLR[3]) {
+                    b.items = h->dItemsLR[3];
+                    k_post_lowrank<RP, 128><<<h->nItemsLR[3], 256, 0, st>>>(T, b);
                 }
                 if (h->nItemsLR[2]) {
                     b.items = h->dItemsLR[2];

@@ -290,7 +290,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // T = double: FP64 throughout.  T = float (mixed precision): the covariance
 // tile, both products and the stored G, B are FP32 with FP32 accumulation.
 template <int RP, int NT, typename T>
-__global__ __launch_bounds__(256) void k_finest_cross(Tree T_, CovCtx cc, Pts K, Pts O, const uint32_t* items,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_finest_cross(Tree T_, CovCtx cc, Pts K, Pts O, const uint32_t* items,
                                                       const uint64_t* goff, const double* Gc, const double* LiT,
                                                       T* Gf, T* Bf, const float* Gc32) {
     const Tree& T0 = T_;
@@ -583,7 +583,7 @@ __device__ __forceinline__ double post_w(const Tree& T, const PostArgs& a, uint6
 }
 
 template <int RP, bool FROM_OBS>
-__global__ __launch_bounds__(256) void k_posterior(Tree T, PostArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_posterior(Tree T, PostArgs a) {
     constexpr int LD = RP + 1;
     __shared__ double Km[RP * LD];
     __shared__ double vv[RP];
@@ -741,27 +741,20 @@ struct LowRankArgs {
     int ldo;
     double* zo;
     double* dterm;
-    double* ws;  // workspace of regions with more than 64 rows
-    const uint64_t* ws_off;
+    double* ws;  // W = L^{-1} X_l^T of each region: RP x n_s at ws + row_s * RP
     int* bad_flag;
 };
 
+// Regions with at most NB (16, 32, 64 or 128) rows: Q in LDS, W through the
+// global workspace, X' stored straight from the product to the output rows.
 template <int RP, int NB>
 __global__ __launch_bounds__(256) void k_post_lowrank(Tree T, LowRankArgs a) {
-    // NB > 0: regions with at most NB rows, buffers in LDS; NB = 0: larger
-    // regions, buffers in the global workspace
-    constexpr int NBL = NB > 0 ? NB : 1;
-    constexpr int LDS_LD = NBL + 1;
-    constexpr int DIM = RP > NBL ? RP : NBL;
-    constexpr int SA = DIM * LDS_LD;
-    constexpr int SB = (DIM * LDS_LD > NBL * 65) ? DIM * LDS_LD : NBL * 65;
-    __shared__ double bufA[NB > 0 ? SA : 1];
-    __shared__ double bufB[NB > 0 ? SB : 1];
-    __shared__ double zbuf[NBL];
+    constexpr int LDA = NB + 1;
+    __shared__ double Aq[NB * LDA];
+    __shared__ double dbuf[NB >= 48 ? 1 : NB * 16];
     __shared__ double red[MRA_WG];
     __shared__ int bad;
-    const uint32_t bid = blockIdx.x;
-    const uint32_t loc = a.items[bid];
+    const uint32_t loc = a.items[blockIdx.x];
     const uint64_t R = a.start_l + loc;
     const uint64_t os = a.row_s[loc];
     const int ns = (int)(a.row_e[loc] - os);
@@ -771,82 +764,39 @@ __global__ __launch_bounds__(256) void k_post_lowrank(Tree T, LowRankArgs a) {
         if (threadIdx.x == 0) a.dterm[R] = 0.0;
         return;
     }
-    const bool small = NB > 0;
-    const int dim = max(ns, RP);
-    const int ldw = small ? LDS_LD : dim;
-    const int ldo = small ? 65 : dim;  // row stride of the output tile in B
-    double* A = small ? bufA : a.ws + a.ws_off[bid];
-    double* B = small ? bufB : A + (uint64_t)dim * ldw;
-    double* zs = small ? zbuf : B + (uint64_t)dim * ldw;
+    const int nsp = (ns + 15) & ~15, nbk = nsp >> 4;
     const double* Xr = a.X + os * (uint64_t)a.ldx;
-    // 1. A = X_l^T (RP x ns), B = 0, zs = z
-    for (int idx = threadIdx.x; idx < RP * ns; idx += MRA_WG) {
-        const int k = idx % RP, o = idx / RP;
-        A[k * ldw + o] = Xr[(uint64_t)o * a.ldx + l * RP + k];
-        B[k * ldw + o] = 0.0;
-    }
-    for (int o = threadIdx.x; o < ns; o += MRA_WG) zs[o] = a.z[os + o];
-    __syncthreads();
-    // 2. B = W = L^{-1} X_l^T
-    const double* LiTR = a.LiT + R * RP * RP;
-    for (int o0 = 0; o0 < ns; o0 += 64) {
-        const int nv = min(64, ns - o0);
-        wg_gemm_tn(B + o0, ldw, RP, (nv + 15) & ~15, RP, 1.0, LiTR, RP, RP, A + o0, ldw, nv);
+    double* W = a.ws + os * (uint64_t)RP;  // RP x ns, row-major
+    // 1. W = L^{-1} X_l^T:  W[i][o] = sum_k LiT[k*RP + i] X_l[o][k]
+    wg_gemm_tn_impl<double, false, false, double, double, GEMM_ANY, double, true>(
+        W, ns, RP, nsp, RP, 1.0, a.LiT + R * RP * RP, RP, RP, Xr + l * RP, a.ldx, ns);
+    // identity (and padding) of Q's block while W is written
+    for (int idx = threadIdx.x; idx < nsp * nsp; idx += MRA_WG) {
+        const int o = idx / nsp, q = idx % nsp;
+        Aq[o * LDA + q] = (o == q) ? 1.0 : 0.0;
     }
     __syncthreads();
-    // 3. A = I + W^T W (ns x ns)
-    for (int idx = threadIdx.x; idx < ns * ns; idx += MRA_WG) {
-        const int o = idx / ns, q = idx % ns;
-        A[o * ldw + q] = (o == q) ? 1.0 : 0.0;
-    }
-    __syncthreads();
-    for (int ti = 0; ti < ns; ti += 64)
-        for (int tj = 0; tj < ns; tj += 64) {
-            const int mv = min(64, ns - ti), nv = min(64, ns - tj);
-            wg_gemm_tn(A + ti * ldw + tj, ldw, (mv + 15) & ~15, (nv + 15) & ~15, RP, 1.0, B + ti, ldw, mv, B + tj, ldw,
-                       nv);
-        }
-    __syncthreads();
-    // 4. Q = chol(A);  5. z' = Q^{-1} z, d_R
-    wg_chol(A, ldw, ns, &bad);
-    wg_trsm(A, ldw, zs, 1, ns, 1);
+    // 2. A = I + W^T W  (ns x ns in LDS)
+    wg_gemm_tn(Aq, LDA, nsp, nsp, RP, 1.0, W, ns, ns, W, ns, ns);
+    // 3. Q = chol(A), 4. Q^{-1} in place
+    wg_chol_blk(Aq, LDA, nbk, dbuf, &bad);
     double part = 0.0;
+    for (int o = threadIdx.x; o < ns; o += MRA_WG) part += 2.0 * log(Aq[o * LDA + o]);
+    wg_trinv_blk(Aq, LDA, nbk, dbuf);
+    // 5. z' = Q^{-1} z, d_R = 2 sum log diag Q - z^T z + z'^T z'
     for (int o = threadIdx.x; o < ns; o += MRA_WG) {
+        double s = 0.0;
+        for (int k = 0; k <= o; k++) s += Aq[o * LDA + k] * a.z[os + k];
         const double z0 = a.z[os + o];
-        part += 2.0 * log(A[o * ldw + o]) + zs[o] * zs[o] - z0 * z0;
-        a.zo[os + o] = zs[o];
+        a.zo[os + o] = s;
+        part += s * s - z0 * z0;
     }
     const double d = wg_sum(part, red);
     if (threadIdx.x == 0) a.dterm[R] = d;
-    // 6. B = Q^{-1};  7. A = Q^{-T} (column-major Q^{-1})
-    for (int idx = threadIdx.x; idx < ns * ns; idx += MRA_WG) {
-        const int o = idx / ns, q = idx % ns;
-        B[o * ldw + q] = (o == q) ? 1.0 : 0.0;
-    }
-    __syncthreads();
-    wg_trsm(A, ldw, B, ldw, ns, ns);
-    for (int idx = threadIdx.x; idx < ns * ns; idx += MRA_WG) {
-        const int o = idx / ns, k = idx % ns;
-        A[k * ldw + o] = B[o * ldw + k];
-    }
-    __syncthreads();
-    // 8. X' = Q^{-1} X[:, 0:l*RP], 64 columns at a time (B is the output tile)
-    const int W = l * RP;
-    for (int c0 = 0; c0 < W; c0 += 64) {
-        const int cv = min(64, W - c0);
-        for (int ti = 0; ti < ns; ti += 64) {
-            const int mv = min(64, ns - ti);
-            for (int idx = threadIdx.x; idx < mv * 64; idx += MRA_WG) B[(idx / 64) * ldo + idx % 64] = 0.0;
-            __syncthreads();
-            wg_gemm_tn(B, ldo, (mv + 15) & ~15, 64, ns, 1.0, A + ti, ldw, mv, Xr + c0, a.ldx, cv);
-            __syncthreads();
-            for (int idx = threadIdx.x; idx < mv * 64; idx += MRA_WG) {
-                const int o = idx / 64, m = idx % 64;
-                if (m < cv) a.Xo[(os + ti + o) * (uint64_t)a.ldo + c0 + m] = B[o * ldo + m];
-            }
-            __syncthreads();
-        }
-    }
+    // 6. X' = Q^{-1} X[:, 0:l*RP] straight to the output rows
+    const int Wd = l * RP;
+    if (Wd > 0)
+        wg_gemm_tn<true, false>(a.Xo + os * (uint64_t)a.ldo, a.ldo, nsp, Wd, ns, 1.0, Aq, LDA, ns, Xr, a.ldx, Wd);
     if (threadIdx.x == 0 && bad) atomicOr(a.bad_flag, 1);
 }
 
