@@ -162,6 +162,7 @@ def main():
     ap.add_argument("--r", type=int, default=64)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -186,8 +187,11 @@ def main():
     h.sample_prior(SEED_FIELD, replace=True)
     log(f"setup {time.time() - t0:.1f}s")
     prec = capi.PRECISION_MIXED if args.precision == "mixed" else capi.PRECISION_FP64
-    # headline: the requested precision, graph replay
     h.set_precision(prec)
+    # work-list shape from the library's autotuner (timed on this workload)
+    tune_cfg, tune_ms = h.autotune() if not args.no_autotune else (h.get_tuning(), None)
+    log(f"autotune {tune_cfg} {tune_ms}")
+    # headline: the requested precision, graph replay
     h.use_graph(graph)
     value, ms_per_step, D = timed(h, args.steps, args.warmup)
     log(f"{args.precision}{' graph' if graph else ''}: {ms_per_step:.2f} ms/eval D={D:.10e}")
@@ -216,6 +220,7 @@ def main():
                                f"nonstationary linear-trend Matern-1/2, planar, {args.npred} prediction points",
                    "n": n, "n_pred": args.npred, "M": args.M, "J": J, "r": args.r, "precision": args.precision,
                    "graph": graph, "max_obs_per_finest_region": int(counts.max()),
+                   "worklist_shape": dict(zip(("x32max", "lrmax", "fb16"), tune_cfg)),
                    "seeds": {"locations": SEED_LOC, "field": SEED_FIELD}},
         "loglik": -0.5 * (D + n * np.log(2 * 3.14159265359)), "D": D,
         "eager_ms_per_step": eager_ms,

@@ -213,6 +213,22 @@ enum { MRA_TREND_NPARAM = 6 };
 int mra_fit_linear_trend(mra_handle* h, int metric, double* theta, const double* lo, const double* hi,
                          double rho_begin, double rho_end, int max_evals, double* D_best, int* n_evals);
 
+/* --------------------------------------------------------------- autotuner */
+/* Work-list shape of a handle (no reference counterpart):
+ *   x32max: segments (observations below one finest-but-one region) with at
+ *           most x32max observations use 32-column tiles in the finest
+ *           conditioning kernel, larger ones 64-column tiles (0, 32 or 64);
+ *   lrmax:  finest-but-one regions with at most lrmax rows use the factored
+ *           posterior, larger ones the dense update (64 or 128);
+ *   fb16:   1 = finest regions of at most 16 observations have their own
+ *           length bucket, 0 = they join the 32 bucket.
+ * Results are the same up to rounding for every shape. */
+int mra_set_tuning(mra_handle* h, int x32max, int lrmax, int fb16);
+int mra_get_tuning(mra_handle* h, int* x32max, int* lrmax, int* fb16);
+/* Times a likelihood evaluation for every candidate shape (covariance and
+ * parameters must be set) and keeps the fastest; returns it and its time. */
+int mra_autotune(mra_handle* h, int* x32max, int* lrmax, int* fb16, double* best_ms);
+
 /* ------------------------------------------------------------------ batch */
 /* Many independent fields (one handle each, any sizes) evaluated as one
  * launch sequence (no reference counterpart; the reference runs one data set

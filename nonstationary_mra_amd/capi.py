@@ -32,7 +32,8 @@ EXPORTED = ["mra_create", "mra_destroy", "mra_last_error", "mra_set_covariance",
             "mra_bobyqa", "mra_fit_linear_trend", "mra_variogram_features", "mra_mlp_create", "mra_mlp_destroy",
             "mra_mlp_last_error", "mra_mlp_param_count", "mra_mlp_set_params", "mra_mlp_get_params",
             "mra_mlp_forward", "mra_mlp_loss_grad", "mra_mlp_train_step", "mra_batch_create", "mra_batch_destroy",
-            "mra_batch_set", "mra_batch_get", "mra_batch_likelihood", "mra_batch_autotune"]
+            "mra_batch_set", "mra_batch_get", "mra_batch_likelihood", "mra_batch_autotune", "mra_set_tuning",
+            "mra_get_tuning", "mra_autotune"]
 OPT_FACTORED_POSTERIOR = 1
 
 
@@ -106,6 +107,9 @@ def lib():
                                     C.c_int, dp, C.POINTER(C.c_int)]
         _lib.mra_fit_linear_trend.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, C.c_double, C.c_double, C.c_int, dp,
                                               C.POINTER(C.c_int)]
+        _lib.mra_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        _lib.mra_get_tuning.argtypes = [C.c_void_p] + [C.POINTER(C.c_int)] * 3
+        _lib.mra_autotune.argtypes = [C.c_void_p] + [C.POINTER(C.c_int)] * 3 + [dp]
         _lib.mra_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]
         _lib.mra_batch_destroy.argtypes = [C.c_void_p]
         _lib.mra_batch_destroy.restype = None
@@ -390,6 +394,21 @@ class Handle:
         if rc < 0:
             check(rc, self.h)
         return th, D.value, ne.value, rc
+
+    def set_tuning(self, x32max, lrmax, fb16):
+        check(lib().mra_set_tuning(self.h, x32max, lrmax, fb16), self.h)
+
+    def get_tuning(self):
+        v = [C.c_int(0) for _ in range(3)]
+        check(lib().mra_get_tuning(self.h, *[C.byref(x) for x in v]), self.h)
+        return tuple(x.value for x in v)
+
+    def autotune(self):
+        """mra_autotune: returns ((x32max, lrmax, fb16), best_ms)."""
+        v = [C.c_int(0) for _ in range(3)]
+        ms = C.c_double(0)
+        check(lib().mra_autotune(self.h, *[C.byref(x) for x in v], C.byref(ms)), self.h)
+        return tuple(x.value for x in v), ms.value
 
     def variogram_features(self):
         """64 log-semivariogram features of the current observation values

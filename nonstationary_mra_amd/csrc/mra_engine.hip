@@ -87,6 +87,13 @@ struct mra_handle {
     static constexpr int NBK = 5;
     uint32_t* dItemsFB[NBK] = {};
     uint32_t nItemsFB[NBK] = {};
+    // work-list shape (mra_autotune / mra_set_tuning): segments of at most
+    // tuneX32 observations use 32-column tiles; finest-but-one regions with
+    // at most tuneLr rows use the factored posterior; tuneFb16 = 0 sends
+    // finest regions of at most 16 observations to the 32 bucket
+    int tuneX32 = 32, tuneLr = 128, tuneFb16 = 1;
+    std::vector<void*> wlAllocs;  // work-list arrays (rebuilt when the shape changes)
+    uint32_t denseCap1 = 0;       // capacity of the dense messages of level L-1
     std::vector<uint32_t*> dItemsPost;
     std::vector<uint32_t> nItemsPost;
     // prediction
@@ -170,6 +177,7 @@ extern "C" void mra_destroy(mra_handle* h) {
     }
     if (h->hPrm) (void)hipHostFree(h->hPrm);
     for (void* p : h->allocs) (void)hipFree(p);
+    for (void* p : h->wlAllocs) (void)hipFree(p);
     for (void* p : h->fieldAllocs) (void)hipFree(p);
     for (auto& e : h->ev)
         if (e) (void)hipEventDestroy(e);
@@ -328,6 +336,93 @@ static int build_structure(mra_handle* h, const mra_tree_desc* t) {
     return MRA_OK;
 }
 
+// Work lists of one evaluation (item arrays per launch), for the current
+// work-list shape; replaces the previous ones.
+template <typename T>
+static int upload_wl(mra_handle* h, T** p, const T* src, size_t count) {
+    int rc = dalloc(h, p, count, h->wlAllocs);
+    if (rc) return rc;
+    if (count && src) HIP_TRY(hipMemcpy(*p, src, sizeof(T) * count, hipMemcpyHostToDevice));
+    return MRA_OK;
+}
+
+static int build_worklists(mra_handle* h) {
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (void* p : h->wlAllocs) (void)hipFree(p);
+    h->wlAllocs.clear();
+    // work lists
+    std::vector<uint32_t> ix, ix32, iF;
+    for (uint64_t f = 0; f < h->nF; f++)
+        if (h->hObsOff[f + 1] > h->hObsOff[f]) iF.push_back((uint32_t)f);
+    for (size_t sgi = 0; sgi < h->hSN.size(); sgi++) {
+        if (h->hSN[sgi] == 0) continue;
+        if (h->hSN[sgi] <= (uint32_t)h->tuneX32) {
+            for (uint64_t c0 = 0; c0 < h->hSN[sgi]; c0 += 32) {
+                ix32.push_back((uint32_t)sgi);
+                ix32.push_back((uint32_t)c0);
+            }
+            continue;
+        }
+        for (uint64_t c0 = 0; c0 < h->hSN[sgi]; c0 += 64) {
+            ix.push_back((uint32_t)sgi);
+            ix.push_back((uint32_t)c0);
+        }
+    }
+    h->nItemsX32 = (uint32_t)(ix32.size() / 2);
+    RC_TRY(upload_wl(h, &h->dItemsX32, ix32.data(), ix32.size()));
+    h->nItemsX = (uint32_t)(ix.size() / 2);
+    h->nItemsF = (uint32_t)iF.size();
+    {
+        std::vector<uint32_t> b[mra_handle::NBK];
+        for (uint32_t f : iF) {
+            const uint64_t n = h->hObsOff[f + 1] - h->hObsOff[f];
+            b[(n <= 16 && h->tuneFb16) ? 0 : n <= 32 ? 1 : n <= 64 ? 2 : n <= 128 ? 3 : 4].push_back(f);
+        }
+        for (int k = 0; k < mra_handle::NBK; k++) {
+            h->nItemsFB[k] = (uint32_t)b[k].size();
+            RC_TRY(upload_wl(h, &h->dItemsFB[k], b[k].data(), b[k].size()));
+        }
+    }
+    RC_TRY(upload_wl(h, &h->dItemsX, ix.data(), ix.size()));
+    RC_TRY(upload_wl(h, &h->dItemsF, iF.data(), iF.size()));
+    h->dItemsPost.assign(h->L, nullptr);
+    h->nItemsPost.assign(h->L, 0);
+    for (int l = 0; l < h->L; l++) {
+        std::vector<uint32_t> it;
+        for (uint64_t R = h->lvlStart[l]; R < h->lvlStart[l + 1]; R++)
+            if (h->hWorking[R]) it.push_back((uint32_t)(R - h->lvlStart[l]));
+        h->nItemsPost[l] = (uint32_t)it.size();
+        RC_TRY(upload_wl(h, &h->dItemsPost[l], it.data(), it.size()));
+    }
+    if (h->lowrank) {
+        const int l1 = h->L - 1;
+        const uint64_t n1 = h->lvlStart[l1 + 1] - h->lvlStart[l1];
+        std::vector<uint32_t> b[mra_handle::NLR], all, big;
+        std::vector<int32_t> slot(n1, -1);
+        for (uint64_t k = 0; k < n1; k++) {
+            if (!h->hWorking[h->lvlStart[l1] + k]) continue;
+            const uint64_t ns = h->hRowE1[k] - h->hRowS1[k];
+            if (ns > (uint64_t)h->tuneLr) {
+                slot[k] = (int32_t)big.size();
+                big.push_back((uint32_t)k);
+            } else {
+                b[ns <= 16 ? 0 : ns <= 32 ? 1 : ns <= 64 ? 2 : 3].push_back((uint32_t)k);
+                all.push_back((uint32_t)k);
+            }
+        }
+        for (int q = 0; q < mra_handle::NLR; q++) {
+            h->nItemsLR[q] = (uint32_t)b[q].size();
+            RC_TRY(upload_wl(h, &h->dItemsLR[q], b[q].data(), b[q].size()));
+        }
+        h->nItemsLRAll = (uint32_t)all.size();
+        h->nItemsBig = (uint32_t)big.size();
+        RC_TRY(upload_wl(h, &h->dItemsLRAll, all.data(), all.size()));
+        RC_TRY(upload_wl(h, &h->dItemsBig, big.data(), big.size()));
+        RC_TRY(upload_wl(h, &h->dSlot1, slot.data(), slot.size()));
+    }
+    return MRA_OK;
+}
+
 static int alloc_engine(mra_handle* h, const mra_tree_desc* t) {
     auto& A = h->allocs;
     const uint64_t RP = h->RP, RP2 = RP * RP, nKp = h->nC * RP;
@@ -440,77 +535,18 @@ static int alloc_engine(mra_handle* h, const mra_tree_desc* t) {
         RC_TRY(dalloc(h, &h->dLrWs, (uint64_t)RP * h->nObs, A));
         RC_TRY(dalloc(h, &h->dZo, h->nObs, A));
     }
-    // work lists
-    std::vector<uint32_t> ix, ix32, iF;
-    for (uint64_t f = 0; f < h->nF; f++)
-        if (h->hObsOff[f + 1] > h->hObsOff[f]) iF.push_back((uint32_t)f);
-    for (size_t sgi = 0; sgi < h->hSN.size(); sgi++) {
-        if (h->hSN[sgi] == 0) continue;
-        if (h->hSN[sgi] <= 32) {
-            ix32.push_back((uint32_t)sgi);
-            ix32.push_back(0);
-            continue;
-        }
-        for (uint64_t c0 = 0; c0 < h->hSN[sgi]; c0 += 64) {
-            ix.push_back((uint32_t)sgi);
-            ix.push_back((uint32_t)c0);
-        }
-    }
-    h->nItemsX32 = (uint32_t)(ix32.size() / 2);
-    RC_TRY(upload(h, &h->dItemsX32, ix32.data(), ix32.size()));
-    h->nItemsX = (uint32_t)(ix.size() / 2);
-    h->nItemsF = (uint32_t)iF.size();
-    {
-        std::vector<uint32_t> b[mra_handle::NBK];
-        for (uint32_t f : iF) {
-            const uint64_t n = h->hObsOff[f + 1] - h->hObsOff[f];
-            b[n <= 16 ? 0 : n <= 32 ? 1 : n <= 64 ? 2 : n <= 128 ? 3 : 4].push_back(f);
-        }
-        for (int k = 0; k < mra_handle::NBK; k++) {
-            h->nItemsFB[k] = (uint32_t)b[k].size();
-            RC_TRY(upload(h, &h->dItemsFB[k], b[k].data(), b[k].size()));
-        }
-    }
-    RC_TRY(upload(h, &h->dItemsX, ix.data(), ix.size()));
-    RC_TRY(upload(h, &h->dItemsF, iF.data(), iF.size()));
-    h->dItemsPost.assign(h->L, nullptr);
-    h->nItemsPost.assign(h->L, 0);
-    for (int l = 0; l < h->L; l++) {
-        std::vector<uint32_t> it;
-        for (uint64_t R = h->lvlStart[l]; R < h->lvlStart[l + 1]; R++)
-            if (h->hWorking[R]) it.push_back((uint32_t)(R - h->lvlStart[l]));
-        h->nItemsPost[l] = (uint32_t)it.size();
-        RC_TRY(upload(h, &h->dItemsPost[l], it.data(), it.size()));
-    }
+    RC_TRY(build_worklists(h));
     if (h->lowrank) {
+        // dense messages of level L-1: capacity for the largest candidate set
+        // (regions with more than 64 rows)
         const int l1 = h->L - 1;
         const uint64_t n1 = h->lvlStart[l1 + 1] - h->lvlStart[l1];
-        std::vector<uint32_t> b[mra_handle::NLR], all, big;
-        std::vector<int32_t> slot(n1, -1);
-        for (uint64_t k = 0; k < n1; k++) {
-            if (!h->hWorking[h->lvlStart[l1] + k]) continue;
-            const uint64_t ns = h->hRowE1[k] - h->hRowS1[k];
-            if (ns > 128) {
-                slot[k] = (int32_t)big.size();
-                big.push_back((uint32_t)k);
-            } else {
-                b[ns <= 16 ? 0 : ns <= 32 ? 1 : ns <= 64 ? 2 : 3].push_back((uint32_t)k);
-                all.push_back((uint32_t)k);
-            }
-        }
-        for (int q = 0; q < mra_handle::NLR; q++) {
-            h->nItemsLR[q] = (uint32_t)b[q].size();
-            RC_TRY(upload(h, &h->dItemsLR[q], b[q].data(), b[q].size()));
-        }
-        h->nItemsLRAll = (uint32_t)all.size();
-        h->nItemsBig = (uint32_t)big.size();
-        RC_TRY(upload(h, &h->dItemsLRAll, all.data(), all.size()));
-        RC_TRY(upload(h, &h->dItemsBig, big.data(), big.size()));
-        RC_TRY(upload(h, &h->dSlot1, slot.data(), slot.size()));
-        // dense messages of the large regions only
-        RC_TRY(dalloc(h, &h->dW[l1], (uint64_t)std::max<uint32_t>(h->nItemsBig, 1) * l1 * RP, A));
-        RC_TRY(dalloc(h, &h->dA[l1],
-                      (uint64_t)std::max<uint32_t>(h->nItemsBig, 1) * (uint64_t)(l1 * (l1 + 1) / 2) * RP2, A));
+        uint32_t cap = 0;
+        for (uint64_t k = 0; k < n1; k++)
+            if (h->hWorking[h->lvlStart[l1] + k] && h->hRowE1[k] - h->hRowS1[k] > 64) cap++;
+        h->denseCap1 = cap;
+        RC_TRY(dalloc(h, &h->dW[l1], (uint64_t)std::max<uint32_t>(cap, 1) * l1 * RP, A));
+        RC_TRY(dalloc(h, &h->dA[l1], (uint64_t)std::max<uint32_t>(cap, 1) * (uint64_t)(l1 * (l1 + 1) / 2) * RP2, A));
     }
     if (h->nPred) {
         // prediction items (F, p0, slot) in chunks bounding the scratch
@@ -1576,6 +1612,61 @@ extern "C" int mra_batch_autotune(mra_batch* b, int* streams, int* graph, double
     b->graph = bg;
     if (streams) *streams = bs;
     if (graph) *graph = bg;
+    if (best_ms) *best_ms = best;
+    return MRA_OK;
+}
+
+// ----------------------------------------------------------------- tuning
+extern "C" int mra_set_tuning(mra_handle* h, int x32max, int lrmax, int fb16) {
+    if (!h || (x32max != 0 && x32max != 32 && x32max != 64) || (lrmax != 64 && lrmax != 128) ||
+        (fb16 != 0 && fb16 != 1))
+        return MRA_E_ARG;
+    h->tuneX32 = x32max;
+    h->tuneLr = lrmax;
+    h->tuneFb16 = fb16;
+    RC_TRY(build_worklists(h));
+    graph_invalidate(h);
+    return MRA_OK;
+}
+
+extern "C" int mra_get_tuning(mra_handle* h, int* x32max, int* lrmax, int* fb16) {
+    if (!h) return MRA_E_ARG;
+    if (x32max) *x32max = h->tuneX32;
+    if (lrmax) *lrmax = h->tuneLr;
+    if (fb16) *fb16 = h->tuneFb16;
+    return MRA_OK;
+}
+
+// Times one likelihood evaluation (after a warm-up) for every candidate
+// work-list shape and keeps the fastest.
+extern "C" int mra_autotune(mra_handle* h, int* x32max, int* lrmax, int* fb16, double* best_ms) {
+    if (!h) return MRA_E_ARG;
+    const int cx[] = {0, 32, 64}, cl[] = {64, 128}, cf[] = {0, 1};
+    double best = 1e300;
+    int bx = h->tuneX32, bl = h->tuneLr, bf = h->tuneFb16;
+    const bool graph = h->useGraph;
+    h->useGraph = false;
+    int rc = MRA_OK;
+    for (int x : cx)
+        for (int l : cl) {
+            if (l != cl[0] && !h->lowrank) continue;
+            for (int f : cf) {
+                if ((rc = mra_set_tuning(h, x, l, f))) goto done;
+                if ((rc = run_eval(h, false))) goto done;
+                const auto t0 = std::chrono::steady_clock::now();
+                if ((rc = run_eval(h, false))) goto done;
+                const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                if (ms < best) best = ms, bx = x, bl = l, bf = f;
+            }
+        }
+done:
+    h->useGraph = graph;
+    const int rc2 = mra_set_tuning(h, bx, bl, bf);
+    if (rc) return rc;
+    if (rc2) return rc2;
+    if (x32max) *x32max = bx;
+    if (lrmax) *lrmax = bl;
+    if (fb16) *fb16 = bf;
     if (best_ms) *best_ms = best;
     return MRA_OK;
 }
