@@ -151,6 +151,66 @@ def calibrate_copy(mib):
     print(json.dumps({"copy_bytes_read": mib << 20, "copy_bytes_written": mib << 20, "launches": 3}), flush=True)
 
 
+def run_c3(args):
+    """Configuration of the uneven batch: independent fields with n drawn
+    log-uniform in [5,000, 200,000] (uniform locations), per-field kernel
+    parameters (log range in [log 0.02, log 0.2], range and variance slopes
+    in [-1, 1], nugget in [0.01, 0.2]), M chosen so that a finest region holds
+    at most 256 points on average, r = 32, mixed precision, fields drawn from
+    the MRA prior; all likelihoods evaluated as one batch (mra_batch_*) whose
+    launch setting comes from the batch autotuner.  One step = one evaluation
+    of every field."""
+    rng = np.random.default_rng(SEED_LOC + 3)
+    hs = []
+    n_tot = 0
+    t0 = time.time()
+    for i in range(args.c3_fields):
+        n = int(np.exp(rng.uniform(np.log(5000), np.log(200000))))
+        M = 1 + max(1, int(np.ceil(np.log(n / 256.0) / np.log(4.0))))
+        x, y = synth.uniform_points(n, rng)
+        tree = capi.partition_program(x, y, np.zeros(n), None, None, M=M, J=4, r=32, bounds=synth.UNIT_BOUNDS)
+        h = capi.Handle(tree, with_pred=False)
+        th = (0.0, rng.uniform(-1, 1), 0.0, rng.uniform(np.log(0.02), np.log(0.2)), rng.uniform(-1, 1),
+              np.log(rng.uniform(0.01, 0.2)))
+        cov = dict(nonstationary=1, metric=synth.METRIC["planar"], wendland_len=1.0,
+                   sigmasq=dict(kind=synth.FIELD_LINEAR, intercept=th[0], slope_x=th[1], slope_y=th[2]),
+                   beta=dict(kind=synth.FIELD_LINEAR, intercept=th[3], slope_x=th[4], slope_y=0.0),
+                   tausq=dict(kind=synth.FIELD_CONSTANT, intercept=th[5]))
+        h.set_covariance(cov)
+        h.set_params(**synth.UNIT_PARAMS)
+        h.sample_prior(SEED_FIELD + i, replace=True)
+        h.set_precision(capi.PRECISION_MIXED)
+        hs.append(h)
+        n_tot += n
+        if i % 32 == 31:
+            log(f"c3: {i + 1} fields ready ({time.time() - t0:.1f}s)")
+    b = capi.Batch(hs)
+    streams, graph, tune_ms = b.autotune()
+    log(f"c3 batch autotune: streams={streams} graph={graph} {tune_ms:.1f} ms")
+    for _ in range(args.warmup):
+        D, st = b.likelihood()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        D, st = b.likelihood()
+    el = time.perf_counter() - t1
+    # the same fields one after another (each synchronised), for comparison
+    t2 = time.perf_counter()
+    Ds = np.array([h.likelihood() for h in hs])
+    seq_ms = 1e3 * (time.perf_counter() - t2)
+    out = {"metric": "C3 batch loglik evals/sec (256 uneven fields, r=32, mixed)", "value": args.c3_fields * args.steps / el,
+           "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32+f64", "data": "synthetic (uniform points, fields drawn from the MRA prior)",
+           "config": {"workload": "256 fields, n log-uniform in [5e3, 2e5], r=32, M per field, mixed precision",
+                      "fields": args.c3_fields, "total_points": n_tot, "streams": streams, "graph": graph},
+           "sequential_ms_per_step": seq_ms, "batch_matches_sequential": bool(np.array_equal(D, Ds)),
+           "not_pd_fields": int((st != 0).sum())}
+    b.close()
+    for h in hs:
+        h.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +223,9 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--config", choices=["c2", "c3"], default="c2",
+                    help="c2: the headline workload; c3: the uneven batch of independent fields")
+    ap.add_argument("--c3-fields", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -174,6 +237,9 @@ def main():
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.config == "c3":
+        run_c3(args)
+        return
     J = 4
     graph = not args.no_graph
     t0 = time.time()
