@@ -99,6 +99,7 @@ struct mra_handle {
     // prediction
     uint32_t* dItemsPred = nullptr;
     std::vector<uint32_t> predChunkStart;  // item index where each chunk starts (+ end)
+    std::vector<int> predChunkNT;          // tile width (points per item) of each chunk
     double *dBt = nullptr, *dGh = nullptr, *dGL = nullptr, *dMean = nullptr, *dVar = nullptr;
     uint64_t* dGlOff = nullptr;
     // model
@@ -549,37 +550,47 @@ static int alloc_engine(mra_handle* h, const mra_tree_desc* t) {
         RC_TRY(dalloc(h, &h->dA[l1], (uint64_t)std::max<uint32_t>(cap, 1) * (uint64_t)(l1 * (l1 + 1) / 2) * RP2, A));
     }
     if (h->nPred) {
-        // prediction items (F, p0, slot) in chunks bounding the scratch
-        const uint32_t CH = 4096;
+        // prediction items (F, p0, slot) in chunks bounding the scratch:
+        // regions with at most 16 prediction points use 16-point tiles
+        // (chunks of CH16 items), larger ones 64-point tiles (CH64)
+        const uint32_t CH16 = 16384, CH64 = 4096;
         std::vector<uint32_t> ip;
         std::vector<uint64_t> gl;
-        uint64_t glCur = 0, glMax = 1;
-        uint32_t inChunk = 0;
+        uint64_t glMax = 1;
         h->predChunkStart.clear();
-        h->predChunkStart.push_back(0);
-        for (uint64_t f = 0; f < h->nF; f++) {
-            const uint64_t p = h->hPredOff[f + 1] - h->hPredOff[f];
-            const uint64_t n = h->hObsOff[f + 1] - h->hObsOff[f];
-            for (uint64_t p0 = 0; p0 < p; p0 += 64) {
-                if (inChunk == CH) {
-                    h->predChunkStart.push_back((uint32_t)(ip.size() / 3));
-                    inChunk = 0;
-                    glCur = 0;
+        h->predChunkNT.clear();
+        for (int pass = 0; pass < 2; pass++) {
+            const int NT = pass == 0 ? 16 : 64;
+            const uint32_t CH = pass == 0 ? CH16 : CH64;
+            uint32_t inChunk = CH;
+            uint64_t glCur = 0;
+            for (uint64_t f = 0; f < h->nF; f++) {
+                const uint64_t p = h->hPredOff[f + 1] - h->hPredOff[f];
+                const uint64_t n = h->hObsOff[f + 1] - h->hObsOff[f];
+                if (p == 0 || (p <= 16) != (pass == 0)) continue;
+                for (uint64_t p0 = 0; p0 < p; p0 += NT) {
+                    if (inChunk == CH) {
+                        h->predChunkStart.push_back((uint32_t)(ip.size() / 3));
+                        h->predChunkNT.push_back(NT);
+                        inChunk = 0;
+                        glCur = 0;
+                    }
+                    ip.push_back((uint32_t)f);
+                    ip.push_back((uint32_t)p0);
+                    ip.push_back(inChunk);
+                    gl.push_back(glCur);
+                    glCur += n * NT;
+                    glMax = std::max(glMax, glCur);
+                    inChunk++;
                 }
-                ip.push_back((uint32_t)f);
-                ip.push_back((uint32_t)p0);
-                ip.push_back(inChunk);
-                gl.push_back(glCur);
-                glCur += n * 64;
-                glMax = std::max(glMax, glCur);
-                inChunk++;
             }
         }
         h->predChunkStart.push_back((uint32_t)(ip.size() / 3));
+        const uint32_t CH = std::max(CH16 * 16, CH64 * 64) / 64;  // scratch rows of 64 points
         // gl offsets indexed by slot within chunk: store per item, kernel reads by slot via chunk base
         RC_TRY(upload(h, &h->dItemsPred, ip.data(), ip.size()));
         RC_TRY(upload(h, &h->dGlOff, gl.data(), gl.size()));
-        RC_TRY(dalloc(h, &h->dGh, (uint64_t)CH * h->LRP * 64, A));
+        RC_TRY(dalloc(h, &h->dGh, (uint64_t)CH * h->LRP * 64, A));  // >= items x LRP x NT of any chunk
         RC_TRY(dalloc(h, &h->dGL, glMax, A));
         RC_TRY(dalloc(h, &h->dBt, (uint64_t)h->LRP * h->nPred, A));
         RC_TRY(dalloc(h, &h->dMean, h->nPred, A));
@@ -922,10 +933,15 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
             if (e == b) continue;
             pa.items = h->dItemsPred + 3 * (uint64_t)b;
             pa.gl_off = h->dGlOff + b;
-            if (mixed)
-                k_pred_pre<RP, float><<<e - b, 256, 0, st>>>(T, cc, K, O, Pp, pa);
+            const bool n16 = h->predChunkNT[c] == 16;
+            if (mixed && n16)
+                k_pred_pre<RP, float, 16><<<e - b, 256, 0, st>>>(T, cc, K, O, Pp, pa);
+            else if (mixed)
+                k_pred_pre<RP, float, 64><<<e - b, 256, 0, st>>>(T, cc, K, O, Pp, pa);
+            else if (n16)
+                k_pred_pre<RP, double, 16><<<e - b, 256, 0, st>>>(T, cc, K, O, Pp, pa);
             else
-                k_pred_pre<RP, double><<<e - b, 256, 0, st>>>(T, cc, K, O, Pp, pa);
+                k_pred_pre<RP, double, 64><<<e - b, 256, 0, st>>>(T, cc, K, O, Pp, pa);
         }
     }
     // posterior, finest-but-one level up to the root
@@ -1032,7 +1048,10 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
             if (e == b) continue;
             pa.items = h->dItemsPred + 3 * (uint64_t)b;
             pa.gl_off = h->dGlOff + b;
-            k_pred_post<RP><<<e - b, 256, 0, st>>>(T, pa);
+            if (h->predChunkNT[c] == 16)
+                k_pred_post<RP, 16><<<e - b, 256, 0, st>>>(T, pa);
+            else
+                k_pred_post<RP, 64><<<e - b, 256, 0, st>>>(T, pa);
         }
     }
     mark(MRA_T_REDUCE);

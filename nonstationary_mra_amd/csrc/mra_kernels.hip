@@ -825,9 +825,10 @@ struct PredArgs {
     const double* v;
 };
 
-template <int RP, typename TG>
+// NT: prediction points per tile (16 for regions with few points, else 64)
+template <int RP, typename TG, int NT>
 __global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts O, Pts Pp, PredArgs a) {
-    constexpr int NT = 64, LD = NT + 1;
+    constexpr int LD = NT + 1;
     __shared__ double Cm[RP * LD];
     __shared__ double Wm[RP * LD];
     const uint32_t F = a.items[3 * blockIdx.x], p0 = a.items[3 * blockIdx.x + 1], slot = a.items[3 * blockIdx.x + 2];
@@ -934,9 +935,9 @@ __global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts 
 // Top-down posterior prediction (src/class_approximation-predict.cpp:123-188):
 //   Z_j = L~_{a_j}^{-1} (B~_j - sum_{j'>j} P_{a_j',j}^T Z_j'),
 //   mean += Z_j^T v_{a_j}, var += colsumsq(Z_j).
-template <int RP>
+template <int RP, int NT>
 __global__ __launch_bounds__(256) void k_pred_post(Tree T, PredArgs a) {
-    constexpr int NT = 64, LD = NT + 1;
+    constexpr int LD = NT + 1;
     __shared__ double Cm[RP * LD];
     __shared__ double Zm[RP * LD];
     const uint32_t F = a.items[3 * blockIdx.x], p0 = a.items[3 * blockIdx.x + 1];
