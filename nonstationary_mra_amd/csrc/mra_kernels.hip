@@ -831,6 +831,7 @@ __global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts 
     constexpr int LD = NT + 1;
     __shared__ double Cm[RP * LD];
     __shared__ double Wm[RP * LD];
+    __shared__ double stage[5 * RP];
     const uint32_t F = a.items[3 * blockIdx.x], p0 = a.items[3 * blockIdx.x + 1], slot = a.items[3 * blockIdx.x + 2];
     const uint64_t pb = T.pred_off[F];
     const int pF = (int)(T.pred_off[F + 1] - pb);
@@ -842,11 +843,8 @@ __global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts 
     double* BtF = a.Bt + (uint64_t)LRP * pb;
     for (int j = 0; j < L; j++) {
         const uint64_t an = ancestor_up(T.parent[F], L - 1, j, T.J);
-        for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
-            const int i = idx / NT, c = idx % NT;
-            Cm[i * LD + c] = (i < r && c < np) ? pcov(cc, K, an * RP + i, Pp, pb + p0 + c) : 0.0;
-            Wm[i * LD + c] = 0.0;
-        }
+        for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) Wm[(idx / NT) * LD + idx % NT] = 0.0;
+        cov_tile<RP, NT>(Cm, LD, cc, K, an * RP, r, Pp, pb + p0, np, false, stage);
         __syncthreads();
         if (j > 0) {
             const double* Ga = a.Gc + a.goff[j] + (an - lvl_start(j, T.J)) * (uint64_t)j * RP * RP;
@@ -878,10 +876,7 @@ __global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts 
         const TG* GF = (const TG*)a.Gf + (uint64_t)LRP * so + (ob - so);
         for (int t0 = 0; t0 < nF; t0 += RP) {  // row tiles of RP (the tile height of Cm)
             const int mv = min(RP, nF - t0);
-            for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
-                const int i = idx / NT, cc2 = idx % NT;
-                Cm[i * LD + cc2] = (i < mv && cc2 < np) ? pcov(cc, O, ob + t0 + i, Pp, pb + p0 + cc2) : 0.0;
-            }
+            cov_tile<RP, NT>(Cm, LD, cc, O, ob + t0, mv, Pp, pb + p0, np, false, stage);
             __syncthreads();
             wg_gemm_tn(Cm, LD, (mv + 15) & ~15, NT, LRP, -1.0, GF + t0, sn, mv, Gh, NT, NT);
             __syncthreads();
