@@ -151,6 +151,61 @@ def calibrate_copy(mib):
     print(json.dumps({"copy_bytes_read": mib << 20, "copy_bytes_written": mib << 20, "launches": 3}), flush=True)
 
 
+def cov_kernel_rate(n1, n2, precision, steps, seed=SEED_LOC + 5):
+    """The pairwise covariance kernel alone (mra_cov_tile_device): an n1 x n2
+    row-major matrix of the C2 kernel (clustered points, linear-trend fields,
+    planar) written to HBM.  Returns (ms per launch from HIP events on the
+    launch stream, bytes written per launch)."""
+    rng = np.random.default_rng(seed)
+    cov = synth.linear_trend_cov()
+
+    def pts(n):
+        x, y = synth.clustered_points(n, rng)
+        s = cov["sigmasq"]
+        b = cov["beta"]
+        ls = s["intercept"] + s["slope_x"] * x + s["slope_y"] * y
+        lb = b["intercept"] + b["slope_x"] * x + b["slope_y"] * y
+        return (capi.DeviceBuffer.from_array(np.concatenate([x, y, np.zeros(n)])),
+                capi.DeviceBuffer.from_array(np.concatenate([np.exp(0.5 * ls), np.exp(lb)])))
+
+    xyz1, sb1 = pts(n1)
+    xyz2, sb2 = pts(n2)
+    esz = 4 if precision == capi.PRECISION_MIXED else 8
+    out = capi.DeviceBuffer(n1 * n2 * esz)
+    try:
+        ms = capi.cov_matrix_timed(precision, synth.METRIC["planar"], xyz1.ptr, sb1.ptr, n1, xyz2.ptr, sb2.ptr, n2,
+                                   out.ptr, steps)
+    finally:
+        for b in (out, xyz1, sb1, xyz2, sb2):
+            b.free()
+    return ms, n1 * n2 * esz
+
+
+def run_cov(args):
+    """The covariance kernel on its own: a 65,536 x 65,536 matrix (4.29e9
+    pairs, about the pair count of one C2 evaluation) in FP32 and FP64."""
+    n1 = n2 = args.cov_n
+    res = {}
+    for name, prec in (("f32", capi.PRECISION_MIXED), ("f64", capi.PRECISION_FP64)):
+        ms, nbytes = cov_kernel_rate(n1, n2, prec, args.steps)
+        res[name] = {"ms_per_launch": ms, "bytes_written": nbytes, "GB_per_s": nbytes / (ms * 1e-3) / 1e9,
+                     "pairs_per_s": n1 * n2 / (ms * 1e-3)}
+        log(f"cov {name}: {ms:.3f} ms, {res[name]['GB_per_s']:.0f} GB/s")
+    f32 = res["f32"]
+    out = {"metric": f"covariance evaluation GB/s ({n1}x{n2} pairs, FP32 output)", "value": f32["GB_per_s"],
+           "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": f32["ms_per_launch"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (clustered points, linear-trend nonstationary fields, planar)",
+           "config": {"workload": f"{n1} x {n2} nonstationary Matern-1/2 covariance matrix, row-major", "n1": n1,
+                      "n2": n2},
+           "roofline": {"bound": "hbm", "kernel": "k_cov_tile_f32", "achieved": f32["GB_per_s"],
+                        "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": f32["GB_per_s"] * 1e9 / PEAK_HBM,
+                        "traffic": load_traffic("cov_f32") if n1 == n2 == 65536 else None,
+                        "avg_launch_ms": f32["ms_per_launch"]},
+           "fp64": res["f64"]}
+    print(json.dumps(out), flush=True)
+
+
 def run_c3(args):
     """Configuration of the uneven batch: independent fields with n drawn
     log-uniform in [5,000, 200,000] (uniform locations), per-field kernel
@@ -223,11 +278,16 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--config", choices=["c2", "c3"], default="c2",
-                    help="c2: the headline workload; c3: the uneven batch of independent fields")
+    ap.add_argument("--tuning", default=None, metavar="X32,LR,FB16",
+                    help="fixed work-list shape instead of the autotuner (e.g. 64,128,1)")
+    ap.add_argument("--config", choices=["c2", "c3", "cov"], default="c2",
+                    help="c2: the headline workload; c3: the uneven batch of independent fields; "
+                         "cov: the covariance kernel alone")
+    ap.add_argument("--cov-n", type=int, default=65536)
     ap.add_argument("--c3-fields", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
+    ap.add_argument("--no-cov-kernel", action="store_true", help="skip the standalone covariance-kernel leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -239,6 +299,9 @@ def main():
         sys.exit(2)
     if args.config == "c3":
         run_c3(args)
+        return
+    if args.config == "cov":
+        run_cov(args)
         return
     J = 4
     graph = not args.no_graph
@@ -255,7 +318,11 @@ def main():
     prec = capi.PRECISION_MIXED if args.precision == "mixed" else capi.PRECISION_FP64
     h.set_precision(prec)
     # work-list shape from the library's autotuner (timed on this workload)
-    tune_cfg, tune_ms = h.autotune() if not args.no_autotune else (h.get_tuning(), None)
+    if args.tuning:
+        h.set_tuning(*[int(v) for v in args.tuning.split(",")])
+        tune_cfg, tune_ms = h.get_tuning(), None
+    else:
+        tune_cfg, tune_ms = h.autotune() if not args.no_autotune else (h.get_tuning(), None)
     log(f"autotune {tune_cfg} {tune_ms}")
     # headline: the requested precision, graph replay
     h.use_graph(graph)
@@ -315,6 +382,15 @@ def main():
                                  "mean_max_abs_err_rel_to_max": float(np.abs(mm - m64).max() / np.abs(m64).max()),
                                  "var_max_rel_err": float(np.max(np.abs(sm - s64) / np.abs(s64)))}
     h.close()
+    if not args.no_cov_kernel:
+        # the pairwise covariance kernel alone, materialised: HBM-bound
+        cn = args.cov_n
+        cms, cbytes = cov_kernel_rate(cn, cn, capi.PRECISION_MIXED, 10)
+        out["covariance_kernel"] = {"bound": "hbm", "kernel": "k_cov_tile_f32", "n1": cn, "n2": cn, "dtype": "f32",
+                                    "ms_per_launch": cms, "achieved": cbytes / (cms * 1e-3) / 1e9,
+                                    "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                                    "frac": cbytes / (cms * 1e-3) / PEAK_HBM,
+                                    "traffic": load_traffic("cov_f32") if cn == 65536 else None}
     if not args.no_cpu_baseline:
         cb = cpu_baseline(None, args.M, J, args.r, args.n)
         cpu_evals = (1.0 / cb["t"]) * (cb["w"] / w_total)

@@ -179,12 +179,30 @@ int mra_cross_covariance(const mra_cov_desc* cov, double sill, double range, dou
 int mra_variance_covariance(const mra_cov_desc* cov, double sill, double range, double nugget,
                             double sill_scale, double range_scale, double nugget_scale,
                             const double* lon, const double* lat, uint64_t n, double* out);
-/* Device-resident covariance tile kernel for throughput measurement:
- * d_out[i*n2+j] for n1 x n2 points whose scaled sqrt-sill, range and
- * coordinates are already on the device (SoA).  precision selects FP64/FP32
- * output (d_out is double* or float*).  stream may be NULL. */
+/* Device-resident form of evaluate_nonstationary_cross_covariance
+ * (src/evaluate_covariance.cpp:155-234), the pairwise covariance kernel on
+ * its own: d_out[i*n2+j] (row-major n1 x n2, as the reference writes it) for
+ * points whose coordinates (d_xyz = [x | y | z], n each) and scaled sqrt-sill
+ * and range (d_sb = [s | b]) are already on the device.  precision selects
+ * FP64 (d_out is double*) or FP32 (MRA_PRECISION_MIXED, float*) output;
+ * stream may be NULL.  n1 is limited to 65535*64 rows per call. */
 int mra_cov_tile_device(int precision, int metric, const double* d_xyz1, const double* d_sb1, uint64_t n1,
                         const double* d_xyz2, const double* d_sb2, uint64_t n2, void* d_out, void* stream);
+/* Throughput measurement of mra_cov_tile_device (no counterpart in the
+ * reference, which prints wall-clock seconds per phase,
+ * src/class_approximation.cpp:161-164): one untimed launch, then `launches`
+ * launches on a private stream between two HIP events; *ms_per_launch is
+ * the average.  Synchronous. */
+int mra_cov_tile_timed(int precision, int metric, const double* d_xyz1, const double* d_sb1, uint64_t n1,
+                       const double* d_xyz2, const double* d_sb2, uint64_t n2, void* d_out, int launches,
+                       double* ms_per_launch);
+/* Device-buffer utilities for callers without their own HIP allocator
+ * (no counterpart in the reference, whose arrays are host memory): plain
+ * hipMalloc / hipFree and a synchronous copy (to_device = 1: host to
+ * device, 0: device to host). */
+int mra_device_alloc(uint64_t bytes, void** d_ptr);
+int mra_device_free(void* d_ptr);
+int mra_device_copy(void* dst, const void* src, uint64_t bytes, int to_device);
 
 /* ------------------------------------------------------------ optimizer */
 /* Positive status: the minimiser stopped at its evaluation limit (the

@@ -33,7 +33,8 @@ EXPORTED = ["mra_create", "mra_destroy", "mra_last_error", "mra_set_covariance",
             "mra_mlp_last_error", "mra_mlp_param_count", "mra_mlp_set_params", "mra_mlp_get_params",
             "mra_mlp_forward", "mra_mlp_loss_grad", "mra_mlp_train_step", "mra_batch_create", "mra_batch_destroy",
             "mra_batch_set", "mra_batch_get", "mra_batch_likelihood", "mra_batch_autotune", "mra_set_tuning",
-            "mra_get_tuning", "mra_autotune"]
+            "mra_get_tuning", "mra_autotune", "mra_cov_tile_timed", "mra_device_alloc", "mra_device_free",
+            "mra_device_copy"]
 OPT_FACTORED_POSTERIOR = 1
 
 
@@ -92,6 +93,11 @@ def lib():
         _lib.mra_variance_covariance.argtypes = [C.POINTER(CovDesc)] + [C.c_double] * 6 + [dp, dp, C.c_uint64, dp]
         _lib.mra_cov_tile_device.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
                                              C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]
+        _lib.mra_cov_tile_timed.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
+                                            C.c_void_p, C.c_uint64, C.c_void_p, C.c_int, dp]
+        _lib.mra_device_alloc.argtypes = [C.c_uint64, C.POINTER(C.c_void_p)]
+        _lib.mra_device_free.argtypes = [C.c_void_p]
+        _lib.mra_device_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int]
         _lib.mra_partition_program_sizes.argtypes = [dp, dp, C.c_uint64, dp, dp, C.c_uint64, C.c_int, C.c_int,
                                                      C.c_int, C.c_double, dp, u64p]
         _lib.mra_partition_program.argtypes = [dp, dp, dp, C.c_uint64, dp, dp, C.c_uint64, C.c_int, C.c_int,
@@ -504,6 +510,62 @@ def variance_covariance(cov, prm, lon, lat):
                                         prm.get("range_scale", 1.0), prm.get("nugget_scale", 1.0), keep.d(lon),
                                         keep.d(lat), n, _d(out)))
     return out.reshape(n, n)
+
+
+class DeviceBuffer:
+    """A device allocation made through the library (mra_device_alloc), so
+    that callers need no HIP runtime of their own."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(lib().mra_device_alloc(self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        check(lib().mra_device_copy(b.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes, 1))
+        return b
+
+    def to_array(self, dtype, count, offset_bytes=0):
+        out = np.empty(count, dtype=dtype)
+        assert offset_bytes + out.nbytes <= self.nbytes
+        check(lib().mra_device_copy(out.ctypes.data_as(C.c_void_p), self.ptr + offset_bytes, out.nbytes, 0))
+        return out
+
+    def upload(self, a, offset_bytes=0):
+        a = np.ascontiguousarray(a)
+        assert offset_bytes + a.nbytes <= self.nbytes
+        check(lib().mra_device_copy(self.ptr + offset_bytes, a.ctypes.data_as(C.c_void_p), a.nbytes, 1))
+
+    def free(self):
+        if self.ptr:
+            check(lib().mra_device_free(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def cov_matrix_device(precision, metric, xyz1, sb1, n1, xyz2, sb2, n2, out, stream=None):
+    """Device covariance matrix out[i*n2 + j] (mra_cov_tile_device).  The
+    arguments are device addresses (ints): xyz = [x | y | z] and sb = [s | b]
+    (scaled sqrt-sill and range) as FP64 SoA, out FP64 or, for
+    PRECISION_MIXED, FP32; the launch is asynchronous on `stream`."""
+    check(lib().mra_cov_tile_device(precision, metric, xyz1, sb1, n1, xyz2, sb2, n2, out, stream))
+
+
+def cov_matrix_timed(precision, metric, xyz1, sb1, n1, xyz2, sb2, n2, out, launches):
+    """Average milliseconds per launch of mra_cov_tile_device (HIP events on
+    the library's own stream, after one untimed launch)."""
+    ms = C.c_double()
+    check(lib().mra_cov_tile_timed(precision, metric, xyz1, sb1, n1, xyz2, sb2, n2, out, launches, C.byref(ms)))
+    return ms.value
 
 
 def partition_program(lon, lat, val, plon, plat, M, J, r, offset=np.e / 100, bounds=None):

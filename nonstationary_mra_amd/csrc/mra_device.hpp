@@ -108,22 +108,43 @@ __device__ __forceinline__ double pcov_pt(const CovCtx& c, const PtV& a, const P
     return (a.q * b.q) * (t * t * t) * exp(-d * t);
 }
 
-// FP32 evaluation of the same form (mixed precision): inputs rounded to
-// float, hardware rsqrt (about 1 ulp) and expf.
-__device__ __forceinline__ float pcov_ptf(const CovCtx& c, const PtV& a, const PtV& b) {
-    float d;
-    if (c.metric == 1) {
-        d = (float)pdist(c, a.x, a.y, a.z, b.x, b.y, b.z);
-    } else {
-        const float dx = (float)b.x - (float)a.x, dy = (float)b.y - (float)a.y, dz = (float)b.z - (float)a.z;
-        const float d2 = dx * dx + dy * dy + dz * dz;
-        d = d2 > 0.0f ? d2 * __builtin_amdgcn_rsqf(d2) : 0.0f;
-        if (c.metric == 0) d *= 6371.0f;
+// FP32 evaluation of the same form (mixed precision) on point data rounded
+// to float: raw v_sqrt_f32, v_rsq_f32 and v_exp_f32 (about 1 ulp each; the
+// exponential as 2^(-d t log2 e)).  PLANAR: the z coordinates are zero.
+struct PtF {
+    float x, y, z, q, b;
+};
+
+__device__ __forceinline__ PtF to_ptf(const PtV& p) {
+    return PtF{(float)p.x, (float)p.y, (float)p.z, (float)p.q, (float)p.b};
+}
+
+// Explicit fused multiply-adds and no contraction, so every inlined copy
+// rounds identically (vector and scalar store paths give the same bits).
+template <bool PLANAR>
+__device__ __forceinline__ float pcov_ff(const PtF& a, const PtF& b, bool chordal) {
+#pragma clang fp contract(off)
+    const float dx = b.x - a.x, dy = b.y - a.y;
+    float d2 = __builtin_fmaf(dx, dx, dy * dy);
+    if (!PLANAR) {
+        const float dz = b.z - a.z;
+        d2 = __builtin_fmaf(dz, dz, d2);
     }
-    if (c.stationary) return (float)c.sill * expf(-d / (float)c.range);
-    const float ab = (float)a.b, bb = (float)b.b;
-    const float t = __builtin_amdgcn_rsqf((ab * ab + bb * bb) * 0.5f);
-    return ((float)a.q * (float)b.q) * (t * t * t) * expf(-d * t);
+    float d = __builtin_amdgcn_sqrtf(d2);
+    if (chordal) d *= 6371.0f;
+    const float t = __builtin_amdgcn_rsqf(__builtin_fmaf(a.b, a.b, b.b * b.b) * 0.5f);
+    return (a.q * b.q) * (t * t * t) * __builtin_amdgcn_exp2f(-(d * t) * 1.44269504088896341f);
+}
+
+__device__ __forceinline__ float pcov_ptf(const CovCtx& c, const PtV& a, const PtV& b) {
+    if (c.metric == 1 || c.stationary) {
+        const float d = (float)pdist(c, a.x, a.y, a.z, b.x, b.y, b.z);
+        if (c.stationary) return (float)c.sill * expf(-d / (float)c.range);
+        const float ab = (float)a.b, bb = (float)b.b;
+        const float t = __builtin_amdgcn_rsqf((ab * ab + bb * bb) * 0.5f);
+        return ((float)a.q * (float)b.q) * (t * t * t) * expf(-d * t);
+    }
+    return pcov_ff<false>(to_ptf(a), to_ptf(b), c.metric == 0);
 }
 
 // Fill the RP x NC tile C[i*LD + c] (i < RP, c < NC) with the covariance of

@@ -1327,24 +1327,111 @@ extern "C" int mra_variance_covariance(const mra_cov_desc* cov, double sill, dou
     return MRA_OK;
 }
 
-extern "C" int mra_cov_tile_device(int precision, int metric, const double* d_xyz1, const double* d_sb1, uint64_t n1,
-                                   const double* d_xyz2, const double* d_sb2, uint64_t n2, void* d_out,
-                                   void* stream) {
+static int cov_tile_launch(int precision, int metric, const double* d_xyz1, const double* d_sb1, uint64_t n1,
+                           const double* d_xyz2, const double* d_sb2, uint64_t n2, void* d_out, hipStream_t st) {
     if (!d_xyz1 || !d_sb1 || !d_xyz2 || !d_sb2 || !d_out || !n1 || !n2 || metric < 0 || metric > 2) {
         g_err = "invalid argument";
         return MRA_E_ARG;
     }
     CovCtx cc{metric, 0, 1.0, 1.0};
-    // SoA: xyz1 = [x(n1) | y(n1) | z(n1)], sb1 = [s(n1) | b(n1)]
+    // SoA: xyz = [x(n) | y(n) | z(n)], sb = [s(n) | b(n)]
     Pts P{d_xyz1, d_xyz1 + n1, d_xyz1 + 2 * n1, d_sb1, d_sb1 + n1, nullptr};
     Pts Q{d_xyz2, d_xyz2 + n2, d_xyz2 + 2 * n2, d_sb2, d_sb2 + n2, nullptr};
-    dim3 grid((unsigned)((n2 + 63) / 64), (unsigned)((n1 + 63) / 64));
-    hipStream_t st = (hipStream_t)stream;
-    if (precision == MRA_PRECISION_MIXED)
-        k_cov_tile<float><<<grid, 256, 0, st>>>(cc, P, n1, Q, n2, (float*)d_out);
+    const bool f32 = precision == MRA_PRECISION_MIXED;
+    const int vw = f32 ? 4 : 2;  // elements per 16-byte store
+    const uint64_t gx = (n2 + 64 * vw - 1) / (64 * vw), gy = (n1 + COV_ROWS - 1) / COV_ROWS;
+    if (gx > 0x7fffffffull || gy > 65535ull) {
+        g_err = "covariance matrix too large for one launch";
+        return MRA_E_ARG;
+    }
+    dim3 grid((unsigned)gx, (unsigned)gy);
+    const int vec = ((uintptr_t)d_out % 16 == 0) && (n2 % vw == 0);
+    if (f32 && metric == 2)
+        k_cov_tile_f32<true><<<grid, 256, 0, st>>>(P, n1, Q, n2, (float*)d_out, vec);
+    else if (f32 && metric == 0)
+        k_cov_tile_f32<false><<<grid, 256, 0, st>>>(P, n1, Q, n2, (float*)d_out, vec);
+    else if (f32)
+        k_cov_tile<float><<<grid, 256, 0, st>>>(cc, P, n1, Q, n2, (float*)d_out, vec);
     else
-        k_cov_tile<double><<<grid, 256, 0, st>>>(cc, P, n1, Q, n2, (double*)d_out);
+        k_cov_tile<double><<<grid, 256, 0, st>>>(cc, P, n1, Q, n2, (double*)d_out, vec);
     hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        g_err = hipGetErrorString(e);
+        return MRA_E_HIP;
+    }
+    return MRA_OK;
+}
+
+extern "C" int mra_cov_tile_device(int precision, int metric, const double* d_xyz1, const double* d_sb1, uint64_t n1,
+                                   const double* d_xyz2, const double* d_sb2, uint64_t n2, void* d_out,
+                                   void* stream) {
+    return cov_tile_launch(precision, metric, d_xyz1, d_sb1, n1, d_xyz2, d_sb2, n2, d_out, (hipStream_t)stream);
+}
+
+extern "C" int mra_cov_tile_timed(int precision, int metric, const double* d_xyz1, const double* d_sb1, uint64_t n1,
+                                  const double* d_xyz2, const double* d_sb2, uint64_t n2, void* d_out, int launches,
+                                  double* ms_per_launch) {
+    if (launches < 1 || !ms_per_launch) {
+        g_err = "invalid argument";
+        return MRA_E_ARG;
+    }
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = MRA_OK;
+    float ms = 0.0f;
+    if (hipStreamCreate(&st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+        g_err = "stream or event creation failed";
+        rc = MRA_E_HIP;
+    }
+    // one untimed launch, then `launches` launches between two events on the same stream
+    if (rc == MRA_OK) rc = cov_tile_launch(precision, metric, d_xyz1, d_sb1, n1, d_xyz2, d_sb2, n2, d_out, st);
+    if (rc == MRA_OK && hipEventRecord(e0, st) != hipSuccess) rc = MRA_E_HIP;
+    for (int i = 0; i < launches && rc == MRA_OK; i++)
+        rc = cov_tile_launch(precision, metric, d_xyz1, d_sb1, n1, d_xyz2, d_sb2, n2, d_out, st);
+    if (rc == MRA_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                         hipEventElapsedTime(&ms, e0, e1) != hipSuccess)) {
+        g_err = "event timing failed";
+        rc = MRA_E_HIP;
+    }
+    if (st) (void)hipStreamSynchronize(st);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (rc == MRA_OK) *ms_per_launch = (double)ms / launches;
+    return rc;
+}
+
+extern "C" int mra_device_alloc(uint64_t bytes, void** d_ptr) {
+    if (!d_ptr || !bytes) {
+        g_err = "invalid argument";
+        return MRA_E_ARG;
+    }
+    *d_ptr = nullptr;
+    if (hipMalloc(d_ptr, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        *d_ptr = nullptr;
+        g_err = "device allocation failed";
+        return MRA_E_NOMEM;
+    }
+    return MRA_OK;
+}
+
+extern "C" int mra_device_free(void* d_ptr) {
+    if (d_ptr && hipFree(d_ptr) != hipSuccess) {
+        g_err = "hipFree failed";
+        return MRA_E_HIP;
+    }
+    return MRA_OK;
+}
+
+extern "C" int mra_device_copy(void* dst, const void* src, uint64_t bytes, int to_device) {
+    if (!dst || !src) {
+        g_err = "invalid argument";
+        return MRA_E_ARG;
+    }
+    if (!bytes) return MRA_OK;
+    hipError_t e = hipMemcpy(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost);
     if (e != hipSuccess) {
         g_err = hipGetErrorString(e);
         return MRA_E_HIP;

@@ -172,43 +172,98 @@ __global__ void k_fill(double* p, uint64_t n, double val) {
 }
 
 // ------------------------------------------------------ covariance tiles
-// Standalone pairwise covariance tile (throughput entry point).
-template <typename T>
-__global__ __launch_bounds__(256) void k_cov_tile(CovCtx cc, Pts P, uint64_t n1, Pts Q, uint64_t n2, T* out) {
-    // 64 x 64 output tile per work group, each thread 16 outputs, coalesced rows
-    const uint64_t i0 = blockIdx.y * 64ull, j0 = blockIdx.x * 64ull;
-    const int tj = threadIdx.x & 63, ti = threadIdx.x >> 6;
-    const uint64_t j = j0 + tj;
-    if (j >= n2) return;
-    const double x2 = Q.x[j], y2 = Q.y[j], z2 = Q.z[j], s2 = Q.s[j], b2 = Q.b[j];
-    for (int k = 0; k < 16; k++) {
-        const uint64_t i = i0 + ti * 16 + k;
-        if (i >= n1) break;
-        double d = pdist(cc, P.x[i], P.y[i], P.z[i], x2, y2, z2);
-        double c;
-        if (cc.stationary) {
-            c = cc.sill * exp(-d / cc.range);
-        } else {
-            double b1 = P.b[i];
-            double bms = (b1 * b1 + b2 * b2) * 0.5;
-            double ratio = (b1 * b2) / bms;
-            c = (P.s[i] * s2) * (ratio * sqrt(ratio)) * exp(-d / sqrt(bms));
+// Standalone covariance matrix out[i*n2 + j] (row-major n1 x n2, the layout
+// evaluate_nonstationary_cross_covariance writes, src/evaluate_covariance.cpp:
+// 155-234), the pairwise evaluation on its own as a throughput kernel.
+// One work group covers COV_ROWS rows x 64*VW columns, VW = 16 bytes /
+// sizeof(T): each lane keeps VW consecutive column points in registers and
+// writes one 16-byte nontemporal store per row (the output is streamed, not
+// re-read), so each wave writes 1 KiB row segments; the row points are
+// staged in LDS (broadcast reads).  The pair formula is the one of the
+// engine's tiles, q_i q_j t^3 exp(-d t).  vec = 0 when the output is not
+// 16-byte aligned at every row; a work group whose columns reach past n2
+// also takes the scalar-store path.
+constexpr int COV_ROWS = 64;
+typedef double mra_d2 __attribute__((ext_vector_type(2)));
+
+// FP32, nonstationary, planar (PLANAR) or chordal metric: float point data.
+template <bool PLANAR>
+__global__ __launch_bounds__(256) void k_cov_tile_f32(Pts P, uint64_t n1, Pts Q, uint64_t n2, float* out, int vec) {
+    __shared__ PtF rows[COV_ROWS];
+    const uint64_t i0 = (uint64_t)blockIdx.y * COV_ROWS;
+    const uint64_t jb = (uint64_t)blockIdx.x * 256;
+    const uint64_t j0 = jb + (threadIdx.x & 63) * 4;
+    const int w = threadIdx.x >> 6;
+    const int nr = (int)(n1 - i0 < (uint64_t)COV_ROWS ? n1 - i0 : (uint64_t)COV_ROWS);
+    const CovCtx cc{PLANAR ? 2 : 0, 0, 1.0, 1.0};
+    for (int i = threadIdx.x; i < nr; i += MRA_WG) rows[i] = to_ptf(load_pt(cc, P, i0 + i));
+    PtF qf[4];
+    for (int v = 0; v < 4; v++) qf[v] = j0 + v < n2 ? to_ptf(load_pt(cc, Q, j0 + v)) : PtF{0, 0, 0, 0, 1};
+    __syncthreads();
+    if (vec && jb + 256 <= n2) {
+        for (int i = w; i < nr; i += MRA_WG / 64) {
+            const PtF pr = rows[i];
+            mra_f4 c;
+            c.x = pcov_ff<PLANAR>(pr, qf[0], !PLANAR);
+            c.y = pcov_ff<PLANAR>(pr, qf[1], !PLANAR);
+            c.z = pcov_ff<PLANAR>(pr, qf[2], !PLANAR);
+            c.w = pcov_ff<PLANAR>(pr, qf[3], !PLANAR);
+            __builtin_nontemporal_store(c, reinterpret_cast<mra_f4*>(out + (i0 + i) * n2 + j0));
         }
-        out[i * n2 + j] = (T)c;
+    } else {
+        for (int i = w; i < nr; i += MRA_WG / 64) {
+            const PtF pr = rows[i];
+            for (int v = 0; v < 4; v++)
+                if (j0 + v < n2) out[(i0 + i) * n2 + j0 + v] = pcov_ff<PLANAR>(pr, qf[v], !PLANAR);
+        }
     }
 }
 
-template <>
-__global__ __launch_bounds__(256) void k_cov_tile<float>(CovCtx cc, Pts P, uint64_t n1, Pts Q, uint64_t n2,
-                                                          float* out) {
-    const uint64_t i0 = blockIdx.y * 64ull, j0 = blockIdx.x * 64ull;
-    const int tj = threadIdx.x & 63, ti = threadIdx.x >> 6;
-    const uint64_t j = j0 + tj;
-    if (j >= n2) return;
-    for (int k = 0; k < 16; k++) {
-        const uint64_t i = i0 + ti * 16 + k;
-        if (i >= n1) break;
-        out[i * n2 + j] = pcovf(cc, P, i, Q, j);
+// Any metric and precision (FP64 output; FP32 for the great-circle metric).
+template <typename T>
+__global__ __launch_bounds__(256) void k_cov_tile(CovCtx cc, Pts P, uint64_t n1, Pts Q, uint64_t n2, T* out,
+                                                  int vec) {
+    constexpr int VW = 16 / sizeof(T);
+    __shared__ double st[5 * COV_ROWS];
+    const uint64_t i0 = (uint64_t)blockIdx.y * COV_ROWS;
+    const uint64_t jb = (uint64_t)blockIdx.x * (64 * VW);
+    const uint64_t j0 = jb + (threadIdx.x & 63) * VW;
+    const int w = threadIdx.x >> 6;
+    const int nr = (int)(n1 - i0 < (uint64_t)COV_ROWS ? n1 - i0 : (uint64_t)COV_ROWS);
+    const bool full = vec && jb + 64 * VW <= n2;
+    PtV q[VW];
+    for (int v = 0; v < VW; v++) q[v] = j0 + v < n2 ? load_pt(cc, Q, j0 + v) : PtV{0, 0, 0, 0, 1};
+    for (int i = threadIdx.x; i < nr; i += MRA_WG) {
+        const PtV p = load_pt(cc, P, i0 + i);
+        st[i] = p.x;
+        st[COV_ROWS + i] = p.y;
+        st[2 * COV_ROWS + i] = p.z;
+        st[3 * COV_ROWS + i] = p.q;
+        st[4 * COV_ROWS + i] = p.b;
+    }
+    __syncthreads();
+    for (int i = w; i < nr; i += MRA_WG / 64) {
+        const PtV pr = {st[i], st[COV_ROWS + i], st[2 * COV_ROWS + i], st[3 * COV_ROWS + i], st[4 * COV_ROWS + i]};
+        T c[VW];
+        for (int v = 0; v < VW; v++) {
+            if constexpr (sizeof(T) == 4)
+                c[v] = pcov_ptf(cc, pr, q[v]);
+            else
+                c[v] = pcov_pt(cc, pr, q[v]);
+        }
+        T* o = out + (i0 + i) * n2 + j0;
+        if (full) {
+            if constexpr (sizeof(T) == 4) {
+                const mra_f4 cv = {c[0], c[1], c[2], c[3]};
+                __builtin_nontemporal_store(cv, reinterpret_cast<mra_f4*>(o));
+            } else {
+                const mra_d2 cv = {c[0], c[1]};
+                __builtin_nontemporal_store(cv, reinterpret_cast<mra_d2*>(o));
+            }
+        } else {
+            for (int v = 0; v < VW; v++)
+                if (j0 + v < n2) o[v] = c[v];
+        }
     }
 }
 
