@@ -36,6 +36,7 @@ EXPORTED = ["mra_create", "mra_destroy", "mra_last_error", "mra_set_covariance",
             "mra_get_tuning", "mra_autotune", "mra_cov_tile_timed", "mra_device_alloc", "mra_device_free",
             "mra_device_copy"]
 OPT_FACTORED_POSTERIOR = 1
+OPT_GRAM_KERNEL = 2
 
 
 class TreeDesc(C.Structure):

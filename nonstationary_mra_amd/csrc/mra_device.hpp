@@ -402,65 +402,78 @@ __device__ __forceinline__ void wg_gemm_tn_impl(TO* C, int ldc, int M, int N, in
     }
 }
 
-// One wave computes a whole 64 x 64 output C = al1 A1^T B1 + al2 A2^T B2
-// (all operands in range, 4 x 4 MFMA tiles in registers: every operand value
-// loaded feeds four matrix instructions) and stores it to C.  Waves work on
-// different outputs independently (no barrier).  FP64.
-__device__ __forceinline__ void wave_gemm2_store64(double* C, int ldc, int K1, double al1, const double* A1,
-                                                   int64_t lda1, const double* B1, int64_t ldb1, int K2, double al2,
-                                                   const double* A2, int64_t lda2, const double* B2, int64_t ldb2) {
-    typedef Mfma<double>::V V;
+// One wave accumulates a whole 64 x 64 product acc += al A^T B (A, B: K x 64
+// row-major with leading dimensions lda, ldb, all in range): 4 x 4 MFMA tiles
+// in registers, so every operand value loaded feeds four matrix
+// instructions.  Tile (ti, tj), element v of lane t holds row
+// 16 ti + t/16 + 4 v, column 16 tj + t%16.  FP64.  Waves work on different
+// outputs independently (no barrier).
+__device__ __forceinline__ void wave_mm64_acc(mra_d4 (&acc)[16], int K, double al, const double* A, int64_t lda,
+                                              const double* B, int64_t ldb) {
     const int lane = threadIdx.x & 63;
     const int r16 = lane & 15, k4 = lane >> 4;
-    V acc[16];
+    const uint32_t oa = (uint32_t)((k4 * lda + r16) * sizeof(double));
+    const uint32_t ob = (uint32_t)((k4 * ldb + r16) * sizeof(double));
+    int k0 = 0;
+    for (; k0 + 8 <= K; k0 += 8) {
+        double av[2][4], bv[2][4];
 #pragma unroll
-    for (int q = 0; q < 16; q++) acc[q] = V{0, 0, 0, 0};
-    // al scales the A operand as it is loaded (exact for al = +-1)
-    auto run = [&](int K, double al, const double* A, int64_t lda, const double* B, int64_t ldb) {
-        V* part = acc;
-        const uint32_t oa = (uint32_t)((k4 * lda + r16) * sizeof(double));
-        const uint32_t ob = (uint32_t)((k4 * ldb + r16) * sizeof(double));
-        int k0 = 0;
-        for (; k0 + 8 <= K; k0 += 8) {
-            double a[2][4], b[2][4];
-#pragma unroll
-            for (int u = 0; u < 2; u++) {
-                const char* Ak = (const char*)(A + (int64_t)(k0 + 4 * u) * lda);
-                const char* Bk = (const char*)(B + (int64_t)(k0 + 4 * u) * ldb);
-#pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    a[u][t] = al * *(const double*)(Ak + oa + 16 * t * sizeof(double));
-                    b[u][t] = *(const double*)(Bk + ob + 16 * t * sizeof(double));
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 2; u++)
-#pragma unroll
-                for (int ti = 0; ti < 4; ti++)
-#pragma unroll
-                    for (int tj = 0; tj < 4; tj++)
-                        part[ti * 4 + tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][ti], b[u][tj], part[ti * 4 + tj],
-                                                                                 0, 0, 0);
-        }
-        for (; k0 < K; k0 += 4) {
-            const int k = k0 + k4;
-            const bool kv = k < K;
-            const int64_t kc = kv ? k : K - 1;
-            double a[4], b[4];
+        for (int u = 0; u < 2; u++) {
+            const char* Ak = (const char*)(A + (int64_t)(k0 + 4 * u) * lda);
+            const char* Bk = (const char*)(B + (int64_t)(k0 + 4 * u) * ldb);
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                a[t] = kv ? al * A[kc * lda + r16 + 16 * t] : 0.0;
-                b[t] = kv ? B[kc * ldb + r16 + 16 * t] : 0.0;
+                av[u][t] = al * *(const double*)(Ak + oa + 16 * t * sizeof(double));
+                bv[u][t] = *(const double*)(Bk + ob + 16 * t * sizeof(double));
             }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++)
 #pragma unroll
             for (int ti = 0; ti < 4; ti++)
 #pragma unroll
                 for (int tj = 0; tj < 4; tj++)
-                    part[ti * 4 + tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ti], b[tj], part[ti * 4 + tj], 0, 0, 0);
+                    acc[ti * 4 + tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][ti], bv[u][tj], acc[ti * 4 + tj],
+                                                                            0, 0, 0);
+    }
+    // remaining k steps: rows clamped into range, zero beyond K
+    for (; k0 < K; k0 += 4) {
+        const int k = k0 + k4;
+        const bool kv = k < K;
+        const int64_t kc = kv ? k : K - 1;
+        double av[4], bv[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            av[t] = kv ? al * A[kc * lda + r16 + 16 * t] : 0.0;
+            bv[t] = kv ? B[kc * ldb + r16 + 16 * t] : 0.0;
         }
-    };
-    run(K1, al1, A1, lda1, B1, ldb1);
-    run(K2, al2, A2, lda2, B2, ldb2);
+#pragma unroll
+        for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+            for (int tj = 0; tj < 4; tj++)
+                acc[ti * 4 + tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ti], bv[tj], acc[ti * 4 + tj], 0, 0, 0);
+    }
+}
+
+// acc += S (a 64 x 64 row-major block with leading dimension lds), in the
+// register layout of wave_mm64_acc.
+__device__ __forceinline__ void wave_add64(mra_d4 (&acc)[16], const double* S, int lds) {
+    const int lane = threadIdx.x & 63;
+    const int r16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+    for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const double* srow = S + (int64_t)(ti * 16 + k4 + 4 * v) * lds + r16;
+#pragma unroll
+            for (int tj = 0; tj < 4; tj++) acc[ti * 4 + tj][v] += srow[16 * tj];
+        }
+}
+
+// C = acc (64 x 64 row-major, leading dimension ldc).
+__device__ __forceinline__ void wave_store64(double* C, int ldc, const mra_d4 (&acc)[16]) {
+    const int lane = threadIdx.x & 63;
+    const int r16 = lane & 15, k4 = lane >> 4;
 #pragma unroll
     for (int ti = 0; ti < 4; ti++)
 #pragma unroll

@@ -42,6 +42,7 @@ struct mra_handle {
     // factored posterior of the finest-but-one level
     bool lowrank = false;
     int lowrankMode = 1;  // requested: 1 = when possible, 0 = dense everywhere
+    int gramSep = 1;      // message blocks by k_post_gram (RP = 64); 0: inside k_posterior
     bool denseFull = false;  // full-size dense messages of level L-1 allocated
     std::vector<uint64_t> hRowS1, hRowE1, hRowS2, hRowE2;  // rows of levels L-1 and L-2
     uint64_t *dRowS1 = nullptr, *dRowE1 = nullptr, *dRowS2 = nullptr, *dRowE2 = nullptr;
@@ -772,6 +773,11 @@ extern "C" int mra_set_option(mra_handle* h, int option, int value) {
         graph_invalidate(h);
         return MRA_OK;
     }
+    if (option == MRA_OPT_GRAM_KERNEL) {
+        h->gramSep = value ? 1 : 0;
+        graph_invalidate(h);
+        return MRA_OK;
+    }
     set_err(h, "unknown option");
     return MRA_E_ARG;
 }
@@ -780,6 +786,10 @@ extern "C" int mra_get_option(mra_handle* h, int option, int* value) {
     if (!h || !value) return MRA_E_ARG;
     if (option == MRA_OPT_FACTORED_POSTERIOR) {
         *value = use_lowrank(h) ? 1 : 0;
+        return MRA_OK;
+    }
+    if (option == MRA_OPT_GRAM_KERNEL) {
+        *value = (h->RP == 64 && h->gramSep) ? 1 : 0;
         return MRA_OK;
     }
     set_err(h, "unknown option");
@@ -946,6 +956,24 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
     }
     // posterior, finest-but-one level up to the root
     const bool lr = use_lowrank(h);
+    // RP = 64: message blocks by the one-wave-per-block kernel after each
+    // dense posterior launch that writes messages
+    const bool gsep = RP == 64 && h->gramSep;
+    auto post = [&](const PostArgs& pa_, uint32_t nitems, bool from_obs) {
+        PostArgs q = pa_;
+        q.gram_sep = gsep && q.write_msgs && q.l > 0;
+        if (from_obs)
+            k_posterior<RP, true><<<nitems, 256, 0, st>>>(T, q);
+        else
+            k_posterior<RP, false><<<nitems, 256, 0, st>>>(T, q);
+        if (q.gram_sep) {
+            const uint32_t grid = post_gram_grid(nitems, q.l);
+            if (from_obs)
+                k_post_gram<true><<<grid, 256, 0, st>>>(T, q, nitems);
+            else
+                k_post_gram<false><<<grid, 256, 0, st>>>(T, q, nitems);
+        }
+    };
     for (int l = h->L - 1; l >= 0; l--) {
         mark(MRA_T_POST0 + l);
         if (!h->nItemsPost[l]) continue;
@@ -995,7 +1023,7 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
                     d.Xz = h->dXo;
                     d.ldxz = (h->L - 1) * h->RP;
                     d.zz = h->dZo;
-                    k_posterior<RP, true><<<h->nItemsBig, 256, 0, st>>>(T, d);
+                    post(d, h->nItemsBig, true);
                 }
                 if (h->nItemsLR[3]) {
                     b.items = h->dItemsLR[3];
@@ -1018,10 +1046,10 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
                     a.write_msgs = 0;
                     a.write_d = 0;
                     a.items = h->dItemsLRAll;
-                    k_posterior<RP, true><<<h->nItemsLRAll, 256, 0, st>>>(T, a);
+                    post(a, h->nItemsLRAll, true);
                 }
             } else {
-                k_posterior<RP, true><<<h->nItemsPost[l], 256, 0, st>>>(T, a);
+                post(a, h->nItemsPost[l], true);
             }
         } else if (l == h->L - 2 && lr) {
             a.X = h->dXo;
@@ -1033,12 +1061,12 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
             a.wc = h->dW[l + 1];
             a.Ac = h->dA[l + 1];
             a.start_c = h->lvlStart[l + 1];
-            k_posterior<RP, true><<<h->nItemsPost[l], 256, 0, st>>>(T, a);
+            post(a, h->nItemsPost[l], true);
         } else {
             a.wc = h->dW[l + 1];
             a.Ac = h->dA[l + 1];
             a.start_c = h->lvlStart[l + 1];
-            k_posterior<RP, false><<<h->nItemsPost[l], 256, 0, st>>>(T, a);
+            post(a, h->nItemsPost[l], false);
         }
     }
     mark(MRA_T_PRED_POST);

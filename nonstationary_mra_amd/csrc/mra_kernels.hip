@@ -557,6 +557,7 @@ struct PostArgs {
     int ldxz;
     double* zz;
     int* bad_flag;
+    int gram_sep;  // the message blocks A~_jk come from k_post_gram (RP = 64)
 };
 
 template <int RP, bool FROM_OBS>
@@ -720,6 +721,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             return;
         }
         // A~_{jk} = A_{jk} - P_j^T P_k
+        if (a.gram_sep) {
+            if (threadIdx.x == 0 && bad) atomicOr(a.bad_flag, 1);
+            return;
+        }
         const int nsl = l * (l + 1) / 2;
         if (FROM_OBS) {
             // A_{jk} = X_j^T X_k over the stacked rows: both products in
@@ -771,6 +776,70 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             }
     }
     if (threadIdx.x == 0 && bad) atomicOr(a.bad_flag, 1);
+}
+
+// Message blocks A~_jk = A_jk - P_j^T P_k (j <= k < l) of the regions of one
+// posterior level (src/class_approximation.cpp:289-303), RP = 64: one wave
+// per 64 x 64 block with its 4 x 4 MFMA tiles in registers, four blocks of the
+// same region per work group, launched after k_posterior has stored P.
+// FROM_OBS: A_jk = X_j^T X_k over the region's stacked rows, plus the dense
+// messages of children that sent one (cslot); otherwise A_jk is the sum of
+// the children's messages.  Work group g runs on XCD g mod 8 (round-robin
+// dispatch); the mapping below gives every work group of one region the same
+// g mod 8, so the region's rows are re-read from that XCD's L2.
+template <bool FROM_OBS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_post_gram(Tree T, PostArgs a,
+                                                                                           uint32_t nitems) {
+    constexpr int RP = 64;
+    const int l = a.l;
+    const int nsl = l * (l + 1) / 2;
+    const uint32_t nq = (uint32_t)(nsl + 3) / 4;
+    const uint32_t g = blockIdx.x, xcd = g & 7u, t = g >> 3;
+    const uint32_t it = (t / nq) * 8u + xcd;
+    const int b = (int)(t % nq) * 4 + wave_id();
+    if (it >= nitems || b >= nsl) return;
+    // (j, k) of message slot b: (0,0), (0,1), ..., (0,l-1), (1,1), ...
+    int j = 0, rem = b;
+    while (rem >= l - j) {
+        rem -= l - j;
+        j++;
+    }
+    const int k = j + rem;
+    const uint64_t loc = a.items[it];
+    const uint64_t R = a.start_l + loc;
+    const uint64_t msg = a.slot ? (uint64_t)a.slot[loc] : loc;
+    const double* PR = a.P + a.goff[l] + loc * (uint64_t)l * RP * RP;
+    mra_d4 acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) acc[q] = mra_d4{0, 0, 0, 0};
+    const int lc = l + 1;
+    const int nslc = lc * (lc + 1) / 2;
+    const int slc = j * lc - j * (j - 1) / 2 + (k - j);
+    if (FROM_OBS) {
+        const uint64_t os = a.row_s[loc], oe = a.row_e[loc];
+        if (oe > os) {
+            const double* X = a.X + os * (uint64_t)a.ldx;
+            wave_mm64_acc(acc, (int)(oe - os), 1.0, X + j * RP, a.ldx, X + k * RP, a.ldx);
+        }
+        if (a.cslot)
+            for (int c = 0; c < T.J; c++) {
+                const int32_t cs = a.cslot[R * T.J + 1 + c - a.start_c];
+                if (cs >= 0) wave_add64(acc, a.Ac + ((uint64_t)cs * nslc + slc) * (uint64_t)RP * RP, RP);
+            }
+    } else {
+        for (int c = 0; c < T.J; c++) {
+            const uint64_t ch = R * T.J + 1 + c;
+            if (T.working[ch]) wave_add64(acc, a.Ac + ((ch - a.start_c) * nslc + slc) * (uint64_t)RP * RP, RP);
+        }
+    }
+    wave_mm64_acc(acc, RP, -1.0, PR + (uint64_t)j * RP * RP, RP, PR + (uint64_t)k * RP * RP, RP);
+    wave_store64(a.Ao + (msg * nsl + b) * (uint64_t)RP * RP, RP, acc);
+}
+
+// Grid of k_post_gram for nitems regions of level l.
+static inline uint32_t post_gram_grid(uint32_t nitems, int l) {
+    const uint32_t nq = (uint32_t)(l * (l + 1) / 2 + 3) / 4;
+    return ((nitems + 7u) / 8u) * nq * 8u;
 }
 
 // Factored posterior of a finest-but-one region R (level l = L-1).  The

@@ -21,12 +21,13 @@ def rel(a, b):
     return abs(a - b) / max(abs(b), 1e-300)
 
 
-def run_gpu(tree, cov, prm, predict=True, factored=1):
+def run_gpu(tree, cov, prm, predict=True, factored=1, gram=1):
     h = capi.Handle(tree, with_pred=predict)
     try:
         h.set_covariance(cov)
         h.set_params(**prm)
         h.set_option(capi.OPT_FACTORED_POSTERIOR, factored)
+        h.set_option(capi.OPT_GRAM_KERNEL, gram)
         if predict and h.n_pred:
             return h.predict()
         return h.likelihood(), None, None
@@ -34,8 +35,8 @@ def run_gpu(tree, cov, prm, predict=True, factored=1):
         h.close()
 
 
-def check(tree, cov, prm, predict=True, factored=1):
-    D, mean, var = run_gpu(tree, cov, prm, predict, factored)
+def check(tree, cov, prm, predict=True, factored=1, gram=1):
+    D, mean, var = run_gpu(tree, cov, prm, predict, factored, gram)
     Do, mo, vo = oracle.evaluate(tree, cov, prm, predict=predict)
     print(f"D gpu {D!r} oracle {Do!r} rel {rel(D, Do):.3e}", flush=True)
     assert rel(D, Do) < TOL_D
@@ -200,6 +201,26 @@ def test_dense_and_factored_posterior(case, factored, example):
     assert h.get_option(capi.OPT_FACTORED_POSTERIOR) == factored
     h.close()
     check(tree, cov, prm, factored=factored)
+
+
+@pytest.mark.parametrize("factored", [1, 0])
+@pytest.mark.parametrize("gram", [1, 0])
+def test_message_block_kernel(gram, factored):
+    """Posterior message blocks of 64-knot regions from the one-wave-per-block
+    kernel (gram=1) or inside the posterior kernel (gram=0): both match the
+    oracle, with dense messages at three coarse levels and, in the factored
+    form, a rows-mode level mixing factored and dense children."""
+    rng = np.random.default_rng(91)
+    x, y = synth.clustered_points(12000, rng, n_clusters=4, sigma=0.02)
+    px, py = synth.uniform_points(200, rng)
+    tree = capi.partition_program(x, y, rng.standard_normal(12000), px, py, M=5, J=4, r=64,
+                                  bounds=synth.UNIT_BOUNDS)
+    cov, prm = synth.linear_trend_cov(), dict(synth.UNIT_PARAMS)
+    h = capi.Handle(tree)
+    h.set_option(capi.OPT_GRAM_KERNEL, gram)
+    assert h.get_option(capi.OPT_GRAM_KERNEL) == gram
+    h.close()
+    check(tree, cov, prm, factored=factored, gram=gram)
 
 
 TOL_MIXED = 1e-6   # |delta loglik| / n for the mixed-precision path
