@@ -146,8 +146,9 @@ int mra_use_graph(mra_handle* h, int enable);
  * reformulation by the Woodbury identity); 0 uses the reference's dense
  * update at every level.  mra_get_option reports whether the factored form is in effect.
  * MRA_OPT_GRAM_KERNEL: 1 (default) computes the posterior message blocks
- * A~_jk = A_jk - P_j^T P_k of 64-knot regions in a separate kernel, one wave
- * per block; 0 computes them inside the posterior kernel (same values up to
+ * A~_jk = X_j^T X_k - P_j^T P_k of 64-knot regions whose messages come from
+ * stacked rows in a separate kernel, one wave per block; 0 computes them
+ * inside the posterior kernel (same values up to
  * rounding). */
 enum { MRA_OPT_FACTORED_POSTERIOR = 1, MRA_OPT_GRAM_KERNEL = 2 };
 int mra_set_option(mra_handle* h, int option, int value);

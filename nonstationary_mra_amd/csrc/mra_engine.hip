@@ -956,23 +956,21 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
     }
     // posterior, finest-but-one level up to the root
     const bool lr = use_lowrank(h);
-    // RP = 64: message blocks by the one-wave-per-block kernel after each
-    // dense posterior launch that writes messages
+    // RP = 64: message blocks of the levels that read stacked rows (X^T X
+    // products over the observations or factor rows below a region) by the
+    // one-wave-per-block kernel after the posterior launch; the levels above
+    // sum dense child messages inside the posterior kernel
     const bool gsep = RP == 64 && h->gramSep;
     auto post = [&](const PostArgs& pa_, uint32_t nitems, bool from_obs) {
         PostArgs q = pa_;
-        q.gram_sep = gsep && q.write_msgs && q.l > 0;
+        q.pj_sep = gsep && from_obs && q.l > 0;
+        q.gram_sep = q.pj_sep && q.write_msgs;
         if (from_obs)
             k_posterior<RP, true><<<nitems, 256, 0, st>>>(T, q);
         else
             k_posterior<RP, false><<<nitems, 256, 0, st>>>(T, q);
-        if (q.gram_sep) {
-            const uint32_t grid = post_gram_grid(nitems, q.l);
-            if (from_obs)
-                k_post_gram<true><<<grid, 256, 0, st>>>(T, q, nitems);
-            else
-                k_post_gram<false><<<grid, 256, 0, st>>>(T, q, nitems);
-        }
+        if (q.pj_sep) k_post_pj<<<post_pj_grid(nitems, q.l), 256, 0, st>>>(T, q, nitems);
+        if (q.gram_sep) k_post_gram<<<post_gram_grid(nitems, q.l), 256, 0, st>>>(T, q, nitems);
     };
     for (int l = h->L - 1; l >= 0; l--) {
         mark(MRA_T_POST0 + l);

@@ -558,6 +558,7 @@ struct PostArgs {
     double* zz;
     int* bad_flag;
     int gram_sep;  // the message blocks A~_jk come from k_post_gram (RP = 64)
+    int pj_sep;    // P_j and w~_j come from k_post_pj (RP = 64, stacked rows)
 };
 
 template <int RP, bool FROM_OBS>
@@ -693,6 +694,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     for (int i = threadIdx.x; i < r; i += MRA_WG) lq -= vv[i] * vv[i];
     const double post_part = wg_sum(lq, red);
     if (threadIdx.x == 0 && a.write_d) a.dterm[R] = -2.0 * logdet_prior + post_part;
+    if (l > 0 && a.pj_sep) {
+        if (threadIdx.x == 0 && bad) atomicOr(a.bad_flag, 1);
+        return;
+    }
     if (l > 0) {
         double* PR = a.P + a.goff[l] + (R - a.start_l) * (uint64_t)l * RP * RP;
         const double* LpiTR = a.LpiT + R * RP * RP;
@@ -778,16 +783,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     if (threadIdx.x == 0 && bad) atomicOr(a.bad_flag, 1);
 }
 
-// Message blocks A~_jk = A_jk - P_j^T P_k (j <= k < l) of the regions of one
-// posterior level (src/class_approximation.cpp:289-303), RP = 64: one wave
-// per 64 x 64 block with its 4 x 4 MFMA tiles in registers, four blocks of the
-// same region per work group, launched after k_posterior has stored P.
-// FROM_OBS: A_jk = X_j^T X_k over the region's stacked rows, plus the dense
-// messages of children that sent one (cslot); otherwise A_jk is the sum of
-// the children's messages.  Work group g runs on XCD g mod 8 (round-robin
-// dispatch); the mapping below gives every work group of one region the same
-// g mod 8, so the region's rows are re-read from that XCD's L2.
-template <bool FROM_OBS>
+// Message blocks A~_jk = A_jk - P_j^T P_k (j <= k < l) of the regions of a
+// posterior level that reads stacked rows (src/class_approximation.cpp:
+// 289-303), RP = 64: A_jk = X_j^T X_k over the region's rows (observation
+// rows H, or the factor rows of the level below) plus the dense messages of
+// children that sent one (cslot).  One wave per 64 x 64 block with its 4 x 4
+// MFMA tiles in registers (each operand value loaded once per block, not
+// once per wave of a shared block), four blocks of the same region per work
+// group, launched after k_posterior has stored P.  Work group g runs on XCD
+// g mod 8 (round-robin dispatch); the mapping below gives every work group
+// of one region the same g mod 8, so the region's rows are re-read from that
+// XCD's L2.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_post_gram(Tree T, PostArgs a,
                                                                                            uint32_t nitems) {
     constexpr int RP = 64;
@@ -815,25 +821,118 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int lc = l + 1;
     const int nslc = lc * (lc + 1) / 2;
     const int slc = j * lc - j * (j - 1) / 2 + (k - j);
-    if (FROM_OBS) {
-        const uint64_t os = a.row_s[loc], oe = a.row_e[loc];
-        if (oe > os) {
-            const double* X = a.X + os * (uint64_t)a.ldx;
-            wave_mm64_acc(acc, (int)(oe - os), 1.0, X + j * RP, a.ldx, X + k * RP, a.ldx);
+    const uint64_t os = a.row_s[loc], oe = a.row_e[loc];
+    if (oe > os) {
+        const double* X = a.X + os * (uint64_t)a.ldx;
+        wave_mm64_acc(acc, (int)(oe - os), 1.0, X + j * RP, a.ldx, X + k * RP, a.ldx);
+    }
+    if (a.cslot)
+        for (int c = 0; c < T.J; c++) {
+            const int32_t cs = a.cslot[R * T.J + 1 + c - a.start_c];
+            if (cs >= 0) wave_add64(acc, a.Ac + ((uint64_t)cs * nslc + slc) * (uint64_t)RP * RP, RP);
         }
+    wave_mm64_acc(acc, RP, -1.0, PR + (uint64_t)j * RP * RP, RP, PR + (uint64_t)k * RP * RP, RP);
+    wave_store64(a.Ao + (msg * nsl + b) * (uint64_t)RP * RP, RP, acc);
+}
+
+// P_j = L~^{-1} A_jl^T and w~_j = w_j - P_j^T v (j < l) of the regions of a
+// posterior level that reads stacked rows (src/class_approximation.cpp:
+// 266-306), RP = 64, after k_posterior has stored L~^{-1} and v: one wave per
+// j.  A_jl^T = X_l^T X_j over the region's rows (plus the transposed dense
+// messages of children that sent one) accumulates in registers; the FP64 MFMA
+// result layout is the B-operand layout, so it feeds L~^{-1} A_jl^T directly
+// (zero upper blocks of L~^{-1} skipped).  Work groups of one region share
+// g mod 8 (one XCD), as in k_post_gram.
+__device__ __forceinline__ void wave_add64_t(mra_d4 (&acc)[16], const double* S, int lds) {
+    // acc += S^T (S 64 x 64 row-major), in the register layout of wave_mm64_acc
+    const int lane = threadIdx.x & 63;
+    const int r16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+    for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const double* scol = S + (int64_t)r16 * lds + ti * 16 + k4 + 4 * v;
+#pragma unroll
+            for (int tj = 0; tj < 4; tj++) acc[ti * 4 + tj][v] += scol[16 * tj * lds];
+        }
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_post_pj(Tree T, PostArgs a,
+                                                                                         uint32_t nitems) {
+    constexpr int RP = 64;
+    const int l = a.l;
+    const uint32_t nq = (uint32_t)(l + 3) / 4;
+    const uint32_t g = blockIdx.x, xcd = g & 7u, t = g >> 3;
+    const uint32_t it = (t / nq) * 8u + xcd;
+    const int j = (int)(t % nq) * 4 + wave_id();
+    if (it >= nitems || j >= l) return;
+    const int lane = threadIdx.x & 63, r16 = lane & 15, k4 = lane >> 4;
+    const uint64_t loc = a.items[it];
+    const uint64_t R = a.start_l + loc;
+    const uint64_t msg = a.slot ? (uint64_t)a.slot[loc] : loc;
+    const int r = T.r;
+    const uint64_t os = a.row_s[loc], oe = a.row_e[loc];
+    const double* X = a.X + os * (uint64_t)a.ldx;
+    const int ns = oe > os ? (int)(oe - os) : 0;
+    const int lc = l + 1;
+    const int nslc = lc * (lc + 1) / 2;
+    const int slc = j * lc - j * (j - 1) / 2 + (l - j);  // child slot of (j, l)
+    mra_d4 acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) acc[q] = mra_d4{0, 0, 0, 0};
+    if (ns > 0) wave_mm64_acc(acc, ns, 1.0, X + l * RP, a.ldx, X + j * RP, a.ldx);
+    if (a.cslot)
+        for (int c = 0; c < T.J; c++) {
+            const int32_t cs = a.cslot[R * T.J + 1 + c - a.start_c];
+            if (cs >= 0) wave_add64_t(acc, a.Ac + ((uint64_t)cs * nslc + slc) * (uint64_t)RP * RP, RP);
+        }
+    // P_j = L~^{-1} acc by row tiles tq (k tiles kt <= tq), stored row-major
+    const char* Li = (const char*)(a.LpiT + R * RP * RP);  // [k*RP + i] = L~^{-1}[i][k]
+    double* Pj = a.P + a.goff[l] + loc * (uint64_t)l * RP * RP + (uint64_t)j * RP * RP;
+    char* Pc = (char*)Pj;
+    uint32_t lo = (uint32_t)(k4 * RP + r16);
+#pragma unroll
+    for (int tq = 0; tq < 4; tq++) {
+        asm volatile("" : "+v"(lo));  // keep the displaced offsets in the loop
+        mra_d4 gq[4];
+#pragma unroll
+        for (int tj = 0; tj < 4; tj++) gq[tj] = mra_d4{0, 0, 0, 0};
+#pragma unroll
+        for (int kt = 0; kt <= tq; kt++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const double av = *(const double*)(Li + 8u * (lo + (16 * kt + 4 * v) * RP + 16 * tq));
+#pragma unroll
+                for (int tj = 0; tj < 4; tj++)
+                    gq[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, acc[kt * 4 + tj][v], gq[tj], 0, 0, 0);
+            }
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+#pragma unroll
+            for (int tj = 0; tj < 4; tj++) *(double*)(Pc + 8u * (lo + (16 * tq + 4 * v) * RP + 16 * tj)) = gq[tj][v];
+    }
+    if (!a.write_msgs) return;
+    // w~_j = w_j - P_j^T v: lane q reads column q of P_j (other lanes' stores)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    const int q = lane;
+    double wj = 0.0;
+    if (q < r) {
+        for (int o = 0; o < ns; o++) wj += X[(uint64_t)o * a.ldx + j * RP + q] * a.z[os + o];
         if (a.cslot)
             for (int c = 0; c < T.J; c++) {
                 const int32_t cs = a.cslot[R * T.J + 1 + c - a.start_c];
-                if (cs >= 0) wave_add64(acc, a.Ac + ((uint64_t)cs * nslc + slc) * (uint64_t)RP * RP, RP);
+                if (cs >= 0) wj += a.wc[(uint64_t)cs * lc * RP + j * RP + q];
             }
-    } else {
-        for (int c = 0; c < T.J; c++) {
-            const uint64_t ch = R * T.J + 1 + c;
-            if (T.working[ch]) wave_add64(acc, a.Ac + ((ch - a.start_c) * nslc + slc) * (uint64_t)RP * RP, RP);
-        }
     }
-    wave_mm64_acc(acc, RP, -1.0, PR + (uint64_t)j * RP * RP, RP, PR + (uint64_t)k * RP * RP, RP);
-    wave_store64(a.Ao + (msg * nsl + b) * (uint64_t)RP * RP, RP, acc);
+    const double* vR = a.v + R * RP;
+    double pv = 0.0;
+    for (int m = 0; m < RP; m++) pv += Pj[m * RP + q] * vR[m];
+    a.wo[msg * (uint64_t)l * RP + j * RP + q] = wj - pv;
+}
+
+static inline uint32_t post_pj_grid(uint32_t nitems, int l) {
+    return ((nitems + 7u) / 8u) * ((uint32_t)(l + 3) / 4) * 8u;
 }
 
 // Grid of k_post_gram for nitems regions of level l.
