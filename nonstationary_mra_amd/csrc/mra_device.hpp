@@ -455,6 +455,64 @@ __device__ __forceinline__ void wave_mm64_acc(mra_d4 (&acc)[16], int K, double a
     }
 }
 
+// wave_mm64_acc with al = 1 that also accumulates w = B^T z (z: K values):
+// z is the A operand of four more matrix instructions per k step, the same
+// in all 16 rows of A, so every row of wacc[tj] holds w[16 tj + column];
+// the B values are the ones already loaded for the product.
+__device__ __forceinline__ void wave_mm64_acc_w(mra_d4 (&acc)[16], mra_d4 (&wacc)[4], int K, const double* A,
+                                                int64_t lda, const double* B, int64_t ldb, const double* z) {
+    const int lane = threadIdx.x & 63;
+    const int r16 = lane & 15, k4 = lane >> 4;
+    const uint32_t oa = (uint32_t)((k4 * lda + r16) * sizeof(double));
+    const uint32_t ob = (uint32_t)((k4 * ldb + r16) * sizeof(double));
+    int k0 = 0;
+    for (; k0 + 8 <= K; k0 += 8) {
+        double av[2][4], bv[2][4], zv[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const char* Ak = (const char*)(A + (int64_t)(k0 + 4 * u) * lda);
+            const char* Bk = (const char*)(B + (int64_t)(k0 + 4 * u) * ldb);
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                av[u][t] = *(const double*)(Ak + oa + 16 * t * sizeof(double));
+                bv[u][t] = *(const double*)(Bk + ob + 16 * t * sizeof(double));
+            }
+            zv[u] = z[k0 + 4 * u + k4];
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+#pragma unroll
+            for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+                for (int tj = 0; tj < 4; tj++)
+                    acc[ti * 4 + tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][ti], bv[u][tj], acc[ti * 4 + tj],
+                                                                            0, 0, 0);
+#pragma unroll
+            for (int tj = 0; tj < 4; tj++)
+                wacc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(zv[u], bv[u][tj], wacc[tj], 0, 0, 0);
+        }
+    }
+    for (; k0 < K; k0 += 4) {
+        const int k = k0 + k4;
+        const bool kv = k < K;
+        const int64_t kc = kv ? k : K - 1;
+        double av[4], bv[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            av[t] = kv ? A[kc * lda + r16 + 16 * t] : 0.0;
+            bv[t] = kv ? B[kc * ldb + r16 + 16 * t] : 0.0;
+        }
+        const double zv = kv ? z[kc] : 0.0;
+#pragma unroll
+        for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+            for (int tj = 0; tj < 4; tj++)
+                acc[ti * 4 + tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ti], bv[tj], acc[ti * 4 + tj], 0, 0, 0);
+#pragma unroll
+        for (int tj = 0; tj < 4; tj++) wacc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(zv, bv[tj], wacc[tj], 0, 0, 0);
+    }
+}
+
 // acc += S (a 64 x 64 row-major block with leading dimension lds), in the
 // register layout of wave_mm64_acc.
 __device__ __forceinline__ void wave_add64(mra_d4 (&acc)[16], const double* S, int lds) {

@@ -877,19 +877,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int lc = l + 1;
     const int nslc = lc * (lc + 1) / 2;
     const int slc = j * lc - j * (j - 1) / 2 + (l - j);  // child slot of (j, l)
-    mra_d4 acc[16];
+    mra_d4 acc[16], wacc[4];
 #pragma unroll
     for (int q = 0; q < 16; q++) acc[q] = mra_d4{0, 0, 0, 0};
-    if (ns > 0) wave_mm64_acc(acc, ns, 1.0, X + l * RP, a.ldx, X + j * RP, a.ldx);
+#pragma unroll
+    for (int q = 0; q < 4; q++) wacc[q] = mra_d4{0, 0, 0, 0};
+    // acc = X_l^T X_j, wacc = X_j^T z (every row)
+    if (ns > 0) wave_mm64_acc_w(acc, wacc, ns, X + l * RP, a.ldx, X + j * RP, a.ldx, a.z + os);
+    double wq[4];  // w_j[16 tj + r16]: every row of wacc holds it
+#pragma unroll
+    for (int tj = 0; tj < 4; tj++) wq[tj] = wacc[tj][0];
     if (a.cslot)
         for (int c = 0; c < T.J; c++) {
             const int32_t cs = a.cslot[R * T.J + 1 + c - a.start_c];
             if (cs >= 0) wave_add64_t(acc, a.Ac + ((uint64_t)cs * nslc + slc) * (uint64_t)RP * RP, RP);
         }
-    // P_j = L~^{-1} acc by row tiles tq (k tiles kt <= tq), stored row-major
+    // P_j = L~^{-1} acc by row tiles tq (k tiles kt <= tq), stored row-major;
+    // pv[tj] collects this lane's part of (P_j^T v)[16 tj + r16]
     const char* Li = (const char*)(a.LpiT + R * RP * RP);  // [k*RP + i] = L~^{-1}[i][k]
     double* Pj = a.P + a.goff[l] + loc * (uint64_t)l * RP * RP + (uint64_t)j * RP * RP;
     char* Pc = (char*)Pj;
+    const double* vR = a.v + R * RP;
+    double pv[4] = {0.0, 0.0, 0.0, 0.0};
     uint32_t lo = (uint32_t)(k4 * RP + r16);
 #pragma unroll
     for (int tq = 0; tq < 4; tq++) {
@@ -906,29 +915,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 for (int tj = 0; tj < 4; tj++)
                     gq[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, acc[kt * 4 + tj][v], gq[tj], 0, 0, 0);
             }
+        uint32_t vo = (uint32_t)k4;
+        asm volatile("" : "+v"(vo));  // v loaded here, not hoisted above the products
 #pragma unroll
-        for (int v = 0; v < 4; v++)
+        for (int v = 0; v < 4; v++) {
+            const double vm = vR[16 * tq + vo + 4 * v];
 #pragma unroll
-            for (int tj = 0; tj < 4; tj++) *(double*)(Pc + 8u * (lo + (16 * tq + 4 * v) * RP + 16 * tj)) = gq[tj][v];
+            for (int tj = 0; tj < 4; tj++) {
+                *(double*)(Pc + 8u * (lo + (16 * tq + 4 * v) * RP + 16 * tj)) = gq[tj][v];
+                pv[tj] += gq[tj][v] * vm;
+            }
+        }
     }
     if (!a.write_msgs) return;
-    // w~_j = w_j - P_j^T v: lane q reads column q of P_j (other lanes' stores)
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    const int q = lane;
-    double wj = 0.0;
-    if (q < r) {
-        for (int o = 0; o < ns; o++) wj += X[(uint64_t)o * a.ldx + j * RP + q] * a.z[os + o];
-        if (a.cslot)
-            for (int c = 0; c < T.J; c++) {
-                const int32_t cs = a.cslot[R * T.J + 1 + c - a.start_c];
-                if (cs >= 0) wj += a.wc[(uint64_t)cs * lc * RP + j * RP + q];
-            }
+    // w~_j = w_j - P_j^T v: the four lane groups' parts summed across lanes
+#pragma unroll
+    for (int tj = 0; tj < 4; tj++) {
+        pv[tj] += __shfl_xor(pv[tj], 16);
+        pv[tj] += __shfl_xor(pv[tj], 32);
     }
-    const double* vR = a.v + R * RP;
-    double pv = 0.0;
-    for (int m = 0; m < RP; m++) pv += Pj[m * RP + q] * vR[m];
-    a.wo[msg * (uint64_t)l * RP + j * RP + q] = wj - pv;
+    if (k4 == 0)
+#pragma unroll
+        for (int tj = 0; tj < 4; tj++) {
+            const int q = 16 * tj + r16;
+            double wj = 0.0;
+            if (q < r) {
+                wj = wq[tj];
+                if (a.cslot)
+                    for (int c = 0; c < T.J; c++) {
+                        const int32_t cs = a.cslot[R * T.J + 1 + c - a.start_c];
+                        if (cs >= 0) wj += a.wc[(uint64_t)cs * lc * RP + j * RP + q];
+                    }
+            }
+            a.wo[msg * (uint64_t)l * RP + j * RP + q] = wj - pv[tj];
+        }
 }
 
 static inline uint32_t post_pj_grid(uint32_t nitems, int l) {
