@@ -149,8 +149,12 @@ int mra_use_graph(mra_handle* h, int enable);
  * A~_jk = X_j^T X_k - P_j^T P_k of 64-knot regions whose messages come from
  * stacked rows in a separate kernel, one wave per block; 0 computes them
  * inside the posterior kernel (same values up to
+ * rounding).
+ * MRA_OPT_WAVE_LOCAL16: 1 (default) runs the finest-level local step
+ * (Sigma_F, its Cholesky, u, H) of regions with at most 16 observations one
+ * wave per region; 0 one work group per region (same values up to
  * rounding). */
-enum { MRA_OPT_FACTORED_POSTERIOR = 1, MRA_OPT_GRAM_KERNEL = 2 };
+enum { MRA_OPT_FACTORED_POSTERIOR = 1, MRA_OPT_GRAM_KERNEL = 2, MRA_OPT_WAVE_LOCAL16 = 3 };
 int mra_set_option(mra_handle* h, int option, int value);
 int mra_get_option(mra_handle* h, int option, int* value);
 

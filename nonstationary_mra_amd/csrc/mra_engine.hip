@@ -43,6 +43,7 @@ struct mra_handle {
     bool lowrank = false;
     int lowrankMode = 1;  // requested: 1 = when possible, 0 = dense everywhere
     int gramSep = 1;      // message blocks by k_post_gram (RP = 64); 0: inside k_posterior
+    int waveLocal16 = 1;  // finest regions with <= 16 observations one wave each
     bool denseFull = false;  // full-size dense messages of level L-1 allocated
     std::vector<uint64_t> hRowS1, hRowE1, hRowS2, hRowE2;  // rows of levels L-1 and L-2
     uint64_t *dRowS1 = nullptr, *dRowE1 = nullptr, *dRowS2 = nullptr, *dRowE2 = nullptr;
@@ -778,6 +779,11 @@ extern "C" int mra_set_option(mra_handle* h, int option, int value) {
         graph_invalidate(h);
         return MRA_OK;
     }
+    if (option == MRA_OPT_WAVE_LOCAL16) {
+        h->waveLocal16 = value ? 1 : 0;
+        graph_invalidate(h);
+        return MRA_OK;
+    }
     set_err(h, "unknown option");
     return MRA_E_ARG;
 }
@@ -790,6 +796,10 @@ extern "C" int mra_get_option(mra_handle* h, int option, int* value) {
     }
     if (option == MRA_OPT_GRAM_KERNEL) {
         *value = (h->RP == 64 && h->gramSep) ? 1 : 0;
+        return MRA_OK;
+    }
+    if (option == MRA_OPT_WAVE_LOCAL16) {
+        *value = h->waveLocal16;
         return MRA_OK;
     }
     set_err(h, "unknown option");
@@ -850,7 +860,12 @@ static void launch_finest_local(mra_handle* h, const Tree& T, const CovCtx& cc, 
     if (h->nItemsFB[1])
         k_finest_local<RP, 32, TG><<<h->nItemsFB[1], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[1], Gf, Bf, h->dSig,
                                                                    h->dH, h->dU, h->dDterm, h->dBad);
-    if (h->nItemsFB[0])
+    if (h->nItemsFB[0] && h->waveLocal16)
+        // one wave per region, four regions per work group
+        k_finest_local_w16<TG><<<(h->nItemsFB[0] + 3) / 4, 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[0],
+                                                                          h->nItemsFB[0], Gf, Bf, h->dSig, h->dH,
+                                                                          h->dU, h->dDterm, h->dBad);
+    else if (h->nItemsFB[0])
         k_finest_local<RP, 16, TG><<<h->nItemsFB[0], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[0], Gf, Bf, h->dSig,
                                                                    h->dH, h->dU, h->dDterm, h->dBad);
 }

@@ -514,6 +514,117 @@ __global__ __launch_bounds__(256) void k_finest_local(Tree T, CovCtx cc, Pts O, 
     if (threadIdx.x == 0 && bad) atomicOr(bad_flag, 1);
 }
 
+// The same local step for finest regions with at most 16 observations, one
+// wave per region and four regions per work group, with no work-group
+// barrier: Sigma_F = V(S) - G_F^T G_F is one 16 x 16 MFMA tile (K = L RP in
+// four interleaved chains, summed in a fixed order; A and B fragments are the
+// same G values, so one load feeds both), factored with its inverse by
+// wave_chol16 in the wave's LDS tile; u = L_F^{-1} y, d_F and
+// H = L_F^{-1} B (16 x 16 output tiles across the L RP columns).
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
+template <typename TG>
+__global__ __launch_bounds__(256) void k_finest_local_w16(Tree T, CovCtx cc, Pts O, const double* yv,
+                                                          const uint32_t* items, uint32_t nitems, const TG* Gf,
+                                                          const TG* Bf, double* Sig, double* H, double* u,
+                                                          double* dterm, int* bad_flag) {
+    constexpr int LDP = 17;
+    __shared__ double Ps[4][16 * LDP];
+    __shared__ double Ds[4][16 * 16];
+    __shared__ int badw[4];
+    const int w = wave_id();
+    const uint32_t it = blockIdx.x * 4u + (uint32_t)w;
+    if (it >= nitems) return;
+    const int lane = threadIdx.x & 63, r16 = lane & 15, k4 = lane >> 4;
+    const uint32_t F = items[it];
+    const uint64_t ob = T.obs_off[F];
+    const int n = (int)(T.obs_off[F + 1] - ob);
+    const int LRP = T.LRP;
+    const uint64_t so = T.f_seg_os[F];
+    const int sn = (int)T.f_seg_n[F];
+    const TG* GF = Gf + (uint64_t)LRP * so + (ob - so);  // G_F(k, i) at GF[k * sn + i]
+    double* P = Ps[w];
+    double* Dinv = Ds[w];
+    if (lane == 0) badw[w] = 0;
+    // -G^T G in four chains
+    const bool iv = r16 < n;
+    const int ic = iv ? r16 : 0;
+    mra_d4 c4[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) c4[q] = mra_d4{0, 0, 0, 0};
+#pragma unroll 4
+    for (int k0 = 0; k0 < LRP; k0 += 16) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const double g = iv ? (double)GF[(uint64_t)(k0 + 4 * q + k4) * sn + ic] : 0.0;
+            c4[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(g, g, c4[q], 0, 0, 0);
+        }
+    }
+    // Sigma_F = V(S) - G^T G, identity padding (element v: row k4 + 4v, column r16)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int i = k4 + 4 * v, c = r16;
+        const double gg = ((c4[0][v] + c4[1][v]) + c4[2][v]) + c4[3][v];
+        double val;
+        if (i < n && c < n) {
+            double cv;
+            if (i == c)
+                cv = O.v[ob + i];
+            else if constexpr (sizeof(TG) == 4)
+                cv = (double)pcovf(cc, O, ob + i, O, ob + c);
+            else
+                cv = pcov(cc, O, ob + i, O, ob + c);
+            val = cv - gg;
+        } else {
+            val = (i == c) ? 1.0 : 0.0;
+        }
+        P[i * LDP + c] = val;
+    }
+    wave_chol16(P, LDP, Dinv, 16, &badw[w]);
+    // log det part, L_F kept (row-major n x n, zeros above)
+    double* LF = Sig + T.sig_off[F];
+    double part = lane < n ? 2.0 * log(P[lane * LDP + lane]) : 0.0;
+    for (int idx = lane; idx < n * n; idx += 64) {
+        const int i = idx / n, c = idx % n;
+        LF[idx] = c <= i ? P[i * LDP + c] : 0.0;
+    }
+    // u = L_F^{-1} y
+    if (lane < n) {
+        double acc = 0.0;
+        for (int k = 0; k <= lane; k++) acc += Dinv[lane * 16 + k] * yv[ob + k];
+        u[ob + lane] = acc;
+        part += acc * acc;
+    }
+    const double dF = wave_sum(part);
+    if (lane == 0) dterm[T.nC + F] = dF;
+    // H = L_F^{-1} B, observation-major rows of L RP
+    const TG* BO = Bf + ob * (uint64_t)LRP;
+    double* HF = H + ob * (uint64_t)LRP;
+    double al[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) al[q] = Dinv[r16 * 16 + 4 * q + k4];  // A(i = r16, k = 4q + k4)
+#pragma unroll 4
+    for (int m0 = 0; m0 < LRP; m0 += 16) {
+        mra_d4 hacc = {0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int k = 4 * q + k4;
+            const double b = k < n ? (double)BO[(uint64_t)k * LRP + m0 + r16] : 0.0;
+            hacc = __builtin_amdgcn_mfma_f64_16x16x4f64(al[q], b, hacc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = k4 + 4 * v;
+            if (i < n) HF[(uint64_t)i * LRP + m0 + r16] = hacc[v];
+        }
+    }
+    if (lane == 0 && badw[w]) atomicOr(bad_flag, 1);
+}
+
 // -------------------------------------------------------------- posterior
 // One work group per working region R of level l:
 //   A_{jk} = sum over children (from the stacked finest rows H when l = L-1,

@@ -223,6 +223,43 @@ def test_message_block_kernel(gram, factored):
     check(tree, cov, prm, factored=factored, gram=gram)
 
 
+@pytest.mark.parametrize("wave", [1, 0])
+@pytest.mark.parametrize("case", ["uniform64", "small9"])
+def test_small_region_local_kernel(case, wave):
+    """Finest regions with at most 16 observations one wave each (wave=1) or
+    one work group each (wave=0): D and predictions match the oracle in FP64,
+    the mixed path stays within its tolerance; regions of 1 to 16
+    observations, plus larger ones in the other buckets."""
+    if case == "uniform64":
+        rng = np.random.default_rng(97)
+        x, y = synth.clustered_points(6000, rng, n_clusters=2, sigma=0.05)
+        px, py = synth.uniform_points(120, rng)
+        tree = capi.partition_program(x, y, rng.standard_normal(6000), px, py, M=5, J=4, r=64,
+                                      bounds=synth.UNIT_BOUNDS)
+        cov, prm = synth.linear_trend_cov(), dict(synth.UNIT_PARAMS)
+    else:
+        tree, cov, prm = cases.synthetic_case(n=1500, npred=80, M=4, J=4, r=9, seed=12, clustered=True)
+    counts = np.diff(tree["obs_offsets"].astype(np.int64))
+    assert ((counts >= 1) & (counts <= 16)).sum() > 20 and counts.max() > 16
+    h = capi.Handle(tree)
+    try:
+        h.set_covariance(cov)
+        h.set_params(**prm)
+        h.set_option(capi.OPT_WAVE_LOCAL16, wave)
+        assert h.get_option(capi.OPT_WAVE_LOCAL16) == wave
+        D, mean, var = h.predict()
+        h.set_precision(capi.PRECISION_MIXED)
+        Dm = h.likelihood()
+    finally:
+        h.close()
+    Do, mo, vo = oracle.evaluate(tree, cov, prm)
+    print(f"{case} wave={wave} D gpu {D!r} oracle {Do!r} rel {rel(D, Do):.3e}", flush=True)
+    assert rel(D, Do) < TOL_D
+    np.testing.assert_allclose(mean, mo, rtol=TOL_PRED, atol=1e-10)
+    np.testing.assert_allclose(var, vo, rtol=TOL_PRED, atol=1e-10)
+    assert abs(Dm - D) / (2 * len(tree["obs_lon"])) < 1e-6
+
+
 TOL_MIXED = 1e-6   # |delta loglik| / n for the mixed-precision path
 
 
