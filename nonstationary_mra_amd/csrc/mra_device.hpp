@@ -185,6 +185,26 @@ __device__ __forceinline__ void cov_tile(T* C, int LD, const CovCtx& cc, const P
     }
 }
 
+// The same tile (no diagonal variant) from row points already staged in LDS
+// (st[i], i < nr) and this thread's column point qc (column threadIdx.x % NC,
+// valid when < nc); the caller synchronises before reuse of C.
+template <int RP, int NC, typename T>
+__device__ __forceinline__ void cov_tile_pre(T* C, int LD, const CovCtx& cc, const PtV* st, int nr, const PtV& qc,
+                                             int nc) {
+    static_assert(MRA_WG % NC == 0, "tile width must divide the work-group size");
+    const int c = threadIdx.x % NC;
+    for (int i = threadIdx.x / NC; i < RP; i += MRA_WG / NC) {
+        T v = T(0);
+        if (i < nr && c < nc) {
+            if constexpr (sizeof(T) == 4)
+                v = pcov_ptf(cc, st[i], qc);
+            else
+                v = pcov_pt(cc, st[i], qc);
+        }
+        C[i * LD + c] = v;
+    }
+}
+
 __device__ __forceinline__ float pcovf(const CovCtx& c, const Pts& P, uint64_t i, const Pts& Q, uint64_t j) {
     float x1 = (float)P.x[i], y1 = (float)P.y[i], z1 = (float)P.z[i];
     float x2 = (float)Q.x[j], y2 = (float)Q.y[j], z2 = (float)Q.z[j];

@@ -351,7 +351,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const Tree& T0 = T_;
     constexpr int LD = NT + 1;
     __shared__ T Cm[RP * LD];
-    __shared__ double stage[5 * RP];
+    __shared__ PtV kst[2][RP];  // ancestor knots, double-buffered
     const uint32_t sg = items[2 * blockIdx.x], c0 = items[2 * blockIdx.x + 1];
     const uint64_t ob = T0.s_os[sg];
     const int nF = (int)T0.s_n[sg];
@@ -359,9 +359,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int r = T0.r, L = T0.L;
     T* GF = Gf + (uint64_t)T0.LRP * ob;
     T* BO = Bf + (uint64_t)T0.LRP * ob;
+    // this thread's observation column (the same for every ancestor step)
+    const int cc_ = threadIdx.x % NT;
+    const PtV qc = cc_ < n ? load_pt(cc, O, ob + c0 + cc_) : PtV{0, 0, 0, 0, 1};
+    // knots of the level-0 ancestor; each later step's knots are loaded into
+    // registers while the current step computes, and staged at its end
+    if (threadIdx.x < r) kst[0][threadIdx.x] = load_pt(cc, K, ancestor_up(T0.s_parent[sg], L - 1, 0, T0.J) * RP + threadIdx.x);
+    __syncthreads();
     for (int j = 0; j < L; j++) {
         const uint64_t a = ancestor_up(T0.s_parent[sg], L - 1, j, T0.J);
-        cov_tile<RP, NT, T>(Cm, LD, cc, K, a * RP, r, O, ob + c0, n, false, stage);
+        // raw fields only (q is formed when staged, so nothing waits on these loads here)
+        double nx = 0.0, ny = 0.0, nz = 0.0, ns_ = 0.0, nb = 1.0;
+        if (j + 1 < L && threadIdx.x < r) {
+            const uint64_t q = ancestor_up(T0.s_parent[sg], L - 1, j + 1, T0.J) * RP + threadIdx.x;
+            nx = K.x[q];
+            ny = K.y[q];
+            nz = K.z[q];
+            ns_ = K.s[q];
+            nb = K.b[q];
+        }
+        cov_tile_pre<RP, NT, T>(Cm, LD, cc, kst[j & 1], r, qc, n);
         __syncthreads();
         if (j > 0) {
             const uint64_t go = goff[j] + (a - lvl_start(j, T0.J)) * (uint64_t)j * RP * RP;
@@ -385,6 +402,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         else
             wg_gemm_tn<false, false, double, double, GEMM_ANY>(GF + (uint64_t)(j * RP) * nF + c0, nF, RP, NT, RP, 1.0,
                                                                LiT + a * RP * RP, RP, RP, (const double*)Cm, LD, n);
+        if (j + 1 < L && threadIdx.x < r)
+            kst[(j + 1) & 1][threadIdx.x] = PtV{nx, ny, nz, cc.stationary ? 0.0 : ns_ * (nb * sqrt(nb)), nb};
         __syncthreads();
     }
 }
