@@ -188,14 +188,18 @@ __device__ __forceinline__ void cov_tile(T* C, int LD, const CovCtx& cc, const P
 // The same tile (no diagonal variant) from row points already staged in LDS
 // (st[i], i < nr) and this thread's column point qc (column threadIdx.x % NC,
 // valid when < nc); the caller synchronises before reuse of C.
+// vdiag (optional): rows and columns are the same point set, the diagonal
+// takes vdiag[i] (identity on padding rows).
 template <int RP, int NC, typename T>
 __device__ __forceinline__ void cov_tile_pre(T* C, int LD, const CovCtx& cc, const PtV* st, int nr, const PtV& qc,
-                                             int nc) {
+                                             int nc, const double* vdiag = nullptr) {
     static_assert(MRA_WG % NC == 0, "tile width must divide the work-group size");
     const int c = threadIdx.x % NC;
     for (int i = threadIdx.x / NC; i < RP; i += MRA_WG / NC) {
         T v = T(0);
-        if (i < nr && c < nc) {
+        if (vdiag && i == c) {
+            v = i < nr ? (T)vdiag[i] : T(1);
+        } else if (i < nr && c < nc) {
             if constexpr (sizeof(T) == 4)
                 v = pcov_ptf(cc, st[i], qc);
             else

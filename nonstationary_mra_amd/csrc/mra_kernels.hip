@@ -294,16 +294,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                                                       int* bad_flag, float* Gc32) {
     constexpr int LD = RP + 1;
     __shared__ double Cm[RP * LD];
-    __shared__ double stage[5 * RP];
+    __shared__ PtV kst[2][RP];  // knots of the current and the next row set
     __shared__ double dbuf[RP >= 48 ? 1 : RP * 16];
     __shared__ int bad;
     const uint64_t R = start_l + blockIdx.x;
     const int r = T.r;
     double* GR = Gc + goff[l] + (uint64_t)blockIdx.x * l * RP * RP;
     if (threadIdx.x == 0) bad = 0;
+    // this thread's column: a knot of R (the same in every step)
+    const int cq = threadIdx.x % RP;
+    const PtV qc = cq < r ? load_pt(cc, K, R * RP + cq) : PtV{0, 0, 0, 0, 1};
+    // row sets: the ancestors at levels 0..l-1, then R itself (for K_R); the
+    // next set's raw fields are loaded while the current step computes
+    auto rows_of = [&](int j) { return j < l ? ancestor_up(R, l, j, T.J) : R; };
+    if (threadIdx.x < r) kst[0][threadIdx.x] = load_pt(cc, K, rows_of(0) * RP + threadIdx.x);
+    __syncthreads();
     for (int j = 0; j < l; j++) {
         const uint64_t a = ancestor_up(R, l, j, T.J);
-        cov_tile<RP, RP>(Cm, LD, cc, K, a * RP, r, K, R * RP, r, false, stage);
+        double nx = 0.0, ny = 0.0, nz = 0.0, ns_ = 0.0, nb = 1.0;
+        if (threadIdx.x < r) {
+            const uint64_t q = rows_of(j + 1) * RP + threadIdx.x;
+            nx = K.x[q];
+            ny = K.y[q];
+            nz = K.z[q];
+            ns_ = K.s[q];
+            nb = K.b[q];
+        }
+        cov_tile_pre<RP, RP, double>(Cm, LD, cc, kst[j & 1], r, qc, r);
         __syncthreads();
         if (j > 0) {
             const double* Ga = Gc + goff[j] + (a - lvl_start(j, T.J)) * (uint64_t)j * RP * RP;
@@ -316,9 +333,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         wg_gemm_tn<false, false, double, double, GEMM_FULL, COPY32>(GR + (uint64_t)j * RP * RP, RP, RP, RP, RP, 1.0,
                                                                     LiT + a * RP * RP, RP, RP, Cm, LD, RP,
                                                                     COPY32 ? Gc32 + o : nullptr);
+        if (threadIdx.x < r)
+            kst[(j + 1) & 1][threadIdx.x] = PtV{nx, ny, nz, cc.stationary ? 0.0 : ns_ * (nb * sqrt(nb)), nb};
         __syncthreads();
     }
-    cov_tile<RP, RP>(Cm, LD, cc, K, R * RP, r, K, R * RP, r, true, stage);
+    // K_R = V(Q_R) - Gcat_R^T Gcat_R: the stage now holds R's own knots
+    cov_tile_pre<RP, RP, double>(Cm, LD, cc, kst[l & 1], r, qc, r, K.v + R * RP);
     __syncthreads();
     if (l > 0) {
         wg_gemm_tn<false, true, double, double, GEMM_FULL>(Cm, LD, RP, RP, l * RP, -1.0, GR, RP, RP, GR, RP, RP);
