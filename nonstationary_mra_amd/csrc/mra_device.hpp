@@ -857,9 +857,11 @@ __device__ __forceinline__ void wg_chol_blk(double* A, int lda, int nb, double* 
     }
 }
 
-// A <- L^{-1} in place after wg_chol_blk (same nb, dbuf); the strictly
-// upper part of the leading 16*nb block is set to zero.
+// A <- L^{-1} in place after wg_chol_blk (same nb <= NBMAX, dbuf); the
+// strictly upper part of the leading 16*nb block is set to zero.
+template <int NBMAX = 8>
 __device__ __forceinline__ void wg_trinv_blk(double* A, int lda, int nb, double* dbuf) {
+    constexpr int NBW = (NBMAX + 3) / 4;  // block columns per wave
     const BlkLayout BL{nb};
     const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, k4 = lane >> 4;
     const int ldd = BL.ldd(lda);
@@ -867,10 +869,12 @@ __device__ __forceinline__ void wg_trinv_blk(double* A, int lda, int nb, double*
     __syncthreads();
     for (int i = 1; i < nb; i++) {
         // T_ij = sum_{k=j}^{i-1} L_ik X_kj (X_jj = D_j; X_kj, k > j, final),
-        // at most two blocks j per wave
-        mra_d4 acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+        // at most NBW blocks j per wave
+        mra_d4 acc[NBW];
 #pragma unroll
-        for (int t = 0; t < 2; t++) {
+        for (int t = 0; t < NBW; t++) acc[t] = mra_d4{0, 0, 0, 0};
+#pragma unroll
+        for (int t = 0; t < NBW; t++) {
             const int j = wave + 4 * t;
             if (j < i)
                 for (int k = j; k < i; k++) {
@@ -881,7 +885,7 @@ __device__ __forceinline__ void wg_trinv_blk(double* A, int lda, int nb, double*
         }
         __syncthreads();  // every L_ij of block row i has been read
 #pragma unroll
-        for (int t = 0; t < 2; t++) {
+        for (int t = 0; t < NBW; t++) {
             const int j = wave + 4 * t;
             if (j < i) {
                 double* Cb = A + i * 16 * lda + j * 16;
@@ -899,9 +903,9 @@ __device__ __forceinline__ void wg_trinv_blk(double* A, int lda, int nb, double*
         __syncthreads();
     }
     // diagonal blocks <- D_b (every D_b is read before any upper block is zeroed)
-    double dv[8];
+    double dv[NBMAX];
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
+    for (int q = 0; q < NBMAX; q++) {
         const int idx = threadIdx.x + q * MRA_WG;
         dv[q] = 0.0;
         if (idx < nb * 256) {
@@ -911,7 +915,7 @@ __device__ __forceinline__ void wg_trinv_blk(double* A, int lda, int nb, double*
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
+    for (int q = 0; q < NBMAX; q++) {
         const int idx = threadIdx.x + q * MRA_WG;
         if (idx < nb * 256) {
             const int b = idx >> 8, e = idx & 255, ii = e >> 4, jj = e & 15;

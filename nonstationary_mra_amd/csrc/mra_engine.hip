@@ -56,6 +56,7 @@ struct mra_handle {
     uint32_t *dItemsLRAll = nullptr, *dItemsBig = nullptr;
     uint32_t nItemsLRAll = 0, nItemsBig = 0;
     int32_t* dSlot1 = nullptr;  // dense-message slot per finest-but-one region (-1: factored)
+    uint64_t *dRowS2f = nullptr, *dRowE2f = nullptr;  // level L-2 rows of factored children only
     std::vector<double> hObsVal;
     // variogram features: sampled observation indices per quadrant
     uint32_t* dVgIdx = nullptr;
@@ -86,7 +87,9 @@ struct mra_handle {
     uint32_t* dItemsX32 = nullptr;  // segments of at most 32 observations (32-column tiles)
     uint32_t nItemsX32 = 0;
     // finest regions by length bucket: n <= 16, 32, 64, 128, larger
-    static constexpr int NBK = 5;
+    static constexpr int NBK = 6;
+    uint64_t* dWsOffBig = nullptr;  // workspace offset of each region of the 128 < n <= 256 bucket
+    double* dWsBig = nullptr;       // padded 16 ceil(n/16) squared workspace per such region
     uint32_t* dItemsFB[NBK] = {};
     uint32_t nItemsFB[NBK] = {};
     // work-list shape (mra_autotune / mra_set_tuning): segments of at most
@@ -379,11 +382,24 @@ static int build_worklists(mra_handle* h) {
         std::vector<uint32_t> b[mra_handle::NBK];
         for (uint32_t f : iF) {
             const uint64_t n = h->hObsOff[f + 1] - h->hObsOff[f];
-            b[(n <= 16 && h->tuneFb16) ? 0 : n <= 32 ? 1 : n <= 64 ? 2 : n <= 128 ? 3 : 4].push_back(f);
+            b[(n <= 16 && h->tuneFb16) ? 0 : n <= 32 ? 1 : n <= 64 ? 2 : n <= 128 ? 3 : n <= 256 ? 4 : 5].push_back(f);
         }
         for (int k = 0; k < mra_handle::NBK; k++) {
             h->nItemsFB[k] = (uint32_t)b[k].size();
             RC_TRY(upload_wl(h, &h->dItemsFB[k], b[k].data(), b[k].size()));
+        }
+        // padded workspace of the 128 < n <= 256 bucket
+        std::vector<uint64_t> wo(b[4].size() + 1, 0);
+        for (size_t k = 0; k < b[4].size(); k++) {
+            const uint64_t n = h->hObsOff[b[4][k] + 1] - h->hObsOff[b[4][k]];
+            const uint64_t n16 = (n + 15) / 16 * 16;
+            wo[k + 1] = wo[k] + n16 * n16;
+        }
+        h->dWsOffBig = nullptr;
+        h->dWsBig = nullptr;
+        if (!b[4].empty()) {
+            RC_TRY(upload_wl(h, &h->dWsOffBig, wo.data(), b[4].size()));
+            RC_TRY(upload_wl(h, &h->dWsBig, (const double*)nullptr, wo.back()));
         }
     }
     RC_TRY(upload_wl(h, &h->dItemsX, ix.data(), ix.size()));
@@ -422,6 +438,27 @@ static int build_worklists(mra_handle* h) {
         RC_TRY(upload_wl(h, &h->dItemsLRAll, all.data(), all.size()));
         RC_TRY(upload_wl(h, &h->dItemsBig, big.data(), big.size()));
         RC_TRY(upload_wl(h, &h->dSlot1, slot.data(), slot.size()));
+        // rows read by level L-2: from the first to the last row of its
+        // factored children (dense children's rows are zero; a region with
+        // only dense children reads none)
+        const int l2 = h->L - 2;
+        const uint64_t n2 = h->lvlStart[l2 + 1] - h->lvlStart[l2];
+        std::vector<uint64_t> rs(n2, 0), re(n2, 0);
+        for (uint64_t k = 0; k < n2; k++) {
+            uint64_t lo = UINT64_MAX, hi = 0;
+            for (int c = 0; c < h->J; c++) {
+                const uint64_t ch = (h->lvlStart[l2] + k) * h->J + 1 + c - h->lvlStart[l1];
+                if (slot[ch] >= 0 || h->hRowE1[ch] <= h->hRowS1[ch]) continue;
+                lo = std::min(lo, h->hRowS1[ch]);
+                hi = std::max(hi, h->hRowE1[ch]);
+            }
+            if (hi > 0) {
+                rs[k] = lo;
+                re[k] = hi;
+            }
+        }
+        RC_TRY(upload_wl(h, &h->dRowS2f, rs.data(), rs.size()));
+        RC_TRY(upload_wl(h, &h->dRowE2f, re.data(), re.size()));
     }
     return MRA_OK;
 }
@@ -848,9 +885,13 @@ static void launch_finest_local(mra_handle* h, const Tree& T, const CovCtx& cc, 
     TG* Gf = (TG*)h->dGf;
     TG* Bf = (TG*)h->dBf;
     // larger buckets first (longest work groups start early)
-    if (h->nItemsFB[4])
-        k_finest_local<RP, 0, TG><<<h->nItemsFB[4], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[4], Gf, Bf, h->dSig,
+    if (h->nItemsFB[5])
+        k_finest_local<RP, 0, TG><<<h->nItemsFB[5], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[5], Gf, Bf, h->dSig,
                                                                   h->dH, h->dU, h->dDterm, h->dBad);
+    if (h->nItemsFB[4])
+        k_finest_local_big<TG><<<h->nItemsFB[4], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[4], h->dWsOffBig,
+                                                               h->dWsBig, Gf, Bf, h->dSig, h->dH, h->dU, h->dDterm,
+                                                               h->dBad);
     if (h->nItemsFB[3])
         k_finest_local<RP, 128, TG><<<h->nItemsFB[3], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[3], Gf, Bf, h->dSig,
                                                                     h->dH, h->dU, h->dDterm, h->dBad);
@@ -1068,8 +1109,8 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
             a.X = h->dXo;
             a.z = h->dZo;
             a.ldx = (h->L - 1) * h->RP;
-            a.row_s = h->dRowS2;
-            a.row_e = h->dRowE2;
+            a.row_s = h->dRowS2f;
+            a.row_e = h->dRowE2f;
             a.cslot = h->dSlot1;
             a.wc = h->dW[l + 1];
             a.Ac = h->dA[l + 1];

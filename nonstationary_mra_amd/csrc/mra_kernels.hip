@@ -553,6 +553,87 @@ __global__ __launch_bounds__(256) void k_finest_local(Tree T, CovCtx cc, Pts O, 
     if (threadIdx.x == 0 && bad) atomicOr(bad_flag, 1);
 }
 
+// The same local step for finest regions with 128 < n_F <= 256 observations,
+// on a padded N16 x N16 workspace in global memory (ws + ws_off[item], row
+// stride N16 = 16 ceil(n_F / 16), identity padding): Sigma_F = V(S) - G^T G
+// on the lower 64 x 64 tiles (MFMA), blocked Cholesky and inverse on 16 x 16
+// blocks (the inverse diagonal blocks kept in the unused upper triangle),
+// u = L_F^{-1} y, d_F and H = L_F^{-1} B (MFMA).  L_F is copied to its
+// n_F x n_F place for prediction and sampling.
+template <typename TG>
+__global__ __launch_bounds__(256) void k_finest_local_big(Tree T, CovCtx cc, Pts O, const double* yv,
+                                                          const uint32_t* items, const uint64_t* ws_off, double* ws,
+                                                          const TG* Gf, const TG* Bf, double* Sig, double* H,
+                                                          double* u, double* dterm, int* bad_flag) {
+    __shared__ double red[MRA_WG];
+    __shared__ double dbuf[1];  // unused: with 3 or more blocks the inverse diagonal blocks live in A
+    __shared__ int bad;
+    const uint32_t F = items[blockIdx.x];
+    const uint64_t ob = T.obs_off[F];
+    const int n = (int)(T.obs_off[F + 1] - ob);
+    const int LRP = T.LRP;
+    const int nbk = (n + 15) >> 4, ld = nbk * 16;
+    double* Sg = ws + ws_off[blockIdx.x];
+    double* LF = Sig + T.sig_off[F];
+    const uint64_t so = T.f_seg_os[F];
+    const int sn = (int)T.f_seg_n[F];
+    const TG* GF = Gf + (uint64_t)LRP * so + (ob - so);
+    if (threadIdx.x == 0) bad = 0;
+    // V(S) on the lower triangle, identity padding (the upper triangle is not read)
+    for (int idx = threadIdx.x; idx < ld * ld; idx += MRA_WG) {
+        const int i = idx / ld, c = idx % ld;
+        if (c > i) continue;
+        double v;
+        if (i < n && c < n) {
+            if (i == c)
+                v = O.v[ob + i];
+            else if constexpr (sizeof(TG) == 4)
+                v = (double)pcovf(cc, O, ob + i, O, ob + c);
+            else
+                v = pcov(cc, O, ob + i, O, ob + c);
+        } else {
+            v = (i == c) ? 1.0 : 0.0;
+        }
+        Sg[(uint64_t)i * ld + c] = v;
+    }
+    __syncthreads();
+    // - G^T G on the lower 64 x 64 tiles (only the valid n x n part changes)
+    for (int ti = 0; ti < n; ti += 64)
+        for (int tj = 0; tj <= ti; tj += 64) {
+            const int mv = min(64, n - ti), nv = min(64, n - tj);
+            wg_gemm_tn(Sg + (uint64_t)ti * ld + tj, ld, (mv + 15) & ~15, (nv + 15) & ~15, LRP, -1.0, GF + ti, sn, mv,
+                       GF + tj, sn, nv);
+        }
+    __syncthreads();
+    wg_chol_blk(Sg, ld, nbk, dbuf, &bad);
+    double part = 0.0;
+    for (int i = threadIdx.x; i < n; i += MRA_WG) part += 2.0 * log(Sg[(uint64_t)i * ld + i]);
+    for (int idx = threadIdx.x; idx < n * n; idx += MRA_WG) {
+        const int i = idx / n, c = idx % n;
+        LF[idx] = c <= i ? Sg[(uint64_t)i * ld + c] : 0.0;
+    }
+    __syncthreads();
+    wg_trinv_blk<16>(Sg, ld, nbk, dbuf);
+    // u = L_F^{-1} y
+    for (int i = threadIdx.x; i < n; i += MRA_WG) {
+        double acc = 0.0;
+        for (int k = 0; k <= i; k++) acc += Sg[(uint64_t)i * ld + k] * yv[ob + k];
+        u[ob + i] = acc;
+        part += acc * acc;
+    }
+    const double dF = wg_sum(part, red);
+    // H = L_F^{-1} B (observation-major rows), 64 columns at a time
+    const TG* BO = Bf + ob * (uint64_t)LRP;
+    for (int c0 = 0; c0 < LRP; c0 += 64) {
+        const int cv = min(64, LRP - c0);
+        wg_gemm_tn<true, false>(H + ob * (uint64_t)LRP + c0, LRP, ld, 64, n, 1.0, Sg, ld, n, BO + c0, LRP, cv);
+    }
+    if (threadIdx.x == 0) {
+        dterm[T.nC + F] = dF;
+        if (bad) atomicOr(bad_flag, 1);
+    }
+}
+
 // The same local step for finest regions with at most 16 observations, one
 // wave per region and four regions per work group, with no work-group
 // barrier: Sigma_F = V(S) - G_F^T G_F is one 16 x 16 MFMA tile (K = L RP in
@@ -718,13 +799,17 @@ __device__ __forceinline__ void post_add_A(const Tree& T, const PostArgs& a, uin
     if (FROM_OBS) {
         const uint64_t loc = R - a.start_l;
         const uint64_t os = a.row_s[loc], oe = a.row_e[loc];
-        if (oe <= os) return;
-        const int ns = (int)(oe - os);
-        const double* X = a.X + os * a.ldx;
-        if (!transpose)
-            wg_gemm_tn<false, true, double, double, GEMM_FULL>(C, ld, RP, RP, ns, 1.0, X + j * RP, a.ldx, RP, X + k * RP, a.ldx, RP);
-        else
-            wg_gemm_tn<false, true, double, double, GEMM_FULL>(C, ld, RP, RP, ns, 1.0, X + k * RP, a.ldx, RP, X + j * RP, a.ldx, RP);
+        if (oe > os) {
+            // (a region may read no rows: all its children sent dense messages)
+            const int ns = (int)(oe - os);
+            const double* X = a.X + os * a.ldx;
+            if (!transpose)
+                wg_gemm_tn<false, true, double, double, GEMM_FULL>(C, ld, RP, RP, ns, 1.0, X + j * RP, a.ldx, RP,
+                                                                  X + k * RP, a.ldx, RP);
+            else
+                wg_gemm_tn<false, true, double, double, GEMM_FULL>(C, ld, RP, RP, ns, 1.0, X + k * RP, a.ldx, RP,
+                                                                  X + j * RP, a.ldx, RP);
+        }
         if (a.cslot) {
             // children that sent a dense message (their rows are zero)
             __syncthreads();

@@ -260,6 +260,42 @@ def test_small_region_local_kernel(case, wave):
     assert abs(Dm - D) / (2 * len(tree["obs_lon"])) < 1e-6
 
 
+@pytest.mark.parametrize("r", [16, 32])
+def test_large_region_local_kernel(r):
+    """Finest regions of 129 to 256 observations (padded workspace, blocked
+    Cholesky and inverse of up to 16 blocks) and above 256 (generic path):
+    D and predictions match the oracle in FP64, mixed within tolerance."""
+    rng = np.random.default_rng(40 + r)
+    x, y = synth.clustered_points(3600, rng, n_clusters=2, sigma=0.08)
+    px, py = synth.uniform_points(60, rng)
+    tree = capi.partition_program(x, y, rng.standard_normal(3600), px, py, M=3, J=4, r=r,
+                                  bounds=synth.UNIT_BOUNDS)
+    cov, prm = synth.linear_trend_cov(log_range=(np.log(0.2), 0.8, 0.0)), dict(synth.UNIT_PARAMS)
+    counts = np.diff(tree["obs_offsets"].astype(np.int64))
+    print("counts", sorted(counts.tolist()), flush=True)
+    assert ((counts > 128) & (counts <= 256)).sum() >= 4
+    h = capi.Handle(tree)
+    try:
+        h.set_covariance(cov)
+        h.set_params(**prm)
+        D, mean, var = h.predict()
+        h.set_precision(capi.PRECISION_MIXED)
+        Dm = h.likelihood()
+    finally:
+        h.close()
+    Do, mo, vo = oracle.evaluate(tree, cov, prm)
+    print(f"r={r} D gpu {D!r} oracle {Do!r} rel {rel(D, Do):.3e}", flush=True)
+    assert rel(D, Do) < TOL_D
+    np.testing.assert_allclose(mean, mo, rtol=TOL_PRED, atol=1e-10)
+    np.testing.assert_allclose(var, vo, rtol=TOL_PRED, atol=1e-10)
+    # mixed: regions of 700-900 strongly correlated observations (range 0.2)
+    # amplify the FP32 covariance rounding through Sigma_F; the 1e-6 target
+    # of the benchmark workload is checked by test_mixed_precision_loglik
+    dl = abs(Dm - D) / (2 * len(tree["obs_lon"]))
+    print(f"mixed |dl|/n {dl:.3e}", flush=True)
+    assert dl < 1e-5
+
+
 TOL_MIXED = 1e-6   # |delta loglik| / n for the mixed-precision path
 
 
