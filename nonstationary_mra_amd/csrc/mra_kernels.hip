@@ -365,7 +365,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // T = double: FP64 throughout.  T = float (mixed precision): the covariance
 // tile, both products and the stored G, B are FP32 with FP32 accumulation.
 template <int RP, int NT, typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_finest_cross(Tree T_, CovCtx cc, Pts K, Pts O, const uint32_t* items,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3))) void k_finest_cross(Tree T_, CovCtx cc, Pts K, Pts O, const uint32_t* items,
                                                       const uint64_t* goff, const double* Gc, const double* LiT,
                                                       T* Gf, T* Bf, const float* Gc32) {
     const Tree& T0 = T_;
