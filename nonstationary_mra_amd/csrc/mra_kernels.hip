@@ -1310,6 +1310,7 @@ __global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts 
     __shared__ double Cm[RP * LD];
     __shared__ double Wm[RP * LD];
     __shared__ double stage[5 * RP];
+    __shared__ double red[MRA_WG];
     const uint32_t F = a.items[3 * blockIdx.x], p0 = a.items[3 * blockIdx.x + 1], slot = a.items[3 * blockIdx.x + 2];
     const uint64_t pb = T.pred_off[F];
     const int pF = (int)(T.pred_off[F + 1] - pb);
@@ -1319,6 +1320,7 @@ __global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts 
     const int r = T.r, L = T.L, LRP = T.LRP;
     double* Gh = a.Gh + (uint64_t)slot * LRP * NT;
     double* BtF = a.Bt + (uint64_t)LRP * pb;
+    double sq = 0.0;  // this thread's part of sum_m G^(m, c)^2, c = threadIdx.x % NT
     for (int j = 0; j < L; j++) {
         const uint64_t an = ancestor_up(T.parent[F], L - 1, j, T.J);
         for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) Wm[(idx / NT) * LD + idx % NT] = 0.0;
@@ -1335,19 +1337,58 @@ __global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts 
         }
         wg_gemm_tn(Wm, LD, RP, NT, RP, 1.0, a.LiT + an * RP * RP, RP, RP, Cm, LD, NT);
         __syncthreads();
-        for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG)
-            Gh[(uint64_t)(j * RP + idx / NT) * NT + idx % NT] = Wm[(idx / NT) * LD + idx % NT];
+        for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
+            const double w = Wm[(idx / NT) * LD + idx % NT];
+            Gh[(uint64_t)(j * RP + idx / NT) * NT + idx % NT] = w;
+            sq += w * w;
+        }
         __syncthreads();
     }
     // var0 and the conditional cross block with the region's observations
+    red[threadIdx.x] = sq;
+    __syncthreads();
     double var0 = 0.0, mean0 = 0.0;
     const int c = threadIdx.x;
     if (c < np) {
         double s = 0.0;
-        for (int m = 0; m < LRP; m++) s += Gh[m * NT + c] * Gh[m * NT + c];
+        for (int q = c; q < MRA_WG; q += NT) s += red[q];
         var0 = Pp.v[pb + p0 + c] - s;
     }
-    if (nF > 0) {
+    if (nF > 0 && nF <= RP) {
+        // one row tile: G^_L stays in LDS (Cm) for the solve and the product
+        cov_tile<RP, NT>(Cm, LD, cc, O, ob, nF, Pp, pb + p0, np, false, stage);
+        __syncthreads();
+        const uint64_t so = T.f_seg_os[F];
+        const int sn = (int)T.f_seg_n[F];
+        const TG* GF = (const TG*)a.Gf + (uint64_t)LRP * so + (ob - so);
+        wg_gemm_tn(Cm, LD, (nF + 15) & ~15, NT, LRP, -1.0, GF, sn, nF, Gh, NT, NT);
+        __syncthreads();
+        // Cm <- L_F^{-1} Cm, one column per thread
+        const double* LF = a.Sig + T.sig_off[F];
+        if (c < NT) {
+            for (int i = 0; i < nF; i++) {
+                double s = Cm[i * LD + c];
+                for (int k = 0; k < i; k++) s -= LF[(uint64_t)i * nF + k] * Cm[k * LD + c];
+                s /= LF[(uint64_t)i * nF + i];
+                Cm[i * LD + c] = s;
+            }
+            if (c < np) {
+                double sv = 0.0, sm = 0.0;
+                for (int i = 0; i < nF; i++) {
+                    const double g = Cm[i * LD + c];
+                    sv += g * g;
+                    sm += g * a.u[ob + i];
+                }
+                var0 -= sv;
+                mean0 = sm;
+            }
+        }
+        __syncthreads();
+        // B~ = E - H^T G^_L for all L*RP rows at once, accumulated in place on
+        // the stored E (rows m = j*RP + i, the tile's np valid columns)
+        wg_gemm_tn(BtF + p0, pF, LRP, NT, nF, -1.0, a.H + ob * LRP, LRP, LRP, Cm, LD, np);
+        __syncthreads();
+    } else if (nF > 0) {
         double* GL = a.GL + a.gl_off[slot];
         const uint64_t so = T.f_seg_os[F];
         const int sn = (int)T.f_seg_n[F];
@@ -1383,21 +1424,10 @@ __global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts 
             }
         }
         __syncthreads();
-        // B~_j = E_j - H_j^T G^_L
-        for (int j = 0; j < L; j++) {
-            for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
-                const int i = idx / NT, cc2 = idx % NT;
-                Cm[i * LD + cc2] = (cc2 < np) ? BtF[(uint64_t)(j * RP + i) * pF + p0 + cc2] : 0.0;
-            }
-            __syncthreads();
-            wg_gemm_tn(Cm, LD, RP, NT, nF, -1.0, a.H + ob * LRP + j * RP, LRP, RP, GL, NT, NT);
-            __syncthreads();
-            for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
-                const int i = idx / NT, cc2 = idx % NT;
-                if (cc2 < np) BtF[(uint64_t)(j * RP + i) * pF + p0 + cc2] = Cm[i * LD + cc2];
-            }
-            __syncthreads();
-        }
+        // B~ = E - H^T G^_L for all L*RP rows at once, accumulated in place on
+        // the stored E (rows m = j*RP + i, the tile's np valid columns)
+        wg_gemm_tn(BtF + p0, pF, LRP, NT, nF, -1.0, a.H + ob * LRP, LRP, LRP, GL, NT, np);
+        __syncthreads();
     }
     if (c < np) {
         a.mean[pb + p0 + c] = mean0;
