@@ -1441,25 +1441,20 @@ __global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts 
 template <int RP, int NT>
 __global__ __launch_bounds__(256) void k_pred_post(Tree T, PredArgs a) {
     constexpr int LD = NT + 1;
-    __shared__ double Cm[RP * LD];
     __shared__ double Zm[RP * LD];
     const uint32_t F = a.items[3 * blockIdx.x], p0 = a.items[3 * blockIdx.x + 1];
     const uint64_t pb = T.pred_off[F];
     const int pF = (int)(T.pred_off[F + 1] - pb);
     const int np = min(NT, pF - (int)p0);
     const int L = T.L, LRP = T.LRP;
-    double* BtF = a.Bt + (uint64_t)LRP * pb;
+    double* BtF = a.Bt + (uint64_t)LRP * pb;  // B~(m, c) at BtF[m * pF + c]
     double macc = 0.0, vacc = 0.0;
     const int c = threadIdx.x;
     for (int jj = L - 1; jj >= 0; jj--) {
         const uint64_t an = ancestor_up(T.parent[F], L - 1, jj, T.J);
-        for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
-            const int i = idx / NT, cc2 = idx % NT;
-            Cm[i * LD + cc2] = (cc2 < np) ? BtF[(uint64_t)(jj * RP + i) * pF + p0 + cc2] : 0.0;
-            Zm[i * LD + cc2] = 0.0;
-        }
-        __syncthreads();
-        wg_gemm_tn(Zm, LD, RP, NT, RP, 1.0, a.LpiT + an * RP * RP, RP, RP, Cm, LD, NT);
+        // Z = L~^{-1} B~_jj, B~ read from global memory (the tile's np columns)
+        wg_gemm_tn<false, false>(Zm, LD, RP, NT, RP, 1.0, a.LpiT + an * RP * RP, RP, RP,
+                                 BtF + (uint64_t)jj * RP * pF + p0, pF, np);
         __syncthreads();
         if (c < np) {
             const double* vA = a.v + an * RP;
@@ -1472,23 +1467,14 @@ __global__ __launch_bounds__(256) void k_pred_post(Tree T, PredArgs a) {
             macc += sm;
             vacc += sv;
         }
+        // B~_jp -= P_{an,jp}^T Z in place (disjoint rows for each jp)
         if (jj > 0) {
             const double* PA = a.P + a.goff[jj] + (an - lvl_start(jj, T.J)) * (uint64_t)jj * RP * RP;
-            for (int jp = 0; jp < jj; jp++) {
-                for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
-                    const int i = idx / NT, cc2 = idx % NT;
-                    Cm[i * LD + cc2] = (cc2 < np) ? BtF[(uint64_t)(jp * RP + i) * pF + p0 + cc2] : 0.0;
-                }
-                __syncthreads();
-                wg_gemm_tn(Cm, LD, RP, NT, RP, -1.0, PA + (uint64_t)jp * RP * RP, RP, RP, Zm, LD, NT);
-                __syncthreads();
-                for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
-                    const int i = idx / NT, cc2 = idx % NT;
-                    if (cc2 < np) BtF[(uint64_t)(jp * RP + i) * pF + p0 + cc2] = Cm[i * LD + cc2];
-                }
-                __syncthreads();
-            }
+            for (int jp = 0; jp < jj; jp++)
+                wg_gemm_tn(BtF + (uint64_t)jp * RP * pF + p0, pF, RP, NT, RP, -1.0, PA + (uint64_t)jp * RP * RP, RP,
+                           RP, Zm, LD, np);
         }
+        __syncthreads();
     }
     if (c < np) {
         a.mean[pb + p0 + c] += macc;
