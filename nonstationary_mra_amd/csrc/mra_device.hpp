@@ -628,9 +628,9 @@ __device__ __forceinline__ void wg_gemm_tn_f32(TC* C, int ldc, int M, int N, int
 // K split over the 4 waves; the partial tiles are summed in wave order
 // through red (LDS: 3*256 doubles for a 16 x 16 output, else 3*4*256), so
 // the result is reproducible.
-template <typename TI = double>
+template <typename TIA = double, typename TIB = TIA>
 __device__ __forceinline__ void wg_gemm_tn_ksplit(double* C, int ldc, int M, int N, int Kv, double alpha,
-                                                  const TI* A, int64_t lda, int Mv, const TI* B, int64_t ldb,
+                                                  const TIA* A, int64_t lda, int Mv, const TIB* B, int64_t ldb,
                                                   int Nv, double* red) {
     const int lane = threadIdx.x & 63, wave = wave_id();
     const int r16 = lane & 15, k4 = lane >> 4;
@@ -641,7 +641,7 @@ __device__ __forceinline__ void wg_gemm_tn_ksplit(double* C, int ldc, int M, int
     for (int q = 0; q < 4; q++) acc[q] = mra_d4{0, 0, 0, 0};
     const int ia = r16, ib = 16 + r16, ja = r16, jb = 16 + r16;
     const bool ha = ia < Mv, hb = tM2 && ib < Mv, hja = ja < Nv, hjb = tN2 && jb < Nv;
-    const BlockOps<false, TI, TI> o(A, lda, B, ldb, ia, ib, ja, jb, ha, hb, hja, hjb, k4);
+    const BlockOps<false, TIA, TIB> o(A, lda, B, ldb, ia, ib, ja, jb, ha, hb, hja, hjb, k4);
     // this wave's contiguous K range, a multiple of 4
     const int kq = ((Kv + 15) / 16) * 4;
     const int kb = min(Kv, wave * kq), ke = min(Kv, kb + kq);

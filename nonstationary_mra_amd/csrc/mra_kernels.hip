@@ -1311,6 +1311,7 @@ __global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts 
     __shared__ double Wm[RP * LD];
     __shared__ double stage[5 * RP];
     __shared__ double red[MRA_WG];
+    __shared__ double kred[NT == 16 ? 3 * 4 * 256 : 1];  // K-split partial tiles
     const uint32_t F = a.items[3 * blockIdx.x], p0 = a.items[3 * blockIdx.x + 1], slot = a.items[3 * blockIdx.x + 2];
     const uint64_t pb = T.pred_off[F];
     const int pF = (int)(T.pred_off[F + 1] - pb);
@@ -1361,7 +1362,10 @@ __global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts 
         const uint64_t so = T.f_seg_os[F];
         const int sn = (int)T.f_seg_n[F];
         const TG* GF = (const TG*)a.Gf + (uint64_t)LRP * so + (ob - so);
-        wg_gemm_tn(Cm, LD, (nF + 15) & ~15, NT, LRP, -1.0, GF, sn, nF, Gh, NT, NT);
+        if (NT == 16 && nF <= 32)  // at most 2 x 1 output tiles: K over the four waves
+            wg_gemm_tn_ksplit<TG, double>(Cm, LD, (nF + 15) & ~15, NT, LRP, -1.0, GF, sn, nF, Gh, NT, NT, kred);
+        else
+            wg_gemm_tn(Cm, LD, (nF + 15) & ~15, NT, LRP, -1.0, GF, sn, nF, Gh, NT, NT);
         __syncthreads();
         // Cm <- L_F^{-1} Cm, one column per thread
         const double* LF = a.Sig + T.sig_off[F];
