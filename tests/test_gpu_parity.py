@@ -296,6 +296,22 @@ def test_large_region_local_kernel(r):
     assert dl < 1e-5
 
 
+@pytest.mark.parametrize("gram", [1, 0])
+def test_binary_tree_padded_knots(gram):
+    """J = 2 with r = 49 knots padded to 64-wide blocks: the one-wave
+    posterior kernels, the factored level, and finest regions in the small,
+    middle and large buckets, against the oracle."""
+    rng = np.random.default_rng(88)
+    x, y = synth.clustered_points(5000, rng, n_clusters=3, sigma=0.04)
+    px, py = synth.uniform_points(100, rng)
+    tree = capi.partition_program(x, y, rng.standard_normal(5000), px, py, M=6, J=2, r=49,
+                                  bounds=synth.UNIT_BOUNDS)
+    counts = np.diff(tree["obs_offsets"].astype(np.int64))
+    print("counts", sorted(counts.tolist()), flush=True)
+    cov, prm = synth.linear_trend_cov(log_range=(np.log(0.1), 0.8, 0.0)), dict(synth.UNIT_PARAMS)
+    check(tree, cov, prm, gram=gram)
+
+
 TOL_MIXED = 1e-6   # |delta loglik| / n for the mixed-precision path
 
 
