@@ -247,6 +247,12 @@ __device__ __forceinline__ float pcovf(const CovCtx& c, const Pts& P, uint64_t i
 // per-lane offsets; out-of-range rows are read at a clamped valid row and
 // replaced by zero with a select.
 enum { GEMM_ANY = 0, GEMM_FULL = 1, GEMM_PART = 2 };
+// TRI (bit set, caller's promise about the operands): GEMM_TRI_A: A(k,i) = 0
+// for k > i (A^T lower triangular: a wave block stops K at its last row);
+// GEMM_TRI_B: B(k,j) = 0 for k > j (stops K at its last column); GEMM_TRI_C:
+// only C on and below the diagonal is wanted (wave blocks strictly above the
+// diagonal are skipped and that part of C is left as it was).
+enum { GEMM_TRI_A = 1, GEMM_TRI_B = 2, GEMM_TRI_C = 4 };
 
 template <typename TC>
 struct Mfma;
@@ -363,7 +369,7 @@ __device__ __forceinline__ void block_kloop_any(typename Mfma<TC>::V (&acc)[4], 
 }
 
 template <typename TC, bool TA, bool ACC, typename TIA, typename TIB, int TILES, typename TO, bool TB = false,
-          bool DUAL = false>
+          bool DUAL = false, int TRI = 0>
 __device__ __forceinline__ void wg_gemm_tn_impl(TO* C, int ldc, int M, int N, int Kv, TC alpha, const TIA* A,
                                                 int64_t lda, int Mv, const TIB* B, int64_t ldb, int Nv,
                                                 float* C32 = nullptr) {
@@ -383,6 +389,10 @@ __device__ __forceinline__ void wg_gemm_tn_impl(TO* C, int ldc, int M, int N, in
         const int ia = ti0 * 16 + r16, ib = ia + 16;
         const int ja = tj0 * 16 + r16, jb = ja + 16;
         const bool tM2 = bh == 2 && ti0 + 1 < tM, tN2 = bw == 2 && tj0 + 1 < tN;
+        if ((TRI & GEMM_TRI_C) && tj0 >= ti0 + (tM2 ? 2 : 1)) continue;  // strictly above the diagonal
+        int ke = Kv;
+        if (TRI & GEMM_TRI_A) ke = min(ke, ti0 * 16 + (tM2 ? 32 : 16));
+        if (TRI & GEMM_TRI_B) ke = min(ke, tj0 * 16 + (tN2 ? 32 : 16));
         const bool ha = ia < Mv, hb = tM2 && ib < Mv;
         const bool hja = ja < Nv, hjb = tN2 && jb < Nv;
         const BlockOps<TA, TIA, TIB, TB> o(A, lda, B, ldb, ia, ib, ja, jb, ha, hb, hja, hjb, k4);
@@ -392,9 +402,9 @@ __device__ __forceinline__ void wg_gemm_tn_impl(TO* C, int ldc, int M, int N, in
         const bool full = TILES == GEMM_FULL ||
                           (TILES == GEMM_ANY && ti0 * 16 + (tM2 ? 32 : 16) <= Mv && tj0 * 16 + (tN2 ? 32 : 16) <= Nv);
         if (TILES != GEMM_PART && full)
-            block_kloop_any<TC, false, 4>(acc, o, tM2, tN2, 0, Kv, k4);
+            block_kloop_any<TC, false, 4>(acc, o, tM2, tN2, 0, ke, k4);
         else if (TILES != GEMM_FULL)
-            block_kloop_any<TC, true, 4>(acc, o, tM2, tN2, 0, Kv, k4);
+            block_kloop_any<TC, true, 4>(acc, o, tM2, tN2, 0, ke, k4);
         // only the valid Mv x Nv part of C is updated
 #pragma unroll
         for (int v = 0; v < 4; v++) {
@@ -609,19 +619,20 @@ __device__ __forceinline__ void wg_gemm2_store(double* C, int ldc, int M, int N,
 
 // FP64 MFMA, FP64 accumulation; operands read as TIA / TIB and widened
 template <bool TA = false, bool ACC = true, typename TIA = double, typename TIB = double, int TILES = GEMM_PART,
-          bool DUAL = false>
+          bool DUAL = false, int TRI = 0>
 __device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int Kv, double alpha, const TIA* A,
                                            int64_t lda, int Mv, const TIB* B, int64_t ldb, int Nv,
                                            float* C32 = nullptr) {
-    wg_gemm_tn_impl<double, TA, ACC, TIA, TIB, TILES, double, false, DUAL>(C, ldc, M, N, Kv, alpha, A, lda, Mv, B,
-                                                                          ldb, Nv, C32);
+    wg_gemm_tn_impl<double, TA, ACC, TIA, TIB, TILES, double, false, DUAL, TRI>(C, ldc, M, N, Kv, alpha, A, lda, Mv,
+                                                                               B, ldb, Nv, C32);
 }
 
 // FP32 MFMA with FP32 accumulation (operands rounded to float); C of type TC
-template <bool ACC = true, typename TIA = float, typename TIB = float, typename TC = float>
+template <bool ACC = true, typename TIA = float, typename TIB = float, typename TC = float, int TRI = 0>
 __device__ __forceinline__ void wg_gemm_tn_f32(TC* C, int ldc, int M, int N, int Kv, float alpha, const TIA* A,
                                                int64_t lda, int Mv, const TIB* B, int64_t ldb, int Nv) {
-    wg_gemm_tn_impl<float, false, ACC, TIA, TIB, GEMM_ANY>(C, ldc, M, N, Kv, alpha, A, lda, Mv, B, ldb, Nv);
+    wg_gemm_tn_impl<float, false, ACC, TIA, TIB, GEMM_ANY, TC, false, false, TRI>(C, ldc, M, N, Kv, alpha, A, lda, Mv,
+                                                                                 B, ldb, Nv);
 }
 
 // Same product for an output of at most 32 x 32 (one 2x2 tile block), with

@@ -330,9 +330,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         // G_{R,j} = L_a^{-1} B, stored straight to global memory
         // (mixed precision: also an FP32 copy for the finest-level products)
         const uint64_t o = goff[l] + (uint64_t)blockIdx.x * l * RP * RP + (uint64_t)j * RP * RP;
-        wg_gemm_tn<false, false, double, double, GEMM_FULL, COPY32>(GR + (uint64_t)j * RP * RP, RP, RP, RP, RP, 1.0,
-                                                                    LiT + a * RP * RP, RP, RP, Cm, LD, RP,
-                                                                    COPY32 ? Gc32 + o : nullptr);
+        wg_gemm_tn<false, false, double, double, GEMM_FULL, COPY32, GEMM_TRI_A>(
+            GR + (uint64_t)j * RP * RP, RP, RP, RP, RP, 1.0, LiT + a * RP * RP, RP, RP, Cm, LD, RP,
+            COPY32 ? Gc32 + o : nullptr);
         if (threadIdx.x < r)
             kst[(j + 1) & 1][threadIdx.x] = PtV{nx, ny, nz, cc.stationary ? 0.0 : ns_ * (nb * sqrt(nb)), nb};
         __syncthreads();
@@ -341,7 +341,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     cov_tile_pre<RP, RP, double>(Cm, LD, cc, kst[l & 1], r, qc, r, K.v + R * RP);
     __syncthreads();
     if (l > 0) {
-        wg_gemm_tn<false, true, double, double, GEMM_FULL>(Cm, LD, RP, RP, l * RP, -1.0, GR, RP, RP, GR, RP, RP);
+        // only the lower part is read by the Cholesky factorisation
+        wg_gemm_tn<false, true, double, double, GEMM_FULL, false, GEMM_TRI_C>(Cm, LD, RP, RP, l * RP, -1.0, GR, RP, RP,
+                                                                              GR, RP, RP);
         __syncthreads();
     }
     wg_chol_blk(Cm, LD, RP / 16, dbuf, &bad);
@@ -417,10 +419,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
         }
         // G_{F,j} = L_a^{-1} B, stored straight to global memory
         if constexpr (sizeof(T) == 4)
-            wg_gemm_tn_f32<false>(GF + (uint64_t)(j * RP) * nF + c0, nF, RP, NT, RP, 1.0f, LiT + a * RP * RP, RP, RP,
-                                  (const float*)Cm, LD, n);
+            wg_gemm_tn_f32<false, double, float, float, GEMM_TRI_A>(GF + (uint64_t)(j * RP) * nF + c0, nF, RP, NT,
+                                                                    RP, 1.0f, LiT + a * RP * RP, RP, RP,
+                                                                    (const float*)Cm, LD, n);
         else
-            wg_gemm_tn<false, false, double, double, GEMM_ANY>(GF + (uint64_t)(j * RP) * nF + c0, nF, RP, NT, RP, 1.0,
+            wg_gemm_tn<false, false, double, double, GEMM_ANY, false, GEMM_TRI_A>(GF + (uint64_t)(j * RP) * nF + c0, nF, RP, NT, RP, 1.0,
                                                                LiT + a * RP * RP, RP, RP, (const double*)Cm, LD, n);
         if (j + 1 < L && threadIdx.x < r)
             kst[(j + 1) & 1][threadIdx.x] = PtV{nx, ny, nz, cc.stationary ? 0.0 : ns_ * (nb * sqrt(nb)), nb};
@@ -898,7 +901,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) Km[(idx / RP) * LD + idx % RP] = 0.0;
     __syncthreads();
     // L L^T
-    wg_gemm_tn<false, true, double, double, GEMM_FULL>(Km, LD, RP, RP, RP, 1.0, LR, RP, RP, LR, RP, RP);
+    wg_gemm_tn<false, true, double, double, GEMM_FULL, false, GEMM_TRI_A | GEMM_TRI_B | GEMM_TRI_C>(
+        Km, LD, RP, RP, RP, 1.0, LR, RP, RP, LR, RP, RP);
     __syncthreads();
     post_add_A<RP, FROM_OBS>(T, a, R, l, l, Km, LD, false);
     __syncthreads();
@@ -944,7 +948,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             __syncthreads();
             // P_j stored straight to global memory (row-major RP x RP)
             double* Pj = PR + (uint64_t)j * RP * RP;
-            wg_gemm_tn<false, false, double, double, GEMM_FULL>(Pj, RP, RP, RP, RP, 1.0, LpiTR, RP, RP, Km, LD, RP);
+            wg_gemm_tn<false, false, double, double, GEMM_FULL, false, GEMM_TRI_A>(Pj, RP, RP, RP, RP, 1.0, LpiTR, RP,
+                                                                                  RP, Km, LD, RP);
             __syncthreads();
             // w~_j = w_j - P_j^T v
             if (a.write_msgs)
@@ -1336,7 +1341,8 @@ __global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts 
             const int i = idx / NT, c = idx % NT;
             if (c < np) BtF[(uint64_t)(j * RP + i) * pF + p0 + c] = Cm[i * LD + c];
         }
-        wg_gemm_tn(Wm, LD, RP, NT, RP, 1.0, a.LiT + an * RP * RP, RP, RP, Cm, LD, NT);
+        wg_gemm_tn<false, true, double, double, GEMM_PART, false, GEMM_TRI_A>(Wm, LD, RP, NT, RP, 1.0,
+                                                                             a.LiT + an * RP * RP, RP, RP, Cm, LD, NT);
         __syncthreads();
         for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
             const double w = Wm[(idx / NT) * LD + idx % NT];
@@ -1457,7 +1463,7 @@ __global__ __launch_bounds__(256) void k_pred_post(Tree T, PredArgs a) {
     for (int jj = L - 1; jj >= 0; jj--) {
         const uint64_t an = ancestor_up(T.parent[F], L - 1, jj, T.J);
         // Z = L~^{-1} B~_jj, B~ read from global memory (the tile's np columns)
-        wg_gemm_tn<false, false>(Zm, LD, RP, NT, RP, 1.0, a.LpiT + an * RP * RP, RP, RP,
+        wg_gemm_tn<false, false, double, double, GEMM_PART, false, GEMM_TRI_A>(Zm, LD, RP, NT, RP, 1.0, a.LpiT + an * RP * RP, RP, RP,
                                  BtF + (uint64_t)jj * RP * pF + p0, pF, np);
         __syncthreads();
         if (c < np) {
