@@ -153,8 +153,12 @@ int mra_use_graph(mra_handle* h, int enable);
  * MRA_OPT_WAVE_LOCAL16: 1 (default) runs the finest-level local step
  * (Sigma_F, its Cholesky, u, H) of regions with at most 16 observations one
  * wave per region; 0 one work group per region (same values up to
- * rounding). */
-enum { MRA_OPT_FACTORED_POSTERIOR = 1, MRA_OPT_GRAM_KERNEL = 2, MRA_OPT_WAVE_LOCAL16 = 3 };
+ * rounding).
+ * MRA_OPT_CROSS16: 1 (default) conditions the observations of finest-level
+ * segments (the observations below one finest-but-one region) with at most
+ * 16 observations in 16-column tiles; 0 in the 32-column tiles of larger
+ * segments (same values). */
+enum { MRA_OPT_FACTORED_POSTERIOR = 1, MRA_OPT_GRAM_KERNEL = 2, MRA_OPT_WAVE_LOCAL16 = 3, MRA_OPT_CROSS16 = 4 };
 int mra_set_option(mra_handle* h, int option, int value);
 int mra_get_option(mra_handle* h, int option, int* value);
 

@@ -38,6 +38,7 @@ EXPORTED = ["mra_create", "mra_destroy", "mra_last_error", "mra_set_covariance",
 OPT_FACTORED_POSTERIOR = 1
 OPT_GRAM_KERNEL = 2
 OPT_WAVE_LOCAL16 = 3
+OPT_CROSS16 = 4
 
 
 class TreeDesc(C.Structure):

@@ -44,6 +44,7 @@ struct mra_handle {
     int lowrankMode = 1;  // requested: 1 = when possible, 0 = dense everywhere
     int gramSep = 1;      // message blocks by k_post_gram (RP = 64); 0: inside k_posterior
     int waveLocal16 = 1;  // finest regions with <= 16 observations one wave each
+    int cross16 = 1;      // segments with <= 16 observations in 16-column cross tiles
     bool denseFull = false;  // full-size dense messages of level L-1 allocated
     std::vector<uint64_t> hRowS1, hRowE1, hRowS2, hRowE2;  // rows of levels L-1 and L-2
     uint64_t *dRowS1 = nullptr, *dRowE1 = nullptr, *dRowS2 = nullptr, *dRowE2 = nullptr;
@@ -86,6 +87,8 @@ struct mra_handle {
     uint32_t nItemsX = 0, nItemsF = 0;
     uint32_t* dItemsX32 = nullptr;  // segments of at most 32 observations (32-column tiles)
     uint32_t nItemsX32 = 0;
+    uint32_t* dItemsX16 = nullptr;  // segments of at most 16 observations (16-column tiles)
+    uint32_t nItemsX16 = 0;
     // finest regions by length bucket: n <= 16, 32, 64, 128, larger
     static constexpr int NBK = 6;
     uint64_t* dWsOffBig = nullptr;  // workspace offset of each region of the 128 < n <= 256 bucket
@@ -357,11 +360,16 @@ static int build_worklists(mra_handle* h) {
     for (void* p : h->wlAllocs) (void)hipFree(p);
     h->wlAllocs.clear();
     // work lists
-    std::vector<uint32_t> ix, ix32, iF;
+    std::vector<uint32_t> ix, ix32, ix16, iF;
     for (uint64_t f = 0; f < h->nF; f++)
         if (h->hObsOff[f + 1] > h->hObsOff[f]) iF.push_back((uint32_t)f);
     for (size_t sgi = 0; sgi < h->hSN.size(); sgi++) {
         if (h->hSN[sgi] == 0) continue;
+        if (h->hSN[sgi] <= 16 && h->cross16) {
+            ix16.push_back((uint32_t)sgi);
+            ix16.push_back(0u);
+            continue;
+        }
         if (h->hSN[sgi] <= (uint32_t)h->tuneX32) {
             for (uint64_t c0 = 0; c0 < h->hSN[sgi]; c0 += 32) {
                 ix32.push_back((uint32_t)sgi);
@@ -376,6 +384,8 @@ static int build_worklists(mra_handle* h) {
     }
     h->nItemsX32 = (uint32_t)(ix32.size() / 2);
     RC_TRY(upload_wl(h, &h->dItemsX32, ix32.data(), ix32.size()));
+    h->nItemsX16 = (uint32_t)(ix16.size() / 2);
+    RC_TRY(upload_wl(h, &h->dItemsX16, ix16.data(), ix16.size()));
     h->nItemsX = (uint32_t)(ix.size() / 2);
     h->nItemsF = (uint32_t)iF.size();
     {
@@ -821,6 +831,12 @@ extern "C" int mra_set_option(mra_handle* h, int option, int value) {
         graph_invalidate(h);
         return MRA_OK;
     }
+    if (option == MRA_OPT_CROSS16) {
+        h->cross16 = value ? 1 : 0;
+        RC_TRY(build_worklists(h));
+        graph_invalidate(h);
+        return MRA_OK;
+    }
     set_err(h, "unknown option");
     return MRA_E_ARG;
 }
@@ -837,6 +853,10 @@ extern "C" int mra_get_option(mra_handle* h, int option, int* value) {
     }
     if (option == MRA_OPT_WAVE_LOCAL16) {
         *value = h->waveLocal16;
+        return MRA_OK;
+    }
+    if (option == MRA_OPT_CROSS16) {
+        *value = h->cross16;
         return MRA_OK;
     }
     set_err(h, "unknown option");
@@ -919,6 +939,9 @@ static void launch_finest_cross(mra_handle* h, const Tree& T, const CovCtx& cc, 
                                                                (TG*)h->dGf, (TG*)h->dBf, h->dGc32);
     if (h->nItemsX32)
         k_finest_cross<RP, 32, TG><<<h->nItemsX32, 256, 0, st>>>(T, cc, K, O, h->dItemsX32, h->dGoff, h->dGc,
+                                                                 h->dLiT, (TG*)h->dGf, (TG*)h->dBf, h->dGc32);
+    if (h->nItemsX16)
+        k_finest_cross<RP, 16, TG><<<h->nItemsX16, 256, 0, st>>>(T, cc, K, O, h->dItemsX16, h->dGoff, h->dGc,
                                                                  h->dLiT, (TG*)h->dGf, (TG*)h->dBf, h->dGc32);
 }
 

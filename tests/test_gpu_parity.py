@@ -260,6 +260,40 @@ def test_small_region_local_kernel(case, wave):
     assert abs(Dm - D) / (2 * len(tree["obs_lon"])) < 1e-6
 
 
+@pytest.mark.parametrize("cross16", [1, 0])
+def test_small_segment_cross_tiles(cross16):
+    """Finest-level segments (observations below one finest-but-one region)
+    of at most 16 observations conditioned in 16-column tiles (cross16=1) or
+    in 32-column tiles (0), next to 32- and 64-column tiles of larger
+    segments: D and predictions match the oracle in FP64, mixed within
+    tolerance."""
+    rng = np.random.default_rng(16)
+    x, y = synth.clustered_points(1500, rng, n_clusters=3, sigma=0.1)
+    px, py = synth.uniform_points(60, rng)
+    tree = capi.partition_program(x, y, rng.standard_normal(1500), px, py, M=5, J=4, r=16,
+                                  bounds=synth.UNIT_BOUNDS)
+    cov, prm = synth.linear_trend_cov(), dict(synth.UNIT_PARAMS)
+    seg = np.diff(tree["obs_offsets"].astype(np.int64)).reshape(-1, 4).sum(1)
+    assert ((seg >= 1) & (seg <= 16)).sum() > 10 and (seg > 32).sum() > 5
+    h = capi.Handle(tree)
+    try:
+        h.set_covariance(cov)
+        h.set_params(**prm)
+        h.set_option(capi.OPT_CROSS16, cross16)
+        assert h.get_option(capi.OPT_CROSS16) == cross16
+        D, mean, var = h.predict()
+        h.set_precision(capi.PRECISION_MIXED)
+        Dm = h.likelihood()
+    finally:
+        h.close()
+    Do, mo, vo = oracle.evaluate(tree, cov, prm)
+    print(f"cross16={cross16} D gpu {D!r} oracle {Do!r} rel {rel(D, Do):.3e}", flush=True)
+    assert rel(D, Do) < TOL_D
+    np.testing.assert_allclose(mean, mo, rtol=TOL_PRED, atol=1e-10)
+    np.testing.assert_allclose(var, vo, rtol=TOL_PRED, atol=1e-10)
+    assert abs(Dm - D) / (2 * len(tree["obs_lon"])) < 1e-6
+
+
 @pytest.mark.parametrize("r", [16, 32])
 def test_large_region_local_kernel(r):
     """Finest regions of 129 to 256 observations (padded workspace, blocked
