@@ -154,10 +154,10 @@ int mra_use_graph(mra_handle* h, int enable);
  * (Sigma_F, its Cholesky, u, H) of regions with at most 16 observations one
  * wave per region; 0 one work group per region (same values up to
  * rounding).
- * MRA_OPT_CROSS16: 1 (default) conditions the observations of finest-level
- * segments (the observations below one finest-but-one region) with at most
- * 16 observations in 16-column tiles; 0 in the 32-column tiles of larger
- * segments (same values). */
+ * MRA_OPT_CROSS16: 1 (default) conditions the last observations of each
+ * finest-level segment (the observations below one finest-but-one region)
+ * in the narrowest tile of 16, 32 or 64 columns that holds them; 0 uses the
+ * segment's tile width throughout (same values). */
 enum { MRA_OPT_FACTORED_POSTERIOR = 1, MRA_OPT_GRAM_KERNEL = 2, MRA_OPT_WAVE_LOCAL16 = 3, MRA_OPT_CROSS16 = 4 };
 int mra_set_option(mra_handle* h, int option, int value);
 int mra_get_option(mra_handle* h, int option, int* value);

@@ -364,22 +364,16 @@ static int build_worklists(mra_handle* h) {
     for (uint64_t f = 0; f < h->nF; f++)
         if (h->hObsOff[f + 1] > h->hObsOff[f]) iF.push_back((uint32_t)f);
     for (size_t sgi = 0; sgi < h->hSN.size(); sgi++) {
-        if (h->hSN[sgi] == 0) continue;
-        if (h->hSN[sgi] <= 16 && h->cross16) {
-            ix16.push_back((uint32_t)sgi);
-            ix16.push_back(0u);
-            continue;
-        }
-        if (h->hSN[sgi] <= (uint32_t)h->tuneX32) {
-            for (uint64_t c0 = 0; c0 < h->hSN[sgi]; c0 += 32) {
-                ix32.push_back((uint32_t)sgi);
-                ix32.push_back((uint32_t)c0);
-            }
-            continue;
-        }
-        for (uint64_t c0 = 0; c0 < h->hSN[sgi]; c0 += 64) {
-            ix.push_back((uint32_t)sgi);
-            ix.push_back((uint32_t)c0);
+        // tiles of 64 (segments above tuneX32) or 32 columns; with cross16 the
+        // last tile is the narrowest of 16, 32, 64 columns that holds the rest
+        const uint32_t n = h->hSN[sgi], w = n <= (uint32_t)h->tuneX32 ? 32u : 64u;
+        for (uint32_t c0 = 0; c0 < n;) {
+            const uint32_t rest = n - c0;
+            const uint32_t t = !h->cross16 ? w : rest > 32 ? w : rest > 16 ? 32u : 16u;
+            std::vector<uint32_t>& lst = t == 64 ? ix : t == 32 ? ix32 : ix16;
+            lst.push_back((uint32_t)sgi);
+            lst.push_back(c0);
+            c0 += t;
         }
     }
     h->nItemsX32 = (uint32_t)(ix32.size() / 2);

@@ -263,10 +263,9 @@ def test_small_region_local_kernel(case, wave):
 @pytest.mark.parametrize("cross16", [1, 0])
 def test_small_segment_cross_tiles(cross16):
     """Finest-level segments (observations below one finest-but-one region)
-    of at most 16 observations conditioned in 16-column tiles (cross16=1) or
-    in 32-column tiles (0), next to 32- and 64-column tiles of larger
-    segments: D and predictions match the oracle in FP64, mixed within
-    tolerance."""
+    conditioned with a last tile of 16, 32 or 64 columns, the narrowest that
+    holds the rest (cross16=1), or in tiles of one width (0): D and
+    predictions match the oracle in FP64, mixed within tolerance."""
     rng = np.random.default_rng(16)
     x, y = synth.clustered_points(1500, rng, n_clusters=3, sigma=0.1)
     px, py = synth.uniform_points(60, rng)
