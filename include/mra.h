@@ -157,8 +157,20 @@ int mra_use_graph(mra_handle* h, int enable);
  * MRA_OPT_CROSS16: 1 (default) conditions the last observations of each
  * finest-level segment (the observations below one finest-but-one region)
  * in the narrowest tile of 16, 32 or 64 columns that holds them; 0 uses the
- * segment's tile width throughout (same values). */
-enum { MRA_OPT_FACTORED_POSTERIOR = 1, MRA_OPT_GRAM_KERNEL = 2, MRA_OPT_WAVE_LOCAL16 = 3, MRA_OPT_CROSS16 = 4 };
+ * segment's tile width throughout (same values).
+ * MRA_OPT_CONCURRENT: 1 (default) issues the independent launches of one
+ * step (the finest-level tile widths and length buckets, the dense and the
+ * factored posterior regions of the finest-but-one level) on two more
+ * streams of the handle, forked and joined by events; 0 issues everything
+ * on the handle's stream (same values).  Handles evaluated through
+ * mra_batch_likelihood use their one stream. */
+enum {
+    MRA_OPT_FACTORED_POSTERIOR = 1,
+    MRA_OPT_GRAM_KERNEL = 2,
+    MRA_OPT_WAVE_LOCAL16 = 3,
+    MRA_OPT_CROSS16 = 4,
+    MRA_OPT_CONCURRENT = 5
+};
 int mra_set_option(mra_handle* h, int option, int value);
 int mra_get_option(mra_handle* h, int option, int* value);
 

@@ -39,6 +39,7 @@ OPT_FACTORED_POSTERIOR = 1
 OPT_GRAM_KERNEL = 2
 OPT_WAVE_LOCAL16 = 3
 OPT_CROSS16 = 4
+OPT_CONCURRENT = 5
 
 
 class TreeDesc(C.Structure):

@@ -44,6 +44,14 @@ struct mra_handle {
     int lowrankMode = 1;  // requested: 1 = when possible, 0 = dense everywhere
     int gramSep = 1;      // message blocks by k_post_gram (RP = 64); 0: inside k_posterior
     int waveLocal16 = 1;  // finest regions with <= 16 observations one wave each
+    // independent launches of one step (length buckets, the dense and the
+    // factored posterior of the finest-but-one level) on two more streams,
+    // forked from and joined back to `stream` by events (also under capture)
+    int concurrent = 1;
+    bool noFork = false;  // handles of a batch: the batch runs fields concurrently instead
+    static constexpr int NFORK = 4;
+    hipStream_t side[2] = {};
+    hipEvent_t evFork[NFORK] = {}, evJoin[NFORK][2] = {};
     int cross16 = 1;      // segments with <= 16 observations in 16-column cross tiles
     bool denseFull = false;  // full-size dense messages of level L-1 allocated
     std::vector<uint64_t> hRowS1, hRowE1, hRowS2, hRowE2;  // rows of levels L-1 and L-2
@@ -190,6 +198,13 @@ extern "C" void mra_destroy(mra_handle* h) {
     for (void* p : h->fieldAllocs) (void)hipFree(p);
     for (auto& e : h->ev)
         if (e) (void)hipEventDestroy(e);
+    for (int i = 0; i < mra_handle::NFORK; i++) {
+        if (h->evFork[i]) (void)hipEventDestroy(h->evFork[i]);
+        for (auto& e : h->evJoin[i])
+            if (e) (void)hipEventDestroy(e);
+    }
+    for (auto& s : h->side)
+        if (s) (void)hipStreamDestroy(s);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -831,6 +846,11 @@ extern "C" int mra_set_option(mra_handle* h, int option, int value) {
         graph_invalidate(h);
         return MRA_OK;
     }
+    if (option == MRA_OPT_CONCURRENT) {
+        h->concurrent = value ? 1 : 0;
+        graph_invalidate(h);
+        return MRA_OK;
+    }
     set_err(h, "unknown option");
     return MRA_E_ARG;
 }
@@ -851,6 +871,10 @@ extern "C" int mra_get_option(mra_handle* h, int option, int* value) {
     }
     if (option == MRA_OPT_CROSS16) {
         *value = h->cross16;
+        return MRA_OK;
+    }
+    if (option == MRA_OPT_CONCURRENT) {
+        *value = h->concurrent;
         return MRA_OK;
     }
     set_err(h, "unknown option");
@@ -893,27 +917,65 @@ static CovCtx make_cc(mra_handle* h) {
 
 static Pts pts(double* const* c, double* const* s) { return Pts{c[0], c[1], c[2], s[0], s[1], s[2]}; }
 
+// Side streams and their events, created on first use (default priority).
+static int ensure_side(mra_handle* h) {
+    if (!h->concurrent || h->noFork || h->side[1]) return MRA_OK;
+    for (auto& s : h->side)
+        if (!s) HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    for (int i = 0; i < mra_handle::NFORK; i++) {
+        if (!h->evFork[i]) HIP_TRY(hipEventCreateWithFlags(&h->evFork[i], hipEventDisableTiming));
+        for (auto& e : h->evJoin[i])
+            if (!e) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    return MRA_OK;
+}
+
+// Fork point k of one evaluation: the side streams wait for the work issued
+// so far on the main stream; join(k) makes the main stream wait for them.
+// Without side streams every stream() is the main stream.
+struct Fork {
+    mra_handle* h;
+    int k;
+    bool on;
+    Fork(mra_handle* h_, int k_) : h(h_), k(k_), on(h_->concurrent && !h_->noFork && h_->side[1]) {
+        if (!on) return;
+        (void)hipEventRecord(h->evFork[k], h->stream);
+        for (auto s : h->side) (void)hipStreamWaitEvent(s, h->evFork[k], 0);
+    }
+    hipStream_t stream(int i) const { return on && i > 0 ? h->side[i - 1] : h->stream; }
+    ~Fork() {
+        if (!on) return;
+        for (int i = 0; i < 2; i++) {
+            (void)hipEventRecord(h->evJoin[k][i], h->side[i]);
+            (void)hipStreamWaitEvent(h->stream, h->evJoin[k][i], 0);
+        }
+    }
+};
+
 template <int RP, typename TG>
 static void launch_finest_local(mra_handle* h, const Tree& T, const CovCtx& cc, const Pts& O) {
-    hipStream_t st = h->stream;
+    // the longest work groups (largest buckets) on one side stream, the
+    // middle buckets on the other, the many small regions on the main stream
+    const Fork fk(h, 0);
+    const hipStream_t s1 = fk.stream(1), s2 = fk.stream(2), st = fk.stream(0);
     TG* Gf = (TG*)h->dGf;
     TG* Bf = (TG*)h->dBf;
     // larger buckets first (longest work groups start early)
     if (h->nItemsFB[5])
-        k_finest_local<RP, 0, TG><<<h->nItemsFB[5], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[5], Gf, Bf, h->dSig,
+        k_finest_local<RP, 0, TG><<<h->nItemsFB[5], 256, 0, s1>>>(T, cc, O, h->dY, h->dItemsFB[5], Gf, Bf, h->dSig,
                                                                   h->dH, h->dU, h->dDterm, h->dBad);
     if (h->nItemsFB[4])
-        k_finest_local_big<TG><<<h->nItemsFB[4], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[4], h->dWsOffBig,
+        k_finest_local_big<TG><<<h->nItemsFB[4], 256, 0, s1>>>(T, cc, O, h->dY, h->dItemsFB[4], h->dWsOffBig,
                                                                h->dWsBig, Gf, Bf, h->dSig, h->dH, h->dU, h->dDterm,
                                                                h->dBad);
     if (h->nItemsFB[3])
-        k_finest_local<RP, 128, TG><<<h->nItemsFB[3], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[3], Gf, Bf, h->dSig,
+        k_finest_local<RP, 128, TG><<<h->nItemsFB[3], 256, 0, s1>>>(T, cc, O, h->dY, h->dItemsFB[3], Gf, Bf, h->dSig,
                                                                     h->dH, h->dU, h->dDterm, h->dBad);
     if (h->nItemsFB[2])
-        k_finest_local<RP, 64, TG><<<h->nItemsFB[2], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[2], Gf, Bf, h->dSig,
+        k_finest_local<RP, 64, TG><<<h->nItemsFB[2], 256, 0, s2>>>(T, cc, O, h->dY, h->dItemsFB[2], Gf, Bf, h->dSig,
                                                                    h->dH, h->dU, h->dDterm, h->dBad);
     if (h->nItemsFB[1])
-        k_finest_local<RP, 32, TG><<<h->nItemsFB[1], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[1], Gf, Bf, h->dSig,
+        k_finest_local<RP, 32, TG><<<h->nItemsFB[1], 256, 0, s2>>>(T, cc, O, h->dY, h->dItemsFB[1], Gf, Bf, h->dSig,
                                                                    h->dH, h->dU, h->dDterm, h->dBad);
     if (h->nItemsFB[0] && h->waveLocal16)
         // one wave per region, four regions per work group
@@ -927,15 +989,16 @@ static void launch_finest_local(mra_handle* h, const Tree& T, const CovCtx& cc, 
 
 template <int RP, typename TG>
 static void launch_finest_cross(mra_handle* h, const Tree& T, const CovCtx& cc, const Pts& K, const Pts& O) {
-    hipStream_t st = h->stream;
+    const Fork fk(h, 1);
+    const hipStream_t st = fk.stream(0);
     if (h->nItemsX)
-        k_finest_cross<RP, 64, TG><<<h->nItemsX, 256, 0, st>>>(T, cc, K, O, h->dItemsX, h->dGoff, h->dGc, h->dLiT,
+        k_finest_cross<RP, 64, TG><<<h->nItemsX, 256, 0, fk.stream(1)>>>(T, cc, K, O, h->dItemsX, h->dGoff, h->dGc, h->dLiT,
                                                                (TG*)h->dGf, (TG*)h->dBf, h->dGc32);
     if (h->nItemsX32)
         k_finest_cross<RP, 32, TG><<<h->nItemsX32, 256, 0, st>>>(T, cc, K, O, h->dItemsX32, h->dGoff, h->dGc,
                                                                  h->dLiT, (TG*)h->dGf, (TG*)h->dBf, h->dGc32);
     if (h->nItemsX16)
-        k_finest_cross<RP, 16, TG><<<h->nItemsX16, 256, 0, st>>>(T, cc, K, O, h->dItemsX16, h->dGoff, h->dGc,
+        k_finest_cross<RP, 16, TG><<<h->nItemsX16, 256, 0, fk.stream(2)>>>(T, cc, K, O, h->dItemsX16, h->dGoff, h->dGc,
                                                                  h->dLiT, (TG*)h->dGf, (TG*)h->dBf, h->dGc32);
 }
 
@@ -1034,16 +1097,16 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
     // one-wave-per-block kernel after the posterior launch; the levels above
     // sum dense child messages inside the posterior kernel
     const bool gsep = RP == 64 && h->gramSep;
-    auto post = [&](const PostArgs& pa_, uint32_t nitems, bool from_obs) {
+    auto post = [&](const PostArgs& pa_, uint32_t nitems, bool from_obs, hipStream_t ps) {
         PostArgs q = pa_;
         q.pj_sep = gsep && from_obs && q.l > 0;
         q.gram_sep = q.pj_sep && q.write_msgs;
         if (from_obs)
-            k_posterior<RP, true><<<nitems, 256, 0, st>>>(T, q);
+            k_posterior<RP, true><<<nitems, 256, 0, ps>>>(T, q);
         else
-            k_posterior<RP, false><<<nitems, 256, 0, st>>>(T, q);
-        if (q.pj_sep) k_post_pj<<<post_pj_grid(nitems, q.l), 256, 0, st>>>(T, q, nitems);
-        if (q.gram_sep) k_post_gram<<<post_gram_grid(nitems, q.l), 256, 0, st>>>(T, q, nitems);
+            k_posterior<RP, false><<<nitems, 256, 0, ps>>>(T, q);
+        if (q.pj_sep) k_post_pj<<<post_pj_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
+        if (q.gram_sep) k_post_gram<<<post_gram_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
     };
     for (int l = h->L - 1; l >= 0; l--) {
         mark(MRA_T_POST0 + l);
@@ -1086,41 +1149,46 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
                 b.dterm = h->dDterm;
                 b.ws = h->dLrWs;
                 b.bad_flag = h->dBad;
-                if (h->nItemsBig) {
-                    // large regions: dense update into compact slots, zero factor rows
-                    PostArgs d = a;
-                    d.items = h->dItemsBig;
-                    d.slot = h->dSlot1;
-                    d.Xz = h->dXo;
-                    d.ldxz = (h->L - 1) * h->RP;
-                    d.zz = h->dZo;
-                    post(d, h->nItemsBig, true);
-                }
-                if (h->nItemsLR[3]) {
-                    b.items = h->dItemsLR[3];
-                    k_post_lowrank<RP, 128><<<h->nItemsLR[3], 256, 0, st>>>(T, b);
-                }
-                if (h->nItemsLR[2]) {
-                    b.items = h->dItemsLR[2];
-                    k_post_lowrank<RP, 64><<<h->nItemsLR[2], 256, 0, st>>>(T, b);
-                }
-                if (h->nItemsLR[1]) {
-                    b.items = h->dItemsLR[1];
-                    k_post_lowrank<RP, 32><<<h->nItemsLR[1], 256, 0, st>>>(T, b);
-                }
-                if (h->nItemsLR[0]) {
-                    b.items = h->dItemsLR[0];
-                    k_post_lowrank<RP, 16><<<h->nItemsLR[0], 256, 0, st>>>(T, b);
+                {
+                    // independent regions: the dense chain of the large ones on
+                    // one side stream, the row-count buckets on the other two
+                    const Fork fk(h, 2);
+                    if (h->nItemsBig) {
+                        // large regions: dense update into compact slots, zero factor rows
+                        PostArgs d = a;
+                        d.items = h->dItemsBig;
+                        d.slot = h->dSlot1;
+                        d.Xz = h->dXo;
+                        d.ldxz = (h->L - 1) * h->RP;
+                        d.zz = h->dZo;
+                        post(d, h->nItemsBig, true, fk.stream(1));
+                    }
+                    if (h->nItemsLR[3]) {
+                        b.items = h->dItemsLR[3];
+                        k_post_lowrank<RP, 128><<<h->nItemsLR[3], 256, 0, fk.stream(2)>>>(T, b);
+                    }
+                    if (h->nItemsLR[2]) {
+                        b.items = h->dItemsLR[2];
+                        k_post_lowrank<RP, 64><<<h->nItemsLR[2], 256, 0, fk.stream(2)>>>(T, b);
+                    }
+                    if (h->nItemsLR[1]) {
+                        b.items = h->dItemsLR[1];
+                        k_post_lowrank<RP, 32><<<h->nItemsLR[1], 256, 0, fk.stream(0)>>>(T, b);
+                    }
+                    if (h->nItemsLR[0]) {
+                        b.items = h->dItemsLR[0];
+                        k_post_lowrank<RP, 16><<<h->nItemsLR[0], 256, 0, fk.stream(0)>>>(T, b);
+                    }
                 }
                 if (predicting && h->nItemsLRAll) {
                     // posterior factors, v and P for the top-down prediction
                     a.write_msgs = 0;
                     a.write_d = 0;
                     a.items = h->dItemsLRAll;
-                    post(a, h->nItemsLRAll, true);
+                    post(a, h->nItemsLRAll, true, st);
                 }
             } else {
-                post(a, h->nItemsPost[l], true);
+                post(a, h->nItemsPost[l], true, st);
             }
         } else if (l == h->L - 2 && lr) {
             a.X = h->dXo;
@@ -1132,12 +1200,12 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
             a.wc = h->dW[l + 1];
             a.Ac = h->dA[l + 1];
             a.start_c = h->lvlStart[l + 1];
-            post(a, h->nItemsPost[l], true);
+            post(a, h->nItemsPost[l], true, st);
         } else {
             a.wc = h->dW[l + 1];
             a.Ac = h->dA[l + 1];
             a.start_c = h->lvlStart[l + 1];
-            post(a, h->nItemsPost[l], false);
+            post(a, h->nItemsPost[l], false, st);
         }
     }
     mark(MRA_T_PRED_POST);
@@ -1241,6 +1309,7 @@ static int run_eval(mra_handle* h, bool predicting) {
     const double prm[6] = {h->sill, h->range, h->nugget, h->ss, h->rs, h->ns};
     for (int i = 0; i < 6; i++) h->hPrm[i] = prm[i];
     HIP_TRY(hipMemcpyAsync(h->dPrm, h->hPrm, sizeof(prm), hipMemcpyHostToDevice, h->stream));
+    RC_TRY(ensure_side(h));
     if (h->useGraph) {
         RC_TRY(run_graph(h, predicting));
     } else {
@@ -1709,6 +1778,8 @@ extern "C" int mra_batch_create(mra_handle* const* handles, int n, mra_batch** o
         }
         b->h.push_back(handles[i]);
     }
+    // the batch's streams already run fields side by side: no side streams
+    for (mra_handle* h : b->h) h->noFork = true;
     b->st.assign(MRA_BATCH_MAX_STREAMS, nullptr);
     for (auto& s : b->st)
         if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {

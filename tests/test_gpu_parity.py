@@ -345,6 +345,45 @@ def test_binary_tree_padded_knots(gram):
     check(tree, cov, prm, gram=gram)
 
 
+def test_concurrent_streams():
+    """Independent launches on forked streams (default) against one stream:
+    the same D to the last bit, eager and graph replay, FP64 and mixed, and
+    the oracle's D; a tree whose finest-but-one level has both dense and
+    factored regions and whose finest level fills every length bucket."""
+    rng = np.random.default_rng(89)
+    x, y = synth.clustered_points(8000, rng, n_clusters=3, sigma=0.03)
+    tree = capi.partition_program(x, y, rng.standard_normal(8000), None, None, M=9, J=2, r=49,
+                                  bounds=synth.UNIT_BOUNDS)
+    counts = np.diff(tree["obs_offsets"].astype(np.int64))
+    assert all(np.histogram(counts, [1, 17, 33, 65, 129, 257, 10**9])[0] > 0)
+    rows = counts.reshape(-1, 2).sum(1)
+    assert (rows > 128).sum() > 0 and ((rows > 0) & (rows <= 128)).sum() > 0
+    cov, prm = synth.linear_trend_cov(log_range=(np.log(0.1), 0.8, 0.0)), dict(synth.UNIT_PARAMS)
+    out = {}
+    for conc in (1, 0):
+        h = capi.Handle(tree, with_pred=False)
+        try:
+            h.set_covariance(cov)
+            h.set_params(**prm)
+            h.set_option(capi.OPT_CONCURRENT, conc)
+            assert h.get_option(capi.OPT_CONCURRENT) == conc
+            for prec in (capi.PRECISION_FP64, capi.PRECISION_MIXED):
+                h.set_precision(prec)
+                for graph in (False, True):
+                    h.use_graph(graph)
+                    out[(conc, prec, graph)] = [h.likelihood() for _ in range(2)]
+        finally:
+            h.close()
+    print(out, flush=True)
+    for prec in (capi.PRECISION_FP64, capi.PRECISION_MIXED):
+        ref = out[(0, prec, False)][0]
+        for k, v in out.items():
+            if k[1] == prec:
+                assert v == [ref, ref], k
+    Do = oracle.evaluate(tree, cov, prm, predict=False)[0]
+    assert rel(out[(1, capi.PRECISION_FP64, True)][0], Do) < TOL_D
+
+
 TOL_MIXED = 1e-6   # |delta loglik| / n for the mixed-precision path
 
 
