@@ -163,13 +163,20 @@ int mra_use_graph(mra_handle* h, int enable);
  * factored posterior regions of the finest-but-one level) on two more
  * streams of the handle, forked and joined by events; 0 issues everything
  * on the handle's stream (same values).  Handles evaluated through
- * mra_batch_likelihood use their one stream. */
+ * mra_batch_likelihood use their one stream.
+ * MRA_OPT_SUM_MESSAGES: 1 (default) has the message-block kernel
+ * (MRA_OPT_GRAM_KERNEL) store the sum of the messages of each parent's
+ * children when the parent level sums dense child messages, so that the
+ * parent reads one block set instead of one per child; 0 stores every
+ * region's message (same values up to rounding).  mra_get_option reports
+ * whether the summed form is in effect. */
 enum {
     MRA_OPT_FACTORED_POSTERIOR = 1,
     MRA_OPT_GRAM_KERNEL = 2,
     MRA_OPT_WAVE_LOCAL16 = 3,
     MRA_OPT_CROSS16 = 4,
-    MRA_OPT_CONCURRENT = 5
+    MRA_OPT_CONCURRENT = 5,
+    MRA_OPT_SUM_MESSAGES = 6
 };
 int mra_set_option(mra_handle* h, int option, int value);
 int mra_get_option(mra_handle* h, int option, int* value);

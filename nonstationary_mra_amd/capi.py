@@ -40,6 +40,7 @@ OPT_GRAM_KERNEL = 2
 OPT_WAVE_LOCAL16 = 3
 OPT_CROSS16 = 4
 OPT_CONCURRENT = 5
+OPT_SUM_MESSAGES = 6
 
 
 class TreeDesc(C.Structure):

@@ -53,6 +53,7 @@ struct mra_handle {
     hipStream_t side[2] = {};
     hipEvent_t evFork[NFORK] = {}, evJoin[NFORK][2] = {};
     int cross16 = 1;      // segments with <= 16 observations in 16-column cross tiles
+    int sumMsgs = 1;      // message blocks summed per parent by k_post_gram
     bool denseFull = false;  // full-size dense messages of level L-1 allocated
     std::vector<uint64_t> hRowS1, hRowE1, hRowS2, hRowE2;  // rows of levels L-1 and L-2
     uint64_t *dRowS1 = nullptr, *dRowE1 = nullptr, *dRowS2 = nullptr, *dRowE2 = nullptr;
@@ -851,6 +852,11 @@ extern "C" int mra_set_option(mra_handle* h, int option, int value) {
         graph_invalidate(h);
         return MRA_OK;
     }
+    if (option == MRA_OPT_SUM_MESSAGES) {
+        h->sumMsgs = value ? 1 : 0;
+        graph_invalidate(h);
+        return MRA_OK;
+    }
     set_err(h, "unknown option");
     return MRA_E_ARG;
 }
@@ -875,6 +881,10 @@ extern "C" int mra_get_option(mra_handle* h, int option, int* value) {
     }
     if (option == MRA_OPT_CONCURRENT) {
         *value = h->concurrent;
+        return MRA_OK;
+    }
+    if (option == MRA_OPT_SUM_MESSAGES) {
+        *value = (h->RP == 64 && h->gramSep && h->sumMsgs) ? 1 : 0;
         return MRA_OK;
     }
     set_err(h, "unknown option");
@@ -1097,6 +1107,9 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
     // one-wave-per-block kernel after the posterior launch; the levels above
     // sum dense child messages inside the posterior kernel
     const bool gsep = RP == 64 && h->gramSep;
+    // summed[l]: the message blocks of level l were stored summed per parent
+    // (the parent level sums dense child messages, so it reads one block set)
+    bool summed[64] = {};
     auto post = [&](const PostArgs& pa_, uint32_t nitems, bool from_obs, hipStream_t ps) {
         PostArgs q = pa_;
         q.pj_sep = gsep && from_obs && q.l > 0;
@@ -1106,7 +1119,19 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
         else
             k_posterior<RP, false><<<nitems, 256, 0, ps>>>(T, q);
         if (q.pj_sep) k_post_pj<<<post_pj_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
-        if (q.gram_sep) k_post_gram<<<post_gram_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
+        if (!q.gram_sep) return;
+        // the parent level reads rows (factored form) when it is L-2 of a factored tree
+        const bool parent_dense = !(lr && q.l - 1 == h->L - 2);
+        if (h->sumMsgs && !q.slot && parent_dense) {
+            PostArgs g = q;
+            g.items = h->dItemsPost[q.l - 1];
+            g.start_p = h->lvlStart[q.l - 1];
+            const uint32_t np = h->nItemsPost[q.l - 1];
+            k_post_gram<true><<<post_gram_grid(np, q.l), 256, 0, ps>>>(T, g, np);
+            summed[q.l] = true;
+        } else {
+            k_post_gram<false><<<post_gram_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
+        }
     };
     for (int l = h->L - 1; l >= 0; l--) {
         mark(MRA_T_POST0 + l);
@@ -1205,6 +1230,7 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
             a.wc = h->dW[l + 1];
             a.Ac = h->dA[l + 1];
             a.start_c = h->lvlStart[l + 1];
+            a.csum = summed[l + 1] ? 1 : 0;
             post(a, h->nItemsPost[l], false, st);
         }
     }

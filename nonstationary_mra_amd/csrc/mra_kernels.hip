@@ -793,6 +793,11 @@ struct PostArgs {
     int* bad_flag;
     int gram_sep;  // the message blocks A~_jk come from k_post_gram (RP = 64)
     int pj_sep;    // P_j and w~_j come from k_post_pj (RP = 64, stacked rows)
+    // dense children: Ac holds the sum of the children's message blocks, one
+    // block set per region of this level (written by k_post_gram<true>)
+    int csum;
+    // k_post_gram<true>: items are the parents (level l-1, from start_p)
+    uint64_t start_p;
 };
 
 template <int RP, bool FROM_OBS>
@@ -835,6 +840,15 @@ __device__ __forceinline__ void post_add_A(const Tree& T, const PostArgs& a, uin
         const int lc = a.l + 1;
         const int nslc = lc * (lc + 1) / 2;
         const int sl = j * lc - j * (j - 1) / 2 + (k - j);
+        if (a.csum) {
+            const double* S = a.Ac + ((R - a.start_l) * nslc + sl) * (uint64_t)RP * RP;
+            for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
+                const int i = idx / RP, q = idx % RP;
+                C[i * ld + q] += transpose ? S[q * RP + i] : S[idx];
+            }
+            __syncthreads();
+            return;
+        }
         for (int c = 0; c < T.J; c++) {
             const uint64_t ch = R * T.J + 1 + c;
             if (!T.working[ch]) continue;
@@ -1034,6 +1048,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // g mod 8 (round-robin dispatch); the mapping below gives every work group
 // of one region the same g mod 8, so the region's rows are re-read from that
 // XCD's L2.
+// SUM: the items are parents (level l-1); one wave sums block (j, k) of the
+// messages of the parent's working children (in child order) and stores one
+// block set per parent, which is all the parent reads (PostArgs::csum).
+template <bool SUM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_post_gram(Tree T, PostArgs a,
                                                                                            uint32_t nitems) {
     constexpr int RP = 64;
@@ -1051,28 +1069,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         j++;
     }
     const int k = j + rem;
-    const uint64_t loc = a.items[it];
-    const uint64_t R = a.start_l + loc;
-    const uint64_t msg = a.slot ? (uint64_t)a.slot[loc] : loc;
-    const double* PR = a.P + a.goff[l] + loc * (uint64_t)l * RP * RP;
     mra_d4 acc[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) acc[q] = mra_d4{0, 0, 0, 0};
     const int lc = l + 1;
     const int nslc = lc * (lc + 1) / 2;
     const int slc = j * lc - j * (j - 1) / 2 + (k - j);
-    const uint64_t os = a.row_s[loc], oe = a.row_e[loc];
-    if (oe > os) {
-        const double* X = a.X + os * (uint64_t)a.ldx;
-        wave_mm64_acc(acc, (int)(oe - os), 1.0, X + j * RP, a.ldx, X + k * RP, a.ldx);
-    }
-    if (a.cslot)
-        for (int c = 0; c < T.J; c++) {
-            const int32_t cs = a.cslot[R * T.J + 1 + c - a.start_c];
-            if (cs >= 0) wave_add64(acc, a.Ac + ((uint64_t)cs * nslc + slc) * (uint64_t)RP * RP, RP);
+    // A~_jk of region R (local index loc) added to acc
+    auto add_region = [&](uint64_t loc) {
+        const uint64_t R = a.start_l + loc;
+        const double* PR = a.P + a.goff[l] + loc * (uint64_t)l * RP * RP;
+        const uint64_t os = a.row_s[loc], oe = a.row_e[loc];
+        if (oe > os) {
+            const double* X = a.X + os * (uint64_t)a.ldx;
+            wave_mm64_acc(acc, (int)(oe - os), 1.0, X + j * RP, a.ldx, X + k * RP, a.ldx);
         }
-    wave_mm64_acc(acc, RP, -1.0, PR + (uint64_t)j * RP * RP, RP, PR + (uint64_t)k * RP * RP, RP);
-    wave_store64(a.Ao + (msg * nsl + b) * (uint64_t)RP * RP, RP, acc);
+        if (a.cslot)
+            for (int c = 0; c < T.J; c++) {
+                const int32_t cs = a.cslot[R * T.J + 1 + c - a.start_c];
+                if (cs >= 0) wave_add64(acc, a.Ac + ((uint64_t)cs * nslc + slc) * (uint64_t)RP * RP, RP);
+            }
+        wave_mm64_acc(acc, RP, -1.0, PR + (uint64_t)j * RP * RP, RP, PR + (uint64_t)k * RP * RP, RP);
+    };
+    if constexpr (SUM) {
+        const uint64_t ploc = a.items[it];
+        const uint64_t Q = a.start_p + ploc;
+        for (int c = 0; c < T.J; c++) {
+            const uint64_t R = Q * T.J + 1 + c;
+            if (T.working[R]) add_region(R - a.start_l);
+        }
+        wave_store64(a.Ao + (ploc * nsl + b) * (uint64_t)RP * RP, RP, acc);
+    } else {
+        const uint64_t loc = a.items[it];
+        add_region(loc);
+        const uint64_t msg = a.slot ? (uint64_t)a.slot[loc] : loc;
+        wave_store64(a.Ao + (msg * nsl + b) * (uint64_t)RP * RP, RP, acc);
+    }
 }
 
 // P_j = L~^{-1} A_jl^T and w~_j = w_j - P_j^T v (j < l) of the regions of a

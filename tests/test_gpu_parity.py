@@ -21,13 +21,14 @@ def rel(a, b):
     return abs(a - b) / max(abs(b), 1e-300)
 
 
-def run_gpu(tree, cov, prm, predict=True, factored=1, gram=1):
+def run_gpu(tree, cov, prm, predict=True, factored=1, gram=1, sum_msgs=1):
     h = capi.Handle(tree, with_pred=predict)
     try:
         h.set_covariance(cov)
         h.set_params(**prm)
         h.set_option(capi.OPT_FACTORED_POSTERIOR, factored)
         h.set_option(capi.OPT_GRAM_KERNEL, gram)
+        h.set_option(capi.OPT_SUM_MESSAGES, sum_msgs)
         if predict and h.n_pred:
             return h.predict()
         return h.likelihood(), None, None
@@ -35,8 +36,8 @@ def run_gpu(tree, cov, prm, predict=True, factored=1, gram=1):
         h.close()
 
 
-def check(tree, cov, prm, predict=True, factored=1, gram=1):
-    D, mean, var = run_gpu(tree, cov, prm, predict, factored, gram)
+def check(tree, cov, prm, predict=True, factored=1, gram=1, sum_msgs=1):
+    D, mean, var = run_gpu(tree, cov, prm, predict, factored, gram, sum_msgs)
     Do, mo, vo = oracle.evaluate(tree, cov, prm, predict=predict)
     print(f"D gpu {D!r} oracle {Do!r} rel {rel(D, Do):.3e}", flush=True)
     assert rel(D, Do) < TOL_D
@@ -221,6 +222,29 @@ def test_message_block_kernel(gram, factored):
     assert h.get_option(capi.OPT_GRAM_KERNEL) == gram
     h.close()
     check(tree, cov, prm, factored=factored, gram=gram)
+
+
+@pytest.mark.parametrize("factored", [1, 0])
+@pytest.mark.parametrize("sum_msgs", [1, 0])
+def test_summed_child_messages(sum_msgs, factored):
+    """Message blocks of the rows-reading level stored summed per parent
+    (sum_msgs=1: the parent reads one block set) or per region (0), in the
+    factored form (summed at the finest-but-two level) and the dense form
+    (summed at the finest-but-one level): both match the oracle, and the
+    option reports the form in effect."""
+    rng = np.random.default_rng(92)
+    x, y = synth.clustered_points(12000, rng, n_clusters=4, sigma=0.02)
+    px, py = synth.uniform_points(200, rng)
+    tree = capi.partition_program(x, y, rng.standard_normal(12000), px, py, M=5, J=4, r=64,
+                                  bounds=synth.UNIT_BOUNDS)
+    cov, prm = synth.linear_trend_cov(), dict(synth.UNIT_PARAMS)
+    h = capi.Handle(tree)
+    h.set_option(capi.OPT_SUM_MESSAGES, sum_msgs)
+    assert h.get_option(capi.OPT_SUM_MESSAGES) == sum_msgs
+    h.set_option(capi.OPT_GRAM_KERNEL, 0)
+    assert h.get_option(capi.OPT_SUM_MESSAGES) == 0
+    h.close()
+    check(tree, cov, prm, factored=factored, sum_msgs=sum_msgs)
 
 
 @pytest.mark.parametrize("wave", [1, 0])
