@@ -35,9 +35,14 @@ struct mra_handle {
     std::vector<uint64_t> lvlStart;  // L+1 entries
     std::vector<uint64_t> hObsOff, hPredOff, hParent, hChS, hChE, hSigOff, hGoff;
     std::vector<uint8_t> hWorking;
-    std::vector<uint64_t> hFSegOs, hSOs, hSParent;
+    std::vector<uint64_t> hFSegOs, hSOs, hSParent, hPSegOs;
+    std::vector<uint32_t> hPSegN;
     std::vector<uint32_t> hFSegN, hSN;
-    uint64_t *dFSegOs = nullptr, *dSOs = nullptr, *dSParent = nullptr;
+    uint64_t *dFSegOs = nullptr, *dSOs = nullptr, *dSParent = nullptr, *dPSegOs = nullptr;
+    uint32_t* dPSegN = nullptr;
+    // prediction post-pass items (F, p0): tiles of the points of one segment
+    uint32_t *dItemsPP16 = nullptr, *dItemsPP64 = nullptr;
+    uint32_t nItemsPP16 = 0, nItemsPP64 = 0;
     uint32_t *dFSegN = nullptr, *dSN = nullptr;
     // factored posterior of the finest-but-one level
     bool lowrank = false;
@@ -298,6 +303,8 @@ static int build_structure(mra_handle* h, const mra_tree_desc* t) {
     // segments: maximal runs of consecutive finest regions with one parent
     h->hFSegOs.assign(h->nF, 0);
     h->hFSegN.assign(h->nF, 0);
+    h->hPSegOs.assign(h->nF, 0);
+    h->hPSegN.assign(h->nF, 0);
     h->hSOs.clear();
     h->hSN.clear();
     h->hSParent.clear();
@@ -312,6 +319,17 @@ static int build_structure(mra_handle* h, const mra_tree_desc* t) {
         for (uint64_t g = f; g < e; g++) {
             h->hFSegOs[g] = os;
             h->hFSegN[g] = (uint32_t)n;
+        }
+        if (h->nPred) {
+            const uint64_t ps = h->hPredOff[f], pn = h->hPredOff[e] - ps;
+            if (pn > 0xFFFFFFFFull) {
+                set_err(h, "too many prediction points below one region");
+                return MRA_E_UNSUPPORTED;
+            }
+            for (uint64_t g = f; g < e; g++) {
+                h->hPSegOs[g] = ps;
+                h->hPSegN[g] = (uint32_t)pn;
+            }
         }
         h->hSOs.push_back(os);
         h->hSN.push_back((uint32_t)n);
@@ -547,7 +565,11 @@ static int alloc_engine(mra_handle* h, const mra_tree_desc* t) {
     }
     RC_TRY(upload(h, &h->dParent, h->hParent.data(), h->nF));
     RC_TRY(upload(h, &h->dObsOff, h->hObsOff.data(), h->nF + 1));
-    if (h->nPred) RC_TRY(upload(h, &h->dPredOff, h->hPredOff.data(), h->nF + 1));
+    if (h->nPred) {
+        RC_TRY(upload(h, &h->dPredOff, h->hPredOff.data(), h->nF + 1));
+        RC_TRY(upload(h, &h->dPSegOs, h->hPSegOs.data(), h->nF));
+        RC_TRY(upload(h, &h->dPSegN, h->hPSegN.data(), h->nF));
+    }
     RC_TRY(upload(h, &h->dChS, h->hChS.data(), h->hChS.size()));
     RC_TRY(upload(h, &h->dChE, h->hChE.data(), h->hChE.size()));
     RC_TRY(upload(h, &h->dWorking, h->hWorking.data(), h->nC));
@@ -645,6 +667,27 @@ static int alloc_engine(mra_handle* h, const mra_tree_desc* t) {
             }
         }
         h->predChunkStart.push_back((uint32_t)(ip.size() / 3));
+        {
+            // post-pass tiles per segment: 16 points for segments with at most
+            // 16, else 64
+            std::vector<uint32_t> p16, p64;
+            for (uint64_t f = 0; f < h->nF;) {
+                uint64_t e = f + 1;
+                while (e < h->nF && h->hParent[e] == h->hParent[f]) e++;
+                const uint32_t pn = h->hPSegN[f];
+                std::vector<uint32_t>& lst = pn <= 16 ? p16 : p64;
+                const uint32_t nt = pn <= 16 ? 16u : 64u;
+                for (uint32_t p0 = 0; p0 < pn; p0 += nt) {
+                    lst.push_back((uint32_t)f);
+                    lst.push_back(p0);
+                }
+                f = e;
+            }
+            h->nItemsPP16 = (uint32_t)(p16.size() / 2);
+            h->nItemsPP64 = (uint32_t)(p64.size() / 2);
+            RC_TRY(upload(h, &h->dItemsPP16, p16.data(), p16.size()));
+            RC_TRY(upload(h, &h->dItemsPP64, p64.data(), p64.size()));
+        }
         const uint32_t CH = std::max(CH16 * 16, CH64 * 64) / 64;  // scratch rows of 64 points
         // gl offsets indexed by slot within chunk: store per item, kernel reads by slot via chunk base
         RC_TRY(upload(h, &h->dItemsPred, ip.data(), ip.size()));
@@ -910,6 +953,8 @@ static Tree make_tree(mra_handle* h) {
     T.sig_off = h->dSigOff;
     T.f_seg_os = h->dFSegOs;
     T.f_seg_n = h->dFSegN;
+    T.p_seg_os = h->dPSegOs;
+    T.p_seg_n = h->dPSegN;
     T.s_os = h->dSOs;
     T.s_n = h->dSN;
     T.s_parent = h->dSParent;
@@ -1236,15 +1281,14 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
     }
     mark(MRA_T_PRED_POST);
     if (predicting) {
-        for (size_t c = 0; c + 1 < h->predChunkStart.size(); c++) {
-            const uint32_t b = h->predChunkStart[c], e = h->predChunkStart[c + 1];
-            if (e == b) continue;
-            pa.items = h->dItemsPred + 3 * (uint64_t)b;
-            pa.gl_off = h->dGlOff + b;
-            if (h->predChunkNT[c] == 16)
-                k_pred_post<RP, 16><<<e - b, 256, 0, st>>>(T, pa);
-            else
-                k_pred_post<RP, 64><<<e - b, 256, 0, st>>>(T, pa);
+        pa.gl_off = nullptr;
+        if (h->nItemsPP64) {
+            pa.items = h->dItemsPP64;
+            k_pred_post<RP, 64><<<h->nItemsPP64, 256, 0, st>>>(T, pa);
+        }
+        if (h->nItemsPP16) {
+            pa.items = h->dItemsPP16;
+            k_pred_post<RP, 16><<<h->nItemsPP16, 256, 0, st>>>(T, pa);
         }
     }
     mark(MRA_T_REDUCE);

@@ -42,6 +42,11 @@ struct Tree {
     const uint64_t* s_os;      // [nSeg] first observation of the segment
     const uint32_t* s_n;       // [nSeg]
     const uint64_t* s_parent;  // [nSeg] global coarse id of the shared parent
+    // prediction points of F's segment (the same runs of finest regions):
+    // B~ of a segment is stored together, element (m, c) at
+    // LRP*p_seg_os + m*p_seg_n + (c - p_seg_os)
+    const uint64_t* p_seg_os;  // [nF] first prediction point of F's segment
+    const uint32_t* p_seg_n;   // [nF] prediction points in F's segment
 };
 
 __device__ __forceinline__ uint64_t lvl_start(int l, int J) {
@@ -1357,7 +1362,11 @@ __global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts 
     const int nF = (int)(T.obs_off[F + 1] - ob);
     const int r = T.r, L = T.L, LRP = T.LRP;
     double* Gh = a.Gh + (uint64_t)slot * LRP * NT;
-    double* BtF = a.Bt + (uint64_t)LRP * pb;
+    // B~ of the segment: column c of this tile at BtS[m * pS + cb + c]
+    const uint64_t ps = T.p_seg_os[F];
+    const int pS = (int)T.p_seg_n[F];
+    double* BtS = a.Bt + (uint64_t)LRP * ps;
+    const int cb = (int)(pb - ps) + (int)p0;
     double sq = 0.0;  // this thread's part of sum_m G^(m, c)^2, c = threadIdx.x % NT
     for (int j = 0; j < L; j++) {
         const uint64_t an = ancestor_up(T.parent[F], L - 1, j, T.J);
@@ -1371,7 +1380,7 @@ __global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts 
         }
         for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
             const int i = idx / NT, c = idx % NT;
-            if (c < np) BtF[(uint64_t)(j * RP + i) * pF + p0 + c] = Cm[i * LD + c];
+            if (c < np) BtS[(uint64_t)(j * RP + i) * pS + cb + c] = Cm[i * LD + c];
         }
         wg_gemm_tn<false, true, double, double, GEMM_PART, false, GEMM_TRI_A>(Wm, LD, RP, NT, RP, 1.0,
                                                                              a.LiT + an * RP * RP, RP, RP, Cm, LD, NT);
@@ -1428,7 +1437,7 @@ __global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts 
         __syncthreads();
         // B~ = E - H^T G^_L for all L*RP rows at once, accumulated in place on
         // the stored E (rows m = j*RP + i, the tile's np valid columns)
-        wg_gemm_tn(BtF + p0, pF, LRP, NT, nF, -1.0, a.H + ob * LRP, LRP, LRP, Cm, LD, np);
+        wg_gemm_tn(BtS + cb, pS, LRP, NT, nF, -1.0, a.H + ob * LRP, LRP, LRP, Cm, LD, np);
         __syncthreads();
     } else if (nF > 0) {
         double* GL = a.GL + a.gl_off[slot];
@@ -1468,7 +1477,7 @@ __global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts 
         __syncthreads();
         // B~ = E - H^T G^_L for all L*RP rows at once, accumulated in place on
         // the stored E (rows m = j*RP + i, the tile's np valid columns)
-        wg_gemm_tn(BtF + p0, pF, LRP, NT, nF, -1.0, a.H + ob * LRP, LRP, LRP, GL, NT, np);
+        wg_gemm_tn(BtS + cb, pS, LRP, NT, nF, -1.0, a.H + ob * LRP, LRP, LRP, GL, NT, np);
         __syncthreads();
     }
     if (c < np) {
@@ -1481,12 +1490,14 @@ __global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts 
 //   Z_j = L~_{a_j}^{-1} (B~_j - sum_{j'>j} P_{a_j',j}^T Z_j'),
 //   mean += Z_j^T v_{a_j}, var += colsumsq(Z_j).
 template <int RP, int NT>
+// One tile of NT points of a segment (all finest regions below one parent
+// share every ancestor): items (F, p0), F any region of the segment.
 __global__ __launch_bounds__(256) void k_pred_post(Tree T, PredArgs a) {
     constexpr int LD = NT + 1;
     __shared__ double Zm[RP * LD];
-    const uint32_t F = a.items[3 * blockIdx.x], p0 = a.items[3 * blockIdx.x + 1];
-    const uint64_t pb = T.pred_off[F];
-    const int pF = (int)(T.pred_off[F + 1] - pb);
+    const uint32_t F = a.items[2 * blockIdx.x], p0 = a.items[2 * blockIdx.x + 1];
+    const uint64_t pb = T.p_seg_os[F];
+    const int pF = (int)T.p_seg_n[F];
     const int np = min(NT, pF - (int)p0);
     const int L = T.L, LRP = T.LRP;
     double* BtF = a.Bt + (uint64_t)LRP * pb;  // B~(m, c) at BtF[m * pF + c]
