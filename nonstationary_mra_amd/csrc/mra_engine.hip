@@ -688,11 +688,10 @@ static int alloc_engine(mra_handle* h, const mra_tree_desc* t) {
             RC_TRY(upload(h, &h->dItemsPP16, p16.data(), p16.size()));
             RC_TRY(upload(h, &h->dItemsPP64, p64.data(), p64.size()));
         }
-        const uint32_t CH = std::max(CH16 * 16, CH64 * 64) / 64;  // scratch rows of 64 points
-        // gl offsets indexed by slot within chunk: store per item, kernel reads by slot via chunk base
+        // GL offsets per item (the chunk's scratch, reused chunk after chunk)
         RC_TRY(upload(h, &h->dItemsPred, ip.data(), ip.size()));
         RC_TRY(upload(h, &h->dGlOff, gl.data(), gl.size()));
-        RC_TRY(dalloc(h, &h->dGh, (uint64_t)CH * h->LRP * 64, A));  // >= items x LRP x NT of any chunk
+        RC_TRY(dalloc(h, &h->dGh, (uint64_t)h->LRP * h->nPred, A));  // G^ of every point (segment layout)
         RC_TRY(dalloc(h, &h->dGL, glMax, A));
         RC_TRY(dalloc(h, &h->dBt, (uint64_t)h->LRP * h->nPred, A));
         RC_TRY(dalloc(h, &h->dMean, h->nPred, A));
@@ -1129,6 +1128,16 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
         pa.LpiT = h->dLpiT;
         pa.P = h->dP;
         pa.v = h->dV;
+        // the ancestor chain per segment tile, then each region's own part
+        pa.gl_off = nullptr;
+        if (h->nItemsPP64) {
+            pa.items = h->dItemsPP64;
+            k_pred_chain<RP, 64><<<h->nItemsPP64, 256, 0, st>>>(T, cc, K, Pp, pa);
+        }
+        if (h->nItemsPP16) {
+            pa.items = h->dItemsPP16;
+            k_pred_chain<RP, 16><<<h->nItemsPP16, 256, 0, st>>>(T, cc, K, Pp, pa);
+        }
         for (size_t c = 0; c + 1 < h->predChunkStart.size(); c++) {
             const uint32_t b = h->predChunkStart[c], e = h->predChunkStart[c + 1];
             if (e == b) continue;
@@ -1136,13 +1145,13 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
             pa.gl_off = h->dGlOff + b;
             const bool n16 = h->predChunkNT[c] == 16;
             if (mixed && n16)
-                k_pred_pre<RP, float, 16><<<e - b, 256, 0, st>>>(T, cc, K, O, Pp, pa);
+                k_pred_local<RP, float, 16><<<e - b, 256, 0, st>>>(T, cc, O, Pp, pa);
             else if (mixed)
-                k_pred_pre<RP, float, 64><<<e - b, 256, 0, st>>>(T, cc, K, O, Pp, pa);
+                k_pred_local<RP, float, 64><<<e - b, 256, 0, st>>>(T, cc, O, Pp, pa);
             else if (n16)
-                k_pred_pre<RP, double, 16><<<e - b, 256, 0, st>>>(T, cc, K, O, Pp, pa);
+                k_pred_local<RP, double, 16><<<e - b, 256, 0, st>>>(T, cc, O, Pp, pa);
             else
-                k_pred_pre<RP, double, 64><<<e - b, 256, 0, st>>>(T, cc, K, O, Pp, pa);
+                k_pred_local<RP, double, 64><<<e - b, 256, 0, st>>>(T, cc, O, Pp, pa);
         }
     }
     // posterior, finest-but-one level up to the root

@@ -1333,8 +1333,8 @@ struct PredArgs {
     const double* Sig;
     const double* H;
     const double* u;
-    double* Bt;    // per prediction point block: L*RP*pred_off[F], (m, p) at m*p_F + p
-    double* Gh;    // scratch: slot * L*RP*64
+    double* Bt;    // B~ per segment: (m, p) at L*RP*p_seg_os + m*p_seg_n + (p - p_seg_os)
+    double* Gh;    // G^ of the ancestor chain, the same layout
     double* GL;    // scratch: gl_off[slot] (n_F x 64)
     const uint64_t* gl_off;
     double* mean;
@@ -1345,145 +1345,139 @@ struct PredArgs {
     const double* v;
 };
 
-// NT: prediction points per tile (16 for regions with few points, else 64)
-template <int RP, typename TG, int NT>
-__global__ __launch_bounds__(256) void k_pred_pre(Tree T, CovCtx cc, Pts K, Pts O, Pts Pp, PredArgs a) {
+// Prediction pre-pass, part 1: the conditioning chain on the ancestors,
+// shared by all prediction points of a segment (the finest regions below one
+// parent), in tiles of NT points (items (F, p0), F any region of the
+// segment).  Per level j: E_j = C(Q_{a_j}, p) - G_{a_j}^T G^ (stored as the
+// rows of B~), G^_j = L_{a_j}^{-1} E_j (stored, segment layout), and
+// var = V(p) - sum_m G^(m, p)^2, mean = 0.
+// (src/class_approximation-predict.cpp:35-96)
+template <int RP, int NT>
+__global__ __launch_bounds__(256) void k_pred_chain(Tree T, CovCtx cc, Pts K, Pts Pp, PredArgs a) {
     constexpr int LD = NT + 1;
     __shared__ double Cm[RP * LD];
     __shared__ double Wm[RP * LD];
     __shared__ double stage[5 * RP];
     __shared__ double red[MRA_WG];
-    __shared__ double kred[NT == 16 ? 3 * 4 * 256 : 1];  // K-split partial tiles
-    const uint32_t F = a.items[3 * blockIdx.x], p0 = a.items[3 * blockIdx.x + 1], slot = a.items[3 * blockIdx.x + 2];
-    const uint64_t pb = T.pred_off[F];
-    const int pF = (int)(T.pred_off[F + 1] - pb);
-    const int np = min(NT, pF - (int)p0);
-    const uint64_t ob = T.obs_off[F];
-    const int nF = (int)(T.obs_off[F + 1] - ob);
-    const int r = T.r, L = T.L, LRP = T.LRP;
-    double* Gh = a.Gh + (uint64_t)slot * LRP * NT;
-    // B~ of the segment: column c of this tile at BtS[m * pS + cb + c]
+    const uint32_t F = a.items[2 * blockIdx.x], p0 = a.items[2 * blockIdx.x + 1];
     const uint64_t ps = T.p_seg_os[F];
     const int pS = (int)T.p_seg_n[F];
-    double* BtS = a.Bt + (uint64_t)LRP * ps;
-    const int cb = (int)(pb - ps) + (int)p0;
+    const int np = min(NT, pS - (int)p0);
+    const int r = T.r, L = T.L, LRP = T.LRP;
+    double* BtS = a.Bt + (uint64_t)LRP * ps + p0;  // (m, c) at BtS[m * pS + c]
+    double* GhS = a.Gh + (uint64_t)LRP * ps + p0;  // the same layout
     double sq = 0.0;  // this thread's part of sum_m G^(m, c)^2, c = threadIdx.x % NT
     for (int j = 0; j < L; j++) {
         const uint64_t an = ancestor_up(T.parent[F], L - 1, j, T.J);
         for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) Wm[(idx / NT) * LD + idx % NT] = 0.0;
-        cov_tile<RP, NT>(Cm, LD, cc, K, an * RP, r, Pp, pb + p0, np, false, stage);
+        cov_tile<RP, NT>(Cm, LD, cc, K, an * RP, r, Pp, ps + p0, np, false, stage);
         __syncthreads();
         if (j > 0) {
             const double* Ga = a.Gc + a.goff[j] + (an - lvl_start(j, T.J)) * (uint64_t)j * RP * RP;
-            wg_gemm_tn(Cm, LD, RP, NT, j * RP, -1.0, Ga, RP, RP, Gh, NT, NT);
+            wg_gemm_tn(Cm, LD, RP, NT, j * RP, -1.0, Ga, RP, RP, GhS, pS, np);
             __syncthreads();
         }
         for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
             const int i = idx / NT, c = idx % NT;
-            if (c < np) BtS[(uint64_t)(j * RP + i) * pS + cb + c] = Cm[i * LD + c];
+            if (c < np) BtS[(uint64_t)(j * RP + i) * pS + c] = Cm[i * LD + c];
         }
         wg_gemm_tn<false, true, double, double, GEMM_PART, false, GEMM_TRI_A>(Wm, LD, RP, NT, RP, 1.0,
                                                                              a.LiT + an * RP * RP, RP, RP, Cm, LD, NT);
         __syncthreads();
         for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
-            const double w = Wm[(idx / NT) * LD + idx % NT];
-            Gh[(uint64_t)(j * RP + idx / NT) * NT + idx % NT] = w;
+            const int i = idx / NT, c = idx % NT;
+            const double w = Wm[i * LD + c];
+            if (c < np) GhS[(uint64_t)(j * RP + i) * pS + c] = w;
             sq += w * w;
         }
         __syncthreads();
     }
-    // var0 and the conditional cross block with the region's observations
     red[threadIdx.x] = sq;
     __syncthreads();
-    double var0 = 0.0, mean0 = 0.0;
     const int c = threadIdx.x;
     if (c < np) {
         double s = 0.0;
         for (int q = c; q < MRA_WG; q += NT) s += red[q];
-        var0 = Pp.v[pb + p0 + c] - s;
+        a.var[ps + p0 + c] = Pp.v[ps + p0 + c] - s;
+        a.mean[ps + p0 + c] = 0.0;
     }
-    if (nF > 0 && nF <= RP) {
-        // one row tile: G^_L stays in LDS (Cm) for the solve and the product
+}
+
+// Prediction pre-pass, part 2, per finest region F with observations, in
+// tiles of NT of its points (items (F, p0, scratch slot)): the conditional
+// cross block with F's observations G^_L = L_F^{-1} (C(S_F, p) - G_F^T G^),
+// mean = u^T G^_L, var -= |G^_L|^2, and B~ = E - H^T G^_L in place.
+// (src/class_approximation-predict.cpp:97-122)
+template <int RP, typename TG, int NT>
+__global__ __launch_bounds__(256) void k_pred_local(Tree T, CovCtx cc, Pts O, Pts Pp, PredArgs a) {
+    constexpr int LD = NT + 1;
+    __shared__ double Cm[RP * LD];
+    __shared__ double stage[5 * RP];
+    __shared__ double kred[NT == 16 ? 3 * 4 * 256 : 1];  // K-split partial tiles
+    const uint32_t F = a.items[3 * blockIdx.x], p0 = a.items[3 * blockIdx.x + 1], slot = a.items[3 * blockIdx.x + 2];
+    const uint64_t ob = T.obs_off[F];
+    const int nF = (int)(T.obs_off[F + 1] - ob);
+    if (nF == 0) return;
+    const uint64_t pb = T.pred_off[F];
+    const int pF = (int)(T.pred_off[F + 1] - pb);
+    const int np = min(NT, pF - (int)p0);
+    const int LRP = T.LRP;
+    const uint64_t ps = T.p_seg_os[F];
+    const int pS = (int)T.p_seg_n[F];
+    const uint64_t cb = (pb - ps) + p0;                 // the tile's first column in the segment
+    double* BtS = a.Bt + (uint64_t)LRP * ps + cb;       // (m, c) at BtS[m * pS + c]
+    const double* GhS = a.Gh + (uint64_t)LRP * ps + cb;
+    const uint64_t so = T.f_seg_os[F];
+    const int sn = (int)T.f_seg_n[F];
+    const TG* GF = (const TG*)a.Gf + (uint64_t)LRP * so + (ob - so);
+    const double* LF = a.Sig + T.sig_off[F];
+    const int c = threadIdx.x;
+    // G^_L in Cm (one row tile) or in the scratch GL (ld NT)
+    double* GL = nF <= RP ? Cm : a.GL + a.gl_off[slot];
+    const int ldg = nF <= RP ? LD : NT;
+    if (nF <= RP) {
         cov_tile<RP, NT>(Cm, LD, cc, O, ob, nF, Pp, pb + p0, np, false, stage);
         __syncthreads();
-        const uint64_t so = T.f_seg_os[F];
-        const int sn = (int)T.f_seg_n[F];
-        const TG* GF = (const TG*)a.Gf + (uint64_t)LRP * so + (ob - so);
         if (NT == 16 && nF <= 32)  // at most 2 x 1 output tiles: K over the four waves
-            wg_gemm_tn_ksplit<TG, double>(Cm, LD, (nF + 15) & ~15, NT, LRP, -1.0, GF, sn, nF, Gh, NT, NT, kred);
+            wg_gemm_tn_ksplit<TG, double>(Cm, LD, (nF + 15) & ~15, NT, LRP, -1.0, GF, sn, nF, GhS, pS, np, kred);
         else
-            wg_gemm_tn(Cm, LD, (nF + 15) & ~15, NT, LRP, -1.0, GF, sn, nF, Gh, NT, NT);
+            wg_gemm_tn(Cm, LD, (nF + 15) & ~15, NT, LRP, -1.0, GF, sn, nF, GhS, pS, np);
         __syncthreads();
-        // Cm <- L_F^{-1} Cm, one column per thread
-        const double* LF = a.Sig + T.sig_off[F];
-        if (c < NT) {
-            for (int i = 0; i < nF; i++) {
-                double s = Cm[i * LD + c];
-                for (int k = 0; k < i; k++) s -= LF[(uint64_t)i * nF + k] * Cm[k * LD + c];
-                s /= LF[(uint64_t)i * nF + i];
-                Cm[i * LD + c] = s;
-            }
-            if (c < np) {
-                double sv = 0.0, sm = 0.0;
-                for (int i = 0; i < nF; i++) {
-                    const double g = Cm[i * LD + c];
-                    sv += g * g;
-                    sm += g * a.u[ob + i];
-                }
-                var0 -= sv;
-                mean0 = sm;
-            }
-        }
-        __syncthreads();
-        // B~ = E - H^T G^_L for all L*RP rows at once, accumulated in place on
-        // the stored E (rows m = j*RP + i, the tile's np valid columns)
-        wg_gemm_tn(BtS + cb, pS, LRP, NT, nF, -1.0, a.H + ob * LRP, LRP, LRP, Cm, LD, np);
-        __syncthreads();
-    } else if (nF > 0) {
-        double* GL = a.GL + a.gl_off[slot];
-        const uint64_t so = T.f_seg_os[F];
-        const int sn = (int)T.f_seg_n[F];
-        const TG* GF = (const TG*)a.Gf + (uint64_t)LRP * so + (ob - so);
+    } else {
         for (int t0 = 0; t0 < nF; t0 += RP) {  // row tiles of RP (the tile height of Cm)
             const int mv = min(RP, nF - t0);
             cov_tile<RP, NT>(Cm, LD, cc, O, ob + t0, mv, Pp, pb + p0, np, false, stage);
             __syncthreads();
-            wg_gemm_tn(Cm, LD, (mv + 15) & ~15, NT, LRP, -1.0, GF + t0, sn, mv, Gh, NT, NT);
+            wg_gemm_tn(Cm, LD, (mv + 15) & ~15, NT, LRP, -1.0, GF + t0, sn, mv, GhS, pS, np);
             __syncthreads();
             for (int idx = threadIdx.x; idx < mv * NT; idx += MRA_WG)
                 GL[(uint64_t)(t0 + idx / NT) * NT + idx % NT] = Cm[(idx / NT) * LD + idx % NT];
             __syncthreads();
         }
-        // GL <- L_F^{-1} GL, one column per thread
-        const double* LF = a.Sig + T.sig_off[F];
-        if (c < NT) {
-            for (int i = 0; i < nF; i++) {
-                double s = GL[(uint64_t)i * NT + c];
-                for (int k = 0; k < i; k++) s -= LF[(uint64_t)i * nF + k] * GL[(uint64_t)k * NT + c];
-                s /= LF[(uint64_t)i * nF + i];
-                GL[(uint64_t)i * NT + c] = s;
-            }
-            if (c < np) {
-                double sv = 0.0, sm = 0.0;
-                for (int i = 0; i < nF; i++) {
-                    const double g = GL[(uint64_t)i * NT + c];
-                    sv += g * g;
-                    sm += g * a.u[ob + i];
-                }
-                var0 -= sv;
-                mean0 = sm;
-            }
+    }
+    // G^_L <- L_F^{-1} G^_L, one column per thread
+    if (c < NT) {
+        for (int i = 0; i < nF; i++) {
+            double s = GL[(uint64_t)i * ldg + c];
+            for (int k = 0; k < i; k++) s -= LF[(uint64_t)i * nF + k] * GL[(uint64_t)k * ldg + c];
+            s /= LF[(uint64_t)i * nF + i];
+            GL[(uint64_t)i * ldg + c] = s;
         }
-        __syncthreads();
-        // B~ = E - H^T G^_L for all L*RP rows at once, accumulated in place on
-        // the stored E (rows m = j*RP + i, the tile's np valid columns)
-        wg_gemm_tn(BtS + cb, pS, LRP, NT, nF, -1.0, a.H + ob * LRP, LRP, LRP, GL, NT, np);
-        __syncthreads();
+        if (c < np) {
+            double sv = 0.0, sm = 0.0;
+            for (int i = 0; i < nF; i++) {
+                const double g = GL[(uint64_t)i * ldg + c];
+                sv += g * g;
+                sm += g * a.u[ob + i];
+            }
+            a.var[pb + p0 + c] -= sv;
+            a.mean[pb + p0 + c] = sm;
+        }
     }
-    if (c < np) {
-        a.mean[pb + p0 + c] = mean0;
-        a.var[pb + p0 + c] = var0;
-    }
+    __syncthreads();
+    // B~ = E - H^T G^_L for all L*RP rows at once, accumulated in place on
+    // the stored E (rows m = j*RP + i, the tile's np valid columns)
+    wg_gemm_tn(BtS, pS, LRP, NT, nF, -1.0, a.H + ob * LRP, LRP, LRP, GL, ldg, np);
 }
 
 // Top-down posterior prediction (src/class_approximation-predict.cpp:123-188):
