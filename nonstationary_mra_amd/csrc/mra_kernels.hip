@@ -1321,11 +1321,12 @@ __global__ __launch_bounds__(256) void k_post_lowrank(Tree T, LowRankArgs a) {
 }
 
 // ------------------------------------------------------------- prediction
-// Pre-pass per (finest region, tile of 64 prediction points)
-// (…finest-level-in-prior.cpp:271-392): G^_j, E_j (into Bt), var0, mean0, and
-// B~_j = E_j - H_j^T G^_L.
+// Pre-pass (…finest-level-in-prior.cpp:271-386) in two launches: the
+// ancestor chain per tile of a segment's points (G^_j, E_j into Bt, var0) and
+// each finest region's own part per tile of its points (G^_L, mean0, var0,
+// B~_j = E_j - H_j^T G^_L); then the top-down post-pass per segment tile.
 struct PredArgs {
-    const uint32_t* items;  // (F, p0, scratch slot) triples
+    const uint32_t* items;  // (F, p0) pairs or, for k_pred_local, (F, p0, scratch slot) triples
     const uint64_t* goff;
     const double* Gc;
     const double* LiT;
@@ -1351,7 +1352,7 @@ struct PredArgs {
 // segment).  Per level j: E_j = C(Q_{a_j}, p) - G_{a_j}^T G^ (stored as the
 // rows of B~), G^_j = L_{a_j}^{-1} E_j (stored, segment layout), and
 // var = V(p) - sum_m G^(m, p)^2, mean = 0.
-// (src/class_approximation-predict.cpp:35-96)
+// (…finest-level-in-prior.cpp:271-317)
 template <int RP, int NT>
 __global__ __launch_bounds__(256) void k_pred_chain(Tree T, CovCtx cc, Pts K, Pts Pp, PredArgs a) {
     constexpr int LD = NT + 1;
@@ -1407,7 +1408,7 @@ __global__ __launch_bounds__(256) void k_pred_chain(Tree T, CovCtx cc, Pts K, Pt
 // tiles of NT of its points (items (F, p0, scratch slot)): the conditional
 // cross block with F's observations G^_L = L_F^{-1} (C(S_F, p) - G_F^T G^),
 // mean = u^T G^_L, var -= |G^_L|^2, and B~ = E - H^T G^_L in place.
-// (src/class_approximation-predict.cpp:97-122)
+// (…finest-level-in-prior.cpp:318-386)
 template <int RP, typename TG, int NT>
 __global__ __launch_bounds__(256) void k_pred_local(Tree T, CovCtx cc, Pts O, Pts Pp, PredArgs a) {
     constexpr int LD = NT + 1;
@@ -1483,9 +1484,9 @@ __global__ __launch_bounds__(256) void k_pred_local(Tree T, CovCtx cc, Pts O, Pt
 // Top-down posterior prediction (src/class_approximation-predict.cpp:123-188):
 //   Z_j = L~_{a_j}^{-1} (B~_j - sum_{j'>j} P_{a_j',j}^T Z_j'),
 //   mean += Z_j^T v_{a_j}, var += colsumsq(Z_j).
-template <int RP, int NT>
 // One tile of NT points of a segment (all finest regions below one parent
 // share every ancestor): items (F, p0), F any region of the segment.
+template <int RP, int NT>
 __global__ __launch_bounds__(256) void k_pred_post(Tree T, PredArgs a) {
     constexpr int LD = NT + 1;
     __shared__ double Zm[RP * LD];
