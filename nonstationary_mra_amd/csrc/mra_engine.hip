@@ -973,7 +973,7 @@ static Pts pts(double* const* c, double* const* s) { return Pts{c[0], c[1], c[2]
 
 // Side streams and their events, created on first use (default priority).
 static int ensure_side(mra_handle* h) {
-    if (!h->concurrent || h->noFork || h->side[1]) return MRA_OK;
+    if (!h->concurrent || h->noFork || h->evJoin[mra_handle::NFORK - 1][1]) return MRA_OK;
     for (auto& s : h->side)
         if (!s) HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     for (int i = 0; i < mra_handle::NFORK; i++) {
@@ -991,7 +991,8 @@ struct Fork {
     mra_handle* h;
     int k;
     bool on;
-    Fork(mra_handle* h_, int k_) : h(h_), k(k_), on(h_->concurrent && !h_->noFork && h_->side[1]) {
+    Fork(mra_handle* h_, int k_)
+        : h(h_), k(k_), on(h_->concurrent && !h_->noFork && h_->evJoin[mra_handle::NFORK - 1][1]) {
         if (!on) return;
         (void)hipEventRecord(h->evFork[k], h->stream);
         for (auto s : h->side) (void)hipStreamWaitEvent(s, h->evFork[k], 0);
