@@ -267,7 +267,8 @@ int mra_fit_linear_trend(mra_handle* h, int metric, double* theta, const double*
 /* Work-list shape of a handle (no reference counterpart):
  *   x32max: segments (observations below one finest-but-one region) with at
  *           most x32max observations use 32-column tiles in the finest
- *           conditioning kernel, larger ones 64-column tiles (0, 32 or 64);
+ *           conditioning kernel, larger ones 64-column tiles (0, 32 or 64;
+ *           -1: every segment uses 32-column tiles);
  *   lrmax:  finest-but-one regions with at most lrmax rows use the factored
  *           posterior, larger ones the dense update (64 or 128);
  *   fb16:   1 = finest regions of at most 16 observations have their own

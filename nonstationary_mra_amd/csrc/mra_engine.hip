@@ -400,7 +400,8 @@ static int build_worklists(mra_handle* h) {
     for (size_t sgi = 0; sgi < h->hSN.size(); sgi++) {
         // tiles of 64 (segments above tuneX32) or 32 columns; with cross16 the
         // last tile is the narrowest of 16, 32, 64 columns that holds the rest
-        const uint32_t n = h->hSN[sgi], w = n <= (uint32_t)h->tuneX32 ? 32u : 64u;
+        const uint32_t n = h->hSN[sgi];
+        const uint32_t w = (h->tuneX32 < 0 || n <= (uint32_t)h->tuneX32) ? 32u : 64u;
         for (uint32_t c0 = 0; c0 < n;) {
             const uint32_t rest = n - c0;
             const uint32_t t = !h->cross16 ? w : rest > 32 ? w : rest > 16 ? 32u : 16u;
@@ -1974,7 +1975,7 @@ extern "C" int mra_batch_autotune(mra_batch* b, int* streams, int* graph, double
 
 // ----------------------------------------------------------------- tuning
 extern "C" int mra_set_tuning(mra_handle* h, int x32max, int lrmax, int fb16) {
-    if (!h || (x32max != 0 && x32max != 32 && x32max != 64) || (lrmax != 64 && lrmax != 128) ||
+    if (!h || (x32max != 0 && x32max != 32 && x32max != 64 && x32max != -1) || (lrmax != 64 && lrmax != 128) ||
         (fb16 != 0 && fb16 != 1))
         return MRA_E_ARG;
     h->tuneX32 = x32max;
@@ -1997,7 +1998,7 @@ extern "C" int mra_get_tuning(mra_handle* h, int* x32max, int* lrmax, int* fb16)
 // work-list shape and keeps the fastest.
 extern "C" int mra_autotune(mra_handle* h, int* x32max, int* lrmax, int* fb16, double* best_ms) {
     if (!h) return MRA_E_ARG;
-    const int cx[] = {0, 32, 64}, cl[] = {64, 128}, cf[] = {0, 1};
+    const int cx[] = {0, 32, 64, -1}, cl[] = {64, 128}, cf[] = {0, 1};
     double best = 1e300;
     int bx = h->tuneX32, bl = h->tuneLr, bf = h->tuneFb16;
     const bool graph = h->useGraph;

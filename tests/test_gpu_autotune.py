@@ -30,7 +30,7 @@ def test_every_shape_matches_oracle_and_autotune_keeps_results():
         h.set_params(**prm)
         assert h.get_tuning() == (32, 128, 1)
         D0, m0, v0 = h.predict()
-        for x32, lr, fb in itertools.product((0, 32, 64), (64, 128), (0, 1)):
+        for x32, lr, fb in itertools.product((0, 32, 64, -1), (64, 128), (0, 1)):
             h.set_tuning(x32, lr, fb)
             D, m, v = h.predict()
             print(x32, lr, fb, abs(D - Do) / abs(Do), flush=True)
@@ -39,7 +39,7 @@ def test_every_shape_matches_oracle_and_autotune_keeps_results():
             np.testing.assert_allclose(v, vo, rtol=1e-9, atol=1e-10)
         cfg, ms = h.autotune()
         print("autotune", cfg, ms, flush=True)
-        assert cfg[0] in (0, 32, 64) and cfg[1] in (64, 128) and cfg[2] in (0, 1) and ms > 0
+        assert cfg[0] in (0, 32, 64, -1) and cfg[1] in (64, 128) and cfg[2] in (0, 1) and ms > 0
         assert h.get_tuning() == cfg
         D1 = h.likelihood()
         assert abs(D1 - D0) / abs(D0) < 1e-12
