@@ -364,16 +364,23 @@ def main():
     }
     if args.precision == "mixed" and not args.no_fp64:
         h.set_precision(capi.PRECISION_FP64)
+        # the FP64 path has its own best work-list shape (tile costs differ)
+        tune64 = tune_cfg
+        if not args.tuning and not args.no_autotune:
+            tune64, _ = h.autotune()
+            log(f"autotune fp64 {tune64}")
         h.use_graph(graph)
         v64, ms64, D64 = timed(h, args.steps, args.warmup)
         log(f"fp64: {ms64:.2f} ms/eval D={D64:.10e}")
-        out["fp64_gpu"] = {"value": v64, "ms_per_step": ms64, "D": D64}
+        out["fp64_gpu"] = {"value": v64, "ms_per_step": ms64, "D": D64,
+                           "worklist_shape": dict(zip(("x32max", "lrmax", "fb16"), tune64))}
         out["mixed_vs_fp64"] = {"speedup": value / v64, "abs_dloglik_per_n": abs(D - D64) / (2 * n)}
         if args.npred:
             t1 = time.perf_counter()
             _, m64, s64 = h.predict()
             tp64 = time.perf_counter() - t1
             h.set_precision(capi.PRECISION_MIXED)
+            h.set_tuning(*tune_cfg)
             h.predict()
             t1 = time.perf_counter()
             _, mm, sm = h.predict()
