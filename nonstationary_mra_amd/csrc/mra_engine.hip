@@ -103,9 +103,9 @@ struct mra_handle {
     uint32_t nItemsX32 = 0;
     uint32_t* dItemsX16 = nullptr;  // segments of at most 16 observations (16-column tiles)
     uint32_t nItemsX16 = 0;
-    // finest regions by length bucket: n <= 16, 32, 64, 128, larger
+    // finest regions by length bucket: n <= 16, 32, 64, (3: unused), 256, larger
     static constexpr int NBK = 6;
-    uint64_t* dWsOffBig = nullptr;  // workspace offset of each region of the 128 < n <= 256 bucket
+    uint64_t* dWsOffBig = nullptr;  // workspace offset of each region of the 64 < n <= 256 bucket
     double* dWsBig = nullptr;       // padded 16 ceil(n/16) squared workspace per such region
     uint32_t* dItemsFB[NBK] = {};
     uint32_t nItemsFB[NBK] = {};
@@ -421,13 +421,15 @@ static int build_worklists(mra_handle* h) {
         std::vector<uint32_t> b[mra_handle::NBK];
         for (uint32_t f : iF) {
             const uint64_t n = h->hObsOff[f + 1] - h->hObsOff[f];
-            b[(n <= 16 && h->tuneFb16) ? 0 : n <= 32 ? 1 : n <= 64 ? 2 : n <= 128 ? 3 : n <= 256 ? 4 : 5].push_back(f);
+            // (64 < n <= 256: the workspace kernel; its low LDS use keeps
+            // several work groups per CU where bucket 3's 134 KB kept one)
+            b[(n <= 16 && h->tuneFb16) ? 0 : n <= 32 ? 1 : n <= 64 ? 2 : n <= 256 ? 4 : 5].push_back(f);
         }
         for (int k = 0; k < mra_handle::NBK; k++) {
             h->nItemsFB[k] = (uint32_t)b[k].size();
             RC_TRY(upload_wl(h, &h->dItemsFB[k], b[k].data(), b[k].size()));
         }
-        // padded workspace of the 128 < n <= 256 bucket
+        // padded workspace of the 64 < n <= 256 bucket
         std::vector<uint64_t> wo(b[4].size() + 1, 0);
         for (size_t k = 0; k < b[4].size(); k++) {
             const uint64_t n = h->hObsOff[b[4][k] + 1] - h->hObsOff[b[4][k]];
@@ -1024,9 +1026,6 @@ static void launch_finest_local(mra_handle* h, const Tree& T, const CovCtx& cc, 
         k_finest_local_big<TG><<<h->nItemsFB[4], 256, 0, s1>>>(T, cc, O, h->dY, h->dItemsFB[4], h->dWsOffBig,
                                                                h->dWsBig, Gf, Bf, h->dSig, h->dH, h->dU, h->dDterm,
                                                                h->dBad);
-    if (h->nItemsFB[3])
-        k_finest_local<RP, 128, TG><<<h->nItemsFB[3], 256, 0, s1>>>(T, cc, O, h->dY, h->dItemsFB[3], Gf, Bf, h->dSig,
-                                                                    h->dH, h->dU, h->dDterm, h->dBad);
     if (h->nItemsFB[2])
         k_finest_local<RP, 64, TG><<<h->nItemsFB[2], 256, 0, s2>>>(T, cc, O, h->dY, h->dItemsFB[2], Gf, Bf, h->dSig,
                                                                    h->dH, h->dU, h->dDterm, h->dBad);

@@ -561,7 +561,7 @@ __global__ __launch_bounds__(256) void k_finest_local(Tree T, CovCtx cc, Pts O, 
     if (threadIdx.x == 0 && bad) atomicOr(bad_flag, 1);
 }
 
-// The same local step for finest regions with 128 < n_F <= 256 observations,
+// The same local step for finest regions with 64 < n_F <= 256 observations,
 // on a padded N16 x N16 workspace in global memory (ws + ws_off[item], row
 // stride N16 = 16 ceil(n_F / 16), identity padding): Sigma_F = V(S) - G^T G
 // on the lower 64 x 64 tiles (MFMA), blocked Cholesky and inverse on 16 x 16

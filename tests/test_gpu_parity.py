@@ -319,7 +319,7 @@ def test_small_segment_cross_tiles(cross16):
 
 @pytest.mark.parametrize("r", [16, 32])
 def test_large_region_local_kernel(r):
-    """Finest regions of 129 to 256 observations (padded workspace, blocked
+    """Finest regions of 65 to 256 observations (padded workspace, blocked
     Cholesky and inverse of up to 16 blocks) and above 256 (generic path):
     D and predictions match the oracle in FP64, mixed within tolerance."""
     rng = np.random.default_rng(40 + r)
