@@ -54,7 +54,7 @@ struct mra_handle {
     // forked from and joined back to `stream` by events (also under capture)
     int concurrent = 1;
     bool noFork = false;  // handles of a batch: the batch runs fields concurrently instead
-    static constexpr int NFORK = 4;
+    static constexpr int NFORK = 3;  // fork points of one evaluation
     hipStream_t side[2] = {};
     hipEvent_t evFork[NFORK] = {}, evJoin[NFORK][2] = {};
     int cross16 = 1;      // segments with <= 16 observations in 16-column cross tiles
