@@ -634,10 +634,10 @@ static int alloc_engine(mra_handle* h, const mra_tree_desc* t) {
         RC_TRY(dalloc(h, &h->dA[l1], (uint64_t)std::max<uint32_t>(cap, 1) * (uint64_t)(l1 * (l1 + 1) / 2) * RP2, A));
     }
     if (h->nPred) {
-        // prediction items (F, p0, slot) in chunks bounding the scratch:
-        // regions with at most 16 prediction points use 16-point tiles
-        // (chunks of CH16 items), larger ones 64-point tiles (CH64)
-        const uint32_t CH16 = 16384, CH64 = 4096;
+        // items (F, p0, slot) of the per-region part of the pre-pass, for the
+        // regions with observations: regions with at most 16 prediction points
+        // use 16-point tiles, larger ones 64-point tiles (one launch each);
+        // regions with more than RP observations keep G^_L in the scratch GL
         std::vector<uint32_t> ip;
         std::vector<uint64_t> gl;
         uint64_t glMax = 1;
@@ -645,27 +645,21 @@ static int alloc_engine(mra_handle* h, const mra_tree_desc* t) {
         h->predChunkNT.clear();
         for (int pass = 0; pass < 2; pass++) {
             const int NT = pass == 0 ? 16 : 64;
-            const uint32_t CH = pass == 0 ? CH16 : CH64;
-            uint32_t inChunk = CH;
+            uint32_t slot = 0;
             uint64_t glCur = 0;
+            h->predChunkStart.push_back((uint32_t)(ip.size() / 3));
+            h->predChunkNT.push_back(NT);
             for (uint64_t f = 0; f < h->nF; f++) {
                 const uint64_t p = h->hPredOff[f + 1] - h->hPredOff[f];
                 const uint64_t n = h->hObsOff[f + 1] - h->hObsOff[f];
-                if (p == 0 || (p <= 16) != (pass == 0)) continue;
+                if (p == 0 || n == 0 || (p <= 16) != (pass == 0)) continue;
                 for (uint64_t p0 = 0; p0 < p; p0 += NT) {
-                    if (inChunk == CH) {
-                        h->predChunkStart.push_back((uint32_t)(ip.size() / 3));
-                        h->predChunkNT.push_back(NT);
-                        inChunk = 0;
-                        glCur = 0;
-                    }
                     ip.push_back((uint32_t)f);
                     ip.push_back((uint32_t)p0);
-                    ip.push_back(inChunk);
+                    ip.push_back(slot++);
                     gl.push_back(glCur);
-                    glCur += n * NT;
+                    if (n > (uint64_t)RP) glCur += n * NT;
                     glMax = std::max(glMax, glCur);
-                    inChunk++;
                 }
             }
         }
@@ -691,7 +685,7 @@ static int alloc_engine(mra_handle* h, const mra_tree_desc* t) {
             RC_TRY(upload(h, &h->dItemsPP16, p16.data(), p16.size()));
             RC_TRY(upload(h, &h->dItemsPP64, p64.data(), p64.size()));
         }
-        // GL offsets per item (the chunk's scratch, reused chunk after chunk)
+        // GL offsets per item (the two launches run in order and share GL)
         RC_TRY(upload(h, &h->dItemsPred, ip.data(), ip.size()));
         RC_TRY(upload(h, &h->dGlOff, gl.data(), gl.size()));
         RC_TRY(dalloc(h, &h->dGh, (uint64_t)h->LRP * h->nPred, A));  // G^ of every point (segment layout)
