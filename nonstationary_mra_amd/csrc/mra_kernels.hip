@@ -1370,7 +1370,6 @@ __global__ __launch_bounds__(256) void k_pred_chain(Tree T, CovCtx cc, Pts K, Pt
     double sq = 0.0;  // this thread's part of sum_m G^(m, c)^2, c = threadIdx.x % NT
     for (int j = 0; j < L; j++) {
         const uint64_t an = ancestor_up(T.parent[F], L - 1, j, T.J);
-        for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) Wm[(idx / NT) * LD + idx % NT] = 0.0;
         cov_tile<RP, NT>(Cm, LD, cc, K, an * RP, r, Pp, ps + p0, np, false, stage);
         __syncthreads();
         if (j > 0) {
@@ -1382,8 +1381,9 @@ __global__ __launch_bounds__(256) void k_pred_chain(Tree T, CovCtx cc, Pts K, Pt
             const int i = idx / NT, c = idx % NT;
             if (c < np) BtS[(uint64_t)(j * RP + i) * pS + c] = Cm[i * LD + c];
         }
-        wg_gemm_tn<false, true, double, double, GEMM_PART, false, GEMM_TRI_A>(Wm, LD, RP, NT, RP, 1.0,
-                                                                             a.LiT + an * RP * RP, RP, RP, Cm, LD, NT);
+        // Wm = L^{-1} Cm (every element stored: no clearing needed)
+        wg_gemm_tn<false, false, double, double, GEMM_FULL, false, GEMM_TRI_A>(Wm, LD, RP, NT, RP, 1.0,
+                                                                              a.LiT + an * RP * RP, RP, RP, Cm, LD, NT);
         __syncthreads();
         for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
             const int i = idx / NT, c = idx % NT;
