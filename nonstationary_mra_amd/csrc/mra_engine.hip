@@ -35,11 +35,16 @@ struct mra_handle {
     std::vector<uint64_t> lvlStart;  // L+1 entries
     std::vector<uint64_t> hObsOff, hPredOff, hParent, hChS, hChE, hSigOff, hGoff;
     std::vector<uint8_t> hWorking;
-    std::vector<uint64_t> hFSegOs, hSOs, hSParent, hPSegOs;
-    std::vector<uint32_t> hPSegN;
+    std::vector<uint64_t> hFSegOs, hSOs, hSParent, hPSegOs, hPGrpOs;
+    std::vector<uint32_t> hPSegN, hPGrpN;
     std::vector<uint32_t> hFSegN, hSN;
     uint64_t *dFSegOs = nullptr, *dSOs = nullptr, *dSParent = nullptr, *dPSegOs = nullptr;
     uint32_t* dPSegN = nullptr;
+    uint64_t* dPGrpOs = nullptr;
+    uint32_t* dPGrpN = nullptr;
+    // prediction chain items (first region of a group, p0): 16-point tiles of a group
+    uint32_t* dItemsPG = nullptr;
+    uint32_t nItemsPG = 0;
     // prediction post-pass items (F, p0): tiles of the points of one segment
     uint32_t *dItemsPP16 = nullptr, *dItemsPP64 = nullptr;
     uint32_t nItemsPP16 = 0, nItemsPP64 = 0;
@@ -305,6 +310,26 @@ static int build_structure(mra_handle* h, const mra_tree_desc* t) {
     h->hFSegN.assign(h->nF, 0);
     h->hPSegOs.assign(h->nF, 0);
     h->hPSegN.assign(h->nF, 0);
+    h->hPGrpOs.assign(h->nF, 0);
+    h->hPGrpN.assign(h->nF, 0);
+    if (h->nPred) {
+        // prediction groups: maximal runs of finest regions with one grandparent
+        auto gp = [&](uint64_t f) { const uint64_t p = h->hParent[f]; return p > 0 ? (p - 1) / (uint64_t)h->J : 0; };
+        for (uint64_t f = 0; f < h->nF;) {
+            uint64_t e = f + 1;
+            while (e < h->nF && gp(e) == gp(f)) e++;
+            const uint64_t ps = h->hPredOff[f], pn = h->hPredOff[e] - ps;
+            if (pn > 0xFFFFFFFFull) {
+                set_err(h, "too many prediction points below one region");
+                return MRA_E_UNSUPPORTED;
+            }
+            for (uint64_t g = f; g < e; g++) {
+                h->hPGrpOs[g] = ps;
+                h->hPGrpN[g] = (uint32_t)pn;
+            }
+            f = e;
+        }
+    }
     h->hSOs.clear();
     h->hSN.clear();
     h->hSParent.clear();
@@ -572,6 +597,8 @@ static int alloc_engine(mra_handle* h, const mra_tree_desc* t) {
         RC_TRY(upload(h, &h->dPredOff, h->hPredOff.data(), h->nF + 1));
         RC_TRY(upload(h, &h->dPSegOs, h->hPSegOs.data(), h->nF));
         RC_TRY(upload(h, &h->dPSegN, h->hPSegN.data(), h->nF));
+        RC_TRY(upload(h, &h->dPGrpOs, h->hPGrpOs.data(), h->nF));
+        RC_TRY(upload(h, &h->dPGrpN, h->hPGrpN.data(), h->nF));
     }
     RC_TRY(upload(h, &h->dChS, h->hChS.data(), h->hChS.size()));
     RC_TRY(upload(h, &h->dChE, h->hChE.data(), h->hChE.size()));
@@ -684,6 +711,20 @@ static int alloc_engine(mra_handle* h, const mra_tree_desc* t) {
             h->nItemsPP64 = (uint32_t)(p64.size() / 2);
             RC_TRY(upload(h, &h->dItemsPP16, p16.data(), p16.size()));
             RC_TRY(upload(h, &h->dItemsPP64, p64.data(), p64.size()));
+            // chain tiles: 16 points of a group, from the group's first region
+            std::vector<uint32_t> pgi;
+            auto gp = [&](uint64_t f) { const uint64_t p = h->hParent[f]; return p > 0 ? (p - 1) / (uint64_t)h->J : 0; };
+            for (uint64_t f = 0; f < h->nF;) {
+                uint64_t e = f + 1;
+                while (e < h->nF && gp(e) == gp(f)) e++;
+                for (uint32_t p0 = 0; p0 < h->hPGrpN[f]; p0 += 16) {
+                    pgi.push_back((uint32_t)f);
+                    pgi.push_back(p0);
+                }
+                f = e;
+            }
+            h->nItemsPG = (uint32_t)(pgi.size() / 2);
+            RC_TRY(upload(h, &h->dItemsPG, pgi.data(), pgi.size()));
         }
         // GL offsets per item (the two launches run in order and share GL)
         RC_TRY(upload(h, &h->dItemsPred, ip.data(), ip.size()));
@@ -951,6 +992,8 @@ static Tree make_tree(mra_handle* h) {
     T.f_seg_n = h->dFSegN;
     T.p_seg_os = h->dPSegOs;
     T.p_seg_n = h->dPSegN;
+    T.p_grp_os = h->dPGrpOs;
+    T.p_grp_n = h->dPGrpN;
     T.s_os = h->dSOs;
     T.s_n = h->dSN;
     T.s_parent = h->dSParent;
@@ -1123,15 +1166,11 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
         pa.LpiT = h->dLpiT;
         pa.P = h->dP;
         pa.v = h->dV;
-        // the ancestor chain per segment tile, then each region's own part
+        // the ancestor chain per group tile, then each region's own part
         pa.gl_off = nullptr;
-        if (h->nItemsPP64) {
-            pa.items = h->dItemsPP64;
-            k_pred_chain<RP, 64><<<h->nItemsPP64, 256, 0, st>>>(T, cc, K, Pp, pa);
-        }
-        if (h->nItemsPP16) {
-            pa.items = h->dItemsPP16;
-            k_pred_chain<RP, 16><<<h->nItemsPP16, 256, 0, st>>>(T, cc, K, Pp, pa);
+        if (h->nItemsPG) {
+            pa.items = h->dItemsPG;
+            k_pred_chain<RP, 16><<<h->nItemsPG, 256, 0, st>>>(T, cc, K, Pp, pa);
         }
         for (size_t c = 0; c + 1 < h->predChunkStart.size(); c++) {
             const uint32_t b = h->predChunkStart[c], e = h->predChunkStart[c + 1];

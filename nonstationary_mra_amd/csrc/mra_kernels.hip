@@ -42,11 +42,13 @@ struct Tree {
     const uint64_t* s_os;      // [nSeg] first observation of the segment
     const uint32_t* s_n;       // [nSeg]
     const uint64_t* s_parent;  // [nSeg] global coarse id of the shared parent
-    // prediction points of F's segment (the same runs of finest regions):
-    // B~ of a segment is stored together, element (m, c) at
-    // LRP*p_seg_os + m*p_seg_n + (c - p_seg_os)
+    // prediction points of F's segment (the same runs of finest regions)
     const uint64_t* p_seg_os;  // [nF] first prediction point of F's segment
     const uint32_t* p_seg_n;   // [nF] prediction points in F's segment
+    // groups: maximal runs of finest regions with the same grandparent; B~
+    // and G^ of a group are stored together (see k_pred_chain)
+    const uint64_t* p_grp_os;  // [nF] first prediction point of F's group
+    const uint32_t* p_grp_n;   // [nF] prediction points in F's group
 };
 
 __device__ __forceinline__ uint64_t lvl_start(int l, int J) {
@@ -1334,7 +1336,7 @@ struct PredArgs {
     const double* Sig;
     const double* H;
     const double* u;
-    double* Bt;    // B~ per segment: (m, p) at L*RP*p_seg_os + m*p_seg_n + (p - p_seg_os)
+    double* Bt;    // B~ per group: (m, p) at L*RP*p_grp_os + m*p_grp_n + (p - p_grp_os)
     double* Gh;    // G^ of the ancestor chain, the same layout
     double* GL;    // scratch: gl_off[slot] (n_F x 64)
     const uint64_t* gl_off;
@@ -1346,61 +1348,78 @@ struct PredArgs {
     const double* v;
 };
 
-// Prediction pre-pass, part 1: the conditioning chain on the ancestors,
-// shared by all prediction points of a segment (the finest regions below one
-// parent), in tiles of NT points (items (F, p0), F any region of the
-// segment).  Per level j: E_j = C(Q_{a_j}, p) - G_{a_j}^T G^ (stored as the
-// rows of B~), G^_j = L_{a_j}^{-1} E_j (stored, segment layout), and
-// var = V(p) - sum_m G^(m, p)^2, mean = 0.
+// Prediction pre-pass, part 1: the conditioning chain on the ancestors, in
+// tiles of NT prediction points of a group (the finest regions below one
+// grandparent: every ancestor but the parent is shared), items (F, p0), F
+// the group's first finest region.  Per level j: E_j = C(Q_{a_j}, p) -
+// G_{a_j}^T G^ (stored as the rows of B~), G^_j = L_{a_j}^{-1} E_j (stored),
+// var = V(p) - sum_m G^(m, p)^2, mean = 0.  Levels below the parent's run
+// once on the whole tile; the parent's level once per segment (the points
+// below one parent) of the tile, on its column range.  B~ and G^ of a group
+// are stored together: (m, c) at LRP*p_grp_os + m*p_grp_n + (c - p_grp_os).
 // (…finest-level-in-prior.cpp:271-317)
 template <int RP, int NT>
 __global__ __launch_bounds__(256) void k_pred_chain(Tree T, CovCtx cc, Pts K, Pts Pp, PredArgs a) {
-    constexpr int LD = NT + 1;
+    constexpr int LD = 2 * NT + 1;  // room for a segment's column offset (< NT) plus a tile
     __shared__ double Cm[RP * LD];
     __shared__ double Wm[RP * LD];
     __shared__ double stage[5 * RP];
     __shared__ double red[MRA_WG];
-    const uint32_t F = a.items[2 * blockIdx.x], p0 = a.items[2 * blockIdx.x + 1];
-    const uint64_t ps = T.p_seg_os[F];
-    const int pS = (int)T.p_seg_n[F];
-    const int np = min(NT, pS - (int)p0);
+    const uint32_t F0 = a.items[2 * blockIdx.x], p0 = a.items[2 * blockIdx.x + 1];
+    const uint64_t pg = T.p_grp_os[F0];
+    const int pG = (int)T.p_grp_n[F0];
+    const int np = min(NT, pG - (int)p0);
     const int r = T.r, L = T.L, LRP = T.LRP;
-    double* BtS = a.Bt + (uint64_t)LRP * ps + p0;  // (m, c) at BtS[m * pS + c]
-    double* GhS = a.Gh + (uint64_t)LRP * ps + p0;  // the same layout
-    double sq = 0.0;  // this thread's part of sum_m G^(m, c)^2, c = threadIdx.x % NT
-    for (int j = 0; j < L; j++) {
-        const uint64_t an = ancestor_up(T.parent[F], L - 1, j, T.J);
-        cov_tile<RP, NT>(Cm, LD, cc, K, an * RP, r, Pp, ps + p0, np, false, stage);
+    double* BtG = a.Bt + (uint64_t)LRP * pg + p0;  // (m, c) at BtG[m * pG + c], c tile column
+    double* GhG = a.Gh + (uint64_t)LRP * pg + p0;  // the same layout
+    const int tc = threadIdx.x % NT;               // this thread's tile column in every loop below
+    double sq = 0.0;                               // its part of sum_m G^(m, tc)^2
+    // one level j on the tile columns [off, off + cnt) with ancestor an
+    auto level = [&](int j, uint64_t an, int off, int cnt) {
+        cov_tile<RP, NT>(Cm + off, LD, cc, K, an * RP, r, Pp, pg + p0 + off, cnt, false, stage);
         __syncthreads();
         if (j > 0) {
             const double* Ga = a.Gc + a.goff[j] + (an - lvl_start(j, T.J)) * (uint64_t)j * RP * RP;
-            wg_gemm_tn(Cm, LD, RP, NT, j * RP, -1.0, Ga, RP, RP, GhS, pS, np);
+            wg_gemm_tn(Cm + off, LD, RP, NT, j * RP, -1.0, Ga, RP, RP, GhG + off, pG, cnt);
             __syncthreads();
         }
         for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
-            const int i = idx / NT, c = idx % NT;
-            if (c < np) BtS[(uint64_t)(j * RP + i) * pS + c] = Cm[i * LD + c];
+            const int i = idx / NT;
+            if (tc >= off && tc < off + cnt) BtG[(uint64_t)(j * RP + i) * pG + tc] = Cm[i * LD + tc];
         }
-        // Wm = L^{-1} Cm (every element stored: no clearing needed)
-        wg_gemm_tn<false, false, double, double, GEMM_FULL, false, GEMM_TRI_A>(Wm, LD, RP, NT, RP, 1.0,
-                                                                              a.LiT + an * RP * RP, RP, RP, Cm, LD, NT);
+        // Wm = L^{-1} Cm on the range (every element of the NT columns stored)
+        wg_gemm_tn<false, false, double, double, GEMM_FULL, false, GEMM_TRI_A>(Wm + off, LD, RP, NT, RP, 1.0,
+                                                                              a.LiT + an * RP * RP, RP, RP,
+                                                                              Cm + off, LD, NT);
         __syncthreads();
         for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
-            const int i = idx / NT, c = idx % NT;
-            const double w = Wm[i * LD + c];
-            if (c < np) GhS[(uint64_t)(j * RP + i) * pS + c] = w;
-            sq += w * w;
+            const int i = idx / NT;
+            if (tc >= off && tc < off + cnt) {
+                const double w = Wm[i * LD + tc];
+                GhG[(uint64_t)(j * RP + i) * pG + tc] = w;
+                sq += w * w;
+            }
         }
         __syncthreads();
+    };
+    for (int j = 0; j + 1 < L; j++) level(j, ancestor_up(T.parent[F0], L - 1, j, T.J), 0, np);
+    // the parent's level, segment by segment over the tile's columns
+    uint32_t f = F0;
+    for (int c = 0; c < np;) {
+        const uint64_t pc = pg + p0 + c;  // global index of the point at tile column c
+        while (T.pred_off[f + 1] <= pc) f++;
+        const uint64_t se = T.p_seg_os[f] + T.p_seg_n[f];
+        const int cnt = (int)min((uint64_t)(np - c), se - pc);
+        level(L - 1, T.parent[f], c, cnt);
+        c += cnt;
     }
     red[threadIdx.x] = sq;
     __syncthreads();
-    const int c = threadIdx.x;
-    if (c < np) {
+    if (threadIdx.x < np) {
         double s = 0.0;
-        for (int q = c; q < MRA_WG; q += NT) s += red[q];
-        a.var[ps + p0 + c] = Pp.v[ps + p0 + c] - s;
-        a.mean[ps + p0 + c] = 0.0;
+        for (int q = threadIdx.x; q < MRA_WG; q += NT) s += red[q];
+        a.var[pg + p0 + threadIdx.x] = Pp.v[pg + p0 + threadIdx.x] - s;
+        a.mean[pg + p0 + threadIdx.x] = 0.0;
     }
 }
 
@@ -1423,9 +1442,9 @@ __global__ __launch_bounds__(256) void k_pred_local(Tree T, CovCtx cc, Pts O, Pt
     const int pF = (int)(T.pred_off[F + 1] - pb);
     const int np = min(NT, pF - (int)p0);
     const int LRP = T.LRP;
-    const uint64_t ps = T.p_seg_os[F];
-    const int pS = (int)T.p_seg_n[F];
-    const uint64_t cb = (pb - ps) + p0;                 // the tile's first column in the segment
+    const uint64_t ps = T.p_grp_os[F];
+    const int pS = (int)T.p_grp_n[F];
+    const uint64_t cb = (pb - ps) + p0;                 // the tile's first column in the group
     double* BtS = a.Bt + (uint64_t)LRP * ps + cb;       // (m, c) at BtS[m * pS + c]
     const double* GhS = a.Gh + (uint64_t)LRP * ps + cb;
     const uint64_t so = T.f_seg_os[F];
@@ -1492,10 +1511,12 @@ __global__ __launch_bounds__(256) void k_pred_post(Tree T, PredArgs a) {
     __shared__ double Zm[RP * LD];
     const uint32_t F = a.items[2 * blockIdx.x], p0 = a.items[2 * blockIdx.x + 1];
     const uint64_t pb = T.p_seg_os[F];
-    const int pF = (int)T.p_seg_n[F];
-    const int np = min(NT, pF - (int)p0);
+    const int np = min(NT, (int)T.p_seg_n[F] - (int)p0);
     const int L = T.L, LRP = T.LRP;
-    double* BtF = a.Bt + (uint64_t)LRP * pb;  // B~(m, c) at BtF[m * pF + c]
+    // B~ in the group layout: column c of the segment at BtF[m * pF + c]
+    const uint64_t pg = T.p_grp_os[F];
+    const int pF = (int)T.p_grp_n[F];
+    double* BtF = a.Bt + (uint64_t)LRP * pg + (pb - pg);
     double macc = 0.0, vacc = 0.0;
     const int c = threadIdx.x;
     for (int jj = L - 1; jj >= 0; jj--) {
