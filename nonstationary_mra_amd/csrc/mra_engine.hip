@@ -42,12 +42,9 @@ struct mra_handle {
     uint32_t* dPSegN = nullptr;
     uint64_t* dPGrpOs = nullptr;
     uint32_t* dPGrpN = nullptr;
-    // prediction chain items (first region of a group, p0): 16-point tiles of a group
+    // prediction chain and post-pass items (first region of a group, p0): 16-point tiles of a group
     uint32_t* dItemsPG = nullptr;
     uint32_t nItemsPG = 0;
-    // prediction post-pass items (F, p0): tiles of the points of one segment
-    uint32_t *dItemsPP16 = nullptr, *dItemsPP64 = nullptr;
-    uint32_t nItemsPP16 = 0, nItemsPP64 = 0;
     uint32_t *dFSegN = nullptr, *dSN = nullptr;
     // factored posterior of the finest-but-one level
     bool lowrank = false;
@@ -692,26 +689,7 @@ static int alloc_engine(mra_handle* h, const mra_tree_desc* t) {
         }
         h->predChunkStart.push_back((uint32_t)(ip.size() / 3));
         {
-            // post-pass tiles per segment: 16 points for segments with at most
-            // 16, else 64
-            std::vector<uint32_t> p16, p64;
-            for (uint64_t f = 0; f < h->nF;) {
-                uint64_t e = f + 1;
-                while (e < h->nF && h->hParent[e] == h->hParent[f]) e++;
-                const uint32_t pn = h->hPSegN[f];
-                std::vector<uint32_t>& lst = pn <= 16 ? p16 : p64;
-                const uint32_t nt = pn <= 16 ? 16u : 64u;
-                for (uint32_t p0 = 0; p0 < pn; p0 += nt) {
-                    lst.push_back((uint32_t)f);
-                    lst.push_back(p0);
-                }
-                f = e;
-            }
-            h->nItemsPP16 = (uint32_t)(p16.size() / 2);
-            h->nItemsPP64 = (uint32_t)(p64.size() / 2);
-            RC_TRY(upload(h, &h->dItemsPP16, p16.data(), p16.size()));
-            RC_TRY(upload(h, &h->dItemsPP64, p64.data(), p64.size()));
-            // chain tiles: 16 points of a group, from the group's first region
+            // chain and post-pass tiles: 16 points of a group, from the group's first region
             std::vector<uint32_t> pgi;
             auto gp = [&](uint64_t f) { const uint64_t p = h->hParent[f]; return p > 0 ? (p - 1) / (uint64_t)h->J : 0; };
             for (uint64_t f = 0; f < h->nF;) {
@@ -1323,16 +1301,10 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
         }
     }
     mark(MRA_T_PRED_POST);
-    if (predicting) {
+    if (predicting && h->nItemsPG) {
         pa.gl_off = nullptr;
-        if (h->nItemsPP64) {
-            pa.items = h->dItemsPP64;
-            k_pred_post<RP, 64><<<h->nItemsPP64, 256, 0, st>>>(T, pa);
-        }
-        if (h->nItemsPP16) {
-            pa.items = h->dItemsPP16;
-            k_pred_post<RP, 16><<<h->nItemsPP16, 256, 0, st>>>(T, pa);
-        }
+        pa.items = h->dItemsPG;
+        k_pred_post<RP, 16><<<h->nItemsPG, 256, 0, st>>>(T, pa);
     }
     mark(MRA_T_REDUCE);
     k_reduce<<<1, 256, 0, st>>>(h->dDterm + h->nC, h->nF, h->dLev);

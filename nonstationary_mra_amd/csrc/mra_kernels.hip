@@ -1503,29 +1503,29 @@ __global__ __launch_bounds__(256) void k_pred_local(Tree T, CovCtx cc, Pts O, Pt
 // Top-down posterior prediction (src/class_approximation-predict.cpp:123-188):
 //   Z_j = L~_{a_j}^{-1} (B~_j - sum_{j'>j} P_{a_j',j}^T Z_j'),
 //   mean += Z_j^T v_{a_j}, var += colsumsq(Z_j).
-// One tile of NT points of a segment (all finest regions below one parent
-// share every ancestor): items (F, p0), F any region of the segment.
+// In 16-point tiles of a group (items as k_pred_chain): the parent's level
+// first, once per segment of the tile on its column range, then the levels
+// above on the whole tile (every column is independent of the others).
 template <int RP, int NT>
 __global__ __launch_bounds__(256) void k_pred_post(Tree T, PredArgs a) {
-    constexpr int LD = NT + 1;
+    constexpr int LD = 2 * NT + 1;  // room for a segment's column offset (< NT) plus a tile
     __shared__ double Zm[RP * LD];
-    const uint32_t F = a.items[2 * blockIdx.x], p0 = a.items[2 * blockIdx.x + 1];
-    const uint64_t pb = T.p_seg_os[F];
-    const int np = min(NT, (int)T.p_seg_n[F] - (int)p0);
+    const uint32_t F0 = a.items[2 * blockIdx.x], p0 = a.items[2 * blockIdx.x + 1];
+    const uint64_t pg = T.p_grp_os[F0];
+    const int pG = (int)T.p_grp_n[F0];
+    const int np = min(NT, pG - (int)p0);
     const int L = T.L, LRP = T.LRP;
-    // B~ in the group layout: column c of the segment at BtF[m * pF + c]
-    const uint64_t pg = T.p_grp_os[F];
-    const int pF = (int)T.p_grp_n[F];
-    double* BtF = a.Bt + (uint64_t)LRP * pg + (pb - pg);
+    double* BtG = a.Bt + (uint64_t)LRP * pg + p0;  // B~(m, c) at BtG[m * pG + c], c tile column
     double macc = 0.0, vacc = 0.0;
     const int c = threadIdx.x;
-    for (int jj = L - 1; jj >= 0; jj--) {
-        const uint64_t an = ancestor_up(T.parent[F], L - 1, jj, T.J);
-        // Z = L~^{-1} B~_jj, B~ read from global memory (the tile's np columns)
-        wg_gemm_tn<false, false, double, double, GEMM_PART, false, GEMM_TRI_A>(Zm, LD, RP, NT, RP, 1.0, a.LpiT + an * RP * RP, RP, RP,
-                                 BtF + (uint64_t)jj * RP * pF + p0, pF, np);
+    // level jj with ancestor an on the tile columns [off, off + cnt)
+    auto step = [&](int jj, uint64_t an, int off, int cnt) {
+        // Z = L~^{-1} B~_jj, B~ read from global memory
+        wg_gemm_tn<false, false, double, double, GEMM_PART, false, GEMM_TRI_A>(
+            Zm + off, LD, RP, NT, RP, 1.0, a.LpiT + an * RP * RP, RP, RP, BtG + (uint64_t)jj * RP * pG + off, pG,
+            cnt);
         __syncthreads();
-        if (c < np) {
+        if (c >= off && c < off + cnt) {
             const double* vA = a.v + an * RP;
             double sm = 0.0, sv = 0.0;
             for (int m = 0; m < RP; m++) {
@@ -1540,14 +1540,24 @@ __global__ __launch_bounds__(256) void k_pred_post(Tree T, PredArgs a) {
         if (jj > 0) {
             const double* PA = a.P + a.goff[jj] + (an - lvl_start(jj, T.J)) * (uint64_t)jj * RP * RP;
             for (int jp = 0; jp < jj; jp++)
-                wg_gemm_tn(BtF + (uint64_t)jp * RP * pF + p0, pF, RP, NT, RP, -1.0, PA + (uint64_t)jp * RP * RP, RP,
-                           RP, Zm, LD, np);
+                wg_gemm_tn(BtG + (uint64_t)jp * RP * pG + off, pG, RP, NT, RP, -1.0, PA + (uint64_t)jp * RP * RP, RP,
+                           RP, Zm + off, LD, cnt);
         }
         __syncthreads();
+    };
+    uint32_t f = F0;
+    for (int c0 = 0; c0 < np;) {
+        const uint64_t pc = pg + p0 + c0;  // global index of the point at tile column c0
+        while (T.pred_off[f + 1] <= pc) f++;
+        const uint64_t se = T.p_seg_os[f] + T.p_seg_n[f];
+        const int cnt = (int)min((uint64_t)(np - c0), se - pc);
+        step(L - 1, T.parent[f], c0, cnt);
+        c0 += cnt;
     }
+    for (int jj = L - 2; jj >= 0; jj--) step(jj, ancestor_up(T.parent[F0], L - 1, jj, T.J), 0, np);
     if (c < np) {
-        a.mean[pb + p0 + c] += macc;
-        a.var[pb + p0 + c] += vacc;
+        a.mean[pg + p0 + c] += macc;
+        a.var[pg + p0 + c] += vacc;
     }
 }
 
