@@ -408,6 +408,37 @@ def test_concurrent_streams():
     assert rel(out[(1, capi.PRECISION_FP64, True)][0], Do) < TOL_D
 
 
+@pytest.mark.parametrize("case", [(1500, 3000, 4, 71), (2000, 400, 5, 72)])
+def test_prediction_group_tiles(case):
+    """Prediction tiles of 16 points of a group (the finest regions below one
+    grandparent): groups of several hundred points, so that segments (the
+    points below one parent) span tile boundaries and tiles hold segment
+    boundaries, next to groups of a few points; means and variances match
+    the oracle in FP64, the mixed path stays within its tolerance."""
+    n, npred, M, seed = case
+    rng = np.random.default_rng(seed)
+    x, y = synth.clustered_points(n, rng, n_clusters=3, sigma=0.08)
+    px, py = synth.clustered_points(npred, rng, n_clusters=2, sigma=0.1)
+    tree = capi.partition_program(x, y, rng.standard_normal(n), px, py, M=M, J=4, r=16,
+                                  bounds=synth.UNIT_BOUNDS)
+    cov, prm = synth.linear_trend_cov(), dict(synth.UNIT_PARAMS)
+    pc = np.diff(tree["pred_offsets"].astype(np.int64))
+    print("largest groups", np.sort(pc.reshape(-1, 16).sum(1))[-4:], flush=True)
+    check(tree, cov, prm)
+    h = capi.Handle(tree)
+    try:
+        h.set_covariance(cov)
+        h.set_params(**prm)
+        h.set_precision(capi.PRECISION_MIXED)
+        Dm, mm, vm = h.predict()
+    finally:
+        h.close()
+    Do, mo, vo = oracle.evaluate(tree, cov, prm, predict=True)
+    assert abs(Dm - Do) / (2 * n) < TOL_MIXED
+    assert np.abs(mm - mo).max() <= 1e-3 * np.abs(mo).max()
+    assert np.max(np.abs(vm - vo) / np.abs(vo)) < 1e-3
+
+
 TOL_MIXED = 1e-6   # |delta loglik| / n for the mixed-precision path
 
 
