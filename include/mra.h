@@ -283,13 +283,20 @@ int mra_autotune(mra_handle* h, int* x32max, int* lrmax, int* fb16, double* best
 /* ------------------------------------------------------------------ batch */
 /* Many independent fields (one handle each, any sizes) evaluated as one
  * launch sequence (no reference counterpart; the reference runs one data set
- * per MPI job): handle i's evaluation is issued on batch stream i mod
- * `streams` (1..8), as a graph replay (graph = 1) or direct launches, and the
- * host waits once for all.  The handles must stay alive while the batch
- * exists and must not be used concurrently from elsewhere. */
+ * per MPI job): one kernel scatters every field's parameters, the fields'
+ * evaluations run on `streams` (1..8) batch streams (largest first, each to
+ * the least loaded stream), one kernel gathers every D and status flag.  With
+ * graph = 1 the whole sequence is captured once as one HIP graph and
+ * replayed; it is captured again when a handle's work lists, options,
+ * covariance model or precision change (or, for a stationary model, its
+ * sill or range).  The host waits once per call.  The handles must be
+ * distinct, must stay alive while the batch exists and must not be used
+ * concurrently from elsewhere; the batch does not use their own streams. */
 typedef struct mra_batch mra_batch;
 int mra_batch_create(mra_handle* const* handles, int n, mra_batch** out);
 void mra_batch_destroy(mra_batch* b);
+/* Message of the last error of this batch (or of mra_batch_create if b is NULL). */
+const char* mra_batch_last_error(const mra_batch* b);
 int mra_batch_set(mra_batch* b, int streams, int graph);
 int mra_batch_get(mra_batch* b, int* streams, int* graph);
 /* D[i] of handle i; status[i] = MRA_OK or MRA_E_NOT_PD (status may be NULL). */

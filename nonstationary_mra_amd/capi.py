@@ -34,7 +34,7 @@ EXPORTED = ["mra_create", "mra_destroy", "mra_last_error", "mra_set_covariance",
             "mra_mlp_forward", "mra_mlp_loss_grad", "mra_mlp_train_step", "mra_batch_create", "mra_batch_destroy",
             "mra_batch_set", "mra_batch_get", "mra_batch_likelihood", "mra_batch_autotune", "mra_set_tuning",
             "mra_get_tuning", "mra_autotune", "mra_cov_tile_timed", "mra_device_alloc", "mra_device_free",
-            "mra_device_copy"]
+            "mra_device_copy", "mra_batch_last_error"]
 OPT_FACTORED_POSTERIOR = 1
 OPT_GRAM_KERNEL = 2
 OPT_WAVE_LOCAL16 = 3
@@ -128,6 +128,8 @@ def lib():
         _lib.mra_batch_get.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         _lib.mra_batch_likelihood.argtypes = [C.c_void_p, dp, C.POINTER(C.c_int)]
         _lib.mra_batch_autotune.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), dp]
+        _lib.mra_batch_last_error.argtypes = [C.c_void_p]
+        _lib.mra_batch_last_error.restype = C.c_char_p
         fp_ = C.POINTER(C.c_float)
         _lib.mra_variogram_features.argtypes = [C.c_void_p, fp_]
         _lib.mra_mlp_create.argtypes = [C.c_uint64, C.POINTER(C.c_void_p)]
@@ -185,24 +187,29 @@ class Batch:
         check(lib().mra_batch_create(arr, len(self.handles), C.byref(b)))
         self.b = b
 
+    def _check(self, rc):
+        if rc != MRA_OK:
+            msg = lib().mra_batch_last_error(self.b)
+            raise MraError(rc, msg.decode() if msg else "")
+
     def set(self, streams, graph):
-        check(lib().mra_batch_set(self.b, streams, int(graph)))
+        self._check(lib().mra_batch_set(self.b, streams, int(graph)))
 
     def get(self):
         s, g = C.c_int(0), C.c_int(0)
-        check(lib().mra_batch_get(self.b, C.byref(s), C.byref(g)))
+        self._check(lib().mra_batch_get(self.b, C.byref(s), C.byref(g)))
         return s.value, g.value
 
     def likelihood(self):
         n = len(self.handles)
         D = np.zeros(n)
         st = (C.c_int * n)()
-        check(lib().mra_batch_likelihood(self.b, _d(D), st))
+        self._check(lib().mra_batch_likelihood(self.b, _d(D), st))
         return D, np.array(st[:], dtype=np.int32)
 
     def autotune(self):
         s, g, ms = C.c_int(0), C.c_int(0), C.c_double(0)
-        check(lib().mra_batch_autotune(self.b, C.byref(s), C.byref(g), C.byref(ms)))
+        self._check(lib().mra_batch_autotune(self.b, C.byref(s), C.byref(g), C.byref(ms)))
         return s.value, g.value, ms.value
 
     def close(self):

@@ -150,6 +150,9 @@ struct mra_handle {
     hipGraph_t graph[NGRAPH] = {};
     hipGraphExec_t gexec[NGRAPH] = {};
     double graphSill[NGRAPH] = {}, graphRange[NGRAPH] = {};
+    // incremented whenever captured launch sequences become invalid (work
+    // lists, options, covariance model): batch graphs compare it
+    uint64_t gen = 0;
 };
 
 static void graph_invalidate(mra_handle* h);
@@ -1326,6 +1329,7 @@ static int ensure_mixed_buffers(mra_handle* h) {
 }
 
 static void graph_invalidate(mra_handle* h) {
+    h->gen++;
     for (int g = 0; g < mra_handle::NGRAPH; g++) {
         if (h->gexec[g]) (void)hipGraphExecDestroy(h->gexec[g]);
         if (h->graph[g]) (void)hipGraphDestroy(h->graph[g]);
@@ -1342,13 +1346,19 @@ static void launch_eval_rp(mra_handle* h, bool predicting, bool capturing) {
     }
 }
 
+// A stationary model passes sill and range to the kernels by value, so a
+// graph captured for other values of them is stale.
+static bool graph_stale(const mra_handle* h, int g) {
+    return h->gexec[g] && !h->nonstationary && (h->graphSill[g] != h->sill || h->graphRange[g] != h->range);
+}
+
 // One evaluation as a captured HIP graph: every launch of launch_eval is
 // recorded once per (predicting, precision) and replayed.  The graph holds
 // kernel nodes only (zeroing is done by fill kernels); parameters reach the
 // kernels through dPrm, written before each replay.
 static int run_graph(mra_handle* h, bool predicting) {
     const int g = (predicting ? 2 : 0) + (h->precision == MRA_PRECISION_MIXED ? 1 : 0);
-    if (h->gexec[g] && !h->nonstationary && (h->graphSill[g] != h->sill || h->graphRange[g] != h->range)) {
+    if (graph_stale(h, g)) {
         (void)hipGraphExecDestroy(h->gexec[g]);
         (void)hipGraphDestroy(h->graph[g]);
         h->gexec[g] = nullptr;
@@ -1825,64 +1835,161 @@ extern "C" int mra_variogram_features(mra_handle* h, float* features) {
 }
 
 // ------------------------------------------------------------------ batch
-// Independent handles (fields) evaluated as one launch sequence: every
-// handle's evaluation is issued on one of `streams` batch streams (handle i
-// on stream i mod streams), graph replay or direct launches, with no host
-// synchronisation until all are issued.  mra_batch_autotune times the
-// candidate settings on the actual batch and keeps the fastest.
+// Independent handles (fields) evaluated as one launch sequence.  Every
+// field's evaluation is issued on one of `streams` batch streams (fields in
+// decreasing order of work, each to the stream with the least work so far),
+// between one parameter-scatter kernel and one result-gather kernel on the
+// first stream; with graph = 1 the whole sequence, all fields, is captured
+// once as a single HIP graph and replayed by one launch per call.  Host
+// traffic per call: one copy of all parameters in, one copy of all D and
+// status flags out, one synchronisation.  The handles' own streams and side
+// streams are not used by the batch.  mra_batch_autotune times the candidate
+// settings on the actual batch and keeps the fastest.
 struct mra_batch {
     std::vector<mra_handle*> h;
-    std::vector<hipStream_t> st;
+    std::vector<int> order;   // issue order: decreasing work
+    std::vector<double> work; // work estimate per field
+    static constexpr int MAXS = 8;
+    hipStream_t st[MAXS] = {};
+    hipEvent_t evRoot = nullptr, evDone[MAXS] = {};
     int streams = 4;
     int graph = 1;
-    double* hOut = nullptr;  // pinned: D and bad flag per handle
+    int n = 0;
+    double* hPrm = nullptr;    // pinned: 6 parameters per field
+    double* dPrm = nullptr;    // device copy, scattered to the handles' parameter buffers
+    double** dPrmPtr = nullptr;
+    const double** dDPtr = nullptr;
+    const int** dBadPtr = nullptr;
+    double* dOut = nullptr;    // gathered D and status flag per field
+    double* hOut = nullptr;    // pinned
+    // the captured batch graph and the state of every handle it was captured for
+    hipGraph_t g = nullptr;
+    hipGraphExec_t gx = nullptr;
+    int gStreams = 0;
+    std::vector<uint64_t> gGen;
+    std::vector<int> gPrec;
+    std::vector<double> gSill, gRange;
     std::string err;
 };
+
+static void batch_err(mra_batch* b, const std::string& s) {
+    if (b) b->err = s;
+    g_err = s;
+}
+
+#define BATCH_HIP_TRY(x)                                                              \
+    do {                                                                              \
+        hipError_t e_ = (x);                                                          \
+        if (e_ != hipSuccess) {                                                       \
+            batch_err(b, std::string(#x) + ": " + hipGetErrorString(e_));             \
+            return MRA_E_HIP;                                                         \
+        }                                                                             \
+    } while (0)
+
+static void batch_drop_graph(mra_batch* b) {
+    if (b->gx) (void)hipGraphExecDestroy(b->gx);
+    if (b->g) (void)hipGraphDestroy(b->g);
+    b->gx = nullptr;
+    b->g = nullptr;
+}
+
+extern "C" const char* mra_batch_last_error(const mra_batch* b) { return b ? b->err.c_str() : g_err.c_str(); }
 
 extern "C" void mra_batch_destroy(mra_batch* b) {
     if (!b) return;
     for (hipStream_t s : b->st)
-        if (s) {
-            (void)hipStreamSynchronize(s);
-            (void)hipStreamDestroy(s);
-        }
+        if (s) (void)hipStreamSynchronize(s);
+    batch_drop_graph(b);
+    for (hipStream_t s : b->st)
+        if (s) (void)hipStreamDestroy(s);
+    if (b->evRoot) (void)hipEventDestroy(b->evRoot);
+    for (hipEvent_t e : b->evDone)
+        if (e) (void)hipEventDestroy(e);
+    if (b->hPrm) (void)hipHostFree(b->hPrm);
     if (b->hOut) (void)hipHostFree(b->hOut);
+    for (void* p : {(void*)b->dPrm, (void*)b->dPrmPtr, (void*)b->dDPtr, (void*)b->dBadPtr, (void*)b->dOut})
+        if (p) (void)hipFree(p);
     delete b;
 }
 
-static constexpr int MRA_BATCH_MAX_STREAMS = 8;
+static constexpr int MRA_BATCH_MAX_STREAMS = mra_batch::MAXS;
 
 extern "C" int mra_batch_create(mra_handle* const* handles, int n, mra_batch** out) {
-    if (!handles || n < 1 || !out) return MRA_E_ARG;
+    if (!handles || n < 1 || !out) {
+        g_err = "mra_batch_create: null argument or n < 1";
+        return MRA_E_ARG;
+    }
     *out = nullptr;
-    mra_batch* b = new mra_batch();
-    for (int i = 0; i < n; i++) {
-        if (!handles[i]) {
-            delete b;
+    {
+        std::vector<mra_handle*> seen(handles, handles + n);
+        std::sort(seen.begin(), seen.end());
+        if (!seen[0] || std::adjacent_find(seen.begin(), seen.end()) != seen.end()) {
+            g_err = "mra_batch_create: null or repeated handle";
             return MRA_E_ARG;
         }
-        b->h.push_back(handles[i]);
     }
-    // the batch's streams already run fields side by side: no side streams
-    for (mra_handle* h : b->h) h->noFork = true;
-    b->st.assign(MRA_BATCH_MAX_STREAMS, nullptr);
-    for (auto& s : b->st)
-        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
-            g_err = "hipStreamCreate failed";
-            mra_batch_destroy(b);
-            return MRA_E_HIP;
+    mra_batch* b = new mra_batch();
+    b->n = n;
+    b->h.assign(handles, handles + n);
+    for (mra_handle* h : b->h) {
+        // work estimate: conditioning rows times columns plus the local squares
+        double w = (double)h->nObs * h->LRP * h->RP + (double)h->nC * h->RP * h->RP * h->RP * h->L;
+        for (uint64_t f = 0; f < h->nF; f++) {
+            const double c = (double)(h->hObsOff[f + 1] - h->hObsOff[f]);
+            w += c * c * (c / 3 + h->LRP);
         }
-    if (hipHostMalloc((void**)&b->hOut, sizeof(double) * 2 * n, hipHostMallocDefault) != hipSuccess) {
-        g_err = "hipHostMalloc failed";
+        b->work.push_back(w);
+    }
+    b->order.resize(n);
+    for (int i = 0; i < n; i++) b->order[i] = i;
+    std::stable_sort(b->order.begin(), b->order.end(), [&](int a, int c) { return b->work[a] > b->work[c]; });
+    int rc = MRA_OK;
+    auto fail = [&](const char* what, int code) {
+        batch_err(b, what);
+        rc = code;
+    };
+    for (auto& s : b->st)
+        if (!rc && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) fail("hipStreamCreate failed", MRA_E_HIP);
+    if (!rc && hipEventCreateWithFlags(&b->evRoot, hipEventDisableTiming) != hipSuccess) fail("hipEventCreate failed", MRA_E_HIP);
+    for (auto& e : b->evDone)
+        if (!rc && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) fail("hipEventCreate failed", MRA_E_HIP);
+    if (!rc && (hipHostMalloc((void**)&b->hPrm, sizeof(double) * 6 * n, hipHostMallocDefault) != hipSuccess ||
+                hipHostMalloc((void**)&b->hOut, sizeof(double) * 2 * n, hipHostMallocDefault) != hipSuccess))
+        fail("hipHostMalloc failed", MRA_E_NOMEM);
+    if (!rc && (hipMalloc((void**)&b->dPrm, sizeof(double) * 6 * n) != hipSuccess ||
+                hipMalloc((void**)&b->dPrmPtr, sizeof(double*) * n) != hipSuccess ||
+                hipMalloc((void**)&b->dDPtr, sizeof(double*) * n) != hipSuccess ||
+                hipMalloc((void**)&b->dBadPtr, sizeof(int*) * n) != hipSuccess ||
+                hipMalloc((void**)&b->dOut, sizeof(double) * 2 * n) != hipSuccess))
+        fail("hipMalloc failed", MRA_E_NOMEM);
+    if (!rc) {
+        std::vector<double*> pp(n);
+        std::vector<const double*> dp(n);
+        std::vector<const int*> bp(n);
+        for (int i = 0; i < n; i++) {
+            pp[i] = b->h[i]->dPrm;
+            dp[i] = b->h[i]->dD;
+            bp[i] = b->h[i]->dBad;
+        }
+        if (hipMemcpy(b->dPrmPtr, pp.data(), sizeof(double*) * n, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(b->dDPtr, dp.data(), sizeof(double*) * n, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(b->dBadPtr, bp.data(), sizeof(int*) * n, hipMemcpyHostToDevice) != hipSuccess)
+            fail("hipMemcpy failed", MRA_E_HIP);
+    }
+    if (rc) {
+        (void)hipGetLastError();
         mra_batch_destroy(b);
-        return MRA_E_HIP;
+        return rc;
     }
     *out = b;
     return MRA_OK;
 }
 
 extern "C" int mra_batch_set(mra_batch* b, int streams, int graph) {
-    if (!b || streams < 1 || streams > MRA_BATCH_MAX_STREAMS) return MRA_E_ARG;
+    if (!b || streams < 1 || streams > MRA_BATCH_MAX_STREAMS) {
+        batch_err(b, "mra_batch_set: streams must be 1..8");
+        return MRA_E_ARG;
+    }
     b->streams = streams;
     b->graph = graph != 0;
     return MRA_OK;
@@ -1895,55 +2002,124 @@ extern "C" int mra_batch_get(mra_batch* b, int* streams, int* graph) {
     return MRA_OK;
 }
 
+// Issues one evaluation of every field: parameter scatter, the fields'
+// launch sequences on the batch streams, result gather (on stream 0).
+static void batch_issue(mra_batch* b) {
+    const int n = b->n, s = b->streams;
+    const hipStream_t s0 = b->st[0];
+    k_batch_params<<<(6 * n + 255) / 256, 256, 0, s0>>>(n, b->dPrm, b->dPrmPtr);
+    if (s > 1) {
+        (void)hipEventRecord(b->evRoot, s0);
+        for (int k = 1; k < s; k++) (void)hipStreamWaitEvent(b->st[k], b->evRoot, 0);
+    }
+    double load[MRA_BATCH_MAX_STREAMS] = {};
+    for (int q = 0; q < n; q++) {
+        const int i = b->order[q];
+        int k = 0;
+        for (int c = 1; c < s; c++)
+            if (load[c] < load[k]) k = c;
+        load[k] += b->work[i];
+        mra_handle* h = b->h[i];
+        const hipStream_t own = h->stream;
+        const bool nf = h->noFork;
+        h->stream = b->st[k];
+        h->noFork = true;  // the batch streams already run fields side by side
+        launch_eval_rp(h, false, true);
+        h->stream = own;
+        h->noFork = nf;
+    }
+    if (s > 1)
+        for (int k = 1; k < s; k++) {
+            (void)hipEventRecord(b->evDone[k], b->st[k]);
+            (void)hipStreamWaitEvent(s0, b->evDone[k], 0);
+        }
+    k_batch_results<<<(n + 255) / 256, 256, 0, s0>>>(n, b->dDPtr, b->dBadPtr, b->dOut);
+}
+
+// The captured graph matches the current state of every handle.
+static bool batch_graph_current(const mra_batch* b) {
+    if (!b->gx || b->gStreams != b->streams) return false;
+    for (int i = 0; i < b->n; i++) {
+        const mra_handle* h = b->h[i];
+        if (b->gGen[i] != h->gen || b->gPrec[i] != h->precision) return false;
+        if (!h->nonstationary && (b->gSill[i] != h->sill || b->gRange[i] != h->range)) return false;
+    }
+    return true;
+}
+
+static int batch_capture(mra_batch* b) {
+    batch_drop_graph(b);
+    BATCH_HIP_TRY(hipStreamBeginCapture(b->st[0], hipStreamCaptureModeThreadLocal));
+    batch_issue(b);
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(b->st[0], &gr);
+    if (e != hipSuccess) {
+        if (gr) (void)hipGraphDestroy(gr);
+        batch_err(b, std::string("batch graph capture failed: ") + hipGetErrorString(e));
+        return MRA_E_HIP;
+    }
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(gr);
+        batch_err(b, std::string("batch graph instantiation failed: ") + hipGetErrorString(e));
+        return MRA_E_HIP;
+    }
+    b->g = gr;
+    b->gx = ex;
+    b->gStreams = b->streams;
+    b->gGen.resize(b->n);
+    b->gPrec.resize(b->n);
+    b->gSill.resize(b->n);
+    b->gRange.resize(b->n);
+    for (int i = 0; i < b->n; i++) {
+        b->gGen[i] = b->h[i]->gen;
+        b->gPrec[i] = b->h[i]->precision;
+        b->gSill[i] = b->h[i]->sill;
+        b->gRange[i] = b->h[i]->range;
+    }
+    return MRA_OK;
+}
+
 // D[i] = D of handle i; status[i] = MRA_OK or MRA_E_NOT_PD (may be NULL).
 // Returns the first error that is not a non-positive pivot.
 extern "C" int mra_batch_likelihood(mra_batch* b, double* D, int* status) {
-    if (!b || !D) return MRA_E_ARG;
-    const int n = (int)b->h.size();
+    if (!b || !D) {
+        batch_err(b, "mra_batch_likelihood: null argument");
+        return MRA_E_ARG;
+    }
+    const int n = b->n;
     for (int i = 0; i < n; i++) {
         mra_handle* h = b->h[i];
         if (!h->covSet || !h->paramsSet) {
-            b->err = "covariance model and parameters must be set on every handle";
+            batch_err(b, "covariance model and parameters must be set on every handle");
             return MRA_E_STATE;
         }
-        if (!use_lowrank(h)) RC_TRY(ensure_dense_msgs(h));
-        RC_TRY(ensure_mixed_buffers(h));
-    }
-    for (int i = 0; i < n; i++) {
-        mra_handle* h = b->h[i];
-        hipStream_t own = h->stream;
-        hipStream_t s = b->st[i % b->streams];
-        const double prm[6] = {h->sill, h->range, h->nugget, h->ss, h->rs, h->ns};
-        for (int k = 0; k < 6; k++) h->hPrm[k] = prm[k];
-        if (b->graph) {
-            // capture (first use) happens on the handle's own stream
-            const int g = h->precision == MRA_PRECISION_MIXED ? 1 : 0;
-            if (!h->gexec[g]) {
-                HIP_TRY(hipMemcpyAsync(h->dPrm, h->hPrm, sizeof(prm), hipMemcpyHostToDevice, own));
-                RC_TRY(run_graph(h, false));
-                HIP_TRY(hipStreamSynchronize(own));
-            }
-            HIP_TRY(hipMemcpyAsync(h->dPrm, h->hPrm, sizeof(prm), hipMemcpyHostToDevice, s));
-            HIP_TRY(hipGraphLaunch(h->gexec[g], s));
-        } else {
-            HIP_TRY(hipMemcpyAsync(h->dPrm, h->hPrm, sizeof(prm), hipMemcpyHostToDevice, s));
-            h->stream = s;
-            launch_eval_rp(h, false, false);
-            h->stream = own;
+        int rc = use_lowrank(h) ? MRA_OK : ensure_dense_msgs(h);
+        if (!rc) rc = ensure_mixed_buffers(h);
+        if (rc) {
+            batch_err(b, "field " + std::to_string(i) + ": " + h->err);
+            return rc;
         }
-        HIP_TRY(hipMemcpyAsync(b->hOut + 2 * i, h->dD, sizeof(double), hipMemcpyDeviceToHost, s));
-        int* hb = (int*)(b->hOut + 2 * i + 1);
-        HIP_TRY(hipMemcpyAsync(hb, h->dBad, sizeof(int), hipMemcpyDeviceToHost, s));
+        // nothing of the handle's own may still run on its buffers
+        BATCH_HIP_TRY(hipStreamSynchronize(h->stream));
+        const double prm[6] = {h->sill, h->range, h->nugget, h->ss, h->rs, h->ns};
+        for (int k = 0; k < 6; k++) b->hPrm[6 * i + k] = prm[k];
     }
-    {
-        mra_handle* const h = nullptr;  // errors below are batch-wide (mra_last_error(NULL))
-        HIP_TRY(hipGetLastError());
-        for (int k = 0; k < b->streams; k++) HIP_TRY(hipStreamSynchronize(b->st[k]));
+    const hipStream_t s0 = b->st[0];
+    BATCH_HIP_TRY(hipMemcpyAsync(b->dPrm, b->hPrm, sizeof(double) * 6 * n, hipMemcpyHostToDevice, s0));
+    if (b->graph) {
+        if (!batch_graph_current(b)) RC_TRY(batch_capture(b));
+        BATCH_HIP_TRY(hipGraphLaunch(b->gx, s0));
+    } else {
+        batch_issue(b);
     }
+    BATCH_HIP_TRY(hipGetLastError());
+    BATCH_HIP_TRY(hipMemcpyAsync(b->hOut, b->dOut, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, s0));
+    BATCH_HIP_TRY(hipStreamSynchronize(s0));
     for (int i = 0; i < n; i++) {
         D[i] = b->hOut[2 * i];
-        const int bad = *(const int*)(b->hOut + 2 * i + 1);
-        if (status) status[i] = bad ? MRA_E_NOT_PD : MRA_OK;
+        if (status) status[i] = b->hOut[2 * i + 1] != 0.0 ? MRA_E_NOT_PD : MRA_OK;
         b->h[i]->nEv = 0;
     }
     return MRA_OK;
@@ -1951,8 +2127,7 @@ extern "C" int mra_batch_likelihood(mra_batch* b, double* D, int* status) {
 
 extern "C" int mra_batch_autotune(mra_batch* b, int* streams, int* graph, double* best_ms) {
     if (!b) return MRA_E_ARG;
-    const int n = (int)b->h.size();
-    std::vector<double> D(n);
+    std::vector<double> D(b->n);
     const int cand_s[] = {1, 2, 4, 8};
     double best = 1e300;
     int bs = b->streams, bg = b->graph;

@@ -1715,6 +1715,25 @@ __global__ __launch_bounds__(256) void k_selftest_f32(const float* A, const floa
     for (int idx = threadIdx.x; idx < 64 * 64; idx += MRA_WG) Cout[idx] = Cm[(idx / 64) * LD + idx % 64];
 }
 
+// ------------------------------------------------------------------ batch
+// Scatter of a batch's parameters: field i's six parameters prm[6i..6i+5]
+// to that field's own parameter buffer dst[i].
+__global__ void k_batch_params(int n, const double* __restrict__ prm, double* const* __restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 6 * n) dst[i / 6][i % 6] = prm[i];
+}
+
+// Gather of a batch's results: out[2i] = D of field i, out[2i+1] = its
+// non-positive-pivot flag.
+__global__ void k_batch_results(int n, const double* const* __restrict__ D, const int* const* __restrict__ bad,
+                                double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        out[2 * i] = *D[i];
+        out[2 * i + 1] = *bad[i] ? 1.0 : 0.0;
+    }
+}
+
 // ------------------------------------------------------------- reductions
 // Deterministic sum of x[0..n) into out[0] (one work group, fixed order).
 __global__ __launch_bounds__(256) void k_reduce(const double* x, uint64_t n, double* out) {

@@ -1,0 +1,171 @@
+"""GPU tests of the benchmark configurations at their full sizes.
+
+C1: the FP64 path against the CPU oracle at n = 250,000 (D to 1e-10,
+posterior mean and variance at 50,000 points to 1e-9).
+C2: n = 2,000,000, M = 10, r = 64 through properties that do not need the
+oracle at that size: mixed vs FP64 |dloglik|/n <= 1e-6, graph replay equal
+to eager launches bit for bit, repeated FP64 evaluations equal bit for bit,
+bounded mixed-vs-FP64 prediction error at the 200,000 prediction points.
+C3: the 256 uneven fields evaluated as one batch equal the fields' own
+evaluations bit for bit; three fields (smallest, median, largest) against
+the oracle in FP64, and their mixed D within the 1e-6 budget.
+C4: the committed warm-start network on held-out fields of the warm-start
+shape (n = 50,000, M = 8, r = 32).
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+from nonstationary_mra_amd import capi, synth, warmstart, workloads
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL_D = 1e-10
+TOL_PRED = 1e-9
+MODEL = os.path.join(os.path.dirname(warmstart.__file__), "data", "warmstart_mlp.npz")
+
+
+def rel(a, b):
+    return abs(a - b) / abs(b)
+
+
+def test_c1_full_size_fp64_matches_oracle():
+    t0 = time.time()
+    tree = workloads.c1_tree()
+    h = workloads.prior_field(tree)
+    try:
+        D, mean, var = h.predict()
+        t_gpu = time.time() - t0
+    finally:
+        h.close()
+    counts = np.diff(tree["obs_offsets"].astype(np.int64))
+    print(f"C1 n={counts.sum()} max_nF={counts.max()} n_pred={len(mean)} gpu+setup {t_gpu:.1f}s", flush=True)
+    oracle.lib().orc_set_blas(1)
+    t1 = time.time()
+    Do, mo, vo = oracle.evaluate(tree, synth.linear_trend_cov(), synth.UNIT_PARAMS)
+    print(f"C1 D gpu {D!r} oracle {Do!r} rel {rel(D, Do):.3e} (oracle {time.time() - t1:.1f}s)", flush=True)
+    assert len(mean) == 50_000 and counts.sum() == 250_000
+    assert rel(D, Do) < TOL_D
+    np.testing.assert_allclose(mean, mo, rtol=TOL_PRED, atol=1e-10)
+    np.testing.assert_allclose(var, vo, rtol=TOL_PRED, atol=1e-10)
+
+
+@pytest.fixture(scope="module")
+def c2():
+    t0 = time.time()
+    tree = workloads.c2_tree()
+    h = workloads.prior_field(tree)
+    print(f"C2 setup {time.time() - t0:.1f}s", flush=True)
+    yield h, len(tree["obs_lon"])
+    h.close()
+
+
+def test_c2_mixed_within_budget_and_graph_bitwise(c2):
+    h, n = c2
+    out = {}
+    for prec in (capi.PRECISION_FP64, capi.PRECISION_MIXED):
+        h.set_precision(prec)
+        h.use_graph(False)
+        eager = [h.likelihood() for _ in range(2)]
+        h.use_graph(True)
+        graph = [h.likelihood() for _ in range(2)]
+        h.use_graph(False)
+        print(f"C2 precision {prec}: eager {eager} graph {graph}", flush=True)
+        assert eager[0] == eager[1] == graph[0] == graph[1]
+        out[prec] = eager[0]
+    dl = abs(out[capi.PRECISION_MIXED] - out[capi.PRECISION_FP64]) / (2 * n)
+    print(f"C2 |dloglik|/n mixed vs FP64 {dl:.3e}", flush=True)
+    assert dl <= 1e-6
+    h.set_precision(capi.PRECISION_FP64)
+
+
+def test_c2_prediction_mixed_vs_fp64(c2):
+    h, n = c2
+    h.set_precision(capi.PRECISION_FP64)
+    D64, m64, v64 = h.predict()
+    D64b, m64b, v64b = h.predict()
+    h.set_precision(capi.PRECISION_MIXED)
+    Dm, mm, vm = h.predict()
+    h.set_precision(capi.PRECISION_FP64)
+    assert len(m64) == 200_000
+    np.testing.assert_array_equal(m64, m64b)
+    np.testing.assert_array_equal(v64, v64b)
+    assert D64 == D64b
+    em = np.abs(mm - m64).max() / np.abs(m64).max()
+    ev = np.max(np.abs(vm - v64) / v64)
+    print(f"C2 prediction mixed vs FP64: mean {em:.3e} (of max |mean|), var {ev:.3e} relative", flush=True)
+    assert np.all(v64 > 0) and np.all(vm > 0)
+    assert em < 1e-4 and ev < 1e-4
+    assert abs(Dm - D64) / (2 * n) <= 1e-6
+
+
+@pytest.fixture(scope="module")
+def c3():
+    t0 = time.time()
+    specs = workloads.c3_specs()
+    hs = [workloads.c3_field(s, i) for i, s in enumerate(specs)]
+    print(f"C3 {len(hs)} fields, {sum(s[0] for s in specs)} points, setup {time.time() - t0:.1f}s", flush=True)
+    yield specs, hs
+    for h in hs:
+        h.close()
+
+
+def test_c3_batch_equals_sequential(c3):
+    specs, hs = c3
+    t0 = time.time()
+    seq = np.array([h.likelihood() for h in hs])
+    t1 = time.time()
+    b = capi.Batch(hs)
+    try:
+        for streams, graph in [(4, 1), (8, 0)]:
+            b.set(streams, graph)
+            D, st = b.likelihood()
+            t2 = time.time()
+            D2, _ = b.likelihood()
+            t3 = time.time()
+            print(f"C3 batch streams={streams} graph={graph}: {1e3 * (t3 - t2):.1f} ms per batch "
+                  f"(sequential {1e3 * (t1 - t0):.1f} ms)", flush=True)
+            assert np.all(st == 0)
+            np.testing.assert_array_equal(D, seq)
+            np.testing.assert_array_equal(D2, seq)
+    finally:
+        b.close()
+
+
+def test_c3_fields_against_oracle(c3):
+    specs, hs = c3
+    ns = np.array([s[0] for s in specs])
+    picks = [int(np.argmin(ns)), int(np.argsort(ns)[len(ns) // 2]), int(np.argmax(ns))]
+    oracle.lib().orc_set_blas(1)
+    for i in picks:
+        h, tree = workloads.c3_field(specs[i], i, precision=capi.PRECISION_FP64, keep_tree=True)
+        try:
+            D64 = h.likelihood()
+            h.set_precision(capi.PRECISION_MIXED)
+            Dm = h.likelihood()
+        finally:
+            h.close()
+        Dm_batch_handle = hs[i].likelihood()
+        Do, _, _ = oracle.evaluate(tree, workloads.trend_cov(specs[i][2]), synth.UNIT_PARAMS, predict=False)
+        dl = abs(Dm - D64) / (2 * specs[i][0])
+        print(f"C3 field {i} n={specs[i][0]} M={specs[i][1]}: FP64 rel {rel(D64, Do):.3e}, "
+              f"mixed |dl|/n {dl:.3e}", flush=True)
+        assert rel(D64, Do) < TOL_D
+        assert Dm == Dm_batch_handle
+        assert dl <= 1e-6
+
+
+def test_c4_committed_network_on_held_out_fields():
+    """The committed network's start against the default start on held-out
+    fields of the warm-start configuration's shape: fewer optimizer
+    evaluations on average, equal final D within the optimizer's resolution."""
+    ws = warmstart.WarmStart.load(MODEL)
+    t0 = time.time()
+    rows = warmstart.compare(ws, 24, 50_000, 8, 32, seed=9001)
+    s = warmstart.summarize(rows)
+    print(f"C4 {s} ({time.time() - t0:.1f}s)", flush=True)
+    assert s["mean_evals_warm"] < s["mean_evals_default"]
+    assert abs(s["mean_D_warm_minus_default"]) < 0.05
