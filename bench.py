@@ -30,7 +30,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from nonstationary_mra_amd import capi, synth, workmodel  # noqa: E402
+from nonstationary_mra_amd import capi, synth, workloads, workmodel  # noqa: E402
 
 PEAK = {"f64": 78.6e12, "f32": 157.3e12}   # MI355X dense matrix FLOP/s (FP64 matrix = vector)
 PEAK_HBM = 8.0e12                         # bytes/s
@@ -212,33 +212,18 @@ def run_c3(args):
     parameters (log range in [log 0.02, log 0.2], range and variance slopes
     in [-1, 1], nugget in [0.01, 0.2]), M chosen so that a finest region holds
     at most 256 points on average, r = 32, mixed precision, fields drawn from
-    the MRA prior; all likelihoods evaluated as one batch (mra_batch_*) whose
-    launch setting comes from the batch autotuner.  One step = one evaluation
-    of every field."""
-    rng = np.random.default_rng(SEED_LOC + 3)
-    hs = []
-    n_tot = 0
+    the MRA prior (workloads.c3_specs / c3_field); all likelihoods evaluated
+    as one batch (mra_batch_*: one captured graph for all fields) whose launch
+    setting comes from the batch autotuner.  One step = one evaluation of
+    every field."""
     t0 = time.time()
-    for i in range(args.c3_fields):
-        n = int(np.exp(rng.uniform(np.log(5000), np.log(200000))))
-        M = 1 + max(1, int(np.ceil(np.log(n / 256.0) / np.log(4.0))))
-        x, y = synth.uniform_points(n, rng)
-        tree = capi.partition_program(x, y, np.zeros(n), None, None, M=M, J=4, r=32, bounds=synth.UNIT_BOUNDS)
-        h = capi.Handle(tree, with_pred=False)
-        th = (0.0, rng.uniform(-1, 1), 0.0, rng.uniform(np.log(0.02), np.log(0.2)), rng.uniform(-1, 1),
-              np.log(rng.uniform(0.01, 0.2)))
-        cov = dict(nonstationary=1, metric=synth.METRIC["planar"], wendland_len=1.0,
-                   sigmasq=dict(kind=synth.FIELD_LINEAR, intercept=th[0], slope_x=th[1], slope_y=th[2]),
-                   beta=dict(kind=synth.FIELD_LINEAR, intercept=th[3], slope_x=th[4], slope_y=0.0),
-                   tausq=dict(kind=synth.FIELD_CONSTANT, intercept=th[5]))
-        h.set_covariance(cov)
-        h.set_params(**synth.UNIT_PARAMS)
-        h.sample_prior(SEED_FIELD + i, replace=True)
-        h.set_precision(capi.PRECISION_MIXED)
-        hs.append(h)
-        n_tot += n
+    specs = workloads.c3_specs(args.c3_fields)
+    hs = []
+    for i, spec in enumerate(specs):
+        hs.append(workloads.c3_field(spec, i))
         if i % 32 == 31:
             log(f"c3: {i + 1} fields ready ({time.time() - t0:.1f}s)")
+    n_tot = sum(s[0] for s in specs)
     b = capi.Batch(hs)
     streams, graph, tune_ms = b.autotune()
     log(f"c3 batch autotune: streams={streams} graph={graph} {tune_ms:.1f} ms")
