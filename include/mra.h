@@ -119,6 +119,14 @@ int mra_likelihood(mra_handle* h, double* D);
  * prediction points in the handle's (finest-region) order. */
 int mra_predict(mra_handle* h, double* D, double* mean, double* var);
 
+/* likelihood() and the log-likelihood the reference prints
+ * (src/class_approximation-posterior-inference.cpp:244):
+ * loglik = -0.5 (D + n log(2*3.14159265359)) with n = n_observations
+ * (NUM_OBSERVATIONS: every observation loaded, including any the program
+ * partition dropped on ancestor knots) or, when 0, the handle's observation
+ * count.  D may be NULL. */
+int mra_loglikelihood(mra_handle* h, uint64_t n_observations, double* D, double* loglik);
+
 /* Per-level partial sums of D: [0] finest level, [1..M-1] levels M-2..0. */
 int mra_likelihood_levels(mra_handle* h, double* terms);
 
@@ -211,6 +219,27 @@ int mra_cross_covariance(const mra_cov_desc* cov, double sill, double range, dou
 int mra_variance_covariance(const mra_cov_desc* cov, double sill, double range, double nugget,
                             double sill_scale, double range_scale, double nugget_scale,
                             const double* lon, const double* lat, uint64_t n, double* out);
+/* The same with the caller's unit-sphere coordinates x, y, z (the
+ * reference's evaluate_* take them, src/evaluate_covariance.cpp:155-313):
+ * used as given for the chordal metric; NULL (or another metric) derives
+ * them from lon/lat. */
+int mra_cross_covariance_xyz(const mra_cov_desc* cov, double sill, double range, double nugget,
+                             double sill_scale, double range_scale, double nugget_scale,
+                             const double* lon1, const double* lat1, const double* x1, const double* y1,
+                             const double* z1, uint64_t n1, const double* lon2, const double* lat2,
+                             const double* x2, const double* y2, const double* z2, uint64_t n2, double* out);
+int mra_variance_covariance_xyz(const mra_cov_desc* cov, double sill, double range, double nugget,
+                                double sill_scale, double range_scale, double nugget_scale,
+                                const double* lon, const double* lat, const double* x, const double* y,
+                                const double* z, uint64_t n, double* out);
+/* Nonstationary_covariance::get_sigmasq / get_beta / get_tausq, both
+ * overloads (src/class_nonstationary_covariance.cpp:479-573): the fields
+ * exp(intercept + sum_i w_i psi(d_i / wendland_len)) (or the linear /
+ * constant forms) at n points, unscaled.  lon/lat are required except for
+ * the chordal metric with x, y, z given (used as given); any output may be
+ * NULL. */
+int mra_fields(const mra_cov_desc* cov, const double* lon, const double* lat, const double* x, const double* y,
+               const double* z, uint64_t n, double* sigmasq, double* beta, double* tausq);
 /* Device-resident form of evaluate_nonstationary_cross_covariance
  * (src/evaluate_covariance.cpp:155-234), the pairwise covariance kernel on
  * its own: d_out[i*n2+j] (row-major n1 x n2, as the reference writes it) for
@@ -350,6 +379,116 @@ int mra_partition_program(const double* lon, const double* lat, const double* va
                           uint64_t* obs_offsets, double* obs_lon, double* obs_lat, double* obs_val,
                           uint64_t* obs_index, uint64_t* pred_offsets, double* pred_lon,
                           double* pred_lat, uint64_t* pred_index);
+
+/* ------------------------------------------------ inputs and outputs (host) */
+/* The reference's run configuration (the globals of include/constants.hpp set
+ * from the `user_parameters` file by src/read_user_parameters.cpp:10-341).
+ * Flags are 0/1; num_levels_m = MRA_LEVELS_DEFAULT means "default" (chosen
+ * by mra_default_levels at partition time). */
+enum { MRA_LEVELS_DEFAULT = -99 };
+typedef struct mra_user_params {
+    char data_file_name[1024];
+    char prediction_location_file[1024];
+    char prediction_results_file_name[1024];
+    char sigmasq_file_name[1024], beta_file_name[1024], tausq_file_name[1024];
+    char knots_file_name[1024];
+    char parent_region_file_name[1024];
+    char tmp_directory[1024];
+    char calculation_mode[64];   /* prediction | optimization | likelihood | build_structure_only */
+    char prediction_location_mode;  /* 'D', 'N' or 'A' */
+    char pad_[3];
+    int32_t elimination_duplicates_flag, regression_flag, nonstationary_flag, print_detail_flag;
+    int32_t dump_prediction_results_flag, save_to_disk_flag, dynamic_schedule_flag, chordal_distance_flag;
+    int32_t provided_knots_flag, obs_as_knots_flag, various_j_finest_level_flag;
+    int32_t num_partitions_j, num_knots_r, num_levels_m, num_knots_finest, max_iterations;
+    double offset, wendland_len;
+    double sigmasq, beta, tausq, sigmasq_scale, beta_scale, tausq_scale;
+    double sigmasq_initial_guess, sigmasq_lower_bound, sigmasq_upper_bound;
+    double beta_initial_guess, beta_lower_bound, beta_upper_bound;
+    double tausq_initial_guess, tausq_lower_bound, tausq_upper_bound;
+} mra_user_params;
+/* Message of the last failed call of this group (the calling thread's). */
+const char* mra_io_last_error(void);
+/* Zero every field, as the reference's globals start (src/constants.cpp:8-120). */
+void mra_user_params_init(mra_user_params* p);
+/* read_from_file() + check_errors() (src/read_user_parameters.cpp:10-341,
+ * 428-637): `name=value` lines, '#' comments, spaces, tabs and quotes
+ * removed, flags "true"/"false", OFFSET=default -> e/100, NUM_LEVELS_M=default
+ * -> MRA_LEVELS_DEFAULT; unknown names ignored.  Fields not named keep their
+ * value in *p (call mra_user_params_init first for the reference's start). */
+int mra_read_user_parameters(const char* path, mra_user_params* p);
+/* check_errors (src/read_user_parameters.cpp:428-637) for one process. */
+int mra_check_user_parameters(const mra_user_params* p);
+
+/* Data::load_data (src/class_data.cpp:15-358): reads DATA_FILE_NAME
+ * (u64 n; f64 lon[n], lat[n], val[n]; u64 region[n] iff
+ * provided_knots_flag), takes the non-NaN values as observations, the domain
+ * bounds over every location padded by 1e-6 of the extent, and in prediction
+ * mode the prediction locations of PREDICTION_LOCATION_MODE: 'D' every data
+ * location, 'N' the NaN ones, 'A' PREDICTION_LOCATION_FILE (u64 n; f64
+ * lon[n], lat[n]; u64 region[n] iff provided_knots_flag).  Then duplicate
+ * elimination (eliminate_duplicate :400-506: locations sorted by (lon, lat),
+ * the first of equal locations kept; region indices travel with their
+ * locations) and the regression detrend (:333-345: least squares on
+ * [1, lon, lat], residuals replace the values). */
+typedef struct mra_data mra_data;
+typedef struct mra_data_view {
+    uint64_t n_locations, n_obs_raw, n_pred_raw, n_obs, n_pred;
+    const double *obs_lon, *obs_lat, *obs_val;  /* obs_val: residuals when regression = 1 */
+    const uint64_t* obs_region;                 /* NULL without region indices */
+    const double *pred_lon, *pred_lat;
+    const uint64_t* pred_region;
+    double bounds[4];                           /* lonMin, lonMax, latMin, latMax (padded) */
+    double coefficients[3];                     /* intercept, lon, lat (regression = 1) */
+    int32_t regression, predicting;
+} mra_data_view;
+int mra_data_load(const mra_user_params* p, mra_data** out);
+void mra_data_destroy(mra_data* d);
+/* Pointers stay valid while d lives. */
+int mra_data_get(const mra_data* d, mra_data_view* v);
+
+/* Partition (src/class_partition.cpp): the number of levels (the default
+ * rule :782-797 when num_levels_m = MRA_LEVELS_DEFAULT), the finest level
+ * (heap children, or the parent file of VARIOUS_J_FINEST_LEVEL_FLAG :659-770:
+ * u64 start; u64 end; u64 parent[end-start+1]), the knots (program grid
+ * :459-654 through mra_partition_program, or the knots file of
+ * PROVIDED_KNOTS_FLAG :93-263: u64 nr; f64 lon[nr], lat[nr]; u64 region[nr]
+ * in runs of r per coarse region) and the assignment of observations and
+ * prediction locations to finest regions (program: point in half-open box,
+ * observations on an ancestor's knot dropped; user files: the region index
+ * of each location, :263-454).  Locations keep their order within a region. */
+typedef struct mra_partition mra_partition;
+int mra_partition_build(const mra_user_params* p, const mra_data* d, mra_partition** out);
+void mra_partition_destroy(mra_partition* P);
+/* The structure as the descriptor of mra_create; pointers into P. */
+int mra_partition_tree(const mra_partition* P, mra_tree_desc* t);
+/* Index into the loaded data of each observation / prediction location of
+ * the tree (finest-region order); *pred_index is NULL without predictions. */
+int mra_partition_indices(const mra_partition* P, const uint64_t** obs_index, const uint64_t** pred_index);
+/* Partition::nRegionsAtEachLevel: counts[l] for l < num_levels. */
+int mra_partition_level_counts(const mra_partition* P, uint64_t* counts);
+/* Partition::dump_structure_information (src/class_partition.cpp:1433-1498):
+ * the same text; path NULL or empty = "structure_information.txt". */
+int mra_partition_dump_structure(const mra_partition* P, const char* path);
+/* The default number of levels: ceil(log(ceil(n/r)) / log J) + 1
+ * (src/class_partition.cpp:782-797); MRA_E_ARG for n = 0. */
+int mra_default_levels(uint64_t n_obs, int J, int r);
+
+/* Nonstationary_covariance::load_data (src/class_nonstationary_covariance.cpp:
+ * 46-170): the three basis files (u64 nb; f64 intercept; f64 lon[nb], lat[nb],
+ * w[nb]) as Wendland fields, metric chordal or great circle from
+ * chordal_distance_flag, wendland_len; constant fields when
+ * nonstationary_flag = 0.  The descriptor's arrays point into the model. */
+typedef struct mra_cov_model mra_cov_model;
+int mra_cov_model_load(const mra_user_params* p, mra_cov_model** out);
+void mra_cov_model_destroy(mra_cov_model* c);
+int mra_cov_model_desc(const mra_cov_model* c, mra_cov_desc* desc);
+
+/* Approximation::dump_prediction_result's file (src/class_approximation.cpp:
+ * 473-550): u64 n; f64 lon[n], lat[n], mean[n], var[n]; coef non-NULL adds
+ * the regression trend coef[0] + coef[1] lon + coef[2] lat to the mean. */
+int mra_write_prediction_result(const char* path, uint64_t n, const double* lon, const double* lat,
+                                const double* mean, const double* var, const double* coef);
 
 /* Self test of the device building blocks (no reference counterpart; test
  * hook): one work group computes Cout = C0 - A^T B with the FP64 matrix-core

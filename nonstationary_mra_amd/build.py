@@ -13,6 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(BUILD, "libmra_hip.so")
 SHIM = os.path.join(BUILD, "shim_example")
+REFMAIN = os.path.join(BUILD, "reference_main")
 ROOT = os.path.dirname(HERE)
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -22,7 +23,7 @@ ARCH = os.environ.get("MRA_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-mllvm", "-amdgpu-mfma-vgpr-form", "-I", os.path.join(ROOT, "include")]
 HIP_SOURCES = ["mra_engine.hip", "mra_mlp.hip"]
-CPP_SOURCES = ["mra_partition.cpp", "mra_optimize.cpp"]
+CPP_SOURCES = ["mra_partition.cpp", "mra_optimize.cpp", "mra_io.cpp"]
 
 
 def run(cmd):
@@ -45,8 +46,9 @@ def build(verbose=True):
     run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs)
     os.replace(tmp, LIB)
     # the reference-style C++ driver over include/mra_approximation.hpp
-    run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), os.path.join(HERE, "examples",
-         "shim_example.cpp"), "-o", SHIM, "-L", BUILD, "-lmra_hip", "-Wl,-rpath,$ORIGIN"])
+    for src, exe in (("shim_example.cpp", SHIM), ("reference_main.cpp", REFMAIN)):
+        run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+             os.path.join(HERE, "examples", src), "-o", exe, "-L", BUILD, "-lmra_hip", "-Wl,-rpath,$ORIGIN"])
     return LIB
 
 

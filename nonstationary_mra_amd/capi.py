@@ -34,7 +34,14 @@ EXPORTED = ["mra_create", "mra_destroy", "mra_last_error", "mra_set_covariance",
             "mra_mlp_forward", "mra_mlp_loss_grad", "mra_mlp_train_step", "mra_batch_create", "mra_batch_destroy",
             "mra_batch_set", "mra_batch_get", "mra_batch_likelihood", "mra_batch_autotune", "mra_set_tuning",
             "mra_get_tuning", "mra_autotune", "mra_cov_tile_timed", "mra_device_alloc", "mra_device_free",
-            "mra_device_copy", "mra_batch_last_error"]
+            "mra_device_copy", "mra_batch_last_error", "mra_loglikelihood", "mra_cross_covariance_xyz",
+            "mra_variance_covariance_xyz", "mra_fields", "mra_io_last_error", "mra_user_params_init",
+            "mra_read_user_parameters", "mra_check_user_parameters", "mra_data_load", "mra_data_destroy",
+            "mra_data_get", "mra_partition_build", "mra_partition_destroy", "mra_partition_tree",
+            "mra_partition_indices", "mra_partition_level_counts", "mra_partition_dump_structure",
+            "mra_default_levels", "mra_cov_model_load", "mra_cov_model_destroy", "mra_cov_model_desc",
+            "mra_write_prediction_result"]
+LEVELS_DEFAULT = -99
 OPT_FACTORED_POSTERIOR = 1
 OPT_GRAM_KERNEL = 2
 OPT_WAVE_LOCAL16 = 3
@@ -60,6 +67,52 @@ class FieldDesc(C.Structure):
 class CovDesc(C.Structure):
     _fields_ = [("nonstationary", C.c_int32), ("metric", C.c_int32), ("wendland_len", C.c_double),
                 ("sigmasq", FieldDesc), ("beta", FieldDesc), ("tausq", FieldDesc)]
+
+
+_PATHS = ("data_file_name", "prediction_location_file", "prediction_results_file_name", "sigmasq_file_name",
+          "beta_file_name", "tausq_file_name", "knots_file_name", "parent_region_file_name", "tmp_directory")
+_FLAGS = ("elimination_duplicates_flag", "regression_flag", "nonstationary_flag", "print_detail_flag",
+          "dump_prediction_results_flag", "save_to_disk_flag", "dynamic_schedule_flag", "chordal_distance_flag",
+          "provided_knots_flag", "obs_as_knots_flag", "various_j_finest_level_flag")
+_INTS = ("num_partitions_j", "num_knots_r", "num_levels_m", "num_knots_finest", "max_iterations")
+_DBLS = ("offset", "wendland_len", "sigmasq", "beta", "tausq", "sigmasq_scale", "beta_scale", "tausq_scale",
+         "sigmasq_initial_guess", "sigmasq_lower_bound", "sigmasq_upper_bound", "beta_initial_guess",
+         "beta_lower_bound", "beta_upper_bound", "tausq_initial_guess", "tausq_lower_bound", "tausq_upper_bound")
+
+
+class UserParams(C.Structure):
+    """mra_user_params: the reference's run configuration (user_parameters keys)."""
+    _fields_ = ([(k, C.c_char * 1024) for k in _PATHS] + [("calculation_mode", C.c_char * 64),
+                                                           ("prediction_location_mode", C.c_char),
+                                                           ("pad_", C.c_char * 3)] +
+                [(k, C.c_int32) for k in _FLAGS + _INTS] + [(k, C.c_double) for k in _DBLS])
+
+    def to_dict(self):
+        d = {k: getattr(self, k).decode() for k in _PATHS + ("calculation_mode",)}
+        d["prediction_location_mode"] = self.prediction_location_mode.decode()
+        d.update({k: getattr(self, k) for k in _FLAGS + _INTS + _DBLS})
+        return d
+
+    @staticmethod
+    def from_dict(d):
+        p = UserParams()
+        lib().mra_user_params_init(C.byref(p))
+        for k, v in d.items():
+            if k in _PATHS or k == "calculation_mode":
+                setattr(p, k, str(v).encode())
+            elif k == "prediction_location_mode":
+                p.prediction_location_mode = str(v).encode()[:1]
+            else:
+                setattr(p, k, v)
+        return p
+
+
+class DataView(C.Structure):
+    _fields_ = [("n_locations", C.c_uint64), ("n_obs_raw", C.c_uint64), ("n_pred_raw", C.c_uint64),
+                ("n_obs", C.c_uint64), ("n_pred", C.c_uint64), ("obs_lon", dp), ("obs_lat", dp), ("obs_val", dp),
+                ("obs_region", u64p), ("pred_lon", dp), ("pred_lat", dp), ("pred_region", u64p),
+                ("bounds", C.c_double * 4), ("coefficients", C.c_double * 3), ("regression", C.c_int32),
+                ("predicting", C.c_int32)]
 
 
 class MraError(RuntimeError):
@@ -142,7 +195,175 @@ def lib():
         _lib.mra_mlp_forward.argtypes = [C.c_void_p, fp_, C.c_int, fp_, C.c_int]
         _lib.mra_mlp_loss_grad.argtypes = [C.c_void_p, fp_, fp_, C.c_int, fp_, fp_]
         _lib.mra_mlp_train_step.argtypes = [C.c_void_p, fp_, fp_, C.c_int, C.c_float, fp_]
+        _lib.mra_loglikelihood.argtypes = [C.c_void_p, C.c_uint64, dp, dp]
+        _lib.mra_cross_covariance_xyz.argtypes = [C.POINTER(CovDesc)] + [C.c_double] * 6 + [dp] * 5 + [C.c_uint64] + \
+            [dp] * 5 + [C.c_uint64, dp]
+        _lib.mra_variance_covariance_xyz.argtypes = [C.POINTER(CovDesc)] + [C.c_double] * 6 + [dp] * 5 + \
+            [C.c_uint64, dp]
+        _lib.mra_fields.argtypes = [C.POINTER(CovDesc), dp, dp, dp, dp, dp, C.c_uint64, dp, dp, dp]
+        _lib.mra_io_last_error.restype = C.c_char_p
+        _lib.mra_io_last_error.argtypes = []
+        _lib.mra_user_params_init.argtypes = [C.POINTER(UserParams)]
+        _lib.mra_user_params_init.restype = None
+        _lib.mra_read_user_parameters.argtypes = [C.c_char_p, C.POINTER(UserParams)]
+        _lib.mra_check_user_parameters.argtypes = [C.POINTER(UserParams)]
+        _lib.mra_data_load.argtypes = [C.POINTER(UserParams), C.POINTER(C.c_void_p)]
+        _lib.mra_data_destroy.argtypes = [C.c_void_p]
+        _lib.mra_data_destroy.restype = None
+        _lib.mra_data_get.argtypes = [C.c_void_p, C.POINTER(DataView)]
+        _lib.mra_partition_build.argtypes = [C.POINTER(UserParams), C.c_void_p, C.POINTER(C.c_void_p)]
+        _lib.mra_partition_destroy.argtypes = [C.c_void_p]
+        _lib.mra_partition_destroy.restype = None
+        _lib.mra_partition_tree.argtypes = [C.c_void_p, C.POINTER(TreeDesc)]
+        _lib.mra_partition_indices.argtypes = [C.c_void_p, C.POINTER(u64p), C.POINTER(u64p)]
+        _lib.mra_partition_level_counts.argtypes = [C.c_void_p, u64p]
+        _lib.mra_partition_dump_structure.argtypes = [C.c_void_p, C.c_char_p]
+        _lib.mra_default_levels.argtypes = [C.c_uint64, C.c_int, C.c_int]
+        _lib.mra_cov_model_load.argtypes = [C.POINTER(UserParams), C.POINTER(C.c_void_p)]
+        _lib.mra_cov_model_destroy.argtypes = [C.c_void_p]
+        _lib.mra_cov_model_destroy.restype = None
+        _lib.mra_cov_model_desc.argtypes = [C.c_void_p, C.POINTER(CovDesc)]
+        _lib.mra_write_prediction_result.argtypes = [C.c_char_p, C.c_uint64, dp, dp, dp, dp, dp]
     return _lib
+
+
+def check_io(rc):
+    if rc != MRA_OK:
+        raise MraError(rc, lib().mra_io_last_error().decode())
+
+
+def read_user_parameters(path, base=None):
+    """mra_read_user_parameters: the `user_parameters` file as a dict (fields
+    the file does not name keep `base`'s values, else zero)."""
+    p = UserParams.from_dict(base or {})
+    check_io(lib().mra_read_user_parameters(str(path).encode(), C.byref(p)))
+    return p.to_dict()
+
+
+def check_user_parameters(params):
+    check_io(lib().mra_check_user_parameters(C.byref(UserParams.from_dict(params))))
+
+
+def _arr(ptr, n, dtype):
+    if n == 0 or not ptr:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+class DataSet:
+    """mra_data: Data::load_data of a parameter dict (files, NaN split,
+    prediction location mode, duplicates, regression)."""
+
+    def __init__(self, params):
+        self.params = UserParams.from_dict(params)
+        d = C.c_void_p()
+        check_io(lib().mra_data_load(C.byref(self.params), C.byref(d)))
+        self.d = d
+        v = DataView()
+        check_io(lib().mra_data_get(self.d, C.byref(v)))
+        self.n_locations, self.n_obs_raw, self.n_pred_raw = v.n_locations, v.n_obs_raw, v.n_pred_raw
+        self.obs_lon, self.obs_lat = _arr(v.obs_lon, v.n_obs, np.float64), _arr(v.obs_lat, v.n_obs, np.float64)
+        self.obs_val = _arr(v.obs_val, v.n_obs, np.float64)
+        self.obs_region = _arr(v.obs_region, v.n_obs, np.uint64) if v.obs_region else None
+        self.pred_lon, self.pred_lat = _arr(v.pred_lon, v.n_pred, np.float64), _arr(v.pred_lat, v.n_pred, np.float64)
+        self.pred_region = _arr(v.pred_region, v.n_pred, np.uint64) if v.pred_region else None
+        self.bounds = np.array(v.bounds[:])
+        self.coefficients = np.array(v.coefficients[:]) if v.regression else None
+        self.predicting = bool(v.predicting)
+
+    def partition(self, params=None):
+        """mra_partition_build: the tree dict (as partition_program returns)."""
+        p = UserParams.from_dict(params) if params is not None else self.params
+        P = C.c_void_p()
+        check_io(lib().mra_partition_build(C.byref(p), self.d, C.byref(P)))
+        try:
+            t = TreeDesc()
+            check_io(lib().mra_partition_tree(P, C.byref(t)))
+            nC, nF, no, nq = t.n_coarse, t.n_finest, t.n_obs, t.n_pred
+            oi, pi = u64p(), u64p()
+            check_io(lib().mra_partition_indices(P, C.byref(oi), C.byref(pi)))
+            counts = np.zeros(t.num_levels, dtype=np.uint64)
+            check_io(lib().mra_partition_level_counts(P, _u(counts)))
+            tree = dict(M=t.num_levels, J=t.num_partitions, r=t.num_knots, various_j=t.various_j, n_coarse=nC,
+                        n_finest=nF, knot_lon=_arr(t.knot_lon, nC * t.num_knots, np.float64),
+                        knot_lat=_arr(t.knot_lat, nC * t.num_knots, np.float64),
+                        finest_parent=_arr(t.finest_parent, nF, np.uint64),
+                        obs_offsets=_arr(t.obs_offsets, nF + 1, np.uint64), obs_lon=_arr(t.obs_lon, no, np.float64),
+                        obs_lat=_arr(t.obs_lat, no, np.float64), obs_val=_arr(t.obs_val, no, np.float64),
+                        obs_index=_arr(oi, no, np.uint64), level_counts=counts)
+            if t.pred_offsets:
+                tree.update(pred_offsets=_arr(t.pred_offsets, nF + 1, np.uint64),
+                            pred_lon=_arr(t.pred_lon, nq, np.float64), pred_lat=_arr(t.pred_lat, nq, np.float64),
+                            pred_index=_arr(pi, nq, np.uint64))
+            return tree
+        finally:
+            lib().mra_partition_destroy(P)
+
+    def dump_structure(self, path, params=None):
+        """mra_partition_dump_structure of the partition of this data set."""
+        p = UserParams.from_dict(params) if params is not None else self.params
+        P = C.c_void_p()
+        check_io(lib().mra_partition_build(C.byref(p), self.d, C.byref(P)))
+        try:
+            check_io(lib().mra_partition_dump_structure(P, str(path).encode()))
+        finally:
+            lib().mra_partition_destroy(P)
+
+    def close(self):
+        if getattr(self, "d", None):
+            lib().mra_data_destroy(self.d)
+            self.d = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def default_levels(n, J, r):
+    return lib().mra_default_levels(n, J, r)
+
+
+def load_cov_model(params):
+    """mra_cov_model_load: the covariance model of a parameter dict as the
+    cov dict of Handle.set_covariance (arrays copied)."""
+    p = UserParams.from_dict(params)
+    m = C.c_void_p()
+    check_io(lib().mra_cov_model_load(C.byref(p), C.byref(m)))
+    try:
+        c = CovDesc()
+        check_io(lib().mra_cov_model_desc(m, C.byref(c)))
+
+        def fld(f):
+            d = dict(kind=f.kind, intercept=f.intercept, slope_x=f.slope_x, slope_y=f.slope_y)
+            if f.kind == FIELD_WENDLAND:
+                n = f.n_basis
+                d.update(lon=_arr(f.basis_lon, n, np.float64), lat=_arr(f.basis_lat, n, np.float64),
+                         w=_arr(f.basis_weight, n, np.float64))
+            return d
+        return dict(nonstationary=c.nonstationary, metric=c.metric, wendland_len=c.wendland_len,
+                    sigmasq=fld(c.sigmasq), beta=fld(c.beta), tausq=fld(c.tausq))
+    finally:
+        lib().mra_cov_model_destroy(m)
+
+
+def write_prediction_result(path, lon, lat, mean, var, coef=None):
+    keep = Keep()
+    n = len(lon)
+    check_io(lib().mra_write_prediction_result(str(path).encode(), n, keep.d(lon), keep.d(lat), keep.d(mean),
+                                               keep.d(var), keep.d(coef) if coef is not None else None))
+
+
+def fields(cov, lon=None, lat=None, x=None, y=None, z=None):
+    """mra_fields: (sigmasq, beta, tausq) at the points, unscaled."""
+    keep = Keep()
+    c = cov_desc(cov, keep)
+    n = len(lon) if lon is not None else len(x)
+    s, b, t = np.zeros(n), np.zeros(n), np.zeros(n)
+    opt = lambda a: keep.d(a) if a is not None else None
+    check(lib().mra_fields(C.byref(c), opt(lon), opt(lat), opt(x), opt(y), opt(z), n, _d(s), _d(b), _d(t)))
+    return s, b, t
 
 
 def selftest(K, Mv, Nv, A, B, C0, S, Bt):
@@ -472,6 +693,12 @@ class Handle:
         D = C.c_double()
         check(lib().mra_likelihood(self.h, C.byref(D)), self.h)
         return D.value
+
+    def loglikelihood(self, n_observations=0):
+        """mra_loglikelihood: (D, -0.5 (D + n log(2*3.14159265359)))."""
+        D, ell = C.c_double(), C.c_double()
+        check(lib().mra_loglikelihood(self.h, n_observations, C.byref(D), C.byref(ell)), self.h)
+        return D.value, ell.value
 
     def likelihood_levels(self):
         out = np.zeros(self.M)

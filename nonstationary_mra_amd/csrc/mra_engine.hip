@@ -1434,6 +1434,20 @@ extern "C" int mra_likelihood(mra_handle* h, double* D) {
     return MRA_OK;
 }
 
+// The log-likelihood the reference prints
+// (src/class_approximation-posterior-inference.cpp:244):
+// -0.5 (D + n log(2 * 3.14159265359)), n = n_observations or, when 0, the
+// handle's observation count.
+extern "C" int mra_loglikelihood(mra_handle* h, uint64_t n_observations, double* D, double* loglik) {
+    if (!h || !loglik) return MRA_E_ARG;
+    double d = 0.0;
+    RC_TRY(mra_likelihood(h, &d));
+    const uint64_t n = n_observations ? n_observations : h->nObs;
+    *loglik = -0.5 * (d + n * std::log(2 * 3.14159265359));
+    if (D) *D = d;
+    return MRA_OK;
+}
+
 extern "C" int mra_likelihood_levels(mra_handle* h, double* terms) {
     if (!h || !terms) return MRA_E_ARG;
     RC_TRY(run_eval(h, false));
@@ -1505,8 +1519,12 @@ struct TmpPts {
     }
 };
 
+// Point data of a covariance evaluation on temporary device buffers.  With
+// xyz non-NULL and the chordal metric the caller's unit-sphere coordinates
+// are used as given (the reference passes x, y, z computed on the host).
 static int tmp_points(mra_handle* h, const mra_cov_desc* cov, double sill, double range, double nugget, double ss,
-                      double rs, double ns, const double* lon, const double* lat, uint64_t n, TmpPts& t) {
+                      double rs, double ns, const double* lon, const double* lat, uint64_t n, TmpPts& t,
+                      const double* const* xyz = nullptr) {
     double *dl, *da;
     RC_TRY(dalloc(h, &dl, n, t.list));
     RC_TRY(dalloc(h, &da, n, t.list));
@@ -1517,6 +1535,9 @@ static int tmp_points(mra_handle* h, const mra_cov_desc* cov, double sill, doubl
         RC_TRY(dalloc(h, &t.f[k], n, t.list));
         RC_TRY(dalloc(h, &t.s[k], n, t.list));
     }
+    const bool given = xyz && xyz[0] && xyz[1] && xyz[2] && cov->metric == MRA_METRIC_CHORDAL;
+    if (given)
+        for (int k = 0; k < 3; k++) HIP_TRY(hipMemcpy(t.c[k], xyz[k], sizeof(double) * n, hipMemcpyHostToDevice));
     FieldDev f[3];
     std::vector<void*> saved;
     saved.swap(h->fieldAllocs);
@@ -1526,7 +1547,7 @@ static int tmp_points(mra_handle* h, const mra_cov_desc* cov, double sill, doubl
     for (void* p : h->fieldAllocs) t.list.push_back(p);
     h->fieldAllocs.swap(saved);
     if (rc) return rc;
-    k_coords<<<nblk(n), 256>>>(n, dl, da, cov->metric, t.c[0], t.c[1], t.c[2]);
+    if (!given) k_coords<<<nblk(n), 256>>>(n, dl, da, cov->metric, t.c[0], t.c[1], t.c[2]);
     k_fields<<<nblk(n), 256>>>(n, dl, da, cov->metric, cov->wendland_len, f[0], f[1], f[2], t.f[0], t.f[1], t.f[2]);
     double* dp;
     RC_TRY(dalloc(h, &dp, 6, t.list));
@@ -1537,10 +1558,12 @@ static int tmp_points(mra_handle* h, const mra_cov_desc* cov, double sill, doubl
     return MRA_OK;
 }
 
-extern "C" int mra_cross_covariance(const mra_cov_desc* cov, double sill, double range, double nugget,
-                                    double sill_scale, double range_scale, double nugget_scale, const double* lon1,
-                                    const double* lat1, uint64_t n1, const double* lon2, const double* lat2,
-                                    uint64_t n2, double* out) {
+extern "C" int mra_cross_covariance_xyz(const mra_cov_desc* cov, double sill, double range, double nugget,
+                                        double sill_scale, double range_scale, double nugget_scale,
+                                        const double* lon1, const double* lat1, const double* x1, const double* y1,
+                                        const double* z1, uint64_t n1, const double* lon2, const double* lat2,
+                                        const double* x2, const double* y2, const double* z2, uint64_t n2,
+                                        double* out) {
     mra_handle tmp;
     mra_handle* h = &tmp;
     if (!cov || !out || !n1 || !n2 || !lon1 || !lat1 || !lon2 || !lat2) {
@@ -1548,8 +1571,10 @@ extern "C" int mra_cross_covariance(const mra_cov_desc* cov, double sill, double
         return MRA_E_ARG;
     }
     TmpPts a, b;
-    RC_TRY(tmp_points(h, cov, sill, range, nugget, sill_scale, range_scale, nugget_scale, lon1, lat1, n1, a));
-    RC_TRY(tmp_points(h, cov, sill, range, nugget, sill_scale, range_scale, nugget_scale, lon2, lat2, n2, b));
+    const double* xyz1[3] = {x1, y1, z1};
+    const double* xyz2[3] = {x2, y2, z2};
+    RC_TRY(tmp_points(h, cov, sill, range, nugget, sill_scale, range_scale, nugget_scale, lon1, lat1, n1, a, xyz1));
+    RC_TRY(tmp_points(h, cov, sill, range, nugget, sill_scale, range_scale, nugget_scale, lon2, lat2, n2, b, xyz2));
     double* d;
     std::vector<void*> l;
     RC_TRY(dalloc(h, &d, n1 * n2, l));
@@ -1565,9 +1590,18 @@ extern "C" int mra_cross_covariance(const mra_cov_desc* cov, double sill, double
     return MRA_OK;
 }
 
-extern "C" int mra_variance_covariance(const mra_cov_desc* cov, double sill, double range, double nugget,
-                                       double sill_scale, double range_scale, double nugget_scale, const double* lon,
-                                       const double* lat, uint64_t n, double* out) {
+extern "C" int mra_cross_covariance(const mra_cov_desc* cov, double sill, double range, double nugget,
+                                    double sill_scale, double range_scale, double nugget_scale, const double* lon1,
+                                    const double* lat1, uint64_t n1, const double* lon2, const double* lat2,
+                                    uint64_t n2, double* out) {
+    return mra_cross_covariance_xyz(cov, sill, range, nugget, sill_scale, range_scale, nugget_scale, lon1, lat1,
+                                    nullptr, nullptr, nullptr, n1, lon2, lat2, nullptr, nullptr, nullptr, n2, out);
+}
+
+extern "C" int mra_variance_covariance_xyz(const mra_cov_desc* cov, double sill, double range, double nugget,
+                                           double sill_scale, double range_scale, double nugget_scale,
+                                           const double* lon, const double* lat, const double* x, const double* y,
+                                           const double* z, uint64_t n, double* out) {
     mra_handle tmp;
     mra_handle* h = &tmp;
     if (!cov || !out || !n || !lon || !lat) {
@@ -1575,7 +1609,8 @@ extern "C" int mra_variance_covariance(const mra_cov_desc* cov, double sill, dou
         return MRA_E_ARG;
     }
     TmpPts a;
-    RC_TRY(tmp_points(h, cov, sill, range, nugget, sill_scale, range_scale, nugget_scale, lon, lat, n, a));
+    const double* xyz[3] = {x, y, z};
+    RC_TRY(tmp_points(h, cov, sill, range, nugget, sill_scale, range_scale, nugget_scale, lon, lat, n, a, xyz));
     double* d;
     std::vector<void*> l;
     RC_TRY(dalloc(h, &d, n * n, l));
@@ -1587,6 +1622,42 @@ extern "C" int mra_variance_covariance(const mra_cov_desc* cov, double sill, dou
         g_err = hipGetErrorString(e);
         return MRA_E_HIP;
     }
+    return MRA_OK;
+}
+
+extern "C" int mra_variance_covariance(const mra_cov_desc* cov, double sill, double range, double nugget,
+                                       double sill_scale, double range_scale, double nugget_scale, const double* lon,
+                                       const double* lat, uint64_t n, double* out) {
+    return mra_variance_covariance_xyz(cov, sill, range, nugget, sill_scale, range_scale, nugget_scale, lon, lat,
+                                       nullptr, nullptr, nullptr, n, out);
+}
+
+// Nonstationary_covariance::get_sigmasq / get_beta / get_tausq
+// (src/class_nonstationary_covariance.cpp:479-573): exp(log field) at each
+// point, unscaled.  lon/lat are needed by the great-circle, planar and
+// linear forms; x, y, z (chordal metric) are used as given when non-NULL.
+// Any output may be NULL.
+extern "C" int mra_fields(const mra_cov_desc* cov, const double* lon, const double* lat, const double* x,
+                          const double* y, const double* z, uint64_t n, double* sigmasq, double* beta,
+                          double* tausq) {
+    mra_handle tmp;
+    mra_handle* h = &tmp;
+    const bool xyz = x && y && z && cov && cov->metric == MRA_METRIC_CHORDAL;
+    if (!cov || !n || ((!lon || !lat) && !xyz)) {
+        g_err = "invalid argument";
+        return MRA_E_ARG;
+    }
+    mra_cov_desc c = *cov;
+    c.nonstationary = 1;  // the fields themselves, whatever the model
+    std::vector<double> zeros;
+    if (!lon || !lat) zeros.assign(n, 0.0);
+    TmpPts a;
+    const double* pxyz[3] = {x, y, z};
+    RC_TRY(tmp_points(h, &c, 0.0, 1.0, 0.0, 1.0, 1.0, 1.0, lon ? lon : zeros.data(), lat ? lat : zeros.data(), n, a,
+                      pxyz));
+    double* outs[3] = {sigmasq, beta, tausq};
+    for (int k = 0; k < 3; k++)
+        if (outs[k]) HIP_TRY(hipMemcpy(outs[k], a.f[k], sizeof(double) * n, hipMemcpyDeviceToHost));
     return MRA_OK;
 }
 
