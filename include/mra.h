@@ -490,6 +490,21 @@ int mra_cov_model_desc(const mra_cov_model* c, mra_cov_desc* desc);
 int mra_write_prediction_result(const char* path, uint64_t n, const double* lon, const double* lat,
                                 const double* mean, const double* var, const double* coef);
 
+/* The same partition built on the GPU (one thread per location descends the
+ * region boxes; a stable radix sort groups the locations by finest region):
+ * identical knots, assignment and order.  Unlike the host form it needs no
+ * sizes call: obs_lon / obs_lat / obs_val / obs_index must hold n entries and
+ * pred_* np entries; sizes[4] receives what mra_partition_program_sizes
+ * reports (knot count, finest regions, kept observations, kept prediction
+ * locations).  Up to 2^31 - 1 locations of each kind. */
+int mra_partition_program_gpu(const double* lon, const double* lat, const double* val, uint64_t n,
+                              const double* plon, const double* plat, uint64_t np, int M, int J, int r,
+                              double offset, const double* bounds, double* knot_lon, double* knot_lat,
+                              uint64_t* finest_parent, uint64_t* obs_offsets, double* obs_lon, double* obs_lat,
+                              double* obs_val, uint64_t* obs_index, uint64_t* pred_offsets, double* pred_lon,
+                              double* pred_lat, uint64_t* pred_index, uint64_t* sizes);
+const char* mra_partition_gpu_last_error(void);
+
 /* Self test of the device building blocks (no reference counterpart; test
  * hook): one work group computes Cout = C0 - A^T B with the FP64 matrix-core
  * GEMM (A, B: K x 64 row-major; only the Mv x Nv leading part of Cout is

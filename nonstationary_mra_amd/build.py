@@ -22,7 +22,7 @@ ARCH = os.environ.get("MRA_OFFLOAD_ARCH", "gfx950")
 # around every matrix instruction in loops whose accumulators are carried)
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-mllvm", "-amdgpu-mfma-vgpr-form", "-I", os.path.join(ROOT, "include")]
-HIP_SOURCES = ["mra_engine.hip", "mra_mlp.hip"]
+HIP_SOURCES = ["mra_engine.hip", "mra_mlp.hip", "mra_partition_gpu.hip"]
 CPP_SOURCES = ["mra_partition.cpp", "mra_optimize.cpp", "mra_io.cpp"]
 
 
@@ -32,16 +32,19 @@ def run(cmd):
 
 
 def build(verbose=True):
+    from concurrent.futures import ThreadPoolExecutor
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
+    jobs = []
     for s in CPP_SOURCES:
         o = os.path.join(BUILD, os.path.splitext(s)[0] + ".o")
-        run(["g++", "-O2", "-std=c++17", "-fPIC", "-c", os.path.join(CSRC, s), "-o", o])
-        objs.append(o)
+        jobs.append((["g++", "-O2", "-std=c++17", "-fPIC", "-c", os.path.join(CSRC, s), "-o", o], o))
     for s in HIP_SOURCES:
         o = os.path.join(BUILD, os.path.splitext(s)[0] + ".o")
-        run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, s), "-o", o])
-        objs.append(o)
+        jobs.append(([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, s), "-o", o], o))
+    # the translation units compile independently (the engine is the long one)
+    with ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 1)) as ex:
+        list(ex.map(lambda j: run(j[0]), jobs))
+    objs = [o for _, o in jobs]
     tmp = LIB + ".tmp"
     run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs)
     os.replace(tmp, LIB)

@@ -40,7 +40,7 @@ EXPORTED = ["mra_create", "mra_destroy", "mra_last_error", "mra_set_covariance",
             "mra_data_get", "mra_partition_build", "mra_partition_destroy", "mra_partition_tree",
             "mra_partition_indices", "mra_partition_level_counts", "mra_partition_dump_structure",
             "mra_default_levels", "mra_cov_model_load", "mra_cov_model_destroy", "mra_cov_model_desc",
-            "mra_write_prediction_result"]
+            "mra_write_prediction_result", "mra_partition_program_gpu", "mra_partition_gpu_last_error"]
 LEVELS_DEFAULT = -99
 OPT_FACTORED_POSTERIOR = 1
 OPT_GRAM_KERNEL = 2
@@ -224,6 +224,11 @@ def lib():
         _lib.mra_cov_model_destroy.restype = None
         _lib.mra_cov_model_desc.argtypes = [C.c_void_p, C.POINTER(CovDesc)]
         _lib.mra_write_prediction_result.argtypes = [C.c_char_p, C.c_uint64, dp, dp, dp, dp, dp]
+        _lib.mra_partition_program_gpu.argtypes = [dp, dp, dp, C.c_uint64, dp, dp, C.c_uint64, C.c_int, C.c_int,
+                                                   C.c_int, C.c_double, dp, dp, dp, u64p, u64p, dp, dp, dp, u64p,
+                                                   u64p, dp, dp, u64p, u64p]
+        _lib.mra_partition_gpu_last_error.restype = C.c_char_p
+        _lib.mra_partition_gpu_last_error.argtypes = []
     return _lib
 
 
@@ -807,8 +812,11 @@ def cov_matrix_timed(precision, metric, xyz1, sb1, n1, xyz2, sb2, n2, out, launc
     return ms.value
 
 
-def partition_program(lon, lat, val, plon, plat, M, J, r, offset=np.e / 100, bounds=None):
-    """Program-placed partition (host code of the library)."""
+def partition_program(lon, lat, val, plon, plat, M, J, r, offset=np.e / 100, bounds=None, gpu=False):
+    """Program-placed partition: the library's host builder, or with gpu=True
+    its GPU builder (mra_partition_program_gpu; identical results)."""
+    if gpu:
+        return partition_program_gpu(lon, lat, val, plon, plat, M, J, r, offset, bounds)
     lon = np.ascontiguousarray(lon, dtype=np.float64)
     lat = np.ascontiguousarray(lat, dtype=np.float64)
     val = np.ascontiguousarray(val, dtype=np.float64)
@@ -835,6 +843,41 @@ def partition_program(lon, lat, val, plon, plat, M, J, r, offset=np.e / 100, bou
     check(lib().mra_partition_program(_d(lon), _d(lat), _d(val), n, _d(plon), _d(plat), npred, M, J, r, offset,
                                       _d(b), _d(kl), _d(kt), _u(parent), _u(oo), _d(ol), _d(ot), _d(ov), _u(oi),
                                       _u(po), _d(pl), _d(pt), _u(pi)))
+    return dict(M=M, J=J, r=nk // nC, various_j=0, n_coarse=nC, n_finest=nF, knot_lon=kl, knot_lat=kt,
+                finest_parent=parent, obs_offsets=oo, obs_lon=ol[:no], obs_lat=ot[:no], obs_val=ov[:no],
+                obs_index=oi[:no], pred_offsets=po, pred_lon=pl[:nq], pred_lat=pt[:nq], pred_index=pi[:nq], bounds=b)
+
+
+def partition_program_gpu(lon, lat, val, plon, plat, M, J, r, offset=np.e / 100, bounds=None):
+    """mra_partition_program_gpu: the program partition built on the GPU."""
+    lon = np.ascontiguousarray(lon, dtype=np.float64)
+    lat = np.ascontiguousarray(lat, dtype=np.float64)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    plon = np.ascontiguousarray(plon if plon is not None else np.zeros(0), dtype=np.float64)
+    plat = np.ascontiguousarray(plat if plat is not None else np.zeros(0), dtype=np.float64)
+    if bounds is None:
+        mnl, mxl, mna, mxa = lon.min(), lon.max(), lat.min(), lat.max()
+        bounds = np.array([mnl, mxl + 1e-6 * (mxl - mnl), mna, mxa + 1e-6 * (mxa - mna)])
+    b = np.ascontiguousarray(bounds, dtype=np.float64)
+    n, npred = len(lon), len(plon)
+    nC = sum(J ** l for l in range(M - 1))
+    nlon = int(np.ceil(np.sqrt(r)))
+    nk = nC * nlon * (r // nlon)
+    nF = J ** (M - 1)
+    kl, kt = np.zeros(nk), np.zeros(nk)
+    parent = np.zeros(nF, dtype=np.uint64)
+    oo, po = np.zeros(nF + 1, dtype=np.uint64), np.zeros(nF + 1, dtype=np.uint64)
+    ol, ot, ov = np.zeros(max(n, 1)), np.zeros(max(n, 1)), np.zeros(max(n, 1))
+    oi = np.zeros(max(n, 1), dtype=np.uint64)
+    pl, pt = np.zeros(max(npred, 1)), np.zeros(max(npred, 1))
+    pi = np.zeros(max(npred, 1), dtype=np.uint64)
+    sizes = np.zeros(4, dtype=np.uint64)
+    rc = lib().mra_partition_program_gpu(_d(lon), _d(lat), _d(val), n, _d(plon), _d(plat), npred, M, J, r, offset,
+                                         _d(b), _d(kl), _d(kt), _u(parent), _u(oo), _d(ol), _d(ot), _d(ov), _u(oi),
+                                         _u(po), _d(pl), _d(pt), _u(pi), _u(sizes))
+    if rc != MRA_OK:
+        raise MraError(rc, lib().mra_partition_gpu_last_error().decode())
+    no, nq = int(sizes[2]), int(sizes[3])
     return dict(M=M, J=J, r=nk // nC, various_j=0, n_coarse=nC, n_finest=nF, knot_lon=kl, knot_lat=kt,
                 finest_parent=parent, obs_offsets=oo, obs_lon=ol[:no], obs_lat=ot[:no], obs_val=ov[:no],
                 obs_index=oi[:no], pred_offsets=po, pred_lon=pl[:nq], pred_lat=pt[:nq], pred_index=pi[:nq], bounds=b)

@@ -16,89 +16,14 @@
 
 #include "../../include/mra.h"
 
+#include "mra_partition_grid.hpp"
+
 namespace {
 
-struct Grid {
-    int M, J, nLon, nLat, nKnots;
-    uint64_t nCoarse, nFinest, nTotal;
-};
-
-Grid make_grid(int M, int J, int r) {
-    Grid g;
-    g.M = M;
-    g.J = J;
-    g.nLon = (int)std::ceil(std::sqrt((double)r));
-    g.nLat = r / g.nLon;
-    g.nKnots = g.nLon * g.nLat;
-    uint64_t tot = 1, cnt = 1;
-    for (int l = 1; l < M - 1; l++) {
-        cnt *= (uint64_t)J;
-        tot += cnt;
-    }
-    g.nCoarse = tot;
-    g.nFinest = cnt * (uint64_t)J;
-    g.nTotal = tot + g.nFinest;
-    return g;
-}
-
-struct Box {
-    double lonMin, lonMax, latMin, latMax;
-};
-
-// Children boxes of a box, in the reference's child order.
-void split(const Box& b, int J, Box* c) {
-    if (J == 2) {
-        if ((b.lonMax - b.lonMin) >= (b.latMax - b.latMin)) {
-            double mid = (b.lonMax + b.lonMin) / 2;
-            c[0] = {b.lonMin, mid, b.latMin, b.latMax};
-            c[1] = {mid, b.lonMax, b.latMin, b.latMax};
-        } else {
-            double mid = (b.latMax + b.latMin) / 2;
-            c[0] = {b.lonMin, b.lonMax, b.latMin, mid};
-            c[1] = {b.lonMin, b.lonMax, mid, b.latMax};
-        }
-    } else {
-        double lm = (b.lonMax + b.lonMin) / 2, am = (b.latMax + b.latMin) / 2;
-        c[0] = {b.lonMin, lm, b.latMin, am};
-        c[1] = {lm, b.lonMax, b.latMin, am};
-        c[2] = {b.lonMin, lm, am, b.latMax};
-        c[3] = {lm, b.lonMax, am, b.latMax};
-    }
-}
+using namespace mra_part;
 
 inline bool inside(const Box& b, double lon, double lat) {
     return lon >= b.lonMin && lon < b.lonMax && lat >= b.latMin && lat < b.latMax;
-}
-
-// Knot grid of one box: lon values (nLon) and lat values (nLat); knot
-// (i, j) sits at index i*nLat + j.
-void knot_axes(const Box& b, const Grid& g, double offset, double* lonv, double* latv) {
-    double offLon = (b.lonMax - b.lonMin) * offset, offLat = (b.latMax - b.latMin) * offset;
-    double lonStart = b.lonMin + offLon, lonEnd = b.lonMax - offLon;
-    double latStart = b.latMin + offLat, latEnd = b.latMax - offLat;
-    double incLon = (g.nLon == 1 ? 0 : (lonEnd - lonStart) / (g.nLon - 1));
-    double incLat = (g.nLat == 1 ? 0 : (latEnd - latStart) / (g.nLat - 1));
-    for (int i = 0; i < g.nLon; i++) lonv[i] = lonEnd - (g.nLon - i - 1) * incLon;
-    for (int j = 0; j < g.nLat; j++) latv[j] = latEnd - (g.nLat - j - 1) * incLat;
-}
-
-struct Built {
-    Grid g;
-    std::vector<Box> box;           // all regions
-    std::vector<double> klon, klat; // knot axes per coarse region: nLon / nLat values
-};
-
-Built build_boxes(const Grid& g, const double* bounds, double offset) {
-    Built b;
-    b.g = g;
-    b.box.resize(g.nTotal);
-    b.box[0] = {bounds[0], bounds[1], bounds[2], bounds[3]};
-    for (uint64_t R = 0; R < g.nCoarse; R++) split(b.box[R], g.J, &b.box[R * g.J + 1]);
-    b.klon.resize(g.nCoarse * g.nLon);
-    b.klat.resize(g.nCoarse * g.nLat);
-    for (uint64_t R = 0; R < g.nCoarse; R++)
-        knot_axes(b.box[R], g, offset, &b.klon[R * g.nLon], &b.klat[R * g.nLat]);
-    return b;
 }
 
 // Finest region (local index) containing the point, or -1.
@@ -160,16 +85,12 @@ void assign_pred(const Built& b, const double* lon, const double* lat, uint64_t 
     for (int64_t i = 0; i < (int64_t)n; i++) reg[i] = locate(b, lon[i], lat[i]);
 }
 
-bool valid(int M, int J, int r, const double* bounds) {
-    return M >= 2 && M <= 24 && (J == 2 || J == 4) && r >= 1 && bounds != nullptr;
-}
-
 }  // namespace
 
 extern "C" int mra_partition_program_sizes(const double* lon, const double* lat, uint64_t n, const double* plon,
                                            const double* plat, uint64_t np, int M, int J, int r, double offset,
                                            const double* bounds, uint64_t* sizes) {
-    if (!valid(M, J, r, bounds) || !sizes || (n && (!lon || !lat)) || (np && (!plon || !plat))) return MRA_E_ARG;
+    if (!valid_args(M, J, r, bounds) || !sizes || (n && (!lon || !lat)) || (np && (!plon || !plat))) return MRA_E_ARG;
     Grid g = make_grid(M, J, r);
     Built b = build_boxes(g, bounds, offset);
     std::vector<int64_t> reg;
@@ -193,16 +114,11 @@ extern "C" int mra_partition_program(const double* lon, const double* lat, const
                                      uint64_t* finest_parent, uint64_t* obs_offsets, double* obs_lon, double* obs_lat,
                                      double* obs_val, uint64_t* obs_index, uint64_t* pred_offsets, double* pred_lon,
                                      double* pred_lat, uint64_t* pred_index) {
-    if (!valid(M, J, r, bounds) || !knot_lon || !knot_lat || !obs_offsets) return MRA_E_ARG;
+    if (!valid_args(M, J, r, bounds) || !knot_lon || !knot_lat || !obs_offsets) return MRA_E_ARG;
     if (n && (!lon || !lat || !val || !obs_lon || !obs_lat || !obs_val)) return MRA_E_ARG;
     Grid g = make_grid(M, J, r);
     Built b = build_boxes(g, bounds, offset);
-    for (uint64_t R = 0; R < g.nCoarse; R++)
-        for (int i = 0; i < g.nLon; i++)
-            for (int j = 0; j < g.nLat; j++) {
-                knot_lon[R * g.nKnots + i * g.nLat + j] = b.klon[R * g.nLon + i];
-                knot_lat[R * g.nKnots + i * g.nLat + j] = b.klat[R * g.nLat + j];
-            }
+    expand_knots(b, knot_lon, knot_lat);
     if (finest_parent)
         for (uint64_t f = 0; f < g.nFinest; f++) finest_parent[f] = (g.nCoarse + f - 1) / (uint64_t)J;
     std::vector<int64_t> reg;

@@ -192,3 +192,105 @@ def test_great_circle_vs_chord():
     cc = oracle.cross_covariance(dict(base, metric=0), p, lon[:1], lat[:1], lon[1:], lat[1:])[0, 0]
     theta, chord = -np.log(cg), -np.log(cc)
     assert abs(chord - 2 * np.sin(theta / 2)) < 1e-12
+
+
+# ----------------------------------------------------------------------------
+# The oracle's program partition (orc_partition_program, a restatement of
+# src/class_partition.cpp:24-89,459-654) pinned by properties computed here
+# without it: the closed-form knot grid, half-open box membership of every
+# assigned location, and the drop of observations lying on an ancestor's knot.
+
+def _boxes(bounds, M, J):
+    """Region boxes by recursive bisection at midpoints, heap order."""
+    boxes = [tuple(bounds)]
+    n_total = sum(J ** l for l in range(M))
+    for R in range(n_total):
+        if len(boxes) >= n_total:
+            break
+        lo0, lo1, la0, la1 = boxes[R]
+        if J == 4:
+            lm, am = (lo0 + lo1) / 2, (la0 + la1) / 2
+            boxes += [(lo0, lm, la0, am), (lm, lo1, la0, am), (lo0, lm, am, la1), (lm, lo1, am, la1)]
+        elif lo1 - lo0 >= la1 - la0:
+            lm = (lo0 + lo1) / 2
+            boxes += [(lo0, lm, la0, la1), (lm, lo1, la0, la1)]
+        else:
+            am = (la0 + la1) / 2
+            boxes += [(lo0, lo1, la0, am), (lo0, lo1, am, la1)]
+    return boxes[:n_total]
+
+
+@pytest.mark.parametrize("M, J, r", [(3, 4, 64), (4, 2, 16), (3, 4, 32), (2, 4, 9)])
+def test_program_partition_knot_grid_closed_form(M, J, r):
+    """Each coarse region's knots: a ceil(sqrt r) x floor(r / ceil(sqrt r))
+    grid, equally spaced, first and last knot OFFSET x width inside the box."""
+    rng = np.random.default_rng(M + J + r)
+    lon, lat = rng.uniform(-3, 9, 500), rng.uniform(2, 5, 500)
+    bounds = oracle.domain_bounds(lon, lat)
+    tree = oracle.partition_program(lon, lat, np.zeros(500), None, None, M=M, J=J, r=r)
+    nlon = int(np.ceil(np.sqrt(r)))
+    nlat = r // nlon
+    assert tree["r"] == nlon * nlat
+    off = np.e / 100
+    for R, (lo0, lo1, la0, la1) in enumerate(_boxes(bounds, M, J)[:tree["n_coarse"]]):
+        kl = tree["knot_lon"][R * tree["r"]:(R + 1) * tree["r"]].reshape(nlon, nlat)
+        kt = tree["knot_lat"][R * tree["r"]:(R + 1) * tree["r"]].reshape(nlon, nlat)
+        assert np.all(kl == kl[:, :1]) and np.all(kt == kt[:1, :])  # a tensor grid
+        xs, ys = kl[:, 0], kt[0, :]
+        np.testing.assert_allclose([xs[0], xs[-1]], [lo0 + off * (lo1 - lo0), lo1 - off * (lo1 - lo0)], rtol=0,
+                                   atol=1e-12 * (abs(lo0) + abs(lo1)))
+        np.testing.assert_allclose([ys[0], ys[-1]], [la0 + off * (la1 - la0), la1 - off * (la1 - la0)], rtol=0,
+                                   atol=1e-12 * (abs(la0) + abs(la1)))
+        if nlon > 1:
+            np.testing.assert_allclose(np.diff(xs), np.diff(xs).mean(), rtol=1e-9)
+        if nlat > 1:
+            np.testing.assert_allclose(np.diff(ys), np.diff(ys).mean(), rtol=1e-9)
+
+
+@pytest.mark.parametrize("M, J", [(4, 4), (5, 2)])
+def test_program_partition_half_open_membership(M, J):
+    """Every assigned observation / prediction lies in its finest region's
+    half-open box [lonMin, lonMax) x [latMin, latMax); locations keep their
+    input order inside a region; a location on the outer upper edge is
+    outside every box and dropped."""
+    rng = np.random.default_rng(M * J)
+    n = 4000
+    lon, lat = rng.uniform(0, 1, n), rng.uniform(0, 1, n)
+    # points exactly on interior box edges (midpoints of the unit square's bisections)
+    lon[:40] = rng.integers(1, 8, 40) / 8.0
+    lat[40:80] = rng.integers(1, 8, 40) / 8.0
+    bounds = np.array([0.0, 1.0, 0.0, 1.0])
+    lon[80], lat[80] = 1.0, 0.5  # on the outer upper edge: in no box
+    tree = oracle.partition_program(lon, lat, np.arange(n, dtype=float), lon[:200], lat[:200], M=M, J=J, r=4,
+                                    bounds=bounds)
+    boxes = _boxes(bounds, M, J)[tree["n_coarse"]:]
+    for kind, idx_key in (("obs", "obs_index"), ("pred", "pred_index")):
+        off = tree[f"{kind}_offsets"].astype(np.int64)
+        for f, (lo0, lo1, la0, la1) in enumerate(boxes):
+            s = slice(off[f], off[f + 1])
+            x, y = tree[f"{kind}_lon"][s], tree[f"{kind}_lat"][s]
+            assert np.all((x >= lo0) & (x < lo1) & (y >= la0) & (y < la1))
+            assert np.all(np.diff(tree[idx_key][s].astype(np.int64)) > 0)
+    assert 80 not in set(tree["obs_index"].tolist())
+    # every other point is assigned unless it sits on an ancestor's knot (none here: knots are off-grid)
+    assert len(tree["obs_lon"]) == n - 1
+
+
+def test_program_partition_drops_observations_on_ancestor_knots():
+    """Observations placed exactly on a knot of one of their ancestors are
+    dropped (src/class_partition.cpp:547-565); prediction locations at the
+    same places are kept."""
+    M, J, r = 3, 4, 16
+    rng = np.random.default_rng(77)
+    lon, lat = rng.uniform(0, 1, 1000), rng.uniform(0, 1, 1000)
+    bounds = np.array([0.0, 1.0, 0.0, 1.0])
+    probe = oracle.partition_program(lon, lat, np.zeros(1000), None, None, M=M, J=J, r=r, bounds=bounds)
+    # knots of the root and of two level-1 regions, as exact coordinates
+    picks = [0, 5, 17, 16 + 3, 16 * 2 + 9, 16 * 4 + 15]
+    klon, klat = probe["knot_lon"][picks], probe["knot_lat"][picks]
+    lon2, lat2 = np.concatenate([lon, klon]), np.concatenate([lat, klat])
+    tree = oracle.partition_program(lon2, lat2, np.zeros(len(lon2)), klon, klat, M=M, J=J, r=r, bounds=bounds)
+    kept = set(tree["obs_index"].tolist())
+    assert all(1000 + k not in kept for k in range(len(picks)))
+    assert len(tree["obs_lon"]) == 1000
+    assert len(tree["pred_lon"]) == len(picks)
