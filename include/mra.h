@@ -232,6 +232,21 @@ int mra_variance_covariance_xyz(const mra_cov_desc* cov, double sill, double ran
                                 double sill_scale, double range_scale, double nugget_scale,
                                 const double* lon, const double* lat, const double* x, const double* y,
                                 const double* z, uint64_t n, double* out);
+/* Allocation-free form of mra_cross_covariance (no counterpart in the
+ * reference; for callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and the same values bit for bit; every device buffer it needs is
+ * carved from one static device array of
+ * mra_cross_covariance_noalloc_limit() bytes (32 MiB) per GPU, held until
+ * the process ends, so it calls no hipMalloc / hipFree.  Inputs beyond that
+ * array (8 n1 n2 + 88 (n1 + n2) + 48 x basis functions bytes, each piece
+ * rounded up to 256) are refused with MRA_E_ARG.  Calls that overlap on one
+ * GPU share the array and race; run them one at a time per GPU. */
+int mra_cross_covariance_noalloc(const mra_cov_desc* cov, double sill, double range, double nugget,
+                                 double sill_scale, double range_scale, double nugget_scale, const double* lon1,
+                                 const double* lat1, uint64_t n1, const double* lon2, const double* lat2,
+                                 uint64_t n2, double* out);
+uint64_t mra_cross_covariance_noalloc_limit(void);
 /* Nonstationary_covariance::get_sigmasq / get_beta / get_tausq, both
  * overloads (src/class_nonstationary_covariance.cpp:479-573): the fields
  * exp(intercept + sum_i w_i psi(d_i / wendland_len)) (or the linear /

@@ -40,7 +40,8 @@ EXPORTED = ["mra_create", "mra_destroy", "mra_last_error", "mra_set_covariance",
             "mra_data_get", "mra_partition_build", "mra_partition_destroy", "mra_partition_tree",
             "mra_partition_indices", "mra_partition_level_counts", "mra_partition_dump_structure",
             "mra_default_levels", "mra_cov_model_load", "mra_cov_model_destroy", "mra_cov_model_desc",
-            "mra_write_prediction_result", "mra_partition_program_gpu", "mra_partition_gpu_last_error"]
+            "mra_write_prediction_result", "mra_partition_program_gpu", "mra_partition_gpu_last_error",
+            "mra_cross_covariance_noalloc", "mra_cross_covariance_noalloc_limit"]
 LEVELS_DEFAULT = -99
 OPT_FACTORED_POSTERIOR = 1
 OPT_GRAM_KERNEL = 2
@@ -228,6 +229,9 @@ def lib():
                                                    C.c_int, C.c_double, dp, dp, dp, u64p, u64p, dp, dp, dp, u64p,
                                                    u64p, dp, dp, u64p, u64p]
         _lib.mra_partition_gpu_last_error.restype = C.c_char_p
+        _lib.mra_cross_covariance_noalloc.argtypes = _lib.mra_cross_covariance.argtypes
+        _lib.mra_cross_covariance_noalloc_limit.argtypes = []
+        _lib.mra_cross_covariance_noalloc_limit.restype = C.c_uint64
         _lib.mra_partition_gpu_last_error.argtypes = []
     return _lib
 

@@ -1598,6 +1598,160 @@ extern "C" int mra_cross_covariance(const mra_cov_desc* cov, double sill, double
                                     nullptr, nullptr, nullptr, n1, lon2, lat2, nullptr, nullptr, nullptr, n2, out);
 }
 
+// ------------------------------------------------ allocation-free variant
+// mra_cross_covariance_noalloc: the same evaluation as mra_cross_covariance
+// with every device buffer (point coordinates, fields, scaled parameters,
+// basis arrays, the output staging) carved from one file-scope device array
+// instead of hipMalloc.  The array's device address is looked up once per
+// device with hipGetSymbolAddress; the name `scratch` itself is a host-side
+// stand-in and is never passed to a kernel or a copy.  Calls that overlap on
+// one device share the array and race; nothing in it has to start at zero
+// (every byte read is written first in the same call).
+static constexpr uint64_t MRA_NOALLOC_BYTES = 32ull << 20;
+static __device__ __align__(256) unsigned char scratch[MRA_NOALLOC_BYTES];
+static constexpr int MRA_MAX_DEVICES = 64;
+static unsigned char* g_scratch_ptr[MRA_MAX_DEVICES] = {};
+
+static int scratch_base(unsigned char** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= MRA_MAX_DEVICES) {
+        g_err = std::string("hipGetDevice: ") + hipGetErrorString(e);
+        return MRA_E_HIP;
+    }
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            g_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return MRA_E_HIP;
+        }
+        g_scratch_ptr[dev] = (unsigned char*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return MRA_OK;
+}
+
+// Bump allocation of 256-byte aligned slices of the scratch array.
+struct ScratchCarver {
+    unsigned char* base;
+    uint64_t used = 0;
+    template <typename T>
+    T* take(uint64_t count) {
+        T* p = (T*)(base + used);
+        used += (sizeof(T) * count + 255) & ~uint64_t(255);
+        return p;
+    }
+};
+
+static uint64_t noalloc_bytes(const mra_cov_desc* cov, uint64_t n1, uint64_t n2) {
+    auto r = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
+    uint64_t b = r(8 * n1 * n2) + r(8 * 6);
+    for (uint64_t n : {n1, n2}) b += 11 * r(8 * n);
+    for (const mra_field_desc* f : {&cov->sigmasq, &cov->beta, &cov->tausq})
+        if (f->kind == MRA_FIELD_WENDLAND) b += 6 * r(8 * f->n_basis);
+    return b;
+}
+
+extern "C" uint64_t mra_cross_covariance_noalloc_limit(void) { return MRA_NOALLOC_BYTES; }
+
+extern "C" int mra_cross_covariance_noalloc(const mra_cov_desc* cov, double sill, double range, double nugget,
+                                            double sill_scale, double range_scale, double nugget_scale,
+                                            const double* lon1, const double* lat1, uint64_t n1, const double* lon2,
+                                            const double* lat2, uint64_t n2, double* out) {
+    mra_handle* const h = nullptr;
+    if (!cov || !out || !n1 || !n2 || !lon1 || !lat1 || !lon2 || !lat2 || cov->metric < 0 || cov->metric > 2) {
+        g_err = "invalid argument";
+        return MRA_E_ARG;
+    }
+    for (const mra_field_desc* f : {&cov->sigmasq, &cov->beta, &cov->tausq})
+        if (f->kind == MRA_FIELD_WENDLAND && f->n_basis && (!f->basis_lon || !f->basis_lat || !f->basis_weight)) {
+            g_err = "null basis arrays";
+            return MRA_E_ARG;
+        }
+    const uint64_t need = noalloc_bytes(cov, n1, n2);
+    if (n1 > (1ull << 32) || n2 > (1ull << 32) || need > MRA_NOALLOC_BYTES) {
+        g_err = "mra_cross_covariance_noalloc: " + std::to_string(n1) + " x " + std::to_string(n2) +
+                " needs " + std::to_string(need) + " bytes of scratch, more than its " +
+                std::to_string(MRA_NOALLOC_BYTES) + " (use mra_cross_covariance)";
+        return MRA_E_ARG;
+    }
+    unsigned char* base = nullptr;
+    RC_TRY(scratch_base(&base));
+    ScratchCarver sc{base};
+    // basis arrays of the Wendland fields (host unit-sphere coordinates, as field_dev)
+    FieldDev fd[3];
+    const mra_field_desc* fs[3] = {&cov->sigmasq, &cov->beta, &cov->tausq};
+    for (int k = 0; k < 3; k++) {
+        const mra_field_desc& f = *fs[k];
+        FieldDev& d = fd[k];
+        d.kind = f.kind;
+        d.nb = (int)f.n_basis;
+        d.intercept = f.intercept;
+        d.sx = f.slope_x;
+        d.sy = f.slope_y;
+        d.bx = d.by = d.bz = d.blon = d.blat = d.w = nullptr;
+        if (f.kind != MRA_FIELD_WENDLAND || !f.n_basis) continue;
+        const uint64_t nb = f.n_basis;
+        std::vector<double> host(6 * nb);
+        for (uint64_t i = 0; i < nb; i++) {
+            const double cosLat = std::cos(f.basis_lat[i] * M_PI / 180), cosLon = std::cos(f.basis_lon[i] * M_PI / 180);
+            const double sinLat = std::sin(f.basis_lat[i] * M_PI / 180), sinLon = std::sin(f.basis_lon[i] * M_PI / 180);
+            host[i] = cosLat * cosLon;
+            host[nb + i] = cosLat * sinLon;
+            host[2 * nb + i] = sinLat;
+            host[3 * nb + i] = f.basis_lon[i];
+            host[4 * nb + i] = f.basis_lat[i];
+            host[5 * nb + i] = f.basis_weight[i];
+        }
+        double* p[6];
+        for (int c = 0; c < 6; c++) {
+            p[c] = sc.take<double>(nb);
+            HIP_TRY(hipMemcpy(p[c], host.data() + c * nb, sizeof(double) * nb, hipMemcpyHostToDevice));
+        }
+        d.bx = p[0];
+        d.by = p[1];
+        d.bz = p[2];
+        d.blon = p[3];
+        d.blat = p[4];
+        d.w = p[5];
+    }
+    double* prm = sc.take<double>(6);
+    const double hp[6] = {sill, range, nugget, sill_scale, range_scale, nugget_scale};
+    HIP_TRY(hipMemcpy(prm, hp, sizeof(hp), hipMemcpyHostToDevice));
+    Pts P[2];
+    const double* lons[2] = {lon1, lon2};
+    const double* lats[2] = {lat1, lat2};
+    const uint64_t ns[2] = {n1, n2};
+    for (int k = 0; k < 2; k++) {
+        const uint64_t n = ns[k];
+        double* dl = sc.take<double>(n);
+        double* da = sc.take<double>(n);
+        double *c[3], *f[3], *sv[3];
+        for (int q = 0; q < 3; q++) {
+            c[q] = sc.take<double>(n);
+            f[q] = sc.take<double>(n);
+            sv[q] = sc.take<double>(n);
+        }
+        HIP_TRY(hipMemcpy(dl, lons[k], sizeof(double) * n, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(da, lats[k], sizeof(double) * n, hipMemcpyHostToDevice));
+        k_coords<<<nblk(n), 256>>>(n, dl, da, cov->metric, c[0], c[1], c[2]);
+        k_fields<<<nblk(n), 256>>>(n, dl, da, cov->metric, cov->wendland_len, fd[0], fd[1], fd[2], f[0], f[1], f[2]);
+        k_scale<<<nblk(n), 256>>>(n, f[0], f[1], f[2], cov->nonstationary ? 0 : 1, prm, sv[0], sv[1], sv[2]);
+        P[k] = Pts{c[0], c[1], c[2], sv[0], sv[1], sv[2]};
+    }
+    double* d = sc.take<double>(n1 * n2);
+    if (sc.used > MRA_NOALLOC_BYTES) {  // the size check above makes this unreachable
+        g_err = "scratch overflow";
+        return MRA_E_ARG;
+    }
+    CovCtx cc{cov->metric, cov->nonstationary ? 0 : 1, sill, range};
+    k_cross_cov<<<nblk(n1 * n2), 256>>>(cc, P[0], n1, P[1], n2, d);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(out, d, sizeof(double) * n1 * n2, hipMemcpyDeviceToHost));
+    return MRA_OK;
+}
+
 extern "C" int mra_variance_covariance_xyz(const mra_cov_desc* cov, double sill, double range, double nugget,
                                            double sill_scale, double range_scale, double nugget_scale,
                                            const double* lon, const double* lat, const double* x, const double* y,
