@@ -177,14 +177,20 @@ int mra_use_graph(mra_handle* h, int enable);
  * children when the parent level sums dense child messages, so that the
  * parent reads one block set instead of one per child; 0 stores every
  * region's message (same values up to rounding).  mra_get_option reports
- * whether the summed form is in effect. */
+ * whether the summed form is in effect.
+ * MRA_OPT_XCD_MAP: 1 (default) has the coarse prior and finest conditioning
+ * launches hand out their regions in 8 contiguous ranges, one per XCD
+ * (work groups are dispatched to the XCDs round robin), so that neighbouring
+ * regions, which read the same ancestor blocks, run under one L2; 0 takes
+ * them in launch order (same values). */
 enum {
     MRA_OPT_FACTORED_POSTERIOR = 1,
     MRA_OPT_GRAM_KERNEL = 2,
     MRA_OPT_WAVE_LOCAL16 = 3,
     MRA_OPT_CROSS16 = 4,
     MRA_OPT_CONCURRENT = 5,
-    MRA_OPT_SUM_MESSAGES = 6
+    MRA_OPT_SUM_MESSAGES = 6,
+    MRA_OPT_XCD_MAP = 7
 };
 int mra_set_option(mra_handle* h, int option, int value);
 int mra_get_option(mra_handle* h, int option, int* value);

@@ -49,6 +49,7 @@ OPT_WAVE_LOCAL16 = 3
 OPT_CROSS16 = 4
 OPT_CONCURRENT = 5
 OPT_SUM_MESSAGES = 6
+OPT_XCD_MAP = 7
 
 
 class TreeDesc(C.Structure):

@@ -61,6 +61,7 @@ struct mra_handle {
     hipEvent_t evFork[NFORK] = {}, evJoin[NFORK][2] = {};
     int cross16 = 1;      // segments with <= 16 observations in 16-column cross tiles
     int sumMsgs = 1;      // message blocks summed per parent by k_post_gram
+    int xcdMap = 1;       // XCD-contiguous item ranges (Tree::xcd)
     bool denseFull = false;  // full-size dense messages of level L-1 allocated
     std::vector<uint64_t> hRowS1, hRowE1, hRowS2, hRowE2;  // rows of levels L-1 and L-2
     uint64_t *dRowS1 = nullptr, *dRowE1 = nullptr, *dRowS2 = nullptr, *dRowE2 = nullptr;
@@ -918,6 +919,11 @@ extern "C" int mra_set_option(mra_handle* h, int option, int value) {
         graph_invalidate(h);
         return MRA_OK;
     }
+    if (option == MRA_OPT_XCD_MAP) {
+        h->xcdMap = value ? 1 : 0;
+        graph_invalidate(h);
+        return MRA_OK;
+    }
     set_err(h, "unknown option");
     return MRA_E_ARG;
 }
@@ -948,6 +954,10 @@ extern "C" int mra_get_option(mra_handle* h, int option, int* value) {
         *value = (h->RP == 64 && h->gramSep && h->sumMsgs) ? 1 : 0;
         return MRA_OK;
     }
+    if (option == MRA_OPT_XCD_MAP) {
+        *value = h->xcdMap;
+        return MRA_OK;
+    }
     set_err(h, "unknown option");
     return MRA_E_ARG;
 }
@@ -960,6 +970,7 @@ static Tree make_tree(mra_handle* h) {
     T.r = h->r;
     T.RP = h->RP;
     T.LRP = h->LRP;
+    T.xcd = h->xcdMap;
     T.nC = h->nC;
     T.nF = h->nF;
     T.parent = h->dParent;

@@ -26,6 +26,7 @@ namespace mra {
 
 struct Tree {
     int M, J, L, r, RP, LRP;
+    int xcd;                  // 1: work groups take items in XCD-contiguous ranges (xcd_block)
     uint64_t nC, nF;
     const uint64_t* parent;   // [nF] global coarse id
     const uint64_t* obs_off;  // [nF+1]
@@ -50,6 +51,17 @@ struct Tree {
     const uint64_t* p_grp_os;  // [nF] first prediction point of F's group
     const uint32_t* p_grp_n;   // [nF] prediction points in F's group
 };
+
+// Item of work group b among n when xcd = 1: the device dispatches work
+// groups to its 8 XCDs round robin (group b on XCD b mod 8), so group b takes
+// item b/8 of the b mod 8-th of 8 contiguous item ranges, and the items of one
+// range (neighbouring regions, which read the same ancestor blocks) share one
+// XCD's L2.  A bijection of [0, n).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n, int xcd) {
+    if (!xcd) return b;
+    const uint32_t x = b & 7u, i = b >> 3, q = n >> 3, r = n & 7u;
+    return x * q + (x < r ? x : r) + i;
+}
 
 __device__ __forceinline__ uint64_t lvl_start(int l, int J) {
     uint64_t s = 0, c = 1;
@@ -304,9 +316,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     __shared__ PtV kst[2][RP];  // knots of the current and the next row set
     __shared__ double dbuf[RP >= 48 ? 1 : RP * 16];
     __shared__ int bad;
-    const uint64_t R = start_l + blockIdx.x;
+    const uint32_t bid = xcd_block(blockIdx.x, gridDim.x, T.xcd);
+    const uint64_t R = start_l + bid;
     const int r = T.r;
-    double* GR = Gc + goff[l] + (uint64_t)blockIdx.x * l * RP * RP;
+    double* GR = Gc + goff[l] + (uint64_t)bid * l * RP * RP;
     if (threadIdx.x == 0) bad = 0;
     // this thread's column: a knot of R (the same in every step)
     const int cq = threadIdx.x % RP;
@@ -336,7 +349,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         }
         // G_{R,j} = L_a^{-1} B, stored straight to global memory
         // (mixed precision: also an FP32 copy for the finest-level products)
-        const uint64_t o = goff[l] + (uint64_t)blockIdx.x * l * RP * RP + (uint64_t)j * RP * RP;
+        const uint64_t o = goff[l] + (uint64_t)bid * l * RP * RP + (uint64_t)j * RP * RP;
         wg_gemm_tn<false, false, double, double, GEMM_FULL, COPY32, GEMM_TRI_A>(
             GR + (uint64_t)j * RP * RP, RP, RP, RP, RP, 1.0, LiT + a * RP * RP, RP, RP, Cm, LD, RP,
             COPY32 ? Gc32 + o : nullptr);
@@ -381,7 +394,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     constexpr int LD = NT + 1;
     __shared__ T Cm[RP * LD];
     __shared__ PtV kst[2][RP];  // ancestor knots, double-buffered
-    const uint32_t sg = items[2 * blockIdx.x], c0 = items[2 * blockIdx.x + 1];
+    const uint32_t bid = xcd_block(blockIdx.x, gridDim.x, T0.xcd);
+    const uint32_t sg = items[2 * bid], c0 = items[2 * bid + 1];
     const uint64_t ob = T0.s_os[sg];
     const int nF = (int)T0.s_n[sg];
     const int n = min(NT, nF - (int)c0);

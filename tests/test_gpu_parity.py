@@ -517,3 +517,39 @@ def test_graph_replay_matches_eager():
     finally:
         eager.close()
         graph.close()
+
+
+def test_xcd_item_order():
+    """Regions handed out in XCD-contiguous ranges (MRA_OPT_XCD_MAP, the
+    default) or in launch order: the same D and predictions to the last bit,
+    eager and graph, FP64 and mixed, and the oracle's D."""
+    rng = np.random.default_rng(90)
+    x, y = synth.clustered_points(6000, rng, n_clusters=3, sigma=0.03)
+    px, py = synth.uniform_points(300, rng)
+    tree = capi.partition_program(x, y, rng.standard_normal(6000), px, py, M=5, J=4, r=64,
+                                  bounds=synth.UNIT_BOUNDS)
+    cov, prm = synth.linear_trend_cov(log_range=(np.log(0.1), 0.8, 0.0)), dict(synth.UNIT_PARAMS)
+    h = capi.Handle(tree)
+    out = {}
+    try:
+        h.set_covariance(cov)
+        h.set_params(**prm)
+        for prec in (capi.PRECISION_FP64, capi.PRECISION_MIXED):
+            h.set_precision(prec)
+            for xcd in (1, 0):
+                h.set_option(capi.OPT_XCD_MAP, xcd)
+                assert h.get_option(capi.OPT_XCD_MAP) == xcd
+                for graph in (0, 1):
+                    h.use_graph(graph)
+                    out[(prec, xcd, graph)] = h.predict()
+    finally:
+        h.close()
+    for prec in (capi.PRECISION_FP64, capi.PRECISION_MIXED):
+        ref = out[(prec, 1, 0)]
+        for k, v in out.items():
+            if k[0] == prec:
+                assert v[0] == ref[0]
+                np.testing.assert_array_equal(v[1], ref[1])
+                np.testing.assert_array_equal(v[2], ref[2])
+    Do, _, _ = oracle.evaluate(tree, cov, prm, predict=False)
+    assert rel(out[(capi.PRECISION_FP64, 1, 0)][0], Do) < TOL_D
