@@ -295,11 +295,21 @@ typedef double (*mra_objective_fn)(const double* x, void* ctx);
  * (src/optimization.cpp:80,169: dlib::find_min_bobyqa with npt = 2n+1):
  * quadratic interpolation models with least-Frobenius-norm updates, trust
  * region with bounds, resolution rho from rho_begin down to rho_end.  x is
- * the start on entry and the best point on return.  npt: 0 or 2n+1.
+ * the start on entry and the best point on return.  npt: 0 (= 2n+1, as the
+ * reference) or any value from 2n+1 to (n+1)(n+2)/2 (a full quadratic model;
+ * the points beyond 2n+1 step along pairs of coordinates).
  * Requires hi[i] - lo[i] >= 2 rho_begin and max_evals >= 2n+2.  Returns
  * MRA_OK, MRA_W_MAX_EVALS or a negative error. */
 int mra_bobyqa(int n, int npt, mra_objective_fn fn, void* ctx, double* x, const double* lo, const double* hi,
                double rho_begin, double rho_end, int max_evals, double* f_best, int* n_evals);
+
+/* The same with a starting curvature: h0 (n x n row-major, symmetrised; NULL
+ * = none) is the Hessian the first quadratic model starts from, the first
+ * model being the interpolant of the initial points whose Hessian is closest
+ * to h0 in the Frobenius norm (no counterpart in the reference; the learned
+ * warm start supplies it). */
+int mra_bobyqa_h0(int n, int npt, mra_objective_fn fn, void* ctx, double* x, const double* lo, const double* hi,
+                  double rho_begin, double rho_end, int max_evals, const double* h0, double* f_best, int* n_evals);
 
 /* Maximum-likelihood fit of the six parameters of the planar linear-trend
  * nonstationary model, theta = (a_s, bx_s, by_s, a_b, bx_b, log_tau2):
@@ -312,6 +322,12 @@ int mra_bobyqa(int n, int npt, mra_objective_fn fn, void* ctx, double* x, const 
 enum { MRA_TREND_NPARAM = 6 };
 int mra_fit_linear_trend(mra_handle* h, int metric, double* theta, const double* lo, const double* hi,
                          double rho_begin, double rho_end, int max_evals, double* D_best, int* n_evals);
+/* The same with the minimiser's interpolation set size npt (0 = 2n+1 = 13,
+ * up to 28 = a full quadratic model in the six parameters) and an optional
+ * starting curvature h0 (6 x 6, as mra_bobyqa_h0; NULL = none). */
+int mra_fit_linear_trend_npt(mra_handle* h, int metric, double* theta, const double* lo, const double* hi,
+                             double rho_begin, double rho_end, int max_evals, int npt, const double* h0,
+                             double* D_best, int* n_evals);
 
 /* --------------------------------------------------------------- autotuner */
 /* Work-list shape of a handle (no reference counterpart):

@@ -41,7 +41,8 @@ EXPORTED = ["mra_create", "mra_destroy", "mra_last_error", "mra_set_covariance",
             "mra_partition_indices", "mra_partition_level_counts", "mra_partition_dump_structure",
             "mra_default_levels", "mra_cov_model_load", "mra_cov_model_destroy", "mra_cov_model_desc",
             "mra_write_prediction_result", "mra_partition_program_gpu", "mra_partition_gpu_last_error",
-            "mra_cross_covariance_noalloc", "mra_cross_covariance_noalloc_limit"]
+            "mra_cross_covariance_noalloc", "mra_cross_covariance_noalloc_limit", "mra_fit_linear_trend_npt",
+            "mra_bobyqa_h0"]
 LEVELS_DEFAULT = -99
 OPT_FACTORED_POSTERIOR = 1
 OPT_GRAM_KERNEL = 2
@@ -173,6 +174,10 @@ def lib():
                                     C.c_int, dp, C.POINTER(C.c_int)]
         _lib.mra_fit_linear_trend.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, C.c_double, C.c_double, C.c_int, dp,
                                               C.POINTER(C.c_int)]
+        _lib.mra_fit_linear_trend_npt.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, C.c_double, C.c_double, C.c_int,
+                                                  C.c_int, dp, dp, C.POINTER(C.c_int)]
+        _lib.mra_bobyqa_h0.argtypes = [C.c_int, C.c_int, OBJECTIVE, C.c_void_p, dp, dp, dp, C.c_double, C.c_double,
+                                       C.c_int, dp, dp, C.POINTER(C.c_int)]
         _lib.mra_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
         _lib.mra_get_tuning.argtypes = [C.c_void_p] + [C.POINTER(C.c_int)] * 3
         _lib.mra_autotune.argtypes = [C.c_void_p] + [C.POINTER(C.c_int)] * 3 + [dp]
@@ -391,7 +396,7 @@ MRA_W_MAX_EVALS = 1
 TREND_NPARAM = 6
 
 
-def bobyqa(fun, x0, lo, hi, rho_begin, rho_end, max_evals):
+def bobyqa(fun, x0, lo, hi, rho_begin, rho_end, max_evals, npt=0, h0=None):
     """mra_bobyqa on a Python objective fun(x) -> float.  Returns
     (x_best, f_best, n_evals, status) with status MRA_OK or MRA_W_MAX_EVALS."""
     n = len(x0)
@@ -401,8 +406,9 @@ def bobyqa(fun, x0, lo, hi, rho_begin, rho_end, max_evals):
     cb = OBJECTIVE(lambda p, ctx: float(fun(np.ctypeslib.as_array(p, shape=(n,)).copy())))
     fb = C.c_double(0.0)
     ne = C.c_int(0)
-    rc = lib().mra_bobyqa(n, 0, cb, None, _d(x), _d(lo), _d(hi), rho_begin, rho_end, max_evals, C.byref(fb),
-                          C.byref(ne))
+    H0 = np.ascontiguousarray(h0, dtype=np.float64).reshape(-1) if h0 is not None else None
+    rc = lib().mra_bobyqa_h0(n, npt, cb, None, _d(x), _d(lo), _d(hi), rho_begin, rho_end, max_evals,
+                             _d(H0) if H0 is not None else None, C.byref(fb), C.byref(ne))
     if rc < 0:
         check(rc)
     return x, fb.value, ne.value, rc
@@ -630,16 +636,19 @@ class Handle:
         self.n_pred = int(t.n_pred)
         self.M = tree["M"]
 
-    def fit_linear_trend(self, theta0, lo, hi, rho_begin=0.1, rho_end=1e-6, max_evals=400, metric=METRIC_PLANAR):
+    def fit_linear_trend(self, theta0, lo, hi, rho_begin=0.1, rho_end=1e-6, max_evals=400, metric=METRIC_PLANAR,
+                         npt=0, h0=None):
         """Maximum-likelihood fit of the six linear-trend parameters
-        (mra_fit_linear_trend).  Returns (theta, D, n_evals, status)."""
+        (mra_fit_linear_trend_npt; h0 = starting curvature or None).
+        Returns (theta, D, n_evals, status)."""
         th = np.ascontiguousarray(theta0, dtype=np.float64).copy()
         lo = np.ascontiguousarray(lo, dtype=np.float64)
         hi = np.ascontiguousarray(hi, dtype=np.float64)
         D = C.c_double(0.0)
         ne = C.c_int(0)
-        rc = lib().mra_fit_linear_trend(self.h, metric, _d(th), _d(lo), _d(hi), rho_begin, rho_end, max_evals,
-                                        C.byref(D), C.byref(ne))
+        H0 = np.ascontiguousarray(h0, dtype=np.float64).reshape(-1) if h0 is not None else None
+        rc = lib().mra_fit_linear_trend_npt(self.h, metric, _d(th), _d(lo), _d(hi), rho_begin, rho_end, max_evals,
+                                            npt, _d(H0) if H0 is not None else None, C.byref(D), C.byref(ne))
         if rc < 0:
             check(rc, self.h)
         return th, D.value, ne.value, rc

@@ -546,8 +546,8 @@ static int alloc_engine(mra_handle* h, const mra_tree_desc* t) {
     RC_TRY(upload(h, &h->dOlat, t->obs_lat, h->nObs));
     h->hObsVal.assign(t->obs_val, t->obs_val + h->nObs);
     {
-        // quadrants of the observations' bounding box; up to VG_S points per
-        // quadrant, evenly strided in storage order
+        // quadrants of the observations' bounding box; every point of a
+        // quadrant up to VG_S, beyond that VG_S evenly strided in storage order
         std::vector<uint32_t> idx(4 * VG_S, 0), cnt(4, 0);
         std::vector<double> hm(4, 0.0);
         if (h->nObs) {
@@ -2037,7 +2037,7 @@ extern "C" int mra_variogram_features(mra_handle* h, float* features) {
         return MRA_E_UNSUPPORTED;
     }
     k_fill_u64<<<1, 128, 0, h->stream>>>(h->dVgAcc, 4 * VG_BINS * 2);
-    dim3 grid(64, 4);
+    dim3 grid(256, 4);
     k_variogram<<<grid, 256, 0, h->stream>>>(h->dVgIdx, h->dVgCnt, h->dVgHmax, h->dOlon, h->dOlat, h->dY, h->dVgAcc);
     HIP_TRY(hipGetLastError());
     unsigned long long acc[4 * VG_BINS * 2];

@@ -1656,27 +1656,35 @@ __global__ __launch_bounds__(256) void k_selftest(const double* A, const double*
 
 // -------------------------------------------------- variogram features
 // Empirical semivariogram of the observations in 4 quadrants x 16 lag bins
-// (features of the warm-start network).  Pairs of the (at most VG_S) sampled
+// (features of the warm-start network).  All pairs of the (at most VG_S)
 // points of a quadrant; lag d falls in bin floor(16 sqrt(d / hmax)).  Sums of
-// (y_i - y_j)^2 are accumulated in fixed point (2^-24 units) with integer
-// atomics, so the result does not depend on the order of the additions.
-constexpr int VG_S = 1024, VG_BINS = 16;
+// (y_i - y_j)^2 are accumulated in fixed point (2^-24 units): each thread
+// keeps its 16 bins in registers, then one integer atomic per bin and thread,
+// so the result does not depend on the order of the additions.
+constexpr int VG_S = 16384, VG_BINS = 16;
 constexpr double VG_SCALE = 16777216.0;
 
 __global__ __launch_bounds__(256) void k_variogram(const uint32_t* idx, const uint32_t* cnt, const double* hmax,
                                                    const double* x, const double* y, const double* val,
                                                    unsigned long long* acc) {
     __shared__ unsigned long long s_sum[VG_BINS];
-    __shared__ unsigned int s_cnt[VG_BINS];
+    __shared__ unsigned long long s_cnt[VG_BINS];
     const int q = blockIdx.y;
     const int S = (int)cnt[q];
     if (threadIdx.x < VG_BINS) {
         s_sum[threadIdx.x] = 0ull;
-        s_cnt[threadIdx.x] = 0u;
+        s_cnt[threadIdx.x] = 0ull;
     }
     __syncthreads();
-    const uint32_t* id = idx + q * VG_S;
+    const uint32_t* id = idx + (int64_t)q * VG_S;
     const double hm = hmax[q];
+    unsigned long long ls[VG_BINS];
+    unsigned int lc[VG_BINS];
+#pragma unroll
+    for (int k = 0; k < VG_BINS; k++) {
+        ls[k] = 0ull;
+        lc[k] = 0u;
+    }
     const int64_t np = (int64_t)S * S;
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
         const int i = (int)(p / S), j = (int)(p % S);
@@ -1687,13 +1695,24 @@ __global__ __launch_bounds__(256) void k_variogram(const uint32_t* idx, const ui
         if (d >= hm) continue;
         const int bin = min(VG_BINS - 1, (int)(VG_BINS * sqrt(d / hm)));
         const double e = val[a] - val[b];
-        atomicAdd(&s_sum[bin], (unsigned long long)(e * e * VG_SCALE + 0.5));
-        atomicAdd(&s_cnt[bin], 1u);
+        const unsigned long long v = (unsigned long long)(e * e * VG_SCALE + 0.5);
+#pragma unroll
+        for (int k = 0; k < VG_BINS; k++) {
+            const bool m = k == bin;
+            ls[k] += m ? v : 0ull;
+            lc[k] += m ? 1u : 0u;
+        }
     }
+#pragma unroll
+    for (int k = 0; k < VG_BINS; k++)
+        if (lc[k]) {
+            atomicAdd(&s_sum[k], ls[k]);
+            atomicAdd(&s_cnt[k], (unsigned long long)lc[k]);
+        }
     __syncthreads();
     if (threadIdx.x < VG_BINS) {
         atomicAdd(&acc[(q * VG_BINS + threadIdx.x) * 2], s_sum[threadIdx.x]);
-        atomicAdd(&acc[(q * VG_BINS + threadIdx.x) * 2 + 1], (unsigned long long)s_cnt[threadIdx.x]);
+        atomicAdd(&acc[(q * VG_BINS + threadIdx.x) * 2 + 1], s_cnt[threadIdx.x]);
     }
 }
 

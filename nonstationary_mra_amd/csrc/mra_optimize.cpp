@@ -253,15 +253,19 @@ struct Bobyqa {
 
 }  // namespace
 
-// Generic entry point (C callback objective); see mra.h.
-extern "C" int mra_bobyqa(int n, int npt, mra_objective_fn fn, void* ctx, double* x, const double* lo,
-                          const double* hi, double rho_begin, double rho_end, int max_evals, double* f_best,
-                          int* n_evals) {
+// Generic entry point (C callback objective); see mra.h.  h0 (n x n, may be
+// NULL): the Hessian the first quadratic model starts from; the first model
+// is then the interpolant closest to it in the Frobenius norm.
+extern "C" int mra_bobyqa_h0(int n, int npt, mra_objective_fn fn, void* ctx, double* x, const double* lo,
+                             const double* hi, double rho_begin, double rho_end, int max_evals, const double* h0,
+                             double* f_best, int* n_evals) {
     if (n < 1 || n > 64 || !fn || !x || !lo || !hi || !(rho_begin > 0) || !(rho_end > 0) || rho_end > rho_begin ||
         max_evals < 2 * n + 2)
         return MRA_E_ARG;
+    // 2n+1 as the reference (src/optimization.cpp:80,169), up to the
+    // (n+1)(n+2)/2 points that determine a full quadratic
     if (npt <= 0) npt = 2 * n + 1;
-    if (npt != 2 * n + 1) return MRA_E_UNSUPPORTED;  // the reference uses 2n+1 (src/optimization.cpp:80,169)
+    if (npt < 2 * n + 1 || npt > (n + 1) * (n + 2) / 2) return MRA_E_UNSUPPORTED;
     for (int i = 0; i < n; i++)
         if (!(hi[i] - lo[i] >= 2 * rho_begin)) return MRA_E_ARG;
     Bobyqa B;
@@ -281,12 +285,29 @@ extern "C" int mra_bobyqa(int n, int npt, mra_objective_fn fn, void* ctx, double
         B.Y[1 + n + i][i] -= rho;
     }
     B.F.assign(B.m, 0.0);
-    for (int i = 0; i < B.m; i++) B.F[i] = B.eval(B.Y[i]);
+    for (int i = 0; i < 2 * n + 1; i++) B.F[i] = B.eval(B.Y[i]);
+    // beyond 2n+1: x0 + s_i rho e_i + s_j rho e_j for the pairs i < j in
+    // order, s_i the side of the lower value along coordinate i (Powell's
+    // initial points for npt > 2n+1)
+    for (int k = 2 * n + 1, i = 0, j = 1; k < B.m; k++) {
+        Vec y = x0;
+        const double si = B.F[1 + i] <= B.F[1 + n + i] ? rho : -rho;
+        const double sj = B.F[1 + j] <= B.F[1 + n + j] ? rho : -rho;
+        y[i] += si;
+        y[j] += sj;
+        B.Y[k] = y;
+        B.F[k] = B.eval(y);
+        if (++j == n) j = ++i + 1;
+    }
     B.set_opt();
     B.g.assign(n, 0.0);
     B.H.assign((size_t)n * n, 0.0);
     if (!B.refresh_inverse()) return MRA_E_STATE;
-    B.update_model(Vec(n, 0.0), Vec((size_t)n * n, 0.0), B.F[B.kopt]);
+    Vec H0((size_t)n * n, 0.0);
+    if (h0)
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) H0[i * n + j] = 0.5 * (h0[i * n + j] + h0[j * n + i]);
+    B.update_model(Vec(n, 0.0), H0, B.F[B.kopt]);
     int status = MRA_OK;
     int short_steps = 0;
     while (true) {
@@ -421,6 +442,12 @@ extern "C" int mra_bobyqa(int n, int npt, mra_objective_fn fn, void* ctx, double
     return status;
 }
 
+extern "C" int mra_bobyqa(int n, int npt, mra_objective_fn fn, void* ctx, double* x, const double* lo,
+                          const double* hi, double rho_begin, double rho_end, int max_evals, double* f_best,
+                          int* n_evals) {
+    return mra_bobyqa_h0(n, npt, fn, ctx, x, lo, hi, rho_begin, rho_end, max_evals, nullptr, f_best, n_evals);
+}
+
 // ------------------------------------------------------------------------
 // The MRA objective: D as a function of the six parameters of the planar
 // linear-trend model (log sigma^2 = a_s + bx_s x + by_s y, log beta = a_b +
@@ -464,14 +491,21 @@ double trend_objective(const double* th, void* p) {
 }
 }  // namespace
 
-extern "C" int mra_fit_linear_trend(mra_handle* h, int metric, double* theta, const double* lo, const double* hi,
-                                    double rho_begin, double rho_end, int max_evals, double* D_best, int* n_evals) {
+extern "C" int mra_fit_linear_trend_npt(mra_handle* h, int metric, double* theta, const double* lo,
+                                        const double* hi, double rho_begin, double rho_end, int max_evals, int npt,
+                                        const double* h0, double* D_best, int* n_evals) {
     if (!h || !theta || !lo || !hi) return MRA_E_ARG;
     TrendCtx c{h, metric, 0};
-    int rc = mra_bobyqa(MRA_TREND_NPARAM, 0, trend_objective, &c, theta, lo, hi, rho_begin, rho_end, max_evals,
-                        D_best, n_evals);
+    int rc = mra_bobyqa_h0(MRA_TREND_NPARAM, npt, trend_objective, &c, theta, lo, hi, rho_begin, rho_end, max_evals,
+                           h0, D_best, n_evals);
     if (c.failed) return MRA_E_STATE;
     // leave the handle at the best parameters
     (void)trend_objective(theta, &c);
     return rc;
+}
+
+extern "C" int mra_fit_linear_trend(mra_handle* h, int metric, double* theta, const double* lo, const double* hi,
+                                    double rho_begin, double rho_end, int max_evals, double* D_best, int* n_evals) {
+    return mra_fit_linear_trend_npt(h, metric, theta, lo, hi, rho_begin, rho_end, max_evals, 0, nullptr, D_best,
+                                    n_evals);
 }

@@ -37,10 +37,14 @@ BOUND_LO = np.array([-3.0, -3.0, -3.0, np.log(0.005), -3.0, np.log(1e-3)])
 BOUND_HI = np.array([3.0, 3.0, 3.0, np.log(1.0), 3.0, np.log(1.0)])
 DEFAULT_START = np.array([0.0, 0.0, 0.0, np.log(0.05), 0.0, np.log(0.05)])
 RHO_BEGIN, RHO_END, MAX_EVALS = 0.3, 1e-4, 800
+# interpolation points of the minimiser's quadratic models: the full
+# quadratic in six parameters, for both starts (2n+1 = 13 needs about 2.6x
+# more evaluations on these fields from either start)
+NPT = 28
 # initial trust radius from the network's start: of the order of its
 # held-out prediction error in the standardised parameters, so the optimizer
 # does not spend its first evaluations re-exploring the whole box
-RHO_BEGIN_WARM = 0.05
+RHO_BEGIN_WARM = 0.03
 
 
 def trend_cov(theta):
@@ -70,7 +74,7 @@ def simulate(handle, theta, seed):
     return handle.variogram_features()
 
 
-def training_set(n_fields, n, M, r, seed=2026):
+def training_set(n_fields, n, M, r, seed=2026, log=None):
     tree = make_locations(n, M, r, seed)
     h = capi.Handle(tree, with_pred=False)
     rng = np.random.default_rng(seed + 1)
@@ -79,6 +83,8 @@ def training_set(n_fields, n, M, r, seed=2026):
     try:
         for i in range(n_fields):
             X[i] = simulate(h, th[i], seed * 1000 + i)
+            if log and (i + 1) % 1024 == 0:
+                log(f"{i + 1} fields simulated")
     finally:
         h.close()
     return X, th
@@ -141,9 +147,9 @@ def train(X, theta, epochs=300, batch=256, lr=1e-3, seed=0, val_frac=0.1, log=No
     return ws, hist
 
 
-def fit(handle, theta0, max_evals=MAX_EVALS, rho_begin=RHO_BEGIN):
+def fit(handle, theta0, max_evals=MAX_EVALS, rho_begin=RHO_BEGIN, npt=NPT):
     """The optimizer from theta0; returns (theta, D, n_evals, status)."""
-    return handle.fit_linear_trend(theta0, BOUND_LO, BOUND_HI, rho_begin, RHO_END, max_evals)
+    return handle.fit_linear_trend(theta0, BOUND_LO, BOUND_HI, rho_begin, RHO_END, max_evals, npt=npt)
 
 
 def compare(ws, n_fields, n, M, r, seed=77, log=None, max_evals=MAX_EVALS, rho_warm=RHO_BEGIN_WARM):
@@ -202,7 +208,7 @@ def main(argv=None):
     log = lambda s: print(f"[warmstart] {s}", file=sys.stderr, flush=True)
     if a.cmd == "train":
         t0 = time.time()
-        X, th = training_set(a.fields, a.n, a.M, a.r, seed=a.seed)
+        X, th = training_set(a.fields, a.n, a.M, a.r, seed=a.seed, log=log)
         log(f"{a.fields} fields simulated in {time.time() - t0:.1f}s")
         ws, hist = train(X, th, epochs=a.epochs, log=log)
         log(f"trained in {time.time() - t0:.1f}s; final train {hist[-1][0]:.4f} val {hist[-1][1]:.4f}")

@@ -71,10 +71,11 @@ def loss_grad(p, X, T):
     return float(np.mean((Y - T) ** 2)), pack(g)
 
 
-def variogram_features(lon, lat, val, S=1024, bins=16):
+def variogram_features(lon, lat, val, S=16384, bins=16):
     """log semivariogram per quadrant of the bounding box and lag bin
     floor(bins * sqrt(d / hmax)), hmax = half the box's larger side; up to S
-    points per quadrant taken at storage positions k * n_q // S."""
+    points per quadrant taken at storage positions k * n_q // S (all points
+    when the quadrant has at most S).  Pairs are enumerated in row blocks."""
     lon, lat, val = (np.asarray(a, dtype=np.float64) for a in (lon, lat, val))
     x0, x1, y0, y1 = lon.min(), lon.max(), lat.min(), lat.max()
     xm, ym = 0.5 * (x0 + x1), 0.5 * (y0 + y1)
@@ -86,14 +87,18 @@ def variogram_features(lon, lat, val, S=1024, bins=16):
         nq = len(members)
         Sq = min(nq, S)
         sel = members[(np.arange(Sq) * nq) // Sq] if Sq else members[:0]
-        i, j = np.triu_indices(Sq, 1)
-        a, b = sel[i], sel[j]
-        d = np.sqrt((lon[a] - lon[b]) ** 2 + (lat[a] - lat[b]) ** 2)
-        keep = d < hm
-        binix = np.minimum(bins - 1, (bins * np.sqrt(d[keep] / hm)).astype(int))
-        e2 = (val[a] - val[b])[keep] ** 2
-        s = np.bincount(binix, weights=e2, minlength=bins)
-        c = np.bincount(binix, minlength=bins)
+        s = np.zeros(bins)
+        c = np.zeros(bins, dtype=np.int64)
+        for r0 in range(0, Sq, 256):
+            rows = np.arange(r0, min(Sq, r0 + 256))
+            i, j = np.nonzero(np.arange(Sq)[None, :] > rows[:, None])
+            a, b = sel[rows[i]], sel[j]
+            d = np.sqrt((lon[a] - lon[b]) ** 2 + (lat[a] - lat[b]) ** 2)
+            keep = d < hm
+            binix = np.minimum(bins - 1, (bins * np.sqrt(d[keep] / hm)).astype(int))
+            e2 = (val[a] - val[b])[keep] ** 2
+            s += np.bincount(binix, weights=e2, minlength=bins)
+            c += np.bincount(binix, minlength=bins)
         g = np.where(c > 0, s / np.maximum(2 * c, 1), 0.0)
         have = c > 0
         for bix in range(bins):

@@ -160,12 +160,14 @@ def test_c3_fields_against_oracle(c3):
 
 def test_c4_committed_network_on_held_out_fields():
     """The committed network's start against the default start on held-out
-    fields of the warm-start configuration's shape: fewer optimizer
-    evaluations on average, equal final D within the optimizer's resolution."""
+    fields of the warm-start configuration's shape (n = 50,000, M = 8,
+    r = 32), the same optimizer from both: at least 1.5x fewer evaluations on
+    average (1.8x measured on 64 fields) at equal final D within the
+    optimizer's resolution."""
     ws = warmstart.WarmStart.load(MODEL)
     t0 = time.time()
     rows = warmstart.compare(ws, 24, 50_000, 8, 32, seed=9001)
     s = warmstart.summarize(rows)
     print(f"C4 {s} ({time.time() - t0:.1f}s)", flush=True)
-    assert s["mean_evals_warm"] < s["mean_evals_default"]
-    assert abs(s["mean_D_warm_minus_default"]) < 0.05
+    assert s["evals_ratio"] >= 1.5
+    assert abs(s["mean_D_warm_minus_default"]) < 1e-3 and s["max_D_warm_minus_default"] < 1e-2
