@@ -107,7 +107,11 @@ int mra_set_covariance(mra_handle* h, const mra_cov_desc* cov);
 int mra_set_params(mra_handle* h, double sill, double range, double nugget,
                    double sill_scale, double range_scale, double nugget_scale);
 
-/* MRA_PRECISION_FP64 (default) or MRA_PRECISION_MIXED. */
+/* MRA_PRECISION_FP64 (default) or MRA_PRECISION_MIXED: covariance evaluation
+ * and the conditioning products of the finest level in FP32 on FP32 matrix
+ * cores, knot Cholesky factors, conditioning of knots and log-determinants
+ * in FP64.  Finest regions above MRA_OPT_MIXED_FP64_ABOVE observations
+ * (default 128) are conditioned in FP64 in either mode. */
 int mra_set_precision(mra_handle* h, int precision);
 
 /* Approximation::likelihood() (src/class_approximation.cpp:466-470):
@@ -182,7 +186,11 @@ int mra_use_graph(mra_handle* h, int enable);
  * launches hand out their regions in 8 contiguous ranges, one per XCD
  * (work groups are dispatched to the XCDs round robin), so that neighbouring
  * regions, which read the same ancestor blocks, run under one L2; 0 takes
- * them in launch order (same values). */
+ * them in launch order (same values).
+ * MRA_OPT_MIXED_FP64_ABOVE: in mixed precision, finest regions with more
+ * than this many observations (default 128, in [64, 256]) are conditioned
+ * and factored in FP64: in large, strongly correlated regions the FP32
+ * rounding amplified through Sigma_F exceeds |dloglik|/n = 1e-6. */
 enum {
     MRA_OPT_FACTORED_POSTERIOR = 1,
     MRA_OPT_GRAM_KERNEL = 2,
@@ -190,7 +198,8 @@ enum {
     MRA_OPT_CROSS16 = 4,
     MRA_OPT_CONCURRENT = 5,
     MRA_OPT_SUM_MESSAGES = 6,
-    MRA_OPT_XCD_MAP = 7
+    MRA_OPT_XCD_MAP = 7,
+    MRA_OPT_MIXED_FP64_ABOVE = 8
 };
 int mra_set_option(mra_handle* h, int option, int value);
 int mra_get_option(mra_handle* h, int option, int* value);

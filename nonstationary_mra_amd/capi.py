@@ -51,6 +51,7 @@ OPT_CROSS16 = 4
 OPT_CONCURRENT = 5
 OPT_SUM_MESSAGES = 6
 OPT_XCD_MAP = 7
+OPT_MIXED_FP64_ABOVE = 8
 
 
 class TreeDesc(C.Structure):

@@ -106,10 +106,21 @@ struct mra_handle {
     uint32_t nItemsX32 = 0;
     uint32_t* dItemsX16 = nullptr;  // segments of at most 16 observations (16-column tiles)
     uint32_t nItemsX16 = 0;
-    // finest regions by length bucket: n <= 16, 32, 64, (3: unused), 256, larger
+    // mixed precision: 64-column tiles of the segments that hold a finest
+    // region above 256 observations, conditioned again in FP64 into
+    // dGf64 / dBf64 for those regions' local step (allocated on first use)
+    uint32_t* dItemsXBig = nullptr;
+    uint32_t nItemsXBig = 0;
+    double *dGf64 = nullptr, *dBf64 = nullptr;
+    // finest regions by length bucket: n <= 16, 32, 64; 64 < n <= 256 above
+    // mixFp64Min (3) and up to it (4); above 256 (5)
     static constexpr int NBK = 6;
-    uint64_t* dWsOffBig = nullptr;  // workspace offset of each region of the 64 < n <= 256 bucket
+    uint64_t* dWsOffBig = nullptr;  // workspace offset of each region of bucket 4
+    uint64_t* dWsOffBig3 = nullptr; // workspace offset of each region of bucket 3
     double* dWsBig = nullptr;       // padded 16 ceil(n/16) squared workspace per such region
+    // mixed precision: finest regions above this many observations are
+    // conditioned in FP64 (MRA_OPT_MIXED_FP64_ABOVE)
+    uint32_t mixFp64Min = 128;
     uint32_t* dItemsFB[NBK] = {};
     uint32_t nItemsFB[NBK] = {};
     // work-list shape (mra_autotune / mra_set_tuning): segments of at most
@@ -437,6 +448,25 @@ static int build_worklists(mra_handle* h) {
             c0 += t;
         }
     }
+    {
+        // segments holding a finest region of more than mixFp64Min observations
+        std::vector<char> big(h->hSN.size(), 0);
+        for (uint64_t f = 0; f < h->nF; f++)
+            if (h->hObsOff[f + 1] - h->hObsOff[f] > h->mixFp64Min) {
+                const uint64_t o = h->hObsOff[f];
+                const size_t sgi = (size_t)(std::upper_bound(h->hSOs.begin(), h->hSOs.end(), o) - h->hSOs.begin()) - 1;
+                big[sgi] = 1;
+            }
+        std::vector<uint32_t> ixb;
+        for (size_t sgi = 0; sgi < big.size(); sgi++)
+            if (big[sgi])
+                for (uint32_t c0 = 0; c0 < h->hSN[sgi]; c0 += 64) {
+                    ixb.push_back((uint32_t)sgi);
+                    ixb.push_back(c0);
+                }
+        h->nItemsXBig = (uint32_t)(ixb.size() / 2);
+        RC_TRY(upload_wl(h, &h->dItemsXBig, ixb.data(), ixb.size()));
+    }
     h->nItemsX32 = (uint32_t)(ix32.size() / 2);
     RC_TRY(upload_wl(h, &h->dItemsX32, ix32.data(), ix32.size()));
     h->nItemsX16 = (uint32_t)(ix16.size() / 2);
@@ -449,23 +479,29 @@ static int build_worklists(mra_handle* h) {
             const uint64_t n = h->hObsOff[f + 1] - h->hObsOff[f];
             // (64 < n <= 256: the workspace kernel; its low LDS use keeps
             // several work groups per CU where bucket 3's 134 KB kept one)
-            b[(n <= 16 && h->tuneFb16) ? 0 : n <= 32 ? 1 : n <= 64 ? 2 : n <= 256 ? 4 : 5].push_back(f);
+            b[(n <= 16 && h->tuneFb16) ? 0 : n <= 32 ? 1 : n <= 64 ? 2 : n <= 256 ? (n > h->mixFp64Min ? 3 : 4) : 5]
+                .push_back(f);
         }
         for (int k = 0; k < mra_handle::NBK; k++) {
             h->nItemsFB[k] = (uint32_t)b[k].size();
             RC_TRY(upload_wl(h, &h->dItemsFB[k], b[k].data(), b[k].size()));
         }
-        // padded workspace of the 64 < n <= 256 bucket
-        std::vector<uint64_t> wo(b[4].size() + 1, 0);
-        for (size_t k = 0; k < b[4].size(); k++) {
-            const uint64_t n = h->hObsOff[b[4][k] + 1] - h->hObsOff[b[4][k]];
-            const uint64_t n16 = (n + 15) / 16 * 16;
-            wo[k + 1] = wo[k] + n16 * n16;
-        }
+        // padded workspace of the 64 < n <= 256 buckets (4, then 3)
+        std::vector<uint64_t> wo(b[4].size() + b[3].size() + 1, 0);
+        size_t k = 0;
+        for (int q : {4, 3})
+            for (uint32_t f : b[q]) {
+                const uint64_t n = h->hObsOff[f + 1] - h->hObsOff[f];
+                const uint64_t n16 = (n + 15) / 16 * 16;
+                wo[k + 1] = wo[k] + n16 * n16;
+                k++;
+            }
         h->dWsOffBig = nullptr;
+        h->dWsOffBig3 = nullptr;
         h->dWsBig = nullptr;
-        if (!b[4].empty()) {
-            RC_TRY(upload_wl(h, &h->dWsOffBig, wo.data(), b[4].size()));
+        if (k) {
+            RC_TRY(upload_wl(h, &h->dWsOffBig, wo.data(), std::max<size_t>(b[4].size(), 1)));
+            RC_TRY(upload_wl(h, &h->dWsOffBig3, wo.data() + b[4].size(), std::max<size_t>(b[3].size(), 1)));
             RC_TRY(upload_wl(h, &h->dWsBig, (const double*)nullptr, wo.back()));
         }
     }
@@ -924,6 +960,16 @@ extern "C" int mra_set_option(mra_handle* h, int option, int value) {
         graph_invalidate(h);
         return MRA_OK;
     }
+    if (option == MRA_OPT_MIXED_FP64_ABOVE) {
+        if (value < 64 || value > 256) {
+            set_err(h, "MRA_OPT_MIXED_FP64_ABOVE must be in [64, 256]");
+            return MRA_E_ARG;
+        }
+        h->mixFp64Min = (uint32_t)value;
+        RC_TRY(build_worklists(h));
+        graph_invalidate(h);
+        return MRA_OK;
+    }
     set_err(h, "unknown option");
     return MRA_E_ARG;
 }
@@ -956,6 +1002,10 @@ extern "C" int mra_get_option(mra_handle* h, int option, int* value) {
     }
     if (option == MRA_OPT_XCD_MAP) {
         *value = h->xcdMap;
+        return MRA_OK;
+    }
+    if (option == MRA_OPT_MIXED_FP64_ABOVE) {
+        *value = (int)std::min<uint32_t>(h->mixFp64Min, 0x7fffffff);
         return MRA_OK;
     }
     set_err(h, "unknown option");
@@ -1047,10 +1097,24 @@ static void launch_finest_local(mra_handle* h, const Tree& T, const CovCtx& cc, 
     const hipStream_t s1 = fk.stream(1), s2 = fk.stream(2), st = fk.stream(0);
     TG* Gf = (TG*)h->dGf;
     TG* Bf = (TG*)h->dBf;
-    // larger buckets first (longest work groups start early)
-    if (h->nItemsFB[5])
+    // larger buckets first (longest work groups start early); in mixed
+    // precision the regions above mixFp64Min observations (buckets 3 and 5)
+    // use their FP64 G and B
+    if (h->nItemsFB[5] && sizeof(TG) == 4 && h->nItemsXBig && h->dGf64)
+        k_finest_local<RP, 0, double><<<h->nItemsFB[5], 256, 0, s1>>>(T, cc, O, h->dY, h->dItemsFB[5], h->dGf64,
+                                                                      h->dBf64, h->dSig, h->dH, h->dU, h->dDterm,
+                                                                      h->dBad);
+    else if (h->nItemsFB[5])
         k_finest_local<RP, 0, TG><<<h->nItemsFB[5], 256, 0, s1>>>(T, cc, O, h->dY, h->dItemsFB[5], Gf, Bf, h->dSig,
                                                                   h->dH, h->dU, h->dDterm, h->dBad);
+    if (h->nItemsFB[3] && sizeof(TG) == 4 && h->nItemsXBig && h->dGf64)
+        k_finest_local_big<double><<<h->nItemsFB[3], 256, 0, s1>>>(T, cc, O, h->dY, h->dItemsFB[3], h->dWsOffBig3,
+                                                                   h->dWsBig, h->dGf64, h->dBf64, h->dSig, h->dH,
+                                                                   h->dU, h->dDterm, h->dBad);
+    else if (h->nItemsFB[3])
+        k_finest_local_big<TG><<<h->nItemsFB[3], 256, 0, s1>>>(T, cc, O, h->dY, h->dItemsFB[3], h->dWsOffBig3,
+                                                               h->dWsBig, Gf, Bf, h->dSig, h->dH, h->dU, h->dDterm,
+                                                               h->dBad);
     if (h->nItemsFB[4])
         k_finest_local_big<TG><<<h->nItemsFB[4], 256, 0, s1>>>(T, cc, O, h->dY, h->dItemsFB[4], h->dWsOffBig,
                                                                h->dWsBig, Gf, Bf, h->dSig, h->dH, h->dU, h->dDterm,
@@ -1084,6 +1148,9 @@ static void launch_finest_cross(mra_handle* h, const Tree& T, const CovCtx& cc, 
     if (h->nItemsX16)
         k_finest_cross<RP, 16, TG><<<h->nItemsX16, 256, 0, fk.stream(2)>>>(T, cc, K, O, h->dItemsX16, h->dGoff, h->dGc,
                                                                  h->dLiT, (TG*)h->dGf, (TG*)h->dBf, h->dGc32);
+    if (sizeof(TG) == 4 && h->nItemsXBig && h->dGf64)
+        k_finest_cross<RP, 64, double><<<h->nItemsXBig, 256, 0, st>>>(T, cc, K, O, h->dItemsXBig, h->dGoff, h->dGc,
+                                                                     h->dLiT, h->dGf64, h->dBf64, nullptr);
 }
 
 template <int RP>
@@ -1332,10 +1399,18 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
 // FP32 copy of the coarse G blocks, read by the mixed-precision finest-level
 // kernels (allocated at the first mixed-precision evaluation)
 static int ensure_mixed_buffers(mra_handle* h) {
-    if (h->precision != MRA_PRECISION_MIXED || h->dGc32) return MRA_OK;
-    float* p = nullptr;
-    RC_TRY(dalloc(h, &p, h->hGoff[h->L], h->allocs));
-    h->dGc32 = p;
+    if (h->precision != MRA_PRECISION_MIXED) return MRA_OK;
+    if (!h->dGc32) {
+        float* p = nullptr;
+        RC_TRY(dalloc(h, &p, h->hGoff[h->L], h->allocs));
+        h->dGc32 = p;
+    }
+    if (h->nItemsXBig && !h->dGf64) {
+        // FP64 G and B at the observations' own offsets (only the segments
+        // with a region above mixFp64Min observations are written)
+        RC_TRY(dalloc(h, &h->dGf64, (uint64_t)h->LRP * h->nObs, h->allocs));
+        RC_TRY(dalloc(h, &h->dBf64, (uint64_t)h->LRP * h->nObs, h->allocs));
+    }
     return MRA_OK;
 }
 

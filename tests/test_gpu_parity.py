@@ -337,7 +337,16 @@ def test_large_region_local_kernel(r):
         h.set_params(**prm)
         D, mean, var = h.predict()
         h.set_precision(capi.PRECISION_MIXED)
+        assert h.get_option(capi.OPT_MIXED_FP64_ABOVE) == 128
         Dm = h.likelihood()
+        h.set_option(capi.OPT_MIXED_FP64_ABOVE, 256)
+        Dm256 = h.likelihood()
+        h.set_option(capi.OPT_MIXED_FP64_ABOVE, 64)
+        Dm64 = h.likelihood()
+        for bad in (63, 257):
+            with pytest.raises(capi.MraError):
+                h.set_option(capi.OPT_MIXED_FP64_ABOVE, bad)
+        assert h.get_option(capi.OPT_MIXED_FP64_ABOVE) == 64
     finally:
         h.close()
     Do, mo, vo = oracle.evaluate(tree, cov, prm)
@@ -345,12 +354,13 @@ def test_large_region_local_kernel(r):
     assert rel(D, Do) < TOL_D
     np.testing.assert_allclose(mean, mo, rtol=TOL_PRED, atol=1e-10)
     np.testing.assert_allclose(var, vo, rtol=TOL_PRED, atol=1e-10)
-    # mixed: regions of 700-900 strongly correlated observations (range 0.2)
-    # amplify the FP32 covariance rounding through Sigma_F; the 1e-6 target
-    # of the benchmark workload is checked by test_mixed_precision_loglik
-    dl = abs(Dm - D) / (2 * len(tree["obs_lon"]))
-    print(f"mixed |dl|/n {dl:.3e}", flush=True)
-    assert dl < 1e-5
+    # mixed: regions of strongly correlated observations (range 0.2) above
+    # the threshold (default 128) are conditioned in FP64; with FP32 for the
+    # 700-900 observation regions |dl|/n reached 1e-5
+    n2 = 2 * len(tree["obs_lon"])
+    dl, dl256, dl64 = abs(Dm - D) / n2, abs(Dm256 - D) / n2, abs(Dm64 - D) / n2
+    print(f"mixed |dl|/n {dl:.3e} (threshold 256: {dl256:.3e}, 64: {dl64:.3e})", flush=True)
+    assert dl <= 1e-6 and dl64 <= 1e-6 and dl256 < 1e-5
 
 
 @pytest.mark.parametrize("gram", [1, 0])
