@@ -810,17 +810,21 @@ __device__ __forceinline__ void wave_chol16(double* P, int lda, double* Dp, int 
         if (j == i) P[i * lda + i] = sqrt(pj);
     }
     wave_lds_fence();
-    // inverse by forward substitution, one column per lane (lanes 0..15);
-    // the partial column lives in Dp (rolled loops keep register use low)
+    // inverse by forward substitution, one column per lane (lanes 0..15):
+    // the column is kept in registers and the factor entries are read as
+    // LDS broadcasts that do not wait on earlier results
     if (lane < 16) {
         const int c = lane;
-#pragma unroll 1
+        double x[16];
+#pragma unroll
         for (int ii = 0; ii < 16; ii++) {
             double sacc = ii == c ? 1.0 : 0.0;
-#pragma unroll 1
-            for (int kk = 0; kk < ii; kk++) sacc -= P[ii * lda + kk] * Dp[kk * ldd + c];
-            Dp[ii * ldd + c] = sacc / P[ii * lda + ii];
+#pragma unroll
+            for (int kk = 0; kk < ii; kk++) sacc -= P[ii * lda + kk] * x[kk];
+            x[ii] = sacc / P[ii * lda + ii];
         }
+#pragma unroll
+        for (int ii = 0; ii < 16; ii++) Dp[ii * ldd + c] = x[ii];
     }
     wave_lds_fence();
 }

@@ -71,6 +71,8 @@ struct mra_handle {
     static constexpr int NLR = 4;
     uint32_t* dItemsLR[NLR] = {};
     uint32_t nItemsLR[NLR] = {};
+    // (Q^{-1})^T of the 33-64 and 65-128 row buckets (split low-rank form)
+    double* dLrQ[NLR] = {};
     uint32_t *dItemsLRAll = nullptr, *dItemsBig = nullptr;
     uint32_t nItemsLRAll = 0, nItemsBig = 0;
     int32_t* dSlot1 = nullptr;  // dense-message slot per finest-but-one region (-1: factored)
@@ -535,6 +537,10 @@ static int build_worklists(mra_handle* h) {
         for (int q = 0; q < mra_handle::NLR; q++) {
             h->nItemsLR[q] = (uint32_t)b[q].size();
             RC_TRY(upload_wl(h, &h->dItemsLR[q], b[q].data(), b[q].size()));
+            h->dLrQ[q] = nullptr;
+            const uint64_t nb = q == 3 ? 128 : q == 2 ? 64 : 0;
+            if (nb && !b[q].empty())
+                RC_TRY(upload_wl(h, &h->dLrQ[q], (const double*)nullptr, b[q].size() * nb * nb));
         }
         h->nItemsLRAll = (uint32_t)all.size();
         h->nItemsBig = (uint32_t)big.size();
@@ -1335,13 +1341,21 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
                         d.zz = h->dZo;
                         post(d, h->nItemsBig, true, fk.stream(1));
                     }
+                    // 33-128 rows: factors, then X' by column blocks
+                    const int wd = l * RP, nblk = (wd + 63) / 64;
                     if (h->nItemsLR[3]) {
                         b.items = h->dItemsLR[3];
-                        k_post_lowrank<RP, 128><<<h->nItemsLR[3], 256, 0, fk.stream(2)>>>(T, b);
+                        b.qws = h->dLrQ[3];
+                        k_post_lowrank<RP, 128, true><<<h->nItemsLR[3], 256, 0, fk.stream(2)>>>(T, b);
+                        if (nblk)
+                            k_post_lowrank_x<128><<<h->nItemsLR[3] * nblk, 256, 0, fk.stream(2)>>>(b, wd, nblk);
                     }
                     if (h->nItemsLR[2]) {
                         b.items = h->dItemsLR[2];
-                        k_post_lowrank<RP, 64><<<h->nItemsLR[2], 256, 0, fk.stream(2)>>>(T, b);
+                        b.qws = h->dLrQ[2];
+                        k_post_lowrank<RP, 64, true><<<h->nItemsLR[2], 256, 0, fk.stream(2)>>>(T, b);
+                        if (nblk)
+                            k_post_lowrank_x<64><<<h->nItemsLR[2] * nblk, 256, 0, fk.stream(2)>>>(b, wd, nblk);
                     }
                     if (h->nItemsLR[1]) {
                         b.items = h->dItemsLR[1];

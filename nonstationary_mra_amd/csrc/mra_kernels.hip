@@ -1278,12 +1278,17 @@ struct LowRankArgs {
     double* zo;
     double* dterm;
     double* ws;  // W = L^{-1} X_l^T of each region: RP x n_s at ws + row_s * RP
+    double* qws; // split form: (Q^{-1})^T of item k, NB x NB at qws + k NB^2
     int* bad_flag;
 };
 
 // Regions with at most NB (16, 32, 64 or 128) rows: Q in LDS, W through the
 // global workspace, X' stored straight from the product to the output rows.
-template <int RP, int NB>
+// SPLIT: (Q^{-1})^T is stored to qws instead and k_post_lowrank_x forms X'
+// by column blocks in a launch of its own (the product is most of the work
+// and one work group per region leaves most of the chip idle for NB = 128,
+// whose LDS allows one work group per CU).
+template <int RP, int NB, bool SPLIT = false>
 __global__ __launch_bounds__(256) void k_post_lowrank(Tree T, LowRankArgs a) {
     constexpr int LDA = NB + 1;
     __shared__ double Aq[NB * LDA];
@@ -1331,9 +1336,37 @@ __global__ __launch_bounds__(256) void k_post_lowrank(Tree T, LowRankArgs a) {
     if (threadIdx.x == 0) a.dterm[R] = d;
     // 6. X' = Q^{-1} X[:, 0:l*RP] straight to the output rows
     const int Wd = l * RP;
-    if (Wd > 0)
+    if constexpr (SPLIT) {
+        // Qt[k*nsp + i] = Q^{-1}[i][k] (zero for k > i)
+        double* Qt = a.qws + (uint64_t)blockIdx.x * NB * NB;
+        for (int idx = threadIdx.x; idx < nsp * nsp; idx += MRA_WG) {
+            const int k = idx / nsp, i = idx % nsp;
+            Qt[idx] = (i >= k) ? Aq[i * LDA + k] : 0.0;
+        }
+    } else if (Wd > 0) {
         wg_gemm_tn<true, false>(a.Xo + os * (uint64_t)a.ldo, a.ldo, nsp, Wd, ns, 1.0, Aq, LDA, ns, Xr, a.ldx, Wd);
+    }
     if (threadIdx.x == 0 && bad) atomicOr(a.bad_flag, 1);
+}
+
+// X' = Q^{-1} X[:, 0:l*RP] for the regions of one k_post_lowrank<.., NB, true>
+// launch (same items, same order): work group g forms the 64 columns
+// 64 (g mod nblk) .. of item g / nblk from the stored (Q^{-1})^T; the upper
+// zeros of Q^{-1} are skipped (K of each wave block stops at its last row).
+// wd = l*RP columns, nblk = ceil(wd / 64).
+template <int NB>
+__global__ __launch_bounds__(256) void k_post_lowrank_x(LowRankArgs a, int wd, int nblk) {
+    const uint32_t it = blockIdx.x / (uint32_t)nblk;
+    const int cb = (int)(blockIdx.x % (uint32_t)nblk);
+    const uint32_t loc = a.items[it];
+    const uint64_t os = a.row_s[loc];
+    const int ns = (int)(a.row_e[loc] - os);
+    if (ns <= 0) return;
+    const int nsp = (ns + 15) & ~15;
+    const double* Qt = a.qws + (uint64_t)it * NB * NB;
+    wg_gemm_tn<false, false, double, double, GEMM_ANY, false, GEMM_TRI_A>(
+        a.Xo + os * (uint64_t)a.ldo + cb * 64, a.ldo, nsp, 64, ns, 1.0, Qt, nsp, ns,
+        a.X + os * (uint64_t)a.ldx + cb * 64, a.ldx, min(64, wd - cb * 64));
 }
 
 // ------------------------------------------------------------- prediction
