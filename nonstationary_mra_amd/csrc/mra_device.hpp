@@ -448,27 +448,48 @@ __device__ __forceinline__ void wave_mm64_acc(mra_d4 (&acc)[16], int K, double a
     const int r16 = lane & 15, k4 = lane >> 4;
     const uint32_t oa = (uint32_t)((k4 * lda + r16) * sizeof(double));
     const uint32_t ob = (uint32_t)((k4 * ldb + r16) * sizeof(double));
-    int k0 = 0;
-    for (; k0 + 8 <= K; k0 += 8) {
-        double av[2][4], bv[2][4];
+    // operands of 8 rows; the next 8 rows are loaded before the products of
+    // the current ones are issued
+    auto load8 = [&](int k0, double (&av)[2][4], double (&bv)[2][4]) {
 #pragma unroll
         for (int u = 0; u < 2; u++) {
             const char* Ak = (const char*)(A + (int64_t)(k0 + 4 * u) * lda);
             const char* Bk = (const char*)(B + (int64_t)(k0 + 4 * u) * ldb);
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                av[u][t] = al * *(const double*)(Ak + oa + 16 * t * sizeof(double));
+                av[u][t] = *(const double*)(Ak + oa + 16 * t * sizeof(double));
                 bv[u][t] = *(const double*)(Bk + ob + 16 * t * sizeof(double));
             }
         }
+    };
+    auto mm8 = [&](const double (&av)[2][4], const double (&bv)[2][4]) {
 #pragma unroll
         for (int u = 0; u < 2; u++)
 #pragma unroll
             for (int ti = 0; ti < 4; ti++)
 #pragma unroll
                 for (int tj = 0; tj < 4; tj++)
-                    acc[ti * 4 + tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][ti], bv[u][tj], acc[ti * 4 + tj],
-                                                                            0, 0, 0);
+                    acc[ti * 4 + tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(al * av[u][ti], bv[u][tj],
+                                                                            acc[ti * 4 + tj], 0, 0, 0);
+    };
+    int k0 = 0;
+    if (K >= 8) {
+        double ac[2][4], bc[2][4];
+        load8(0, ac, bc);
+        for (; k0 + 16 <= K; k0 += 8) {
+            double an[2][4], bn[2][4];
+            load8(k0 + 8, an, bn);
+            mm8(ac, bc);
+#pragma unroll
+            for (int u = 0; u < 2; u++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    ac[u][t] = an[u][t];
+                    bc[u][t] = bn[u][t];
+                }
+        }
+        mm8(ac, bc);
+        k0 += 8;
     }
     // remaining k steps: rows clamped into range, zero beyond K
     for (; k0 < K; k0 += 4) {
@@ -571,6 +592,227 @@ __device__ __forceinline__ void wave_store64(double* C, int ldc, const mra_d4 (&
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             double* c = C + (int64_t)(ti * 16 + k4 + 4 * v) * ldc + r16;
+#pragma unroll
+            for (int tj = 0; tj < 4; tj++) c[16 * tj] = acc[ti * 4 + tj][v];
+        }
+}
+
+// ---- the same 64 x 64 accumulators in the layout of the FP32 matrix
+// instruction ("L32"): tile (ti, tj), element v of lane t holds row
+// 16 ti + 4 (t/16) + v, column 16 tj + t%16.
+
+// f += A^T B over K rows with FP32 matrix instructions (mixed precision:
+// operands rounded to float, FP32 sums), f in the L32 layout.  TI: the
+// element type of A and B in memory (double, or an FP32 copy).
+template <typename TI>
+__device__ __forceinline__ void wave_mm64_f32(mra_f4 (&f)[16], int K, const TI* A, int64_t lda, const TI* B,
+                                              int64_t ldb) {
+    const int lane = threadIdx.x & 63;
+    const int r16 = lane & 15, k4 = lane >> 4;
+    const uint32_t oa = (uint32_t)((k4 * lda + r16) * sizeof(TI));
+    const uint32_t ob = (uint32_t)((k4 * ldb + r16) * sizeof(TI));
+    // operands of 8 rows (two k steps); the next 8 rows are loaded before the
+    // products of the current ones are issued
+    auto load8 = [&](int k0, TI (&av)[2][4], TI (&bv)[2][4]) {
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const char* Ak = (const char*)(A + (int64_t)(k0 + 4 * u) * lda);
+            const char* Bk = (const char*)(B + (int64_t)(k0 + 4 * u) * ldb);
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                av[u][t] = *(const TI*)(Ak + oa + 16 * t * sizeof(TI));
+                bv[u][t] = *(const TI*)(Bk + ob + 16 * t * sizeof(TI));
+            }
+        }
+    };
+    auto mm8 = [&](const TI (&av)[2][4], const TI (&bv)[2][4]) {
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+                for (int tj = 0; tj < 4; tj++)
+                    f[ti * 4 + tj] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)av[u][ti], (float)bv[u][tj],
+                                                                          f[ti * 4 + tj], 0, 0, 0);
+    };
+    int k0 = 0;
+    if (K >= 8) {
+        TI ac[2][4], bc[2][4];
+        load8(0, ac, bc);
+        for (; k0 + 16 <= K; k0 += 8) {
+            TI an[2][4], bn[2][4];
+            load8(k0 + 8, an, bn);
+            mm8(ac, bc);
+#pragma unroll
+            for (int u = 0; u < 2; u++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    ac[u][t] = an[u][t];
+                    bc[u][t] = bn[u][t];
+                }
+        }
+        mm8(ac, bc);
+        k0 += 8;
+    }
+    // remaining rows: clamped into range, zero beyond K
+    for (; k0 < K; k0 += 4) {
+        const int k = k0 + k4;
+        const bool kv = k < K;
+        const int64_t kk = kv ? k : K - 1;
+        float av[4], bv[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            av[t] = kv ? (float)A[kk * lda + r16 + 16 * t] : 0.0f;
+            bv[t] = kv ? (float)B[kk * ldb + r16 + 16 * t] : 0.0f;
+        }
+#pragma unroll
+        for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+            for (int tj = 0; tj < 4; tj++)
+                f[ti * 4 + tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[ti], bv[tj], f[ti * 4 + tj], 0, 0, 0);
+    }
+}
+
+// wave_mm64_acc_w with the product in FP32 (as wave_mm64_f32, into f; A of
+// element type TIA) and w = B^T z in FP64 (every row of wacc[tj] holds it).
+template <typename TIA>
+__device__ __forceinline__ void wave_mm64_w_f32(mra_f4 (&f)[16], mra_d4 (&wacc)[4], int K, const TIA* A,
+                                                int64_t lda, const double* B, int64_t ldb, const double* z) {
+    const int lane = threadIdx.x & 63;
+    const int r16 = lane & 15, k4 = lane >> 4;
+    const uint32_t oa = (uint32_t)((k4 * lda + r16) * sizeof(TIA));
+    const uint32_t ob = (uint32_t)((k4 * ldb + r16) * sizeof(double));
+    auto load8 = [&](int k0, TIA (&av)[2][4], double (&bv)[2][4], double (&zv)[2]) {
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const char* Ak = (const char*)(A + (int64_t)(k0 + 4 * u) * lda);
+            const char* Bk = (const char*)(B + (int64_t)(k0 + 4 * u) * ldb);
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                av[u][t] = *(const TIA*)(Ak + oa + 16 * t * sizeof(TIA));
+                bv[u][t] = *(const double*)(Bk + ob + 16 * t * sizeof(double));
+            }
+            zv[u] = z[k0 + 4 * u + k4];
+        }
+    };
+    auto mm8 = [&](const TIA (&av)[2][4], const double (&bv)[2][4], const double (&zv)[2]) {
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+#pragma unroll
+            for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+                for (int tj = 0; tj < 4; tj++)
+                    f[ti * 4 + tj] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)av[u][ti], (float)bv[u][tj],
+                                                                          f[ti * 4 + tj], 0, 0, 0);
+#pragma unroll
+            for (int tj = 0; tj < 4; tj++)
+                wacc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(zv[u], bv[u][tj], wacc[tj], 0, 0, 0);
+        }
+    };
+    int k0 = 0;
+    if (K >= 8) {
+        TIA ac[2][4];
+        double bc[2][4], zc[2];
+        load8(0, ac, bc, zc);
+        for (; k0 + 16 <= K; k0 += 8) {
+            TIA an[2][4];
+            double bn[2][4], zn[2];
+            load8(k0 + 8, an, bn, zn);
+            mm8(ac, bc, zc);
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                zc[u] = zn[u];
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    ac[u][t] = an[u][t];
+                    bc[u][t] = bn[u][t];
+                }
+            }
+        }
+        mm8(ac, bc, zc);
+        k0 += 8;
+    }
+    for (; k0 < K; k0 += 4) {
+        const int k = k0 + k4;
+        const bool kv = k < K;
+        const int64_t kk = kv ? k : K - 1;
+        float av[4];
+        double bv[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            av[t] = kv ? (float)A[kk * lda + r16 + 16 * t] : 0.0f;
+            bv[t] = kv ? B[kk * ldb + r16 + 16 * t] : 0.0;
+        }
+        const double zv = kv ? z[kk] : 0.0;
+#pragma unroll
+        for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+            for (int tj = 0; tj < 4; tj++)
+                f[ti * 4 + tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[ti], (float)bv[tj], f[ti * 4 + tj], 0, 0, 0);
+#pragma unroll
+        for (int tj = 0; tj < 4; tj++) wacc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(zv, bv[tj], wacc[tj], 0, 0, 0);
+    }
+}
+
+// acc = f widened to FP64 (L32 layout).
+__device__ __forceinline__ void wave_widen64(mra_d4 (&acc)[16], const mra_f4 (&f)[16]) {
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) acc[q][v] = (double)f[q][v];
+}
+
+// acc += al A^T B in FP64 with the result in the L32 layout: the rows of A
+// are permuted so that the FP64 instruction's row t/16 + 4v of lane t is
+// row 4 (t/16) + v (row r of a tile read as row 4 (r%4) + r/4).
+__device__ __forceinline__ void wave_mm64_acc_l32(mra_d4 (&acc)[16], int K, double al, const double* A,
+                                                  int64_t lda, const double* B, int64_t ldb) {
+    const int lane = threadIdx.x & 63;
+    const int r16 = lane & 15;
+    const int rp = 4 * (r16 & 3) + (r16 >> 2);
+    // A is read at columns permuted by rp: shift A so that wave_mm64_acc's
+    // column offset r16 lands on rp
+    const double* Ap = A + (rp - r16);
+    wave_mm64_acc(acc, K, al, Ap, lda, B, ldb);
+}
+
+// acc += S in the L32 layout (S 64 x 64 row-major, leading dimension lds).
+__device__ __forceinline__ void wave_add64_l32(mra_d4 (&acc)[16], const double* S, int lds) {
+    const int lane = threadIdx.x & 63;
+    const int r16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+    for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const double* srow = S + (int64_t)(ti * 16 + 4 * k4 + v) * lds + r16;
+#pragma unroll
+            for (int tj = 0; tj < 4; tj++) acc[ti * 4 + tj][v] += srow[16 * tj];
+        }
+}
+
+// acc += S^T in the L32 layout.
+__device__ __forceinline__ void wave_add64_t_l32(mra_d4 (&acc)[16], const double* S, int lds) {
+    const int lane = threadIdx.x & 63;
+    const int r16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+    for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const double* scol = S + (int64_t)r16 * lds + ti * 16 + 4 * k4 + v;
+#pragma unroll
+            for (int tj = 0; tj < 4; tj++) acc[ti * 4 + tj][v] += scol[16 * tj * lds];
+        }
+}
+
+// C = acc (L32 layout; 64 x 64 row-major, leading dimension ldc).
+__device__ __forceinline__ void wave_store64_l32(double* C, int ldc, const mra_d4 (&acc)[16]) {
+    const int lane = threadIdx.x & 63;
+    const int r16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+    for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            double* c = C + (int64_t)(ti * 16 + 4 * k4 + v) * ldc + r16;
 #pragma unroll
             for (int tj = 0; tj < 4; tj++) c[16 * tj] = acc[ti * 4 + tj][v];
         }

@@ -66,6 +66,7 @@ struct mra_handle {
     std::vector<uint64_t> hRowS1, hRowE1, hRowS2, hRowE2;  // rows of levels L-1 and L-2
     uint64_t *dRowS1 = nullptr, *dRowE1 = nullptr, *dRowS2 = nullptr, *dRowE2 = nullptr;
     double *dXo = nullptr, *dZo = nullptr, *dLrWs = nullptr;  // factor rows, z', W workspace
+    float* dXo32 = nullptr;  // FP32 copy of the factor rows (mixed precision)
     // finest-but-one regions: factored buckets (rows <= 16, 32, 64, 128), all
     // factored ones, and the larger ones that keep the dense update
     static constexpr int NLR = 4;
@@ -1271,7 +1272,10 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
             k_posterior<RP, true><<<nitems, 256, 0, ps>>>(T, q);
         else
             k_posterior<RP, false><<<nitems, 256, 0, ps>>>(T, q);
-        if (q.pj_sep) k_post_pj<<<post_pj_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
+        if (q.pj_sep && mixed)
+            k_post_pj<true><<<post_pj_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
+        else if (q.pj_sep)
+            k_post_pj<false><<<post_pj_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
         if (!q.gram_sep) return;
         // the parent level reads rows (factored form) when it is L-2 of a factored tree
         const bool parent_dense = !(lr && q.l - 1 == h->L - 2);
@@ -1280,10 +1284,15 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
             g.items = h->dItemsPost[q.l - 1];
             g.start_p = h->lvlStart[q.l - 1];
             const uint32_t np = h->nItemsPost[q.l - 1];
-            k_post_gram<true><<<post_gram_grid(np, q.l), 256, 0, ps>>>(T, g, np);
+            if (mixed)
+                k_post_gram<true, true><<<post_gram_grid(np, q.l), 256, 0, ps>>>(T, g, np);
+            else
+                k_post_gram<true, false><<<post_gram_grid(np, q.l), 256, 0, ps>>>(T, g, np);
             summed[q.l] = true;
+        } else if (mixed) {
+            k_post_gram<false, true><<<post_gram_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
         } else {
-            k_post_gram<false><<<post_gram_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
+            k_post_gram<false, false><<<post_gram_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
         }
     };
     for (int l = h->L - 1; l >= 0; l--) {
@@ -1322,6 +1331,7 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
                 b.z = h->dU;
                 b.LiT = h->dLiT;
                 b.Xo = h->dXo;
+                b.Xo32 = (mixed && gsep) ? h->dXo32 : nullptr;
                 b.ldo = (h->L - 1) * h->RP;
                 b.zo = h->dZo;
                 b.dterm = h->dDterm;
@@ -1337,6 +1347,7 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
                         d.items = h->dItemsBig;
                         d.slot = h->dSlot1;
                         d.Xz = h->dXo;
+                        d.Xz32 = b.Xo32;
                         d.ldxz = (h->L - 1) * h->RP;
                         d.zz = h->dZo;
                         post(d, h->nItemsBig, true, fk.stream(1));
@@ -1378,6 +1389,7 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
             }
         } else if (l == h->L - 2 && lr) {
             a.X = h->dXo;
+            a.X32 = (mixed && gsep) ? h->dXo32 : nullptr;
             a.z = h->dZo;
             a.ldx = (h->L - 1) * h->RP;
             a.row_s = h->dRowS2f;
@@ -1418,6 +1430,11 @@ static int ensure_mixed_buffers(mra_handle* h) {
         float* p = nullptr;
         RC_TRY(dalloc(h, &p, h->hGoff[h->L], h->allocs));
         h->dGc32 = p;
+    }
+    if (h->lowrank && h->RP == 64 && h->L >= 2 && !h->dXo32) {
+        float* p = nullptr;
+        RC_TRY(dalloc(h, &p, (uint64_t)(h->L - 1) * h->RP * h->nObs, h->allocs));
+        h->dXo32 = p;
     }
     if (h->nItemsXBig && !h->dGf64) {
         // FP64 G and B at the observations' own offsets (only the segments

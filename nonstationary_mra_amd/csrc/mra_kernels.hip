@@ -811,6 +811,9 @@ struct PostArgs {
     double* Xz;            // rows mode of a dense region in a factored level: zero its rows
     int ldxz;
     double* zz;
+    float* Xz32;           // (mixed precision) the FP32 copy of those rows, zeroed too
+    // mixed precision: FP32 copy of X (same layout) read by the F32 row products
+    const float* X32;
     int* bad_flag;
     int gram_sep;  // the message blocks A~_jk come from k_post_gram (RP = 64)
     int pj_sep;    // P_j and w~_j come from k_post_pj (RP = 64, stacked rows)
@@ -931,6 +934,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         // this region sends a dense message: its factor rows contribute nothing
         const uint64_t os = a.row_s[R - a.start_l], oe = a.row_e[R - a.start_l];
         for (uint64_t idx = threadIdx.x; idx < (oe - os) * a.ldxz; idx += MRA_WG) a.Xz[os * a.ldxz + idx] = 0.0;
+        if (a.Xz32)
+            for (uint64_t idx = threadIdx.x; idx < (oe - os) * a.ldxz; idx += MRA_WG) a.Xz32[os * a.ldxz + idx] = 0.0f;
         for (uint64_t o = os + threadIdx.x; o < oe; o += MRA_WG) a.zz[o] = 0.0;
     }
     for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) Km[(idx / RP) * LD + idx % RP] = 0.0;
@@ -1072,7 +1077,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // SUM: the items are parents (level l-1); one wave sums block (j, k) of the
 // messages of the parent's working children (in child order) and stores one
 // block set per parent, which is all the parent reads (PostArgs::csum).
-template <bool SUM>
+// F32 (mixed precision): X_j^T X_k over all rows of the item (the rows of
+// every summed child) with FP32 matrix instructions and FP32 sums, widened
+// to FP64 once; the dense child messages and -P_j^T P_k are added in FP64.
+// Only the FP32 accumulators are live during the row loop.
+template <bool SUM, bool F32 = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_post_gram(Tree T, PostArgs a,
                                                                                            uint32_t nitems) {
     constexpr int RP = 64;
@@ -1096,21 +1105,63 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int lc = l + 1;
     const int nslc = lc * (lc + 1) / 2;
     const int slc = j * lc - j * (j - 1) / 2 + (k - j);
-    // A~_jk of region R (local index loc) added to acc
+    // FP32 row products of region loc (into f)
+    auto rows_f32 = [&](mra_f4 (&f)[16], uint64_t loc) {
+        const uint64_t os = a.row_s[loc], oe = a.row_e[loc];
+        if (oe <= os) return;
+        if (a.X32) {
+            const float* X = a.X32 + os * (uint64_t)a.ldx;
+            wave_mm64_f32(f, (int)(oe - os), X + j * RP, a.ldx, X + k * RP, a.ldx);
+        } else {
+            const double* X = a.X + os * (uint64_t)a.ldx;
+            wave_mm64_f32(f, (int)(oe - os), X + j * RP, a.ldx, X + k * RP, a.ldx);
+        }
+    };
+    if constexpr (F32) {
+        mra_f4 f[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) f[q] = mra_f4{0, 0, 0, 0};
+        if constexpr (SUM) {
+            const uint64_t Q = a.start_p + a.items[it];
+            for (int c = 0; c < T.J; c++) {
+                const uint64_t R = Q * T.J + 1 + c;
+                if (T.working[R]) rows_f32(f, R - a.start_l);
+            }
+        } else {
+            rows_f32(f, a.items[it]);
+        }
+        wave_widen64(acc, f);
+    }
+    // A~_jk of region R (local index loc) added to acc (in F32 mode without
+    // the row products, already in acc)
     auto add_region = [&](uint64_t loc) {
         const uint64_t R = a.start_l + loc;
         const double* PR = a.P + a.goff[l] + loc * (uint64_t)l * RP * RP;
         const uint64_t os = a.row_s[loc], oe = a.row_e[loc];
-        if (oe > os) {
+        if (!F32 && oe > os) {
             const double* X = a.X + os * (uint64_t)a.ldx;
             wave_mm64_acc(acc, (int)(oe - os), 1.0, X + j * RP, a.ldx, X + k * RP, a.ldx);
         }
         if (a.cslot)
             for (int c = 0; c < T.J; c++) {
                 const int32_t cs = a.cslot[R * T.J + 1 + c - a.start_c];
-                if (cs >= 0) wave_add64(acc, a.Ac + ((uint64_t)cs * nslc + slc) * (uint64_t)RP * RP, RP);
+                if (cs < 0) continue;
+                const double* S = a.Ac + ((uint64_t)cs * nslc + slc) * (uint64_t)RP * RP;
+                if constexpr (F32)
+                    wave_add64_l32(acc, S, RP);
+                else
+                    wave_add64(acc, S, RP);
             }
-        wave_mm64_acc(acc, RP, -1.0, PR + (uint64_t)j * RP * RP, RP, PR + (uint64_t)k * RP * RP, RP);
+        if constexpr (F32)
+            wave_mm64_acc_l32(acc, RP, -1.0, PR + (uint64_t)j * RP * RP, RP, PR + (uint64_t)k * RP * RP, RP);
+        else
+            wave_mm64_acc(acc, RP, -1.0, PR + (uint64_t)j * RP * RP, RP, PR + (uint64_t)k * RP * RP, RP);
+    };
+    auto store = [&](double* out) {
+        if constexpr (F32)
+            wave_store64_l32(out, RP, acc);
+        else
+            wave_store64(out, RP, acc);
     };
     if constexpr (SUM) {
         const uint64_t ploc = a.items[it];
@@ -1119,12 +1170,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             const uint64_t R = Q * T.J + 1 + c;
             if (T.working[R]) add_region(R - a.start_l);
         }
-        wave_store64(a.Ao + (ploc * nsl + b) * (uint64_t)RP * RP, RP, acc);
+        store(a.Ao + (ploc * nsl + b) * (uint64_t)RP * RP);
     } else {
         const uint64_t loc = a.items[it];
         add_region(loc);
         const uint64_t msg = a.slot ? (uint64_t)a.slot[loc] : loc;
-        wave_store64(a.Ao + (msg * nsl + b) * (uint64_t)RP * RP, RP, acc);
+        store(a.Ao + (msg * nsl + b) * (uint64_t)RP * RP);
     }
 }
 
@@ -1150,6 +1201,11 @@ __device__ __forceinline__ void wave_add64_t(mra_d4 (&acc)[16], const double* S,
         }
 }
 
+// F32 (mixed precision): X_l^T X_j with FP32 matrix instructions (as in
+// k_post_gram), held in the L32 layout; its rows feed L~^{-1} as the k index
+// of the product, so L~^{-1} is read at the same permuted k.  w_j = X_j^T z
+// stays FP64.
+template <bool F32 = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_post_pj(Tree T, PostArgs a,
                                                                                          uint32_t nitems) {
     constexpr int RP = 64;
@@ -1176,14 +1232,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
     for (int q = 0; q < 4; q++) wacc[q] = mra_d4{0, 0, 0, 0};
     // acc = X_l^T X_j, wacc = X_j^T z (every row)
-    if (ns > 0) wave_mm64_acc_w(acc, wacc, ns, X + l * RP, a.ldx, X + j * RP, a.ldx, a.z + os);
+    if constexpr (F32) {
+        // FP32 row products (only f live in the loop), widened once
+        mra_f4 f[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) f[q] = mra_f4{0, 0, 0, 0};
+        if (ns > 0 && a.X32)
+            wave_mm64_w_f32(f, wacc, ns, a.X32 + os * (uint64_t)a.ldx + l * RP, a.ldx, X + j * RP, a.ldx, a.z + os);
+        else if (ns > 0)
+            wave_mm64_w_f32(f, wacc, ns, X + l * RP, a.ldx, X + j * RP, a.ldx, a.z + os);
+        wave_widen64(acc, f);
+    } else if (ns > 0) {
+        wave_mm64_acc_w(acc, wacc, ns, X + l * RP, a.ldx, X + j * RP, a.ldx, a.z + os);
+    }
     double wq[4];  // w_j[16 tj + r16]: every row of wacc holds it
 #pragma unroll
     for (int tj = 0; tj < 4; tj++) wq[tj] = wacc[tj][0];
     if (a.cslot)
         for (int c = 0; c < T.J; c++) {
             const int32_t cs = a.cslot[R * T.J + 1 + c - a.start_c];
-            if (cs >= 0) wave_add64_t(acc, a.Ac + ((uint64_t)cs * nslc + slc) * (uint64_t)RP * RP, RP);
+            if (cs < 0) continue;
+            const double* S = a.Ac + ((uint64_t)cs * nslc + slc) * (uint64_t)RP * RP;
+            if constexpr (F32)
+                wave_add64_t_l32(acc, S, RP);
+            else
+                wave_add64_t(acc, S, RP);
         }
     // P_j = L~^{-1} acc by row tiles tq (k tiles kt <= tq), stored row-major;
     // pv[tj] collects this lane's part of (P_j^T v)[16 tj + r16]
@@ -1192,7 +1265,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     char* Pc = (char*)Pj;
     const double* vR = a.v + R * RP;
     double pv[4] = {0.0, 0.0, 0.0, 0.0};
+    // element v of lane t of acc tile kt is row 16 kt + k4 + 4v (FP64
+    // layout) or 16 kt + 4 k4 + v (L32): the k of L~^{-1} read with it
+    // (the product gq is in the FP64 layout either way: lo addresses its rows)
+    constexpr uint32_t KV = F32 ? 1u : 4u;
     uint32_t lo = (uint32_t)(k4 * RP + r16);
+    const uint32_t lk = F32 ? (uint32_t)(3 * k4 * RP) : 0u;
 #pragma unroll
     for (int tq = 0; tq < 4; tq++) {
         asm volatile("" : "+v"(lo));  // keep the displaced offsets in the loop
@@ -1203,7 +1281,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         for (int kt = 0; kt <= tq; kt++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const double av = *(const double*)(Li + 8u * (lo + (16 * kt + 4 * v) * RP + 16 * tq));
+                const double av = *(const double*)(Li + 8u * (lo + lk + (16 * kt + KV * v) * RP + 16 * tq));
 #pragma unroll
                 for (int tj = 0; tj < 4; tj++)
                     gq[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, acc[kt * 4 + tj][v], gq[tj], 0, 0, 0);
@@ -1274,6 +1352,7 @@ struct LowRankArgs {
     const double* z;
     const double* LiT;  // prior L^{-1}, column-major
     double* Xo;         // output rows, width l*RP
+    float* Xo32;        // mixed precision: their FP32 copy (same layout), or null
     int ldo;
     double* zo;
     double* dterm;
@@ -1343,6 +1422,10 @@ __global__ __launch_bounds__(256) void k_post_lowrank(Tree T, LowRankArgs a) {
             const int k = idx / nsp, i = idx % nsp;
             Qt[idx] = (i >= k) ? Aq[i * LDA + k] : 0.0;
         }
+    } else if (Wd > 0 && a.Xo32) {
+        wg_gemm_tn<true, false, double, double, GEMM_PART, true>(a.Xo + os * (uint64_t)a.ldo, a.ldo, nsp, Wd, ns, 1.0,
+                                                                  Aq, LDA, ns, Xr, a.ldx, Wd,
+                                                                  a.Xo32 + os * (uint64_t)a.ldo);
     } else if (Wd > 0) {
         wg_gemm_tn<true, false>(a.Xo + os * (uint64_t)a.ldo, a.ldo, nsp, Wd, ns, 1.0, Aq, LDA, ns, Xr, a.ldx, Wd);
     }
@@ -1364,9 +1447,15 @@ __global__ __launch_bounds__(256) void k_post_lowrank_x(LowRankArgs a, int wd, i
     if (ns <= 0) return;
     const int nsp = (ns + 15) & ~15;
     const double* Qt = a.qws + (uint64_t)it * NB * NB;
-    wg_gemm_tn<false, false, double, double, GEMM_ANY, false, GEMM_TRI_A>(
-        a.Xo + os * (uint64_t)a.ldo + cb * 64, a.ldo, nsp, 64, ns, 1.0, Qt, nsp, ns,
-        a.X + os * (uint64_t)a.ldx + cb * 64, a.ldx, min(64, wd - cb * 64));
+    const uint64_t oo = os * (uint64_t)a.ldo + cb * 64;
+    if (a.Xo32)
+        wg_gemm_tn<false, false, double, double, GEMM_ANY, true, GEMM_TRI_A>(
+            a.Xo + oo, a.ldo, nsp, 64, ns, 1.0, Qt, nsp, ns, a.X + os * (uint64_t)a.ldx + cb * 64, a.ldx,
+            min(64, wd - cb * 64), a.Xo32 + oo);
+    else
+        wg_gemm_tn<false, false, double, double, GEMM_ANY, false, GEMM_TRI_A>(
+            a.Xo + oo, a.ldo, nsp, 64, ns, 1.0, Qt, nsp, ns, a.X + os * (uint64_t)a.ldx + cb * 64, a.ldx,
+            min(64, wd - cb * 64));
 }
 
 // ------------------------------------------------------------- prediction
