@@ -111,7 +111,13 @@ int mra_set_params(mra_handle* h, double sill, double range, double nugget,
  * and the conditioning products of the finest level in FP32 on FP32 matrix
  * cores, knot Cholesky factors, conditioning of knots and log-determinants
  * in FP64.  Finest regions above MRA_OPT_MIXED_FP64_ABOVE observations
- * (default 128) are conditioned in FP64 in either mode. */
+ * (default 128) are conditioned in FP64 in either mode.  In mra_likelihood
+ * (not mra_predict) with 64-knot blocks the row Gram products of the
+ * posterior (X_j^T X_k, X_l^T X_j) also run on FP32 matrix cores, so the two
+ * entry points' D agree to the mixed tolerance, not bit for bit.  The
+ * rounding error of the mixed mode is relative to D: about 1e-8 of |D| per
+ * observation on data drawn from the model, and larger where D is dominated
+ * by a large quadratic form (data far from the model's parameters). */
 int mra_set_precision(mra_handle* h, int precision);
 
 /* Approximation::likelihood() (src/class_approximation.cpp:466-470):

@@ -1261,6 +1261,10 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
     // one-wave-per-block kernel after the posterior launch; the levels above
     // sum dense child messages inside the posterior kernel
     const bool gsep = RP == 64 && h->gramSep;
+    // FP32 row products (mixed precision) for likelihood evaluations; the
+    // posterior factors of a prediction keep FP64 products (the means carry
+    // the rounding of P_j directly)
+    const bool g32 = mixed && gsep && !predicting;
     // summed[l]: the message blocks of level l were stored summed per parent
     // (the parent level sums dense child messages, so it reads one block set)
     bool summed[64] = {};
@@ -1272,7 +1276,7 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
             k_posterior<RP, true><<<nitems, 256, 0, ps>>>(T, q);
         else
             k_posterior<RP, false><<<nitems, 256, 0, ps>>>(T, q);
-        if (q.pj_sep && mixed)
+        if (q.pj_sep && g32)
             k_post_pj<true><<<post_pj_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
         else if (q.pj_sep)
             k_post_pj<false><<<post_pj_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
@@ -1284,12 +1288,12 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
             g.items = h->dItemsPost[q.l - 1];
             g.start_p = h->lvlStart[q.l - 1];
             const uint32_t np = h->nItemsPost[q.l - 1];
-            if (mixed)
+            if (g32)
                 k_post_gram<true, true><<<post_gram_grid(np, q.l), 256, 0, ps>>>(T, g, np);
             else
                 k_post_gram<true, false><<<post_gram_grid(np, q.l), 256, 0, ps>>>(T, g, np);
             summed[q.l] = true;
-        } else if (mixed) {
+        } else if (g32) {
             k_post_gram<false, true><<<post_gram_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
         } else {
             k_post_gram<false, false><<<post_gram_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
@@ -1331,7 +1335,7 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
                 b.z = h->dU;
                 b.LiT = h->dLiT;
                 b.Xo = h->dXo;
-                b.Xo32 = (mixed && gsep) ? h->dXo32 : nullptr;
+                b.Xo32 = g32 ? h->dXo32 : nullptr;
                 b.ldo = (h->L - 1) * h->RP;
                 b.zo = h->dZo;
                 b.dterm = h->dDterm;
@@ -1389,7 +1393,7 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
             }
         } else if (l == h->L - 2 && lr) {
             a.X = h->dXo;
-            a.X32 = (mixed && gsep) ? h->dXo32 : nullptr;
+            a.X32 = g32 ? h->dXo32 : nullptr;
             a.z = h->dZo;
             a.ldx = (h->L - 1) * h->RP;
             a.row_s = h->dRowS2f;
