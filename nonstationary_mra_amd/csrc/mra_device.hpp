@@ -294,39 +294,67 @@ struct BlockOps {
     }
 };
 
-template <typename TC, bool M2, bool N2, bool PRED, int U, bool TA, typename TIA, typename TIB, bool TB>
+// PIPE: the operands of the next U k steps are loaded before the products of
+// the current ones are issued (more registers; for kernels with spare ones).
+template <typename TC, bool M2, bool N2, bool PRED, int U, bool PIPE = false, bool TA, typename TIA, typename TIB,
+          bool TB>
 __device__ __forceinline__ void block_kloop(typename Mfma<TC>::V (&acc)[4], const BlockOps<TA, TIA, TIB, TB>& o,
                                             int kb, int ke, int k4) {
     int k0 = kb;
-    for (; k0 + 4 * U <= ke; k0 += 4 * U) {
+    // operands of U k steps
+    struct Ops {
         TIA a0[U], a1[U];
         TIB b0[U], b1[U];
+    };
+    auto load = [&](int kk, Ops& p) {
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const char* Ak = o.A + (int64_t)(k0 + 4 * u) * o.sa;
-            const char* Bk = o.B + (int64_t)(k0 + 4 * u) * o.sb;
-            a0[u] = *(const TIA*)(Ak + o.oa0);
-            if (M2) a1[u] = *(const TIA*)(Ak + o.oa1);
-            b0[u] = *(const TIB*)(Bk + o.ob0);
-            if (N2) b1[u] = *(const TIB*)(Bk + o.ob1);
+            const char* Ak = o.A + (int64_t)(kk + 4 * u) * o.sa;
+            const char* Bk = o.B + (int64_t)(kk + 4 * u) * o.sb;
+            p.a0[u] = *(const TIA*)(Ak + o.oa0);
+            if (M2) p.a1[u] = *(const TIA*)(Ak + o.oa1);
+            p.b0[u] = *(const TIB*)(Bk + o.ob0);
+            if (N2) p.b1[u] = *(const TIB*)(Bk + o.ob1);
         }
+    };
+    auto mm = [&](const Ops& p) {
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const TC xa0 = (!PRED || o.ha) ? (TC)a0[u] : TC(0);
-            const TC xb0 = (!PRED || o.hja) ? (TC)b0[u] : TC(0);
+            const TC xa0 = (!PRED || o.ha) ? (TC)p.a0[u] : TC(0);
+            const TC xb0 = (!PRED || o.hja) ? (TC)p.b0[u] : TC(0);
             acc[0] = Mfma<TC>::op(xa0, xb0, acc[0]);
             if (N2) {
-                const TC xb1 = (!PRED || o.hjb) ? (TC)b1[u] : TC(0);
+                const TC xb1 = (!PRED || o.hjb) ? (TC)p.b1[u] : TC(0);
                 acc[1] = Mfma<TC>::op(xa0, xb1, acc[1]);
             }
             if (M2) {
-                const TC xa1 = (!PRED || o.hb) ? (TC)a1[u] : TC(0);
+                const TC xa1 = (!PRED || o.hb) ? (TC)p.a1[u] : TC(0);
                 acc[2] = Mfma<TC>::op(xa1, xb0, acc[2]);
                 if (N2) {
-                    const TC xb1 = (!PRED || o.hjb) ? (TC)b1[u] : TC(0);
+                    const TC xb1 = (!PRED || o.hjb) ? (TC)p.b1[u] : TC(0);
                     acc[3] = Mfma<TC>::op(xa1, xb1, acc[3]);
                 }
             }
+        }
+    };
+    if constexpr (PIPE) {
+        if (k0 + 4 * U <= ke) {
+            Ops cur;
+            load(k0, cur);
+            for (; k0 + 8 * U <= ke; k0 += 4 * U) {
+                Ops nxt;
+                load(k0 + 4 * U, nxt);
+                mm(cur);
+                cur = nxt;
+            }
+            mm(cur);
+            k0 += 4 * U;
+        }
+    } else {
+        for (; k0 + 4 * U <= ke; k0 += 4 * U) {
+            Ops cur;
+            load(k0, cur);
+            mm(cur);
         }
     }
     // remaining k steps: k clamped into range, zero beyond ke
@@ -355,21 +383,21 @@ __device__ __forceinline__ void block_kloop(typename Mfma<TC>::V (&acc)[4], cons
 }
 
 // dispatch on the block shape (wave-uniform)
-template <typename TC, bool PRED, int U, bool TA, typename TIA, typename TIB, bool TB>
+template <typename TC, bool PRED, int U, bool PIPE = false, bool TA, typename TIA, typename TIB, bool TB>
 __device__ __forceinline__ void block_kloop_any(typename Mfma<TC>::V (&acc)[4], const BlockOps<TA, TIA, TIB, TB>& o,
                                                 bool m2, bool n2, int kb, int ke, int k4) {
     if (m2 && n2)
-        block_kloop<TC, true, true, PRED, U>(acc, o, kb, ke, k4);
+        block_kloop<TC, true, true, PRED, U, PIPE>(acc, o, kb, ke, k4);
     else if (m2)
-        block_kloop<TC, true, false, PRED, U>(acc, o, kb, ke, k4);
+        block_kloop<TC, true, false, PRED, U, PIPE>(acc, o, kb, ke, k4);
     else if (n2)
-        block_kloop<TC, false, true, PRED, U>(acc, o, kb, ke, k4);
+        block_kloop<TC, false, true, PRED, U, PIPE>(acc, o, kb, ke, k4);
     else
-        block_kloop<TC, false, false, PRED, U>(acc, o, kb, ke, k4);
+        block_kloop<TC, false, false, PRED, U, PIPE>(acc, o, kb, ke, k4);
 }
 
 template <typename TC, bool TA, bool ACC, typename TIA, typename TIB, int TILES, typename TO, bool TB = false,
-          bool DUAL = false, int TRI = 0>
+          bool DUAL = false, int TRI = 0, bool PIPE = false>
 __device__ __forceinline__ void wg_gemm_tn_impl(TO* C, int ldc, int M, int N, int Kv, TC alpha, const TIA* A,
                                                 int64_t lda, int Mv, const TIB* B, int64_t ldb, int Nv,
                                                 float* C32 = nullptr) {
@@ -402,9 +430,9 @@ __device__ __forceinline__ void wg_gemm_tn_impl(TO* C, int ldc, int M, int N, in
         const bool full = TILES == GEMM_FULL ||
                           (TILES == GEMM_ANY && ti0 * 16 + (tM2 ? 32 : 16) <= Mv && tj0 * 16 + (tN2 ? 32 : 16) <= Nv);
         if (TILES != GEMM_PART && full)
-            block_kloop_any<TC, false, 4>(acc, o, tM2, tN2, 0, ke, k4);
+            block_kloop_any<TC, false, 4, PIPE>(acc, o, tM2, tN2, 0, ke, k4);
         else if (TILES != GEMM_FULL)
-            block_kloop_any<TC, true, 4>(acc, o, tM2, tN2, 0, ke, k4);
+            block_kloop_any<TC, true, 4, PIPE>(acc, o, tM2, tN2, 0, ke, k4);
         // only the valid Mv x Nv part of C is updated
 #pragma unroll
         for (int v = 0; v < 4; v++) {
@@ -861,20 +889,21 @@ __device__ __forceinline__ void wg_gemm2_store(double* C, int ldc, int M, int N,
 
 // FP64 MFMA, FP64 accumulation; operands read as TIA / TIB and widened
 template <bool TA = false, bool ACC = true, typename TIA = double, typename TIB = double, int TILES = GEMM_PART,
-          bool DUAL = false, int TRI = 0>
+          bool DUAL = false, int TRI = 0, bool PIPE = false>
 __device__ __forceinline__ void wg_gemm_tn(double* C, int ldc, int M, int N, int Kv, double alpha, const TIA* A,
                                            int64_t lda, int Mv, const TIB* B, int64_t ldb, int Nv,
                                            float* C32 = nullptr) {
-    wg_gemm_tn_impl<double, TA, ACC, TIA, TIB, TILES, double, false, DUAL, TRI>(C, ldc, M, N, Kv, alpha, A, lda, Mv,
-                                                                               B, ldb, Nv, C32);
+    wg_gemm_tn_impl<double, TA, ACC, TIA, TIB, TILES, double, false, DUAL, TRI, PIPE>(C, ldc, M, N, Kv, alpha, A, lda,
+                                                                                     Mv, B, ldb, Nv, C32);
 }
 
 // FP32 MFMA with FP32 accumulation (operands rounded to float); C of type TC
-template <bool ACC = true, typename TIA = float, typename TIB = float, typename TC = float, int TRI = 0>
+template <bool ACC = true, typename TIA = float, typename TIB = float, typename TC = float, int TRI = 0,
+          bool PIPE = false>
 __device__ __forceinline__ void wg_gemm_tn_f32(TC* C, int ldc, int M, int N, int Kv, float alpha, const TIA* A,
                                                int64_t lda, int Mv, const TIB* B, int64_t ldb, int Nv) {
-    wg_gemm_tn_impl<float, false, ACC, TIA, TIB, GEMM_ANY, TC, false, false, TRI>(C, ldc, M, N, Kv, alpha, A, lda, Mv,
-                                                                                 B, ldb, Nv);
+    wg_gemm_tn_impl<float, false, ACC, TIA, TIB, GEMM_ANY, TC, false, false, TRI, PIPE>(C, ldc, M, N, Kv, alpha, A,
+                                                                                       lda, Mv, B, ldb, Nv);
 }
 
 // Same product for an output of at most 32 x 32 (one 2x2 tile block), with

@@ -427,7 +427,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
             const uint64_t go = goff[j] + (a - lvl_start(j, T0.J)) * (uint64_t)j * RP * RP;
             const double* Ga = Gc + go;
             if constexpr (sizeof(T) == 4)
-                wg_gemm_tn_f32(Cm, LD, RP, NT, j * RP, -1.0f, Gc32 + go, RP, RP, GF + c0, nF, n);
+                wg_gemm_tn_f32<true, float, T, float, 0, true>(Cm, LD, RP, NT, j * RP, -1.0f, Gc32 + go, RP, RP,
+                                                               GF + c0, nF, n);
             else
                 wg_gemm_tn<false, true, double, double, GEMM_ANY>(Cm, LD, RP, NT, j * RP, -1.0, Ga, RP, RP, GF + c0,
                                                                   nF, n);
@@ -440,7 +441,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
         }
         // G_{F,j} = L_a^{-1} B, stored straight to global memory
         if constexpr (sizeof(T) == 4)
-            wg_gemm_tn_f32<false, double, float, float, GEMM_TRI_A>(GF + (uint64_t)(j * RP) * nF + c0, nF, RP, NT,
+            wg_gemm_tn_f32<false, double, float, float, GEMM_TRI_A, true>(GF + (uint64_t)(j * RP) * nF + c0, nF, RP, NT,
                                                                     RP, 1.0f, LiT + a * RP * RP, RP, RP,
                                                                     (const float*)Cm, LD, n);
         else
