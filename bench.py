@@ -93,13 +93,16 @@ def eager_groups(h, steps):
 def cpu_baseline(h_unused, M, J, r, n, budget_s=30.0):
     """The FP64 oracle (a restatement of the reference's CPU algorithm, OpenMP
     over regions, MKL kernels when libmkl_rt is present) on a bounded sample:
-    the same generator at n/16 points and M-2 levels (same points per finest
-    region), scaled to the full workload by the ratio of algorithmic flops.
-    The FP64 GPU path is evaluated on the same sample for its error against
-    the oracle."""
+    the same generator at n/4 points and M-1 levels (same points per finest
+    region; about 4 s per evaluation on 16 cores and 110 GB of host memory,
+    where the full size would need about 4x that memory), scaled to the full
+    workload by the ratio of algorithmic flops.  The oracle's flop rate still
+    rises from n/16 to n/4 (208 to 230 GFLOP/s), so the scaled time is an
+    upper bound on the full-size CPU time by a few percent.  The FP64 GPU path
+    is evaluated on the same sample for its error against the oracle."""
     from oracle import oracle
     mkl = oracle.lib().orc_set_blas(1)
-    ns, Ms = n // 16, M - 2
+    ns, Ms = n // 4, M - 1
     tree = make_workload(ns, 0, Ms, J, r, seed=SEED_LOC + 1)
     cov = synth.linear_trend_cov()
     prm = dict(synth.UNIT_PARAMS)
