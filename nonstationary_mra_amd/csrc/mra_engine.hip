@@ -1270,16 +1270,21 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
     bool summed[64] = {};
     auto post = [&](const PostArgs& pa_, uint32_t nitems, bool from_obs, hipStream_t ps) {
         PostArgs q = pa_;
-        q.pj_sep = gsep && from_obs && q.l > 0;
+        // P_j, w~_j and the message blocks by the one-wave-per-block kernels
+        // (rows levels and dense levels alike)
+        q.pj_sep = gsep && q.l > 0;
         q.gram_sep = q.pj_sep && q.write_msgs;
+        const bool f32 = g32 && from_obs;  // only row products run in FP32
         if (from_obs)
             k_posterior<RP, true><<<nitems, 256, 0, ps>>>(T, q);
         else
             k_posterior<RP, false><<<nitems, 256, 0, ps>>>(T, q);
-        if (q.pj_sep && g32)
-            k_post_pj<true><<<post_pj_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
+        if (q.pj_sep && f32)
+            k_post_pj<true, false><<<post_pj_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
+        else if (q.pj_sep && from_obs)
+            k_post_pj<false, false><<<post_pj_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
         else if (q.pj_sep)
-            k_post_pj<false><<<post_pj_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
+            k_post_pj<false, true><<<post_pj_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
         if (!q.gram_sep) return;
         // the parent level reads rows (factored form) when it is L-2 of a factored tree
         const bool parent_dense = !(lr && q.l - 1 == h->L - 2);
@@ -1288,15 +1293,19 @@ static void launch_eval(mra_handle* h, bool predicting, bool capturing) {
             g.items = h->dItemsPost[q.l - 1];
             g.start_p = h->lvlStart[q.l - 1];
             const uint32_t np = h->nItemsPost[q.l - 1];
-            if (g32)
-                k_post_gram<true, true><<<post_gram_grid(np, q.l), 256, 0, ps>>>(T, g, np);
+            if (f32)
+                k_post_gram<true, true, false><<<post_gram_grid(np, q.l), 256, 0, ps>>>(T, g, np);
+            else if (from_obs)
+                k_post_gram<true, false, false><<<post_gram_grid(np, q.l), 256, 0, ps>>>(T, g, np);
             else
-                k_post_gram<true, false><<<post_gram_grid(np, q.l), 256, 0, ps>>>(T, g, np);
+                k_post_gram<true, false, true><<<post_gram_grid(np, q.l), 256, 0, ps>>>(T, g, np);
             summed[q.l] = true;
-        } else if (g32) {
-            k_post_gram<false, true><<<post_gram_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
+        } else if (f32) {
+            k_post_gram<false, true, false><<<post_gram_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
+        } else if (from_obs) {
+            k_post_gram<false, false, false><<<post_gram_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
         } else {
-            k_post_gram<false, false><<<post_gram_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
+            k_post_gram<false, false, true><<<post_gram_grid(nitems, q.l), 256, 0, ps>>>(T, q, nitems);
         }
     };
     for (int l = h->L - 1; l >= 0; l--) {

@@ -1065,10 +1065,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 }
 
 // Message blocks A~_jk = A_jk - P_j^T P_k (j <= k < l) of the regions of a
-// posterior level that reads stacked rows (src/class_approximation.cpp:
-// 289-303), RP = 64: A_jk = X_j^T X_k over the region's rows (observation
-// rows H, or the factor rows of the level below) plus the dense messages of
-// children that sent one (cslot).  One wave per 64 x 64 block with its 4 x 4
+// posterior level (src/class_approximation.cpp:289-303), RP = 64: at a level
+// that reads stacked rows A_jk = X_j^T X_k over the region's rows
+// (observation rows H, or the factor rows of the level below) plus the dense
+// messages of children that sent one (cslot); at a dense level (a.X null)
+// A_jk is the sum of the children's messages (one block per region when the
+// level below stored them summed, csum).  One wave per 64 x 64 block with its 4 x 4
 // MFMA tiles in registers (each operand value loaded once per block, not
 // once per wave of a shared block), four blocks of the same region per work
 // group, launched after k_posterior has stored P.  Work group g runs on XCD
@@ -1082,7 +1084,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // every summed child) with FP32 matrix instructions and FP32 sums, widened
 // to FP64 once; the dense child messages and -P_j^T P_k are added in FP64.
 // Only the FP32 accumulators are live during the row loop.
-template <bool SUM, bool F32 = false>
+// DENSE: a dense level (no rows; compiled separately to keep the row path's
+// registers).
+template <bool SUM, bool F32 = false, bool DENSE = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_post_gram(Tree T, PostArgs a,
                                                                                            uint32_t nitems) {
     constexpr int RP = 64;
@@ -1135,24 +1139,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
     // A~_jk of region R (local index loc) added to acc (in F32 mode without
     // the row products, already in acc)
+    auto add_block = [&](const double* S) {
+        if constexpr (F32)
+            wave_add64_l32(acc, S, RP);
+        else
+            wave_add64(acc, S, RP);
+    };
     auto add_region = [&](uint64_t loc) {
         const uint64_t R = a.start_l + loc;
         const double* PR = a.P + a.goff[l] + loc * (uint64_t)l * RP * RP;
-        const uint64_t os = a.row_s[loc], oe = a.row_e[loc];
-        if (!F32 && oe > os) {
-            const double* X = a.X + os * (uint64_t)a.ldx;
-            wave_mm64_acc(acc, (int)(oe - os), 1.0, X + j * RP, a.ldx, X + k * RP, a.ldx);
-        }
-        if (a.cslot)
-            for (int c = 0; c < T.J; c++) {
-                const int32_t cs = a.cslot[R * T.J + 1 + c - a.start_c];
-                if (cs < 0) continue;
-                const double* S = a.Ac + ((uint64_t)cs * nslc + slc) * (uint64_t)RP * RP;
-                if constexpr (F32)
-                    wave_add64_l32(acc, S, RP);
-                else
-                    wave_add64(acc, S, RP);
+        if constexpr (!DENSE) {
+            // rows, then the dense messages of children that sent one
+            const uint64_t os = a.row_s[loc], oe = a.row_e[loc];
+            if (!F32 && oe > os) {
+                const double* X = a.X + os * (uint64_t)a.ldx;
+                wave_mm64_acc(acc, (int)(oe - os), 1.0, X + j * RP, a.ldx, X + k * RP, a.ldx);
             }
+            if (a.cslot)
+                for (int c = 0; c < T.J; c++) {
+                    const int32_t cs = a.cslot[R * T.J + 1 + c - a.start_c];
+                    if (cs >= 0) add_block(a.Ac + ((uint64_t)cs * nslc + slc) * (uint64_t)RP * RP);
+                }
+        } else if (a.csum) {
+            // dense level: the children's messages, summed per region below
+            add_block(a.Ac + (loc * nslc + slc) * (uint64_t)RP * RP);
+        } else {
+            for (int c = 0; c < T.J; c++) {
+                const uint64_t ch = R * T.J + 1 + c;
+                if (T.working[ch]) add_block(a.Ac + ((ch - a.start_c) * nslc + slc) * (uint64_t)RP * RP);
+            }
+        }
+        (void)R;
         if constexpr (F32)
             wave_mm64_acc_l32(acc, RP, -1.0, PR + (uint64_t)j * RP * RP, RP, PR + (uint64_t)k * RP * RP, RP);
         else
@@ -1206,7 +1223,8 @@ __device__ __forceinline__ void wave_add64_t(mra_d4 (&acc)[16], const double* S,
 // k_post_gram), held in the L32 layout; its rows feed L~^{-1} as the k index
 // of the product, so L~^{-1} is read at the same permuted k.  w_j = X_j^T z
 // stays FP64.
-template <bool F32 = false>
+// DENSE: a dense level (P_j from the children's messages only).
+template <bool F32 = false, bool DENSE = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_post_pj(Tree T, PostArgs a,
                                                                                          uint32_t nitems) {
     constexpr int RP = 64;
@@ -1221,8 +1239,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const uint64_t R = a.start_l + loc;
     const uint64_t msg = a.slot ? (uint64_t)a.slot[loc] : loc;
     const int r = T.r;
-    const uint64_t os = a.row_s[loc], oe = a.row_e[loc];
-    const double* X = a.X + os * (uint64_t)a.ldx;
+    // rows mode or dense level (children's messages only)
+    const uint64_t os = DENSE ? 0 : a.row_s[loc], oe = DENSE ? 0 : a.row_e[loc];
+    const double* X = DENSE ? nullptr : a.X + os * (uint64_t)a.ldx;
     const int ns = oe > os ? (int)(oe - os) : 0;
     const int lc = l + 1;
     const int nslc = lc * (lc + 1) / 2;
@@ -1233,7 +1252,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
     for (int q = 0; q < 4; q++) wacc[q] = mra_d4{0, 0, 0, 0};
     // acc = X_l^T X_j, wacc = X_j^T z (every row)
-    if constexpr (F32) {
+    if constexpr (DENSE) {
+    } else if constexpr (F32) {
         // FP32 row products (only f live in the loop), widened once
         mra_f4 f[16];
 #pragma unroll
@@ -1249,16 +1269,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     double wq[4];  // w_j[16 tj + r16]: every row of wacc holds it
 #pragma unroll
     for (int tj = 0; tj < 4; tj++) wq[tj] = wacc[tj][0];
-    if (a.cslot)
+    auto add_block_t = [&](const double* S) {
+        if constexpr (F32)
+            wave_add64_t_l32(acc, S, RP);
+        else
+            wave_add64_t(acc, S, RP);
+    };
+    if (!DENSE && a.cslot) {
         for (int c = 0; c < T.J; c++) {
             const int32_t cs = a.cslot[R * T.J + 1 + c - a.start_c];
-            if (cs < 0) continue;
-            const double* S = a.Ac + ((uint64_t)cs * nslc + slc) * (uint64_t)RP * RP;
-            if constexpr (F32)
-                wave_add64_t_l32(acc, S, RP);
-            else
-                wave_add64_t(acc, S, RP);
+            if (cs >= 0) add_block_t(a.Ac + ((uint64_t)cs * nslc + slc) * (uint64_t)RP * RP);
         }
+    } else if (DENSE && a.csum) {
+        add_block_t(a.Ac + (loc * nslc + slc) * (uint64_t)RP * RP);
+    } else if (DENSE) {
+        for (int c = 0; c < T.J; c++) {
+            const uint64_t ch = R * T.J + 1 + c;
+            if (T.working[ch]) add_block_t(a.Ac + ((ch - a.start_c) * nslc + slc) * (uint64_t)RP * RP);
+        }
+    }
     // P_j = L~^{-1} acc by row tiles tq (k tiles kt <= tq), stored row-major;
     // pv[tj] collects this lane's part of (P_j^T v)[16 tj + r16]
     const char* Li = (const char*)(a.LpiT + R * RP * RP);  // [k*RP + i] = L~^{-1}[i][k]
@@ -1311,13 +1340,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         for (int tj = 0; tj < 4; tj++) {
             const int q = 16 * tj + r16;
             double wj = 0.0;
-            if (q < r) {
+            if (q < r && !DENSE) {
                 wj = wq[tj];
                 if (a.cslot)
                     for (int c = 0; c < T.J; c++) {
                         const int32_t cs = a.cslot[R * T.J + 1 + c - a.start_c];
                         if (cs >= 0) wj += a.wc[(uint64_t)cs * lc * RP + j * RP + q];
                     }
+            } else if (q < r) {
+                // dense level: w_j summed over the working children in order
+                bool first = true;
+                for (int c = 0; c < T.J; c++) {
+                    const uint64_t ch = R * T.J + 1 + c;
+                    if (!T.working[ch]) continue;
+                    const double x = a.wc[(ch - a.start_c) * (uint64_t)lc * RP + j * RP + q];
+                    wj = first ? x : wj + x;
+                    first = false;
+                }
             }
             a.wo[msg * (uint64_t)l * RP + j * RP + q] = wj - pv[tj];
         }
