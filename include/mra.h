@@ -196,7 +196,13 @@ int mra_use_graph(mra_handle* h, int enable);
  * MRA_OPT_MIXED_FP64_ABOVE: in mixed precision, finest regions with more
  * than this many observations (default 128, in [64, 256]) are conditioned
  * and factored in FP64: in large, strongly correlated regions the FP32
- * rounding amplified through Sigma_F exceeds |dloglik|/n = 1e-6. */
+ * rounding amplified through Sigma_F exceeds |dloglik|/n = 1e-6.
+ * MRA_OPT_PERSISTENT: 1 launches the finest-level kernels (conditioning tiles
+ * and every local length bucket) on a resident grid (as many work groups as
+ * the device holds at once for that kernel) whose work groups walk the
+ * length-bucketed work lists, longest first, by grid stride; 0 (default) one
+ * work group (one wave for regions of at most 16 observations) per item.
+ * Same values bit for bit. */
 enum {
     MRA_OPT_FACTORED_POSTERIOR = 1,
     MRA_OPT_GRAM_KERNEL = 2,
@@ -205,7 +211,8 @@ enum {
     MRA_OPT_CONCURRENT = 5,
     MRA_OPT_SUM_MESSAGES = 6,
     MRA_OPT_XCD_MAP = 7,
-    MRA_OPT_MIXED_FP64_ABOVE = 8
+    MRA_OPT_MIXED_FP64_ABOVE = 8,
+    MRA_OPT_PERSISTENT = 9
 };
 int mra_set_option(mra_handle* h, int option, int value);
 int mra_get_option(mra_handle* h, int option, int* value);

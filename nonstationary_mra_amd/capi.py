@@ -52,6 +52,7 @@ OPT_CONCURRENT = 5
 OPT_SUM_MESSAGES = 6
 OPT_XCD_MAP = 7
 OPT_MIXED_FP64_ABOVE = 8
+OPT_PERSISTENT = 9
 
 
 class TreeDesc(C.Structure):

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -62,6 +63,7 @@ struct mra_handle {
     int cross16 = 1;      // segments with <= 16 observations in 16-column cross tiles
     int sumMsgs = 1;      // message blocks summed per parent by k_post_gram
     int xcdMap = 1;       // XCD-contiguous item ranges (Tree::xcd)
+    int persistent = 0;   // finest-level launches on a resident grid (MRA_OPT_PERSISTENT)
     bool denseFull = false;  // full-size dense messages of level L-1 allocated
     std::vector<uint64_t> hRowS1, hRowE1, hRowS2, hRowE2;  // rows of levels L-1 and L-2
     uint64_t *dRowS1 = nullptr, *dRowE1 = nullptr, *dRowS2 = nullptr, *dRowE2 = nullptr;
@@ -171,6 +173,7 @@ struct mra_handle {
 };
 
 static void graph_invalidate(mra_handle* h);
+static void prefill_grids(const mra_handle* h);
 
 static void set_err(mra_handle* h, const std::string& s) {
     if (h) h->err = s;
@@ -787,6 +790,7 @@ extern "C" int mra_create(const mra_tree_desc* tree, mra_handle** out) {
         mra_destroy(h);
         return rc;
     }
+    prefill_grids(h);  // resident grids of the finest kernels, outside any capture
     *out = h;
     return MRA_OK;
 }
@@ -967,6 +971,11 @@ extern "C" int mra_set_option(mra_handle* h, int option, int value) {
         graph_invalidate(h);
         return MRA_OK;
     }
+    if (option == MRA_OPT_PERSISTENT) {
+        h->persistent = value ? 1 : 0;
+        graph_invalidate(h);
+        return MRA_OK;
+    }
     if (option == MRA_OPT_MIXED_FP64_ABOVE) {
         if (value < 64 || value > 256) {
             set_err(h, "MRA_OPT_MIXED_FP64_ABOVE must be in [64, 256]");
@@ -1009,6 +1018,10 @@ extern "C" int mra_get_option(mra_handle* h, int option, int* value) {
     }
     if (option == MRA_OPT_XCD_MAP) {
         *value = h->xcdMap;
+        return MRA_OK;
+    }
+    if (option == MRA_OPT_PERSISTENT) {
+        *value = h->persistent;
         return MRA_OK;
     }
     if (option == MRA_OPT_MIXED_FP64_ABOVE) {
@@ -1096,6 +1109,59 @@ struct Fork {
     }
 };
 
+// Work groups of one kernel that are resident on the device at once
+// (occupancy x CUs, rounded down to a multiple of 8 so that work group g
+// keeps XCD g mod 8), per kernel and device.  Filled outside graph capture
+// (prefill_grids at handle creation), read while launching.
+static std::mutex g_grid_mu;
+static std::vector<std::pair<std::pair<const void*, int>, uint32_t>> g_grid;
+
+static uint32_t resident_groups(const void* k) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(g_grid_mu);
+    for (const auto& e : g_grid)
+        if (e.first.first == k && e.first.second == dev) return e.second;
+    int nb = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, 256, 0) != hipSuccess || nb < 1) nb = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
+    uint32_t g = (uint32_t)nb * (uint32_t)cus;
+    if (g >= 8) g = g / 8 * 8;
+    g_grid.push_back({{k, dev}, g});
+    return g;
+}
+
+// Grid of a launch over `units` work units taken `per_wg` per work group:
+// one work group per per_wg units, or (persistent launches) at most the
+// resident work groups, each walking the units by grid stride.
+template <typename K>
+static uint32_t unit_grid(const mra_handle* h, K kernel, uint32_t units, uint32_t per_wg = 1) {
+    const uint32_t full = (units + per_wg - 1) / per_wg;
+    if (!h->persistent) return full;
+    return std::max(1u, std::min(full, resident_groups((const void*)kernel)));
+}
+
+template <int RP, typename TG>
+static void prefill_grids_t() {
+    (void)resident_groups((const void*)&k_finest_cross<RP, 64, TG>);
+    (void)resident_groups((const void*)&k_finest_cross<RP, 32, TG>);
+    (void)resident_groups((const void*)&k_finest_cross<RP, 16, TG>);
+    (void)resident_groups((const void*)&k_finest_local<RP, 0, TG>);
+    (void)resident_groups((const void*)&k_finest_local<RP, 16, TG>);
+    (void)resident_groups((const void*)&k_finest_local<RP, 32, TG>);
+    (void)resident_groups((const void*)&k_finest_local<RP, 64, TG>);
+    (void)resident_groups((const void*)&k_finest_local_big<TG>);
+    (void)resident_groups((const void*)&k_finest_local_w16<TG>);
+}
+
+static void prefill_grids(const mra_handle* h) {
+    switch (h->RP) {
+        case 16: prefill_grids_t<16, float>(); prefill_grids_t<16, double>(); break;
+        case 32: prefill_grids_t<32, float>(); prefill_grids_t<32, double>(); break;
+        default: prefill_grids_t<64, float>(); prefill_grids_t<64, double>(); break;
+    }
+}
+
 template <int RP, typename TG>
 static void launch_finest_local(mra_handle* h, const Tree& T, const CovCtx& cc, const Pts& O) {
     // the longest work groups (largest buckets) on one side stream, the
@@ -1108,38 +1174,37 @@ static void launch_finest_local(mra_handle* h, const Tree& T, const CovCtx& cc, 
     // precision the regions above mixFp64Min observations (buckets 3 and 5)
     // use their FP64 G and B
     if (h->nItemsFB[5] && sizeof(TG) == 4 && h->nItemsXBig && h->dGf64)
-        k_finest_local<RP, 0, double><<<h->nItemsFB[5], 256, 0, s1>>>(T, cc, O, h->dY, h->dItemsFB[5], h->dGf64,
-                                                                      h->dBf64, h->dSig, h->dH, h->dU, h->dDterm,
-                                                                      h->dBad);
+        k_finest_local<RP, 0, double><<<unit_grid(h, &k_finest_local<RP, 0, double>, h->nItemsFB[5]), 256, 0, s1>>>(
+            T, cc, O, h->dY, h->dItemsFB[5], h->nItemsFB[5], h->dGf64, h->dBf64, h->dSig, h->dH, h->dU, h->dDterm,
+            h->dBad);
     else if (h->nItemsFB[5])
-        k_finest_local<RP, 0, TG><<<h->nItemsFB[5], 256, 0, s1>>>(T, cc, O, h->dY, h->dItemsFB[5], Gf, Bf, h->dSig,
-                                                                  h->dH, h->dU, h->dDterm, h->dBad);
+        k_finest_local<RP, 0, TG><<<unit_grid(h, &k_finest_local<RP, 0, TG>, h->nItemsFB[5]), 256, 0, s1>>>(
+            T, cc, O, h->dY, h->dItemsFB[5], h->nItemsFB[5], Gf, Bf, h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
     if (h->nItemsFB[3] && sizeof(TG) == 4 && h->nItemsXBig && h->dGf64)
-        k_finest_local_big<double><<<h->nItemsFB[3], 256, 0, s1>>>(T, cc, O, h->dY, h->dItemsFB[3], h->dWsOffBig3,
-                                                                   h->dWsBig, h->dGf64, h->dBf64, h->dSig, h->dH,
-                                                                   h->dU, h->dDterm, h->dBad);
+        k_finest_local_big<double><<<unit_grid(h, &k_finest_local_big<double>, h->nItemsFB[3]), 256, 0, s1>>>(
+            T, cc, O, h->dY, h->dItemsFB[3], h->nItemsFB[3], h->dWsOffBig3, h->dWsBig, h->dGf64, h->dBf64, h->dSig,
+            h->dH, h->dU, h->dDterm, h->dBad);
     else if (h->nItemsFB[3])
-        k_finest_local_big<TG><<<h->nItemsFB[3], 256, 0, s1>>>(T, cc, O, h->dY, h->dItemsFB[3], h->dWsOffBig3,
-                                                               h->dWsBig, Gf, Bf, h->dSig, h->dH, h->dU, h->dDterm,
-                                                               h->dBad);
+        k_finest_local_big<TG><<<unit_grid(h, &k_finest_local_big<TG>, h->nItemsFB[3]), 256, 0, s1>>>(
+            T, cc, O, h->dY, h->dItemsFB[3], h->nItemsFB[3], h->dWsOffBig3, h->dWsBig, Gf, Bf, h->dSig, h->dH, h->dU,
+            h->dDterm, h->dBad);
     if (h->nItemsFB[4])
-        k_finest_local_big<TG><<<h->nItemsFB[4], 256, 0, s1>>>(T, cc, O, h->dY, h->dItemsFB[4], h->dWsOffBig,
-                                                               h->dWsBig, Gf, Bf, h->dSig, h->dH, h->dU, h->dDterm,
-                                                               h->dBad);
+        k_finest_local_big<TG><<<unit_grid(h, &k_finest_local_big<TG>, h->nItemsFB[4]), 256, 0, s1>>>(
+            T, cc, O, h->dY, h->dItemsFB[4], h->nItemsFB[4], h->dWsOffBig, h->dWsBig, Gf, Bf, h->dSig, h->dH, h->dU,
+            h->dDterm, h->dBad);
     if (h->nItemsFB[2])
-        k_finest_local<RP, 64, TG><<<h->nItemsFB[2], 256, 0, s2>>>(T, cc, O, h->dY, h->dItemsFB[2], Gf, Bf, h->dSig,
-                                                                   h->dH, h->dU, h->dDterm, h->dBad);
+        k_finest_local<RP, 64, TG><<<unit_grid(h, &k_finest_local<RP, 64, TG>, h->nItemsFB[2]), 256, 0, s2>>>(
+            T, cc, O, h->dY, h->dItemsFB[2], h->nItemsFB[2], Gf, Bf, h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
     if (h->nItemsFB[1])
-        k_finest_local<RP, 32, TG><<<h->nItemsFB[1], 256, 0, s2>>>(T, cc, O, h->dY, h->dItemsFB[1], Gf, Bf, h->dSig,
-                                                                   h->dH, h->dU, h->dDterm, h->dBad);
+        k_finest_local<RP, 32, TG><<<unit_grid(h, &k_finest_local<RP, 32, TG>, h->nItemsFB[1]), 256, 0, s2>>>(
+            T, cc, O, h->dY, h->dItemsFB[1], h->nItemsFB[1], Gf, Bf, h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
     if (h->nItemsFB[0] && h->waveLocal16)
         // one wave per region, four regions per work group
-        k_finest_local_w16<TG><<<(h->nItemsFB[0] + 3) / 4, 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[0],
-                                                                          h->nItemsFB[0], Gf, Bf, h->dSig, h->dH,
-                                                                          h->dU, h->dDterm, h->dBad);
+        k_finest_local_w16<TG><<<unit_grid(h, &k_finest_local_w16<TG>, h->nItemsFB[0], 4), 256, 0, st>>>(
+            T, cc, O, h->dY, h->dItemsFB[0], h->nItemsFB[0], Gf, Bf, h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
     else if (h->nItemsFB[0])
-        k_finest_local<RP, 16, TG><<<h->nItemsFB[0], 256, 0, st>>>(T, cc, O, h->dY, h->dItemsFB[0], Gf, Bf, h->dSig,
-                                                                   h->dH, h->dU, h->dDterm, h->dBad);
+        k_finest_local<RP, 16, TG><<<unit_grid(h, &k_finest_local<RP, 16, TG>, h->nItemsFB[0]), 256, 0, st>>>(
+            T, cc, O, h->dY, h->dItemsFB[0], h->nItemsFB[0], Gf, Bf, h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
 }
 
 template <int RP, typename TG>
@@ -1147,17 +1212,17 @@ static void launch_finest_cross(mra_handle* h, const Tree& T, const CovCtx& cc, 
     const Fork fk(h, 1);
     const hipStream_t st = fk.stream(0);
     if (h->nItemsX)
-        k_finest_cross<RP, 64, TG><<<h->nItemsX, 256, 0, fk.stream(1)>>>(T, cc, K, O, h->dItemsX, h->dGoff, h->dGc, h->dLiT,
-                                                               (TG*)h->dGf, (TG*)h->dBf, h->dGc32);
+        k_finest_cross<RP, 64, TG><<<unit_grid(h, &k_finest_cross<RP, 64, TG>, h->nItemsX), 256, 0, fk.stream(1)>>>(
+            T, cc, K, O, h->dItemsX, h->nItemsX, h->dGoff, h->dGc, h->dLiT, (TG*)h->dGf, (TG*)h->dBf, h->dGc32);
     if (h->nItemsX32)
-        k_finest_cross<RP, 32, TG><<<h->nItemsX32, 256, 0, st>>>(T, cc, K, O, h->dItemsX32, h->dGoff, h->dGc,
-                                                                 h->dLiT, (TG*)h->dGf, (TG*)h->dBf, h->dGc32);
+        k_finest_cross<RP, 32, TG><<<unit_grid(h, &k_finest_cross<RP, 32, TG>, h->nItemsX32), 256, 0, st>>>(
+            T, cc, K, O, h->dItemsX32, h->nItemsX32, h->dGoff, h->dGc, h->dLiT, (TG*)h->dGf, (TG*)h->dBf, h->dGc32);
     if (h->nItemsX16)
-        k_finest_cross<RP, 16, TG><<<h->nItemsX16, 256, 0, fk.stream(2)>>>(T, cc, K, O, h->dItemsX16, h->dGoff, h->dGc,
-                                                                 h->dLiT, (TG*)h->dGf, (TG*)h->dBf, h->dGc32);
+        k_finest_cross<RP, 16, TG><<<unit_grid(h, &k_finest_cross<RP, 16, TG>, h->nItemsX16), 256, 0, fk.stream(2)>>>(
+            T, cc, K, O, h->dItemsX16, h->nItemsX16, h->dGoff, h->dGc, h->dLiT, (TG*)h->dGf, (TG*)h->dBf, h->dGc32);
     if (sizeof(TG) == 4 && h->nItemsXBig && h->dGf64)
-        k_finest_cross<RP, 64, double><<<h->nItemsXBig, 256, 0, st>>>(T, cc, K, O, h->dItemsXBig, h->dGoff, h->dGc,
-                                                                     h->dLiT, h->dGf64, h->dBf64, nullptr);
+        k_finest_cross<RP, 64, double><<<unit_grid(h, &k_finest_cross<RP, 64, double>, h->nItemsXBig), 256, 0, st>>>(
+            T, cc, K, O, h->dItemsXBig, h->nItemsXBig, h->dGoff, h->dGc, h->dLiT, h->dGf64, h->dBf64, nullptr);
 }
 
 template <int RP>

@@ -388,68 +388,73 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // tile, both products and the stored G, B are FP32 with FP32 accumulation.
 template <int RP, int NT, typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3))) void k_finest_cross(Tree T_, CovCtx cc, Pts K, Pts O, const uint32_t* items,
-                                                      const uint64_t* goff, const double* Gc, const double* LiT,
-                                                      T* Gf, T* Bf, const float* Gc32) {
+                                                      uint32_t nitems, const uint64_t* goff, const double* Gc,
+                                                      const double* LiT, T* Gf, T* Bf, const float* Gc32) {
     const Tree& T0 = T_;
     constexpr int LD = NT + 1;
     __shared__ T Cm[RP * LD];
     __shared__ PtV kst[2][RP];  // ancestor knots, double-buffered
-    const uint32_t bid = xcd_block(blockIdx.x, gridDim.x, T0.xcd);
-    const uint32_t sg = items[2 * bid], c0 = items[2 * bid + 1];
-    const uint64_t ob = T0.s_os[sg];
-    const int nF = (int)T0.s_n[sg];
-    const int n = min(NT, nF - (int)c0);
-    const int r = T0.r, L = T0.L;
-    T* GF = Gf + (uint64_t)T0.LRP * ob;
-    T* BO = Bf + (uint64_t)T0.LRP * ob;
-    // this thread's observation column (the same for every ancestor step)
-    const int cc_ = threadIdx.x % NT;
-    const PtV qc = cc_ < n ? load_pt(cc, O, ob + c0 + cc_) : PtV{0, 0, 0, 0, 1};
-    // knots of the level-0 ancestor; each later step's knots are loaded into
-    // registers while the current step computes, and staged at its end
-    if (threadIdx.x < r) kst[0][threadIdx.x] = load_pt(cc, K, ancestor_up(T0.s_parent[sg], L - 1, 0, T0.J) * RP + threadIdx.x);
-    __syncthreads();
-    for (int j = 0; j < L; j++) {
-        const uint64_t a = ancestor_up(T0.s_parent[sg], L - 1, j, T0.J);
-        // raw fields only (q is formed when staged, so nothing waits on these loads here)
-        double nx = 0.0, ny = 0.0, nz = 0.0, ns_ = 0.0, nb = 1.0;
-        if (j + 1 < L && threadIdx.x < r) {
-            const uint64_t q = ancestor_up(T0.s_parent[sg], L - 1, j + 1, T0.J) * RP + threadIdx.x;
-            nx = K.x[q];
-            ny = K.y[q];
-            nz = K.z[q];
-            ns_ = K.s[q];
-            nb = K.b[q];
-        }
-        cov_tile_pre<RP, NT, T>(Cm, LD, cc, kst[j & 1], r, qc, n);
+    // one item per work group, or (persistent launch) items k = blockIdx.x +
+    // i * gridDim.x; the j loop ends with a barrier, so LDS is free for the
+    // next item
+    for (uint32_t k = blockIdx.x; k < nitems; k += gridDim.x) {
+        const uint32_t bid = xcd_block(k, nitems, T0.xcd);
+        const uint32_t sg = items[2 * bid], c0 = items[2 * bid + 1];
+        const uint64_t ob = T0.s_os[sg];
+        const int nF = (int)T0.s_n[sg];
+        const int n = min(NT, nF - (int)c0);
+        const int r = T0.r, L = T0.L;
+        T* GF = Gf + (uint64_t)T0.LRP * ob;
+        T* BO = Bf + (uint64_t)T0.LRP * ob;
+        // this thread's observation column (the same for every ancestor step)
+        const int cc_ = threadIdx.x % NT;
+        const PtV qc = cc_ < n ? load_pt(cc, O, ob + c0 + cc_) : PtV{0, 0, 0, 0, 1};
+        // knots of the level-0 ancestor; each later step's knots are loaded into
+        // registers while the current step computes, and staged at its end
+        if (threadIdx.x < r) kst[0][threadIdx.x] = load_pt(cc, K, ancestor_up(T0.s_parent[sg], L - 1, 0, T0.J) * RP + threadIdx.x);
         __syncthreads();
-        if (j > 0) {
-            const uint64_t go = goff[j] + (a - lvl_start(j, T0.J)) * (uint64_t)j * RP * RP;
-            const double* Ga = Gc + go;
+        for (int j = 0; j < L; j++) {
+            const uint64_t a = ancestor_up(T0.s_parent[sg], L - 1, j, T0.J);
+            // raw fields only (q is formed when staged, so nothing waits on these loads here)
+            double nx = 0.0, ny = 0.0, nz = 0.0, ns_ = 0.0, nb = 1.0;
+            if (j + 1 < L && threadIdx.x < r) {
+                const uint64_t q = ancestor_up(T0.s_parent[sg], L - 1, j + 1, T0.J) * RP + threadIdx.x;
+                nx = K.x[q];
+                ny = K.y[q];
+                nz = K.z[q];
+                ns_ = K.s[q];
+                nb = K.b[q];
+            }
+            cov_tile_pre<RP, NT, T>(Cm, LD, cc, kst[j & 1], r, qc, n);
+            __syncthreads();
+            if (j > 0) {
+                const uint64_t go = goff[j] + (a - lvl_start(j, T0.J)) * (uint64_t)j * RP * RP;
+                const double* Ga = Gc + go;
+                if constexpr (sizeof(T) == 4)
+                    wg_gemm_tn_f32<true, float, T, float, 0, true>(Cm, LD, RP, NT, j * RP, -1.0f, Gc32 + go, RP, RP,
+                                                                   GF + c0, nF, n);
+                else
+                    wg_gemm_tn<false, true, double, double, GEMM_ANY>(Cm, LD, RP, NT, j * RP, -1.0, Ga, RP, RP, GF + c0,
+                                                                      nF, n);
+                __syncthreads();
+            }
+            // B is stored observation-major: row o holds B_{F,0..L-1}(:, o)
+            for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
+                const int i = idx % RP, c = idx / RP;
+                if (c < n) BO[(uint64_t)(c0 + c) * T0.LRP + j * RP + i] = Cm[i * LD + c];
+            }
+            // G_{F,j} = L_a^{-1} B, stored straight to global memory
             if constexpr (sizeof(T) == 4)
-                wg_gemm_tn_f32<true, float, T, float, 0, true>(Cm, LD, RP, NT, j * RP, -1.0f, Gc32 + go, RP, RP,
-                                                               GF + c0, nF, n);
+                wg_gemm_tn_f32<false, double, float, float, GEMM_TRI_A, true>(GF + (uint64_t)(j * RP) * nF + c0, nF, RP, NT,
+                                                                        RP, 1.0f, LiT + a * RP * RP, RP, RP,
+                                                                        (const float*)Cm, LD, n);
             else
-                wg_gemm_tn<false, true, double, double, GEMM_ANY>(Cm, LD, RP, NT, j * RP, -1.0, Ga, RP, RP, GF + c0,
-                                                                  nF, n);
+                wg_gemm_tn<false, false, double, double, GEMM_ANY, false, GEMM_TRI_A>(GF + (uint64_t)(j * RP) * nF + c0, nF, RP, NT, RP, 1.0,
+                                                                   LiT + a * RP * RP, RP, RP, (const double*)Cm, LD, n);
+            if (j + 1 < L && threadIdx.x < r)
+                kst[(j + 1) & 1][threadIdx.x] = PtV{nx, ny, nz, cc.stationary ? 0.0 : ns_ * (nb * sqrt(nb)), nb};
             __syncthreads();
         }
-        // B is stored observation-major: row o holds B_{F,0..L-1}(:, o)
-        for (int idx = threadIdx.x; idx < RP * NT; idx += MRA_WG) {
-            const int i = idx % RP, c = idx / RP;
-            if (c < n) BO[(uint64_t)(c0 + c) * T0.LRP + j * RP + i] = Cm[i * LD + c];
-        }
-        // G_{F,j} = L_a^{-1} B, stored straight to global memory
-        if constexpr (sizeof(T) == 4)
-            wg_gemm_tn_f32<false, double, float, float, GEMM_TRI_A, true>(GF + (uint64_t)(j * RP) * nF + c0, nF, RP, NT,
-                                                                    RP, 1.0f, LiT + a * RP * RP, RP, RP,
-                                                                    (const float*)Cm, LD, n);
-        else
-            wg_gemm_tn<false, false, double, double, GEMM_ANY, false, GEMM_TRI_A>(GF + (uint64_t)(j * RP) * nF + c0, nF, RP, NT, RP, 1.0,
-                                                               LiT + a * RP * RP, RP, RP, (const double*)Cm, LD, n);
-        if (j + 1 < L && threadIdx.x < r)
-            kst[(j + 1) & 1][threadIdx.x] = PtV{nx, ny, nz, cc.stationary ? 0.0 : ns_ * (nb * sqrt(nb)), nb};
-        __syncthreads();
     }
 }
 
@@ -477,105 +482,111 @@ struct FinestLds<0> {
 
 template <int RP, int NB, typename TG>
 __global__ __launch_bounds__(256) void k_finest_local(Tree T, CovCtx cc, Pts O, const double* yv,
-                                                      const uint32_t* items, TG* Gf, TG* Bf, double* Sig,
-                                                      double* H, double* u, double* dterm, int* bad_flag) {
+                                                      const uint32_t* items, uint32_t nitems, TG* Gf, TG* Bf,
+                                                      double* Sig, double* H, double* u, double* dterm,
+                                                      int* bad_flag) {
     __shared__ FinestLds<NB> sh;
     __shared__ double red[MRA_WG];
     __shared__ int bad;
-    const uint32_t F = items[blockIdx.x];
-    const uint64_t ob = T.obs_off[F];
-    const int n = (int)(T.obs_off[F + 1] - ob);
-    const int LRP = T.LRP;
-    double* LF = Sig + T.sig_off[F];  // L_F (row-major) kept for prediction and sampling
-    double* Sg = NB > 0 ? sh.S : LF;
-    const int ld = NB > 0 ? FinestLds<NB>::LD : n;
-    const uint64_t so = T.f_seg_os[F];
-    const int sn = (int)T.f_seg_n[F];
-    const TG* GF = Gf + (uint64_t)LRP * so + (ob - so);
-    if (threadIdx.x == 0) bad = 0;
-    for (int idx = threadIdx.x; idx < n * n; idx += MRA_WG) {
-        const int i = idx / n, c = idx % n;
-        double v;
-        if constexpr (sizeof(TG) == 4)
-            v = (i == c) ? O.v[ob + i] : (double)pcovf(cc, O, ob + i, O, ob + c);
-        else
-            v = (i == c) ? O.v[ob + i] : pcov(cc, O, ob + i, O, ob + c);
-        Sg[i * ld + c] = v;
-    }
-    __syncthreads();
-    if (NB > 0 && NB <= 32) {
-        wg_gemm_tn_ksplit(Sg, ld, (n + 15) & ~15, (n + 15) & ~15, LRP, -1.0, GF, sn, n, GF, sn, n, sh.kred);
-    } else {
-        for (int ti = 0; ti < n; ti += 64)
-            for (int tj = 0; tj < n; tj += 64) {
-                const int mv = min(64, n - ti), nv = min(64, n - tj);
-                wg_gemm_tn(Sg + ti * ld + tj, ld, (mv + 15) & ~15, (nv + 15) & ~15, LRP, -1.0, GF + ti, sn, mv,
-                           GF + tj, sn, nv);
-            }
-    }
-    __syncthreads();
-    const int nbk = (n + 15) >> 4;
-    if (NB > 0) {
-        // blocked factorisation of the 16-padded block: identity padding
-        const int N16 = nbk * 16;
-        for (int idx = threadIdx.x; idx < N16 * N16; idx += MRA_WG) {
-            const int i = idx / N16, c = idx % N16;
-            if (i >= n || c >= n) Sg[i * ld + c] = (i == c) ? 1.0 : 0.0;
-        }
-        wg_chol_blk(Sg, ld, nbk, sh.D, &bad);
-    } else {
-        wg_chol(Sg, ld, n, &bad);
-    }
-    // log det part, then keep L_F (lower triangle; zeros above)
-    double part = 0.0;
-    for (int i = threadIdx.x; i < n; i += MRA_WG) part += 2.0 * log(Sg[i * ld + i]);
-    if (NB > 0)
+    // one region per work group, or (persistent launch) regions k =
+    // blockIdx.x + i * gridDim.x with a barrier before the next one
+    for (uint32_t k = blockIdx.x; k < nitems; k += gridDim.x) {
+        const uint32_t F = items[k];
+        const uint64_t ob = T.obs_off[F];
+        const int n = (int)(T.obs_off[F + 1] - ob);
+        const int LRP = T.LRP;
+        double* LF = Sig + T.sig_off[F];  // L_F (row-major) kept for prediction and sampling
+        double* Sg = NB > 0 ? sh.S : LF;
+        const int ld = NB > 0 ? FinestLds<NB>::LD : n;
+        const uint64_t so = T.f_seg_os[F];
+        const int sn = (int)T.f_seg_n[F];
+        const TG* GF = Gf + (uint64_t)LRP * so + (ob - so);
+        if (threadIdx.x == 0) bad = 0;
         for (int idx = threadIdx.x; idx < n * n; idx += MRA_WG) {
             const int i = idx / n, c = idx % n;
-            LF[idx] = c <= i ? Sg[i * ld + c] : 0.0;
+            double v;
+            if constexpr (sizeof(TG) == 4)
+                v = (i == c) ? O.v[ob + i] : (double)pcovf(cc, O, ob + i, O, ob + c);
+            else
+                v = (i == c) ? O.v[ob + i] : pcov(cc, O, ob + i, O, ob + c);
+            Sg[i * ld + c] = v;
         }
-    __syncthreads();
-    if (NB > 0) {
-        // L_F^{-1} in place (zero strict upper part, read by the H product)
-        wg_trinv_blk(Sg, ld, nbk, sh.D);
-        // u = L_F^{-1} y
-        double* uF = u + ob;
-        for (int i = threadIdx.x; i < n; i += MRA_WG) {
-            double acc = 0.0;
-            for (int k = 0; k <= i; k++) acc += Sg[i * ld + k] * yv[ob + k];
-            uF[i] = acc;
-            part += acc * acc;
-        }
-        const double dF = wg_sum(part, red);
-        // H = L_F^{-1} B (observation-major rows), 64 columns at a time
-        const TG* BO = Bf + ob * (uint64_t)LRP;
-        for (int c0 = 0; c0 < LRP; c0 += 64) {
-            const int cv = min(64, LRP - c0);
-            wg_gemm_tn<true, false>(H + ob * (uint64_t)LRP + c0, LRP, (n + 15) & ~15, 64, n, 1.0, Sg, ld, n,
-                                    BO + c0, LRP, cv);
-        }
-        if (threadIdx.x == 0) dterm[T.nC + F] = dF;
-    } else {
-        // large regions: forward substitution in global memory
-        double* uF = u + ob;
-        for (int i = threadIdx.x; i < n; i += MRA_WG) uF[i] = yv[ob + i];
         __syncthreads();
-        wg_trsm(Sg, ld, uF, 1, n, 1);
-        for (int i = threadIdx.x; i < n; i += MRA_WG) part += uF[i] * uF[i];
-        const double dF = wg_sum(part, red);
-        const TG* BO = Bf + ob * (uint64_t)LRP;
-        double* HF = H + ob * (uint64_t)LRP;
-        for (int m = threadIdx.x; m < LRP; m += MRA_WG) {
-            for (int i = 0; i < n; i++) {
-                double acc = (double)BO[(uint64_t)i * LRP + m];
-                for (int k = 0; k < i; k++) acc -= Sg[i * ld + k] * HF[(uint64_t)k * LRP + m];
-                acc /= Sg[i * ld + i];
-                HF[(uint64_t)i * LRP + m] = acc;
-            }
+        if (NB > 0 && NB <= 32) {
+            wg_gemm_tn_ksplit(Sg, ld, (n + 15) & ~15, (n + 15) & ~15, LRP, -1.0, GF, sn, n, GF, sn, n, sh.kred);
+        } else {
+            for (int ti = 0; ti < n; ti += 64)
+                for (int tj = 0; tj < n; tj += 64) {
+                    const int mv = min(64, n - ti), nv = min(64, n - tj);
+                    wg_gemm_tn(Sg + ti * ld + tj, ld, (mv + 15) & ~15, (nv + 15) & ~15, LRP, -1.0, GF + ti, sn, mv,
+                               GF + tj, sn, nv);
+                }
         }
-        if (threadIdx.x == 0) dterm[T.nC + F] = dF;
+        __syncthreads();
+        const int nbk = (n + 15) >> 4;
+        if (NB > 0) {
+            // blocked factorisation of the 16-padded block: identity padding
+            const int N16 = nbk * 16;
+            for (int idx = threadIdx.x; idx < N16 * N16; idx += MRA_WG) {
+                const int i = idx / N16, c = idx % N16;
+                if (i >= n || c >= n) Sg[i * ld + c] = (i == c) ? 1.0 : 0.0;
+            }
+            wg_chol_blk(Sg, ld, nbk, sh.D, &bad);
+        } else {
+            wg_chol(Sg, ld, n, &bad);
+        }
+        // log det part, then keep L_F (lower triangle; zeros above)
+        double part = 0.0;
+        for (int i = threadIdx.x; i < n; i += MRA_WG) part += 2.0 * log(Sg[i * ld + i]);
+        if (NB > 0)
+            for (int idx = threadIdx.x; idx < n * n; idx += MRA_WG) {
+                const int i = idx / n, c = idx % n;
+                LF[idx] = c <= i ? Sg[i * ld + c] : 0.0;
+            }
+        __syncthreads();
+        if (NB > 0) {
+            // L_F^{-1} in place (zero strict upper part, read by the H product)
+            wg_trinv_blk(Sg, ld, nbk, sh.D);
+            // u = L_F^{-1} y
+            double* uF = u + ob;
+            for (int i = threadIdx.x; i < n; i += MRA_WG) {
+                double acc = 0.0;
+                for (int k = 0; k <= i; k++) acc += Sg[i * ld + k] * yv[ob + k];
+                uF[i] = acc;
+                part += acc * acc;
+            }
+            const double dF = wg_sum(part, red);
+            // H = L_F^{-1} B (observation-major rows), 64 columns at a time
+            const TG* BO = Bf + ob * (uint64_t)LRP;
+            for (int c0 = 0; c0 < LRP; c0 += 64) {
+                const int cv = min(64, LRP - c0);
+                wg_gemm_tn<true, false>(H + ob * (uint64_t)LRP + c0, LRP, (n + 15) & ~15, 64, n, 1.0, Sg, ld, n,
+                                        BO + c0, LRP, cv);
+            }
+            if (threadIdx.x == 0) dterm[T.nC + F] = dF;
+        } else {
+            // large regions: forward substitution in global memory
+            double* uF = u + ob;
+            for (int i = threadIdx.x; i < n; i += MRA_WG) uF[i] = yv[ob + i];
+            __syncthreads();
+            wg_trsm(Sg, ld, uF, 1, n, 1);
+            for (int i = threadIdx.x; i < n; i += MRA_WG) part += uF[i] * uF[i];
+            const double dF = wg_sum(part, red);
+            const TG* BO = Bf + ob * (uint64_t)LRP;
+            double* HF = H + ob * (uint64_t)LRP;
+            for (int m = threadIdx.x; m < LRP; m += MRA_WG) {
+                for (int i = 0; i < n; i++) {
+                    double acc = (double)BO[(uint64_t)i * LRP + m];
+                    for (int k = 0; k < i; k++) acc -= Sg[i * ld + k] * HF[(uint64_t)k * LRP + m];
+                    acc /= Sg[i * ld + i];
+                    HF[(uint64_t)i * LRP + m] = acc;
+                }
+            }
+            if (threadIdx.x == 0) dterm[T.nC + F] = dF;
+        }
+        if (threadIdx.x == 0 && bad) atomicOr(bad_flag, 1);
+        __syncthreads();
     }
-    if (threadIdx.x == 0 && bad) atomicOr(bad_flag, 1);
 }
 
 // The same local step for finest regions with 64 < n_F <= 256 observations,
@@ -587,75 +598,81 @@ __global__ __launch_bounds__(256) void k_finest_local(Tree T, CovCtx cc, Pts O, 
 // n_F x n_F place for prediction and sampling.
 template <typename TG>
 __global__ __launch_bounds__(256) void k_finest_local_big(Tree T, CovCtx cc, Pts O, const double* yv,
-                                                          const uint32_t* items, const uint64_t* ws_off, double* ws,
-                                                          const TG* Gf, const TG* Bf, double* Sig, double* H,
-                                                          double* u, double* dterm, int* bad_flag) {
+                                                          const uint32_t* items, uint32_t nitems,
+                                                          const uint64_t* ws_off, double* ws, const TG* Gf,
+                                                          const TG* Bf, double* Sig, double* H, double* u,
+                                                          double* dterm, int* bad_flag) {
     __shared__ double red[MRA_WG];
     __shared__ double dbuf[1];  // unused: with 3 or more blocks the inverse diagonal blocks live in A
     __shared__ int bad;
-    const uint32_t F = items[blockIdx.x];
-    const uint64_t ob = T.obs_off[F];
-    const int n = (int)(T.obs_off[F + 1] - ob);
-    const int LRP = T.LRP;
-    const int nbk = (n + 15) >> 4, ld = nbk * 16;
-    double* Sg = ws + ws_off[blockIdx.x];
-    double* LF = Sig + T.sig_off[F];
-    const uint64_t so = T.f_seg_os[F];
-    const int sn = (int)T.f_seg_n[F];
-    const TG* GF = Gf + (uint64_t)LRP * so + (ob - so);
-    if (threadIdx.x == 0) bad = 0;
-    // V(S) on the lower triangle, identity padding (the upper triangle is not read)
-    for (int idx = threadIdx.x; idx < ld * ld; idx += MRA_WG) {
-        const int i = idx / ld, c = idx % ld;
-        if (c > i) continue;
-        double v;
-        if (i < n && c < n) {
-            if (i == c)
-                v = O.v[ob + i];
-            else if constexpr (sizeof(TG) == 4)
-                v = (double)pcovf(cc, O, ob + i, O, ob + c);
-            else
-                v = pcov(cc, O, ob + i, O, ob + c);
-        } else {
-            v = (i == c) ? 1.0 : 0.0;
+    // one region per work group, or (persistent launch) regions k =
+    // blockIdx.x + i * gridDim.x with a barrier before the next one
+    for (uint32_t k = blockIdx.x; k < nitems; k += gridDim.x) {
+        const uint32_t F = items[k];
+        const uint64_t ob = T.obs_off[F];
+        const int n = (int)(T.obs_off[F + 1] - ob);
+        const int LRP = T.LRP;
+        const int nbk = (n + 15) >> 4, ld = nbk * 16;
+        double* Sg = ws + ws_off[k];
+        double* LF = Sig + T.sig_off[F];
+        const uint64_t so = T.f_seg_os[F];
+        const int sn = (int)T.f_seg_n[F];
+        const TG* GF = Gf + (uint64_t)LRP * so + (ob - so);
+        if (threadIdx.x == 0) bad = 0;
+        // V(S) on the lower triangle, identity padding (the upper triangle is not read)
+        for (int idx = threadIdx.x; idx < ld * ld; idx += MRA_WG) {
+            const int i = idx / ld, c = idx % ld;
+            if (c > i) continue;
+            double v;
+            if (i < n && c < n) {
+                if (i == c)
+                    v = O.v[ob + i];
+                else if constexpr (sizeof(TG) == 4)
+                    v = (double)pcovf(cc, O, ob + i, O, ob + c);
+                else
+                    v = pcov(cc, O, ob + i, O, ob + c);
+            } else {
+                v = (i == c) ? 1.0 : 0.0;
+            }
+            Sg[(uint64_t)i * ld + c] = v;
         }
-        Sg[(uint64_t)i * ld + c] = v;
-    }
-    __syncthreads();
-    // - G^T G on the lower 64 x 64 tiles (only the valid n x n part changes)
-    for (int ti = 0; ti < n; ti += 64)
-        for (int tj = 0; tj <= ti; tj += 64) {
-            const int mv = min(64, n - ti), nv = min(64, n - tj);
-            wg_gemm_tn(Sg + (uint64_t)ti * ld + tj, ld, (mv + 15) & ~15, (nv + 15) & ~15, LRP, -1.0, GF + ti, sn, mv,
-                       GF + tj, sn, nv);
+        __syncthreads();
+        // - G^T G on the lower 64 x 64 tiles (only the valid n x n part changes)
+        for (int ti = 0; ti < n; ti += 64)
+            for (int tj = 0; tj <= ti; tj += 64) {
+                const int mv = min(64, n - ti), nv = min(64, n - tj);
+                wg_gemm_tn(Sg + (uint64_t)ti * ld + tj, ld, (mv + 15) & ~15, (nv + 15) & ~15, LRP, -1.0, GF + ti, sn, mv,
+                           GF + tj, sn, nv);
+            }
+        __syncthreads();
+        wg_chol_blk(Sg, ld, nbk, dbuf, &bad);
+        double part = 0.0;
+        for (int i = threadIdx.x; i < n; i += MRA_WG) part += 2.0 * log(Sg[(uint64_t)i * ld + i]);
+        for (int idx = threadIdx.x; idx < n * n; idx += MRA_WG) {
+            const int i = idx / n, c = idx % n;
+            LF[idx] = c <= i ? Sg[(uint64_t)i * ld + c] : 0.0;
         }
-    __syncthreads();
-    wg_chol_blk(Sg, ld, nbk, dbuf, &bad);
-    double part = 0.0;
-    for (int i = threadIdx.x; i < n; i += MRA_WG) part += 2.0 * log(Sg[(uint64_t)i * ld + i]);
-    for (int idx = threadIdx.x; idx < n * n; idx += MRA_WG) {
-        const int i = idx / n, c = idx % n;
-        LF[idx] = c <= i ? Sg[(uint64_t)i * ld + c] : 0.0;
-    }
-    __syncthreads();
-    wg_trinv_blk<16>(Sg, ld, nbk, dbuf);
-    // u = L_F^{-1} y
-    for (int i = threadIdx.x; i < n; i += MRA_WG) {
-        double acc = 0.0;
-        for (int k = 0; k <= i; k++) acc += Sg[(uint64_t)i * ld + k] * yv[ob + k];
-        u[ob + i] = acc;
-        part += acc * acc;
-    }
-    const double dF = wg_sum(part, red);
-    // H = L_F^{-1} B (observation-major rows), 64 columns at a time
-    const TG* BO = Bf + ob * (uint64_t)LRP;
-    for (int c0 = 0; c0 < LRP; c0 += 64) {
-        const int cv = min(64, LRP - c0);
-        wg_gemm_tn<true, false>(H + ob * (uint64_t)LRP + c0, LRP, ld, 64, n, 1.0, Sg, ld, n, BO + c0, LRP, cv);
-    }
-    if (threadIdx.x == 0) {
-        dterm[T.nC + F] = dF;
-        if (bad) atomicOr(bad_flag, 1);
+        __syncthreads();
+        wg_trinv_blk<16>(Sg, ld, nbk, dbuf);
+        // u = L_F^{-1} y
+        for (int i = threadIdx.x; i < n; i += MRA_WG) {
+            double acc = 0.0;
+            for (int q = 0; q <= i; q++) acc += Sg[(uint64_t)i * ld + q] * yv[ob + q];
+            u[ob + i] = acc;
+            part += acc * acc;
+        }
+        const double dF = wg_sum(part, red);
+        // H = L_F^{-1} B (observation-major rows), 64 columns at a time
+        const TG* BO = Bf + ob * (uint64_t)LRP;
+        for (int c0 = 0; c0 < LRP; c0 += 64) {
+            const int cv = min(64, LRP - c0);
+            wg_gemm_tn<true, false>(H + ob * (uint64_t)LRP + c0, LRP, ld, 64, n, 1.0, Sg, ld, n, BO + c0, LRP, cv);
+        }
+        if (threadIdx.x == 0) {
+            dterm[T.nC + F] = dF;
+            if (bad) atomicOr(bad_flag, 1);
+        }
+        __syncthreads();
     }
 }
 
@@ -682,92 +699,95 @@ __global__ __launch_bounds__(256) void k_finest_local_w16(Tree T, CovCtx cc, Pts
     __shared__ double Ds[4][16 * 16];
     __shared__ int badw[4];
     const int w = wave_id();
-    const uint32_t it = blockIdx.x * 4u + (uint32_t)w;
-    if (it >= nitems) return;
     const int lane = threadIdx.x & 63, r16 = lane & 15, k4 = lane >> 4;
-    const uint32_t F = items[it];
-    const uint64_t ob = T.obs_off[F];
-    const int n = (int)(T.obs_off[F + 1] - ob);
-    const int LRP = T.LRP;
-    const uint64_t so = T.f_seg_os[F];
-    const int sn = (int)T.f_seg_n[F];
-    const TG* GF = Gf + (uint64_t)LRP * so + (ob - so);  // G_F(k, i) at GF[k * sn + i]
-    double* P = Ps[w];
-    double* Dinv = Ds[w];
-    if (lane == 0) badw[w] = 0;
-    // -G^T G in four chains
-    const bool iv = r16 < n;
-    const int ic = iv ? r16 : 0;
-    mra_d4 c4[4];
+    // one region per wave: item blockIdx.x * 4 + w, or (persistent launch)
+    // every 4 * gridDim.x-th item from there; waves share no LDS
+    for (uint32_t it = blockIdx.x * 4u + (uint32_t)w; it < nitems; it += 4u * gridDim.x) {
+        const uint32_t F = items[it];
+        const uint64_t ob = T.obs_off[F];
+        const int n = (int)(T.obs_off[F + 1] - ob);
+        const int LRP = T.LRP;
+        const uint64_t so = T.f_seg_os[F];
+        const int sn = (int)T.f_seg_n[F];
+        const TG* GF = Gf + (uint64_t)LRP * so + (ob - so);  // G_F(k, i) at GF[k * sn + i]
+        double* P = Ps[w];
+        double* Dinv = Ds[w];
+        if (lane == 0) badw[w] = 0;
+        // -G^T G in four chains
+        const bool iv = r16 < n;
+        const int ic = iv ? r16 : 0;
+        mra_d4 c4[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) c4[q] = mra_d4{0, 0, 0, 0};
+        for (int q = 0; q < 4; q++) c4[q] = mra_d4{0, 0, 0, 0};
 #pragma unroll 4
-    for (int k0 = 0; k0 < LRP; k0 += 16) {
+        for (int k0 = 0; k0 < LRP; k0 += 16) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const double g = iv ? (double)GF[(uint64_t)(k0 + 4 * q + k4) * sn + ic] : 0.0;
-            c4[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(g, g, c4[q], 0, 0, 0);
+            for (int q = 0; q < 4; q++) {
+                const double g = iv ? (double)GF[(uint64_t)(k0 + 4 * q + k4) * sn + ic] : 0.0;
+                c4[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(g, g, c4[q], 0, 0, 0);
+            }
         }
-    }
-    // Sigma_F = V(S) - G^T G, identity padding (element v: row k4 + 4v, column r16)
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-        const int i = k4 + 4 * v, c = r16;
-        const double gg = ((c4[0][v] + c4[1][v]) + c4[2][v]) + c4[3][v];
-        double val;
-        if (i < n && c < n) {
-            double cv;
-            if (i == c)
-                cv = O.v[ob + i];
-            else if constexpr (sizeof(TG) == 4)
-                cv = (double)pcovf(cc, O, ob + i, O, ob + c);
-            else
-                cv = pcov(cc, O, ob + i, O, ob + c);
-            val = cv - gg;
-        } else {
-            val = (i == c) ? 1.0 : 0.0;
-        }
-        P[i * LDP + c] = val;
-    }
-    wave_chol16(P, LDP, Dinv, 16, &badw[w]);
-    // log det part, L_F kept (row-major n x n, zeros above)
-    double* LF = Sig + T.sig_off[F];
-    double part = lane < n ? 2.0 * log(P[lane * LDP + lane]) : 0.0;
-    for (int idx = lane; idx < n * n; idx += 64) {
-        const int i = idx / n, c = idx % n;
-        LF[idx] = c <= i ? P[i * LDP + c] : 0.0;
-    }
-    // u = L_F^{-1} y
-    if (lane < n) {
-        double acc = 0.0;
-        for (int k = 0; k <= lane; k++) acc += Dinv[lane * 16 + k] * yv[ob + k];
-        u[ob + lane] = acc;
-        part += acc * acc;
-    }
-    const double dF = wave_sum(part);
-    if (lane == 0) dterm[T.nC + F] = dF;
-    // H = L_F^{-1} B, observation-major rows of L RP
-    const TG* BO = Bf + ob * (uint64_t)LRP;
-    double* HF = H + ob * (uint64_t)LRP;
-    double al[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) al[q] = Dinv[r16 * 16 + 4 * q + k4];  // A(i = r16, k = 4q + k4)
-#pragma unroll 4
-    for (int m0 = 0; m0 < LRP; m0 += 16) {
-        mra_d4 hacc = {0, 0, 0, 0};
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int k = 4 * q + k4;
-            const double b = k < n ? (double)BO[(uint64_t)k * LRP + m0 + r16] : 0.0;
-            hacc = __builtin_amdgcn_mfma_f64_16x16x4f64(al[q], b, hacc, 0, 0, 0);
-        }
+        // Sigma_F = V(S) - G^T G, identity padding (element v: row k4 + 4v, column r16)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            const int i = k4 + 4 * v;
-            if (i < n) HF[(uint64_t)i * LRP + m0 + r16] = hacc[v];
+            const int i = k4 + 4 * v, c = r16;
+            const double gg = ((c4[0][v] + c4[1][v]) + c4[2][v]) + c4[3][v];
+            double val;
+            if (i < n && c < n) {
+                double cv;
+                if (i == c)
+                    cv = O.v[ob + i];
+                else if constexpr (sizeof(TG) == 4)
+                    cv = (double)pcovf(cc, O, ob + i, O, ob + c);
+                else
+                    cv = pcov(cc, O, ob + i, O, ob + c);
+                val = cv - gg;
+            } else {
+                val = (i == c) ? 1.0 : 0.0;
+            }
+            P[i * LDP + c] = val;
         }
+        wave_chol16(P, LDP, Dinv, 16, &badw[w]);
+        // log det part, L_F kept (row-major n x n, zeros above)
+        double* LF = Sig + T.sig_off[F];
+        double part = lane < n ? 2.0 * log(P[lane * LDP + lane]) : 0.0;
+        for (int idx = lane; idx < n * n; idx += 64) {
+            const int i = idx / n, c = idx % n;
+            LF[idx] = c <= i ? P[i * LDP + c] : 0.0;
+        }
+        // u = L_F^{-1} y
+        if (lane < n) {
+            double acc = 0.0;
+            for (int k = 0; k <= lane; k++) acc += Dinv[lane * 16 + k] * yv[ob + k];
+            u[ob + lane] = acc;
+            part += acc * acc;
+        }
+        const double dF = wave_sum(part);
+        if (lane == 0) dterm[T.nC + F] = dF;
+        // H = L_F^{-1} B, observation-major rows of L RP
+        const TG* BO = Bf + ob * (uint64_t)LRP;
+        double* HF = H + ob * (uint64_t)LRP;
+        double al[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) al[q] = Dinv[r16 * 16 + 4 * q + k4];  // A(i = r16, k = 4q + k4)
+#pragma unroll 4
+        for (int m0 = 0; m0 < LRP; m0 += 16) {
+            mra_d4 hacc = {0, 0, 0, 0};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int k = 4 * q + k4;
+                const double b = k < n ? (double)BO[(uint64_t)k * LRP + m0 + r16] : 0.0;
+                hacc = __builtin_amdgcn_mfma_f64_16x16x4f64(al[q], b, hacc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int i = k4 + 4 * v;
+                if (i < n) HF[(uint64_t)i * LRP + m0 + r16] = hacc[v];
+            }
+        }
+        if (lane == 0 && badw[w]) atomicOr(bad_flag, 1);
+        wave_lds_fence();
     }
-    if (lane == 0 && badw[w]) atomicOr(bad_flag, 1);
 }
 
 // -------------------------------------------------------------- posterior

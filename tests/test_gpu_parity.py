@@ -566,3 +566,61 @@ def test_xcd_item_order():
                 np.testing.assert_array_equal(v[2], ref[2])
     Do, _, _ = oracle.evaluate(tree, cov, prm, predict=False)
     assert rel(out[(capi.PRECISION_FP64, 1, 0)][0], Do) < TOL_D
+
+
+@pytest.mark.parametrize("case", ["buckets", "large"])
+def test_persistent_finest_launches(case):
+    """Finest-level kernels on a resident grid walking the length-bucketed
+    work lists (MRA_OPT_PERSISTENT = 1) or one work group per item (0): the
+    same D and predictions to the last bit, eager and graph, FP64 and mixed
+    (which also runs the FP64 large-region path), and the oracle's D.  The
+    trees cover every bucket: <= 16 (one wave each), 17-32, 33-64, 65-128,
+    129-256 and above 256 observations."""
+    rng = np.random.default_rng(91 if case == "buckets" else 56)
+    if case == "buckets":
+        x, y = synth.clustered_points(12000, rng, n_clusters=4, sigma=0.02)
+        ux, uy = synth.uniform_points(2000, rng)
+        x, y = np.concatenate([x, ux]), np.concatenate([y, uy])
+        px, py = synth.uniform_points(300, rng)
+        M, r = 6, 64
+    else:
+        x, y = synth.clustered_points(3600, rng, n_clusters=2, sigma=0.08)
+        px, py = synth.uniform_points(60, rng)
+        M, r = 3, 32
+    tree = capi.partition_program(x, y, rng.standard_normal(len(x)), px, py, M=M, J=4, r=r,
+                                  bounds=synth.UNIT_BOUNDS)
+    counts = np.diff(tree["obs_offsets"].astype(np.int64))
+    sizes = [int(((counts > lo) & (counts <= hi)).sum())
+             for lo, hi in ((0, 16), (16, 32), (32, 64), (64, 128), (128, 256), (256, 1 << 30))]
+    print(case, "bucket sizes", sizes, flush=True)
+    assert all(sizes) if case == "buckets" else sizes[3] and sizes[4] and sizes[5]
+    cov, prm = synth.linear_trend_cov(log_range=(np.log(0.2), 0.8, 0.0)), dict(synth.UNIT_PARAMS)
+    h = capi.Handle(tree)
+    out = {}
+    try:
+        h.set_covariance(cov)
+        h.set_params(**prm)
+        for prec in (capi.PRECISION_FP64, capi.PRECISION_MIXED):
+            h.set_precision(prec)
+            for pers in (0, 1):
+                h.set_option(capi.OPT_PERSISTENT, pers)
+                assert h.get_option(capi.OPT_PERSISTENT) == pers
+                for graph in (0, 1):
+                    h.use_graph(graph)
+                    out[(prec, pers, graph)] = h.predict()
+                    out[(prec, pers, graph, "l")] = h.likelihood()
+    finally:
+        h.close()
+    for prec in (capi.PRECISION_FP64, capi.PRECISION_MIXED):
+        ref = out[(prec, 0, 0)]
+        for k, v in out.items():
+            if k[0] != prec:
+                continue
+            if len(k) == 4:
+                assert v == out[(prec, 0, 0, "l")]
+                continue
+            assert v[0] == ref[0]
+            np.testing.assert_array_equal(v[1], ref[1])
+            np.testing.assert_array_equal(v[2], ref[2])
+    Do, _, _ = oracle.evaluate(tree, cov, prm, predict=False)
+    assert rel(out[(capi.PRECISION_FP64, 1, 0)][0], Do) < TOL_D
