@@ -1134,24 +1134,28 @@ static uint32_t resident_groups(const void* k) {
 // Grid of a launch over `units` work units taken `per_wg` per work group:
 // one work group per per_wg units, or (persistent launches) at most the
 // resident work groups, each walking the units by grid stride.
-template <typename K>
-static uint32_t unit_grid(const mra_handle* h, K kernel, uint32_t units, uint32_t per_wg = 1) {
+// `one` is the one-unit-per-group instance, `walk` its persistent instance.
+template <typename... P, typename... A>
+static void launch_units(const mra_handle* h, void (*one)(P...), void (*walk)(P...), uint32_t units,
+                         uint32_t per_wg, hipStream_t s, A... args) {
     const uint32_t full = (units + per_wg - 1) / per_wg;
-    if (!h->persistent) return full;
-    return std::max(1u, std::min(full, resident_groups((const void*)kernel)));
+    if (!h->persistent)
+        one<<<full, 256, 0, s>>>(args...);
+    else
+        walk<<<std::max(1u, std::min(full, resident_groups((const void*)walk))), 256, 0, s>>>(args...);
 }
 
 template <int RP, typename TG>
 static void prefill_grids_t() {
-    (void)resident_groups((const void*)&k_finest_cross<RP, 64, TG>);
-    (void)resident_groups((const void*)&k_finest_cross<RP, 32, TG>);
-    (void)resident_groups((const void*)&k_finest_cross<RP, 16, TG>);
-    (void)resident_groups((const void*)&k_finest_local<RP, 0, TG>);
-    (void)resident_groups((const void*)&k_finest_local<RP, 16, TG>);
-    (void)resident_groups((const void*)&k_finest_local<RP, 32, TG>);
-    (void)resident_groups((const void*)&k_finest_local<RP, 64, TG>);
-    (void)resident_groups((const void*)&k_finest_local_big<TG>);
-    (void)resident_groups((const void*)&k_finest_local_w16<TG>);
+    (void)resident_groups((const void*)&k_finest_cross<RP, 64, TG, true>);
+    (void)resident_groups((const void*)&k_finest_cross<RP, 32, TG, true>);
+    (void)resident_groups((const void*)&k_finest_cross<RP, 16, TG, true>);
+    (void)resident_groups((const void*)&k_finest_local<RP, 0, TG, true>);
+    (void)resident_groups((const void*)&k_finest_local<RP, 16, TG, true>);
+    (void)resident_groups((const void*)&k_finest_local<RP, 32, TG, true>);
+    (void)resident_groups((const void*)&k_finest_local<RP, 64, TG, true>);
+    (void)resident_groups((const void*)&k_finest_local_big<TG, true>);
+    (void)resident_groups((const void*)&k_finest_local_w16<TG, true>);
 }
 
 static void prefill_grids(const mra_handle* h) {
@@ -1174,36 +1178,45 @@ static void launch_finest_local(mra_handle* h, const Tree& T, const CovCtx& cc, 
     // precision the regions above mixFp64Min observations (buckets 3 and 5)
     // use their FP64 G and B
     if (h->nItemsFB[5] && sizeof(TG) == 4 && h->nItemsXBig && h->dGf64)
-        k_finest_local<RP, 0, double><<<unit_grid(h, &k_finest_local<RP, 0, double>, h->nItemsFB[5]), 256, 0, s1>>>(
+        launch_units(h, &k_finest_local<RP, 0, double>, &k_finest_local<RP, 0, double, true>, h->nItemsFB[5], 1, s1,
+                     
             T, cc, O, h->dY, h->dItemsFB[5], h->nItemsFB[5], h->dGf64, h->dBf64, h->dSig, h->dH, h->dU, h->dDterm,
             h->dBad);
     else if (h->nItemsFB[5])
-        k_finest_local<RP, 0, TG><<<unit_grid(h, &k_finest_local<RP, 0, TG>, h->nItemsFB[5]), 256, 0, s1>>>(
+        launch_units(h, &k_finest_local<RP, 0, TG>, &k_finest_local<RP, 0, TG, true>, h->nItemsFB[5], 1, s1,
+                     
             T, cc, O, h->dY, h->dItemsFB[5], h->nItemsFB[5], Gf, Bf, h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
     if (h->nItemsFB[3] && sizeof(TG) == 4 && h->nItemsXBig && h->dGf64)
-        k_finest_local_big<double><<<unit_grid(h, &k_finest_local_big<double>, h->nItemsFB[3]), 256, 0, s1>>>(
+        launch_units(h, &k_finest_local_big<double>, &k_finest_local_big<double, true>, h->nItemsFB[3], 1, s1,
+                     
             T, cc, O, h->dY, h->dItemsFB[3], h->nItemsFB[3], h->dWsOffBig3, h->dWsBig, h->dGf64, h->dBf64, h->dSig,
             h->dH, h->dU, h->dDterm, h->dBad);
     else if (h->nItemsFB[3])
-        k_finest_local_big<TG><<<unit_grid(h, &k_finest_local_big<TG>, h->nItemsFB[3]), 256, 0, s1>>>(
+        launch_units(h, &k_finest_local_big<TG>, &k_finest_local_big<TG, true>, h->nItemsFB[3], 1, s1,
+                     
             T, cc, O, h->dY, h->dItemsFB[3], h->nItemsFB[3], h->dWsOffBig3, h->dWsBig, Gf, Bf, h->dSig, h->dH, h->dU,
             h->dDterm, h->dBad);
     if (h->nItemsFB[4])
-        k_finest_local_big<TG><<<unit_grid(h, &k_finest_local_big<TG>, h->nItemsFB[4]), 256, 0, s1>>>(
+        launch_units(h, &k_finest_local_big<TG>, &k_finest_local_big<TG, true>, h->nItemsFB[4], 1, s1,
+                     
             T, cc, O, h->dY, h->dItemsFB[4], h->nItemsFB[4], h->dWsOffBig, h->dWsBig, Gf, Bf, h->dSig, h->dH, h->dU,
             h->dDterm, h->dBad);
     if (h->nItemsFB[2])
-        k_finest_local<RP, 64, TG><<<unit_grid(h, &k_finest_local<RP, 64, TG>, h->nItemsFB[2]), 256, 0, s2>>>(
+        launch_units(h, &k_finest_local<RP, 64, TG>, &k_finest_local<RP, 64, TG, true>, h->nItemsFB[2], 1, s2,
+                     
             T, cc, O, h->dY, h->dItemsFB[2], h->nItemsFB[2], Gf, Bf, h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
     if (h->nItemsFB[1])
-        k_finest_local<RP, 32, TG><<<unit_grid(h, &k_finest_local<RP, 32, TG>, h->nItemsFB[1]), 256, 0, s2>>>(
+        launch_units(h, &k_finest_local<RP, 32, TG>, &k_finest_local<RP, 32, TG, true>, h->nItemsFB[1], 1, s2,
+                     
             T, cc, O, h->dY, h->dItemsFB[1], h->nItemsFB[1], Gf, Bf, h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
     if (h->nItemsFB[0] && h->waveLocal16)
         // one wave per region, four regions per work group
-        k_finest_local_w16<TG><<<unit_grid(h, &k_finest_local_w16<TG>, h->nItemsFB[0], 4), 256, 0, st>>>(
+        launch_units(h, &k_finest_local_w16<TG>, &k_finest_local_w16<TG, true>, h->nItemsFB[0], 4, st,
+                     
             T, cc, O, h->dY, h->dItemsFB[0], h->nItemsFB[0], Gf, Bf, h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
     else if (h->nItemsFB[0])
-        k_finest_local<RP, 16, TG><<<unit_grid(h, &k_finest_local<RP, 16, TG>, h->nItemsFB[0]), 256, 0, st>>>(
+        launch_units(h, &k_finest_local<RP, 16, TG>, &k_finest_local<RP, 16, TG, true>, h->nItemsFB[0], 1, st,
+                     
             T, cc, O, h->dY, h->dItemsFB[0], h->nItemsFB[0], Gf, Bf, h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
 }
 
@@ -1212,16 +1225,20 @@ static void launch_finest_cross(mra_handle* h, const Tree& T, const CovCtx& cc, 
     const Fork fk(h, 1);
     const hipStream_t st = fk.stream(0);
     if (h->nItemsX)
-        k_finest_cross<RP, 64, TG><<<unit_grid(h, &k_finest_cross<RP, 64, TG>, h->nItemsX), 256, 0, fk.stream(1)>>>(
+        launch_units(h, &k_finest_cross<RP, 64, TG>, &k_finest_cross<RP, 64, TG, true>, h->nItemsX, 1, fk.stream(1),
+                     
             T, cc, K, O, h->dItemsX, h->nItemsX, h->dGoff, h->dGc, h->dLiT, (TG*)h->dGf, (TG*)h->dBf, h->dGc32);
     if (h->nItemsX32)
-        k_finest_cross<RP, 32, TG><<<unit_grid(h, &k_finest_cross<RP, 32, TG>, h->nItemsX32), 256, 0, st>>>(
+        launch_units(h, &k_finest_cross<RP, 32, TG>, &k_finest_cross<RP, 32, TG, true>, h->nItemsX32, 1, st,
+                     
             T, cc, K, O, h->dItemsX32, h->nItemsX32, h->dGoff, h->dGc, h->dLiT, (TG*)h->dGf, (TG*)h->dBf, h->dGc32);
     if (h->nItemsX16)
-        k_finest_cross<RP, 16, TG><<<unit_grid(h, &k_finest_cross<RP, 16, TG>, h->nItemsX16), 256, 0, fk.stream(2)>>>(
+        launch_units(h, &k_finest_cross<RP, 16, TG>, &k_finest_cross<RP, 16, TG, true>, h->nItemsX16, 1, fk.stream(2),
+                     
             T, cc, K, O, h->dItemsX16, h->nItemsX16, h->dGoff, h->dGc, h->dLiT, (TG*)h->dGf, (TG*)h->dBf, h->dGc32);
     if (sizeof(TG) == 4 && h->nItemsXBig && h->dGf64)
-        k_finest_cross<RP, 64, double><<<unit_grid(h, &k_finest_cross<RP, 64, double>, h->nItemsXBig), 256, 0, st>>>(
+        launch_units(h, &k_finest_cross<RP, 64, double>, &k_finest_cross<RP, 64, double, true>, h->nItemsXBig, 1, st,
+                     
             T, cc, K, O, h->dItemsXBig, h->nItemsXBig, h->dGoff, h->dGc, h->dLiT, h->dGf64, h->dBf64, nullptr);
 }
 

@@ -386,7 +386,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // (…finest-level-in-prior.cpp:116-163)
 // T = double: FP64 throughout.  T = float (mixed precision): the covariance
 // tile, both products and the stored G, B are FP32 with FP32 accumulation.
-template <int RP, int NT, typename T>
+template <int RP, int NT, typename T, bool PERSIST = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3))) void k_finest_cross(Tree T_, CovCtx cc, Pts K, Pts O, const uint32_t* items,
                                                       uint32_t nitems, const uint64_t* goff, const double* Gc,
                                                       const double* LiT, T* Gf, T* Bf, const float* Gc32) {
@@ -394,10 +394,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     constexpr int LD = NT + 1;
     __shared__ T Cm[RP * LD];
     __shared__ PtV kst[2][RP];  // ancestor knots, double-buffered
-    // one item per work group, or (persistent launch) items k = blockIdx.x +
-    // i * gridDim.x; the j loop ends with a barrier, so LDS is free for the
-    // next item
-    for (uint32_t k = blockIdx.x; k < nitems; k += gridDim.x) {
+    // one item per work group; PERSIST: a resident grid walks the items
+    // k = blockIdx.x + i * gridDim.x (the j loop ends with a barrier, so
+    // LDS is free for the next item).  The walk is a separate instance: as
+    // a loop around the one-item body it raised the VGPR count of the
+    // one-item launches.
+    auto item = [&](const uint32_t k) {
         const uint32_t bid = xcd_block(k, nitems, T0.xcd);
         const uint32_t sg = items[2 * bid], c0 = items[2 * bid + 1];
         const uint64_t ob = T0.s_os[sg];
@@ -455,6 +457,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
                 kst[(j + 1) & 1][threadIdx.x] = PtV{nx, ny, nz, cc.stationary ? 0.0 : ns_ * (nb * sqrt(nb)), nb};
             __syncthreads();
         }
+    };
+    if constexpr (PERSIST) {
+        for (uint32_t k = blockIdx.x; k < nitems; k += gridDim.x) item(k);
+    } else if (blockIdx.x < nitems) {
+        item(blockIdx.x);
     }
 }
 
@@ -480,7 +487,7 @@ struct FinestLds<0> {
     double D[1];
 };
 
-template <int RP, int NB, typename TG>
+template <int RP, int NB, typename TG, bool PERSIST = false>
 __global__ __launch_bounds__(256) void k_finest_local(Tree T, CovCtx cc, Pts O, const double* yv,
                                                       const uint32_t* items, uint32_t nitems, TG* Gf, TG* Bf,
                                                       double* Sig, double* H, double* u, double* dterm,
@@ -488,9 +495,9 @@ __global__ __launch_bounds__(256) void k_finest_local(Tree T, CovCtx cc, Pts O, 
     __shared__ FinestLds<NB> sh;
     __shared__ double red[MRA_WG];
     __shared__ int bad;
-    // one region per work group, or (persistent launch) regions k =
-    // blockIdx.x + i * gridDim.x with a barrier before the next one
-    for (uint32_t k = blockIdx.x; k < nitems; k += gridDim.x) {
+    // one region per work group; PERSIST: a resident grid walks the regions
+    // k = blockIdx.x + i * gridDim.x, with a barrier before the next one
+    auto item = [&](const uint32_t k) {
         const uint32_t F = items[k];
         const uint64_t ob = T.obs_off[F];
         const int n = (int)(T.obs_off[F + 1] - ob);
@@ -586,6 +593,11 @@ __global__ __launch_bounds__(256) void k_finest_local(Tree T, CovCtx cc, Pts O, 
         }
         if (threadIdx.x == 0 && bad) atomicOr(bad_flag, 1);
         __syncthreads();
+    };
+    if constexpr (PERSIST) {
+        for (uint32_t k = blockIdx.x; k < nitems; k += gridDim.x) item(k);
+    } else if (blockIdx.x < nitems) {
+        item(blockIdx.x);
     }
 }
 
@@ -596,7 +608,7 @@ __global__ __launch_bounds__(256) void k_finest_local(Tree T, CovCtx cc, Pts O, 
 // blocks (the inverse diagonal blocks kept in the unused upper triangle),
 // u = L_F^{-1} y, d_F and H = L_F^{-1} B (MFMA).  L_F is copied to its
 // n_F x n_F place for prediction and sampling.
-template <typename TG>
+template <typename TG, bool PERSIST = false>
 __global__ __launch_bounds__(256) void k_finest_local_big(Tree T, CovCtx cc, Pts O, const double* yv,
                                                           const uint32_t* items, uint32_t nitems,
                                                           const uint64_t* ws_off, double* ws, const TG* Gf,
@@ -605,9 +617,9 @@ __global__ __launch_bounds__(256) void k_finest_local_big(Tree T, CovCtx cc, Pts
     __shared__ double red[MRA_WG];
     __shared__ double dbuf[1];  // unused: with 3 or more blocks the inverse diagonal blocks live in A
     __shared__ int bad;
-    // one region per work group, or (persistent launch) regions k =
-    // blockIdx.x + i * gridDim.x with a barrier before the next one
-    for (uint32_t k = blockIdx.x; k < nitems; k += gridDim.x) {
+    // one region per work group; PERSIST: a resident grid walks the regions
+    // k = blockIdx.x + i * gridDim.x, with a barrier before the next one
+    auto item = [&](const uint32_t k) {
         const uint32_t F = items[k];
         const uint64_t ob = T.obs_off[F];
         const int n = (int)(T.obs_off[F + 1] - ob);
@@ -673,6 +685,11 @@ __global__ __launch_bounds__(256) void k_finest_local_big(Tree T, CovCtx cc, Pts
             if (bad) atomicOr(bad_flag, 1);
         }
         __syncthreads();
+    };
+    if constexpr (PERSIST) {
+        for (uint32_t k = blockIdx.x; k < nitems; k += gridDim.x) item(k);
+    } else if (blockIdx.x < nitems) {
+        item(blockIdx.x);
     }
 }
 
@@ -689,7 +706,7 @@ __device__ __forceinline__ double wave_sum(double x) {
     return x;
 }
 
-template <typename TG>
+template <typename TG, bool PERSIST = false>
 __global__ __launch_bounds__(256) void k_finest_local_w16(Tree T, CovCtx cc, Pts O, const double* yv,
                                                           const uint32_t* items, uint32_t nitems, const TG* Gf,
                                                           const TG* Bf, double* Sig, double* H, double* u,
@@ -700,9 +717,9 @@ __global__ __launch_bounds__(256) void k_finest_local_w16(Tree T, CovCtx cc, Pts
     __shared__ int badw[4];
     const int w = wave_id();
     const int lane = threadIdx.x & 63, r16 = lane & 15, k4 = lane >> 4;
-    // one region per wave: item blockIdx.x * 4 + w, or (persistent launch)
-    // every 4 * gridDim.x-th item from there; waves share no LDS
-    for (uint32_t it = blockIdx.x * 4u + (uint32_t)w; it < nitems; it += 4u * gridDim.x) {
+    // one region per wave: item blockIdx.x * 4 + w; PERSIST: every
+    // 4 * gridDim.x-th item from there (waves share no LDS)
+    auto item = [&](const uint32_t it) {
         const uint32_t F = items[it];
         const uint64_t ob = T.obs_off[F];
         const int n = (int)(T.obs_off[F + 1] - ob);
@@ -787,6 +804,11 @@ __global__ __launch_bounds__(256) void k_finest_local_w16(Tree T, CovCtx cc, Pts
         }
         if (lane == 0 && badw[w]) atomicOr(bad_flag, 1);
         wave_lds_fence();
+    };
+    if constexpr (PERSIST) {
+        for (uint32_t it = blockIdx.x * 4u + (uint32_t)w; it < nitems; it += 4u * gridDim.x) item(it);
+    } else if (blockIdx.x * 4u + (uint32_t)w < nitems) {
+        item(blockIdx.x * 4u + (uint32_t)w);
     }
 }
 
