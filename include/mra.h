@@ -202,7 +202,12 @@ int mra_use_graph(mra_handle* h, int enable);
  * the device holds at once for that kernel) whose work groups walk the
  * length-bucketed work lists, longest first, by grid stride; 0 (default) one
  * work group (one wave for regions of at most 16 observations) per item.
- * Same values bit for bit. */
+ * Same values bit for bit.
+ * MRA_OPT_PRIOR_WAVE: 0 (default, and the only accepted value) computes the
+ * coarse prior one work group per region.  1 would select the one-wave-per-
+ * region form for 64-knot regions (k_prior_wave + k_prior_chol); it is
+ * refused with MRA_E_UNSUPPORTED: a GPU run with it stopped making progress
+ * and the cause is not established. */
 enum {
     MRA_OPT_FACTORED_POSTERIOR = 1,
     MRA_OPT_GRAM_KERNEL = 2,
@@ -212,7 +217,8 @@ enum {
     MRA_OPT_SUM_MESSAGES = 6,
     MRA_OPT_XCD_MAP = 7,
     MRA_OPT_MIXED_FP64_ABOVE = 8,
-    MRA_OPT_PERSISTENT = 9
+    MRA_OPT_PERSISTENT = 9,
+    MRA_OPT_PRIOR_WAVE = 10
 };
 int mra_set_option(mra_handle* h, int option, int value);
 int mra_get_option(mra_handle* h, int option, int* value);
@@ -227,7 +233,7 @@ enum {
     MRA_T_PRED_POST = 4,
     MRA_T_REDUCE = 5,
     MRA_T_GRAPH = 6,          /* one replay of the captured evaluation */
-    MRA_T_PRIOR0 = 8,         /* + level l: k_prior_coarse of level l (l < 24) */
+    MRA_T_PRIOR0 = 8,         /* + level l: coarse prior of level l (l < 24) */
     MRA_T_POST0 = 32,         /* + level l: k_posterior of level l */
     MRA_T_COUNT = 56
 };

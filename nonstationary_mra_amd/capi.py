@@ -53,6 +53,7 @@ OPT_SUM_MESSAGES = 6
 OPT_XCD_MAP = 7
 OPT_MIXED_FP64_ABOVE = 8
 OPT_PERSISTENT = 9
+OPT_PRIOR_WAVE = 10
 
 
 class TreeDesc(C.Structure):

@@ -209,6 +209,35 @@ __device__ __forceinline__ void cov_tile_pre(T* C, int LD, const CovCtx& cc, con
     }
 }
 
+// A 64 x 64 covariance tile in the accumulator layout of the one-wave
+// products (wave_mm64_acc): element v of acc[ti * 4 + tj] of lane t is
+// C(rows[i], cols[c]) with i = 16 ti + t/16 + 4 v and c = 16 tj + t%16, zero
+// where i or c >= n.  vdiag (rows and columns are one point set): the
+// diagonal takes vdiag[i], identity on padding.  rows, cols: LDS.  Per
+// column tile, each lane evaluates its 16 entries in a loop (one copy of the
+// covariance code per column tile) into buf (16 * 64 doubles of LDS, this
+// wave's own; entry q of lane t at buf[64 q + t]) and reads them back into
+// the accumulators, whose register indices must be compile-time constants.
+__device__ __forceinline__ void wave_cov64(mra_d4 (&acc)[16], const CovCtx& cc, const PtV* rows, const PtV* cols,
+                                           int n, const double* vdiag, double* buf) {
+    const int lane = threadIdx.x & 63;
+    const int r16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+    for (int tj = 0; tj < 4; tj++) {
+        const int c = 16 * tj + r16;
+        const PtV cq = cols[c];
+#pragma unroll 1
+        for (int q = 0; q < 16; q++) {
+            const int i = k4 + 4 * q;  // row 16 ti + k4 + 4 v with q = 4 ti + v
+            double x = (i < n && c < n) ? pcov_pt(cc, rows[i], cq) : 0.0;
+            if (vdiag && i == c) x = i < n ? vdiag[i] : 1.0;
+            buf[64 * q + lane] = x;
+        }
+#pragma unroll
+        for (int q = 0; q < 16; q++) acc[(q >> 2) * 4 + tj][q & 3] = buf[64 * q + lane];
+    }
+}
+
 __device__ __forceinline__ float pcovf(const CovCtx& c, const Pts& P, uint64_t i, const Pts& Q, uint64_t j) {
     float x1 = (float)P.x[i], y1 = (float)P.y[i], z1 = (float)P.z[i];
     float x2 = (float)Q.x[j], y2 = (float)Q.y[j], z2 = (float)Q.z[j];
@@ -469,7 +498,9 @@ __device__ __forceinline__ void wg_gemm_tn_impl(TO* C, int ldc, int M, int N, in
 // in registers, so every operand value loaded feeds four matrix
 // instructions.  Tile (ti, tj), element v of lane t holds row
 // 16 ti + t/16 + 4 v, column 16 tj + t%16.  FP64.  Waves work on different
-// outputs independently (no barrier).
+// outputs independently (no barrier).  LOWER: only the tiles on and below the
+// diagonal (tj <= ti) are accumulated.
+template <bool LOWER = false>
 __device__ __forceinline__ void wave_mm64_acc(mra_d4 (&acc)[16], int K, double al, const double* A, int64_t lda,
                                               const double* B, int64_t ldb) {
     const int lane = threadIdx.x & 63;
@@ -497,8 +528,9 @@ __device__ __forceinline__ void wave_mm64_acc(mra_d4 (&acc)[16], int K, double a
             for (int ti = 0; ti < 4; ti++)
 #pragma unroll
                 for (int tj = 0; tj < 4; tj++)
-                    acc[ti * 4 + tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(al * av[u][ti], bv[u][tj],
-                                                                            acc[ti * 4 + tj], 0, 0, 0);
+                    if (!LOWER || tj <= ti)
+                        acc[ti * 4 + tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(al * av[u][ti], bv[u][tj],
+                                                                                acc[ti * 4 + tj], 0, 0, 0);
     };
     int k0 = 0;
     if (K >= 8) {
@@ -534,7 +566,37 @@ __device__ __forceinline__ void wave_mm64_acc(mra_d4 (&acc)[16], int K, double a
         for (int ti = 0; ti < 4; ti++)
 #pragma unroll
             for (int tj = 0; tj < 4; tj++)
-                acc[ti * 4 + tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ti], bv[tj], acc[ti * 4 + tj], 0, 0, 0);
+                if (!LOWER || tj <= ti)
+                    acc[ti * 4 + tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ti], bv[tj], acc[ti * 4 + tj], 0, 0, 0);
+    }
+}
+
+// acc = A^T acc for a lower triangular A^T: A(k, i) = L[k * 64 + i], zero for
+// k > i (an inverse factor stored transposed).  The accumulators are the B
+// operand: k step kk = 4 tb + v (rows 4 kk .. 4 kk + 3) of column tile tj is
+// element v of acc[tb * 4 + tj], exactly the operand layout of the matrix
+// instruction.  Output row tile ti reads row tiles 0..ti, so the row tiles
+// are produced from the last to the first, each overwriting its own rows
+// once no later output reads them; every A value feeds four instructions.
+__device__ __forceinline__ void wave_trmm64_lt(mra_d4 (&acc)[16], const double* L) {
+    const int lane = threadIdx.x & 63;
+    const int r16 = lane & 15, k4 = lane >> 4;
+    const double* Ll = L + k4 * 64 + r16;
+#pragma unroll
+    for (int ti = 3; ti >= 0; ti--) {
+        double a[16];
+#pragma unroll
+        for (int kk = 0; kk < 4 * ti + 4; kk++) a[kk] = Ll[kk * 256 + 16 * ti];
+        mra_d4 o[4];
+#pragma unroll
+        for (int tj = 0; tj < 4; tj++) o[tj] = mra_d4{0, 0, 0, 0};
+#pragma unroll
+        for (int kk = 0; kk < 4 * ti + 4; kk++)
+#pragma unroll
+            for (int tj = 0; tj < 4; tj++)
+                o[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk], acc[(kk >> 2) * 4 + tj][kk & 3], o[tj], 0, 0, 0);
+#pragma unroll
+        for (int tj = 0; tj < 4; tj++) acc[ti * 4 + tj] = o[tj];
     }
 }
 
@@ -611,17 +673,18 @@ __device__ __forceinline__ void wave_add64(mra_d4 (&acc)[16], const double* S, i
         }
 }
 
-// C = acc (64 x 64 row-major, leading dimension ldc).
-__device__ __forceinline__ void wave_store64(double* C, int ldc, const mra_d4 (&acc)[16]) {
+// C = acc (64 x 64 row-major, leading dimension ldc), TO = double or float.
+template <typename TO>
+__device__ __forceinline__ void wave_store64(TO* C, int ldc, const mra_d4 (&acc)[16]) {
     const int lane = threadIdx.x & 63;
     const int r16 = lane & 15, k4 = lane >> 4;
 #pragma unroll
     for (int ti = 0; ti < 4; ti++)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            double* c = C + (int64_t)(ti * 16 + k4 + 4 * v) * ldc + r16;
+            TO* c = C + (int64_t)(ti * 16 + k4 + 4 * v) * ldc + r16;
 #pragma unroll
-            for (int tj = 0; tj < 4; tj++) c[16 * tj] = acc[ti * 4 + tj][v];
+            for (int tj = 0; tj < 4; tj++) c[16 * tj] = (TO)acc[ti * 4 + tj][v];
         }
 }
 

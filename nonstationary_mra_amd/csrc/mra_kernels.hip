@@ -379,6 +379,79 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     if (threadIdx.x == 0 && bad) atomicOr(bad_flag, 1);
 }
 
+// The same recursion for RP = 64, one wave per region and four regions per
+// work group (the four siblings of one parent when J = 4, which read the same
+// ancestor blocks), with no work-group barrier.  Per ancestor level j < l:
+//   acc = C(Q_{a_j}, Q_R)                  covariance tile in registers
+//   acc -= Gcat_{a_j}^T Gcat_R[0:j]         wave_mm64_acc, K = 64 j
+//   G_{R,j} = L_{a_j}^{-1} acc              wave_trmm64_lt (acc is the B operand)
+// then K_R = V(Q_R) - Gcat_R^T Gcat_R (tiles on and below the diagonal) goes
+// to the L_R slot, where k_prior_chol factors it.
+// (src/class_approximation.cpp:108-159)
+template <bool COPY32>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_prior_wave(
+    Tree T, CovCtx cc, Pts K, int l, uint64_t start_l, uint32_t nreg, const uint64_t* goff, double* Gc,
+    const double* LiT, double* Kr, float* Gc32) {
+    constexpr int RP = 64;
+    __shared__ PtV kw[4][2][RP];  // per wave: R's knots, the current ancestor's knots
+    __shared__ double cbuf[4][16 * 64];  // per wave: covariance entries on their way to the registers
+    const int w = wave_id(), lane = threadIdx.x & 63;
+    const uint32_t bid = xcd_block(blockIdx.x, gridDim.x, T.xcd) * 4u + (uint32_t)w;
+    if (bid >= nreg) return;  // whole waves; no work-group barrier below
+    const uint64_t R = start_l + bid;
+    const int r = T.r;
+    const uint64_t og = goff[l] + (uint64_t)bid * l * RP * RP;
+    double* GR = Gc + og;
+    PtV* own = kw[w][0];
+    PtV* rows = kw[w][1];
+    own[lane] = lane < r ? load_pt(cc, K, R * RP + lane) : PtV{0, 0, 0, 0, 1};
+    mra_d4 acc[16];
+    for (int j = 0; j < l; j++) {
+        const uint64_t a = ancestor_up(R, l, j, T.J);
+        rows[lane] = lane < r ? load_pt(cc, K, a * RP + lane) : PtV{0, 0, 0, 0, 1};
+        wave_lds_fence();
+        wave_cov64(acc, cc, rows, own, r, nullptr, cbuf[w]);
+        if (j > 0)
+            wave_mm64_acc(acc, j * RP, -1.0, Gc + goff[j] + (a - lvl_start(j, T.J)) * (uint64_t)j * RP * RP, RP, GR,
+                          RP);
+        wave_trmm64_lt(acc, LiT + a * RP * RP);
+        wave_store64(GR + (uint64_t)j * RP * RP, RP, acc);
+        if (COPY32) wave_store64(Gc32 + og + (uint64_t)j * RP * RP, RP, acc);
+        // G_{R,j} is read back by this wave's later products
+        __threadfence_block();
+    }
+    wave_lds_fence();
+    wave_cov64(acc, cc, own, own, r, K.v + R * RP, cbuf[w]);
+    if (l > 0) wave_mm64_acc<true>(acc, l * RP, -1.0, GR, RP, GR, RP);
+    wave_store64(Kr + R * RP * RP, RP, acc);
+}
+
+// L_R = chol(K_R) and L_R^{-1} from the K_R that k_prior_wave left in the L_R
+// slot (row-major, lower part read), stored as k_prior_coarse stores them.
+// One work group per region.
+template <int RP>
+__global__ __launch_bounds__(256) void k_prior_chol(uint64_t start_l, double* Lt, double* LiT, int* bad_flag) {
+    constexpr int LD = RP + 1;
+    __shared__ double Cm[RP * LD];
+    __shared__ double dbuf[RP >= 48 ? 1 : RP * 16];
+    __shared__ int bad;
+    const uint64_t R = start_l + blockIdx.x;
+    if (threadIdx.x == 0) bad = 0;
+    for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) Cm[(idx / RP) * LD + idx % RP] = Lt[R * RP * RP + idx];
+    __syncthreads();
+    wg_chol_blk(Cm, LD, RP / 16, dbuf, &bad);
+    for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
+        const int k = idx / RP, i = idx % RP;  // column-major: [k*RP + i] = L[i][k]
+        Lt[R * RP * RP + idx] = (i >= k) ? Cm[i * LD + k] : 0.0;
+    }
+    wg_trinv_blk(Cm, LD, RP / 16, dbuf);
+    for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) {
+        const int k = idx / RP, i = idx % RP;
+        LiT[R * RP * RP + idx] = (i >= k) ? Cm[i * LD + k] : 0.0;
+    }
+    if (threadIdx.x == 0 && bad) atomicOr(bad_flag, 1);
+}
+
 // ---------------------------------------------------------- finest level
 // Observation columns of one segment (all observations below one parent,
 // which share every ancestor), tiles of 64 columns:
