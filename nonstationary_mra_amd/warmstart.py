@@ -12,8 +12,17 @@ library's own kernels on standardised features and targets.  At inference
 the predicted theta (clipped into the optimizer's bounds) replaces the
 default start of the optimizer.
 
+The network's start comes with a learned mean curvature: the average
+Hessian H of D at the optimum over fields of the training distribution
+(central differences).  The six parameters are strongly correlated in D
+(about -0.98 between the intercepts of log sigma^2 and log beta, the same on
+every field), so from the network's start the minimiser runs in whitened
+coordinates z, theta = theta0 + S z with S^T H S = I (S = L^-T, H = L L^T),
+where the trust region is round in the metric of D.
+
 Command line (one GPU):
     python -m nonstationary_mra_amd.warmstart train --fields 2048 --n 50000 --M 8 --r 32 --out FILE.npz
+    python -m nonstationary_mra_amd.warmstart curvature --model FILE.npz --fields 64 --n 50000 --M 8 --r 32
     python -m nonstationary_mra_amd.warmstart evaluate --model FILE.npz --fields 64 --n 50000 --M 8 --r 32
 """
 import argparse
@@ -45,6 +54,14 @@ NPT = 28
 # held-out prediction error in the standardised parameters, so the optimizer
 # does not spend its first evaluations re-exploring the whole box
 RHO_BEGIN_WARM = 0.03
+# trust radii of the whitened minimisation from the network's start (z
+# units: a step of length t changes D by about t^2/2 along any direction);
+# 1e-3 reaches at least the final D of the default start's rho_end = 1e-4
+RHO_BEGIN_Z, RHO_END_Z = 3.0, 1e-3
+# box of the whitened coordinates (theta is also clipped into the bounds)
+Z_BOX = 200.0
+# central-difference step of the curvature estimate (theta units)
+HESS_STEP = 0.02
 
 
 def trend_cov(theta):
@@ -90,13 +107,50 @@ def training_set(n_fields, n, M, r, seed=2026, log=None):
     return X, th
 
 
-class WarmStart:
-    """Trained network plus the standardisation of features and targets."""
+def trend_D(handle, theta):
+    """D at theta (clipped into the optimizer's bounds); inf where a
+    factorisation fails, as the library's own objective."""
+    th = np.clip(np.asarray(theta, dtype=np.float64), BOUND_LO, BOUND_HI)
+    handle.set_covariance(trend_cov(th))
+    handle.set_params(**synth.UNIT_PARAMS)
+    try:
+        return handle.likelihood()
+    except capi.MraError as e:
+        if e.code == capi.MRA_E_NOT_PD:
+            return float("inf")
+        raise
 
-    def __init__(self, params, fmean, fstd, tmean, tstd):
+
+def hessian(handle, theta, step=HESS_STEP):
+    """Central-difference Hessian of D at theta (1 + 2k + 2k(k-1) evaluations)."""
+    k = len(theta)
+    E = np.eye(k) * step
+    D0 = trend_D(handle, theta)
+    H = np.zeros((k, k))
+    for i in range(k):
+        H[i, i] = (trend_D(handle, theta + E[i]) - 2.0 * D0 + trend_D(handle, theta - E[i])) / step ** 2
+        for j in range(i):
+            H[i, j] = H[j, i] = (trend_D(handle, theta + E[i] + E[j]) - trend_D(handle, theta + E[i] - E[j])
+                                 - trend_D(handle, theta - E[i] + E[j])
+                                 + trend_D(handle, theta - E[i] - E[j])) / (4.0 * step ** 2)
+    return H
+
+
+def whitening(H):
+    """S with S^T H S = I for a symmetric positive definite H (S = L^-T)."""
+    L = np.linalg.cholesky(0.5 * (H + H.T))
+    return np.linalg.inv(L).T
+
+
+class WarmStart:
+    """Trained network plus the standardisation of features and targets, and
+    (optional) the mean curvature of D at the optimum."""
+
+    def __init__(self, params, fmean, fstd, tmean, tstd, hess=None):
         self.params = np.asarray(params, dtype=np.float32)
         self.fmean, self.fstd = np.asarray(fmean, np.float32), np.asarray(fstd, np.float32)
         self.tmean, self.tstd = np.asarray(tmean, np.float64), np.asarray(tstd, np.float64)
+        self.hess = None if hess is None else np.asarray(hess, np.float64)
         self.mlp = capi.Mlp(0)
         self.mlp.set_params(self.params)
 
@@ -107,12 +161,24 @@ class WarmStart:
         return np.clip(th, BOUND_LO + 1e-6, BOUND_HI - 1e-6)
 
     def save(self, path):
-        np.savez(path, params=self.params, fmean=self.fmean, fstd=self.fstd, tmean=self.tmean, tstd=self.tstd)
+        extra = {} if self.hess is None else dict(hess=self.hess)
+        np.savez(path, params=self.params, fmean=self.fmean, fstd=self.fstd, tmean=self.tmean, tstd=self.tstd,
+                 **extra)
 
     @staticmethod
     def load(path):
         z = np.load(path, allow_pickle=False)
-        return WarmStart(z["params"], z["fmean"], z["fstd"], z["tmean"], z["tstd"])
+        return WarmStart(z["params"], z["fmean"], z["fstd"], z["tmean"], z["tstd"],
+                         z["hess"] if "hess" in z.files else None)
+
+    def fit(self, handle, features, max_evals=MAX_EVALS):
+        """The optimizer from the network's start: in whitened coordinates
+        when the model carries a curvature, else from theta0 in the plain
+        ones.  Returns (theta, D, n_evals, status)."""
+        t0 = self.predict(features)[0]
+        if self.hess is None:
+            return fit(handle, t0, max_evals, RHO_BEGIN_WARM)
+        return fit_whitened(handle, t0, whitening(self.hess), max_evals)
 
 
 def train(X, theta, epochs=300, batch=256, lr=1e-3, seed=0, val_frac=0.1, log=None):
@@ -152,9 +218,46 @@ def fit(handle, theta0, max_evals=MAX_EVALS, rho_begin=RHO_BEGIN, npt=NPT):
     return handle.fit_linear_trend(theta0, BOUND_LO, BOUND_HI, rho_begin, RHO_END, max_evals, npt=npt)
 
 
+def fit_whitened(handle, theta0, S, max_evals=MAX_EVALS, rho_begin=RHO_BEGIN_Z, rho_end=RHO_END_Z, npt=NPT):
+    """The optimizer (mra_bobyqa) over z with theta = theta0 + S z, from z = 0;
+    leaves the handle at the best theta.  Returns (theta, D, n_evals, status)."""
+    theta0 = np.asarray(theta0, dtype=np.float64)
+    S = np.asarray(S, dtype=np.float64)
+    box = np.full(NTHETA, Z_BOX)
+    z, D, ne, st = capi.bobyqa(lambda z: trend_D(handle, theta0 + S @ z), np.zeros(NTHETA), -box, box, rho_begin,
+                               rho_end, max_evals, npt=npt)
+    th = np.clip(theta0 + S @ z, BOUND_LO, BOUND_HI)
+    trend_D(handle, th)
+    return th, D, ne, st
+
+
+def curvature(ws, n_fields, n, M, r, seed=31, log=None):
+    """Mean Hessian of D at the optimum over fields of the training
+    distribution (each fitted from the network's start); returns (H, the
+    per-field Hessians)."""
+    tree = make_locations(n, M, r, seed)
+    h = capi.Handle(tree, with_pred=False)
+    rng = np.random.default_rng(seed + 1)
+    th = draw_theta(rng, n_fields)
+    Hs = []
+    try:
+        for i in range(n_fields):
+            f = simulate(h, th[i], seed * 1000 + i)
+            ts, _, ne, _ = fit(h, ws.predict(f)[0], rho_begin=RHO_BEGIN_WARM)
+            Hs.append(hessian(h, ts))
+            if log:
+                ev = np.linalg.eigvalsh(Hs[-1])
+                log(json.dumps(dict(field=i, evals=ne, eig_min=float(ev[0]), eig_max=float(ev[-1]))))
+    finally:
+        h.close()
+    Hs = np.array(Hs)
+    return Hs.mean(axis=0), Hs
+
+
 def compare(ws, n_fields, n, M, r, seed=77, log=None, max_evals=MAX_EVALS, rho_warm=RHO_BEGIN_WARM):
     """Held-out fields: optimizer evaluations and final D from the network's
-    start vs the default start."""
+    start (whitened when the model carries a curvature) vs the default
+    start."""
     tree = make_locations(n, M, r, seed)
     h = capi.Handle(tree, with_pred=False)
     rng = np.random.default_rng(seed + 1)
@@ -167,7 +270,10 @@ def compare(ws, n_fields, n, M, r, seed=77, log=None, max_evals=MAX_EVALS, rho_w
             t0 = ws.predict(f)[0] if ws is not None else None
             _, Dd, ned, sd = fit(h, DEFAULT_START, max_evals)
             row = dict(field=i, theta=th[i].tolist(), D_default=Dd, evals_default=ned, status_default=sd)
-            if t0 is not None:
+            if t0 is not None and ws.hess is not None:
+                _, Dw, new, sw = fit_whitened(h, t0, whitening(ws.hess), max_evals)
+                row.update(D_warm=Dw, evals_warm=new, status_warm=sw, theta0=t0.tolist())
+            elif t0 is not None:
                 _, Dw, new, sw = fit(h, t0, max_evals, rho_warm)
                 row.update(D_warm=Dw, evals_warm=new, status_warm=sw, theta0=t0.tolist())
             rows.append(row)
@@ -193,7 +299,7 @@ def summarize(rows):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("cmd", choices=["train", "evaluate"])
+    ap.add_argument("cmd", choices=["train", "curvature", "evaluate"])
     ap.add_argument("--fields", type=int, default=2048)
     ap.add_argument("--n", type=int, default=50000)
     ap.add_argument("--M", type=int, default=8)
@@ -217,6 +323,19 @@ def main(argv=None):
         print(json.dumps(dict(fields=a.fields, n=a.n, M=a.M, r=a.r, epochs=a.epochs, train_loss=hist[-1][0],
                               val_loss=hist[-1][1], best_val_loss=min(v for _, v in hist),
                               seconds=time.time() - t0)), flush=True)
+    elif a.cmd == "curvature":
+        # the mean curvature at the optimum, stored in the model file
+        t0 = time.time()
+        ws = WarmStart.load(a.model)
+        H, Hs = curvature(ws, a.fields, a.n, a.M, a.r, seed=a.seed, log=log)
+        Si = np.linalg.cholesky(H)
+        conds = [float(np.linalg.cond(np.linalg.solve(Si, np.linalg.solve(Si, Hf).T))) for Hf in Hs]
+        ws.hess = H
+        ws.save(a.out or a.model)
+        print(json.dumps(dict(fields=a.fields, n=a.n, M=a.M, r=a.r, seconds=time.time() - t0,
+                              mean_cond=float(np.mean([np.linalg.cond(Hf) for Hf in Hs])),
+                              mean_cond_whitened=float(np.mean(conds)), max_cond_whitened=float(np.max(conds)))),
+              flush=True)
     else:
         ws = WarmStart.load(a.model) if a.model else None
         rows = compare(ws, a.fields, a.n, a.M, a.r, log=log, max_evals=a.max_evals, rho_warm=a.rho_warm)

@@ -159,15 +159,18 @@ def test_c3_fields_against_oracle(c3):
 
 
 def test_c4_committed_network_on_held_out_fields():
-    """The committed network's start against the default start on held-out
-    fields of the warm-start configuration's shape (n = 50,000, M = 8,
-    r = 32), the same optimizer from both: at least 1.5x fewer evaluations on
-    average (1.8x measured on 64 fields) at equal final D within the
-    optimizer's resolution."""
+    """The committed warm start (network start and mean curvature, the
+    minimiser in whitened coordinates) against the default start on
+    held-out fields of the warm-start configuration's shape (n = 50,000,
+    M = 8, r = 32): at least 2x fewer evaluations on average (2.10x measured
+    on 64 fields) at a final D no worse than the default start's beyond the
+    optimizer's resolution.  Fixed seeds and bitwise-repeatable evaluations
+    make the counts repeatable."""
     ws = warmstart.WarmStart.load(MODEL)
+    assert ws.hess is not None
     t0 = time.time()
     rows = warmstart.compare(ws, 24, 50_000, 8, 32, seed=9001)
     s = warmstart.summarize(rows)
     print(f"C4 {s} ({time.time() - t0:.1f}s)", flush=True)
-    assert s["evals_ratio"] >= 1.5
-    assert abs(s["mean_D_warm_minus_default"]) < 1e-3 and s["max_D_warm_minus_default"] < 1e-2
+    assert s["evals_ratio"] >= 2.0
+    assert s["mean_D_warm_minus_default"] < 1e-6 and s["max_D_warm_minus_default"] < 1e-5

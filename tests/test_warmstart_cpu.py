@@ -32,3 +32,24 @@ def test_feature_definition_constant_and_scaled_fields():
     np.testing.assert_allclose(f2 - f1, np.log(9.0), atol=1e-12)  # gamma scales with the square
     # white noise of variance 1: semivariance 1 at every lag (sample average)
     assert abs(np.exp(f1).mean() - 1.0) < 0.05
+
+
+def test_whitening_makes_the_curvature_the_identity():
+    """The whitened coordinates of the warm start: theta = theta0 + S z with
+    S^T H S = I, on a curvature with the correlations of D (-0.98 between
+    intercepts) and scales spread over two decades."""
+    from nonstationary_mra_amd import warmstart
+    rng = np.random.default_rng(6)
+    C = np.full((6, 6), 0.5) + 0.5 * np.eye(6)
+    C[0, 3] = C[3, 0] = -0.98
+    C[1, 4] = C[4, 1] = -0.97
+    C = C + 6 * np.eye(6) * (np.linalg.eigvalsh(C)[0] < 0.05)  # keep it positive definite
+    d = np.exp(rng.uniform(np.log(1e2), np.log(1e4), 6))
+    H = np.sqrt(np.outer(d, d)) * C
+    S = warmstart.whitening(H)
+    np.testing.assert_allclose(S.T @ H @ S, np.eye(6), atol=1e-10)
+    # a step of length t in z changes the quadratic D by t^2 / 2 in every direction
+    z = rng.standard_normal(6)
+    z /= np.linalg.norm(z)
+    dth = S @ (0.3 * z)
+    assert abs(0.5 * dth @ H @ dth - 0.5 * 0.09) < 1e-12
