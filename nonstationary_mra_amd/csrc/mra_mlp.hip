@@ -1,7 +1,8 @@
 // Warm-start network for the nonstationary parameters: an MLP
 // 64 -> 256 -> 256 -> 6 with GELU, trained in FP32 on the GPU with its own
 // forward, backward and Adam kernels; inference optionally with weights and
-// activations rounded to bfloat16 (FP32 accumulation).  No reference
+// activations rounded to bfloat16 on the bfloat16 matrix instructions (FP32
+// accumulation).  No reference
 // counterpart: the reference starts its optimizer from a fixed guess
 // (src/optimization.cpp:160-169).
 //
@@ -40,6 +41,53 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const float* 
     const float* Bb = TB ? B + (int64_t)j0 * ldb : B + j0;
     mra::wg_gemm_tn_impl<float, TA, false, float, float, mra::GEMM_ANY, float, TB>(
         C + (int64_t)i0 * ldc + j0, ldc, (mv + 15) & ~15, (nv + 15) & ~15, K, 1.0f, Ab, lda, mv, Bb, ldb, nv);
+}
+
+// C[i][j] = sum_k A[i][k] B[j][k] (M x N, both operands k-contiguous) on the
+// bfloat16 matrix instruction v_mfma_f32_16x16x16_bf16 (FP32 accumulation),
+// for inference on operands already rounded to bfloat16 (the upper 16 bits of
+// each float are the bfloat16 value).  K is a multiple of 16.  One 64 x 64
+// output tile per work group, 32 x 32 per wave.  Fragments: lane t holds
+// A[t%16][4(t/16) .. 4(t/16)+3] and B[4(t/16) .. +3][t%16]; D element v of
+// lane t is row 4(t/16) + v, column t%16.
+typedef short mra_s4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ mra_s4 bf16x4(const float* p) {
+    const float4 v = *reinterpret_cast<const float4*>(p);
+    return mra_s4{(short)(__float_as_uint(v.x) >> 16), (short)(__float_as_uint(v.y) >> 16),
+                  (short)(__float_as_uint(v.z) >> 16), (short)(__float_as_uint(v.w) >> 16)};
+}
+__global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const float* A, int lda, const float* B,
+                                                   int ldb, float* C, int ldc) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r16 = lane & 15, k4 = lane >> 4;
+    const int i0 = blockIdx.y * 64 + (w >> 1) * 32, j0 = blockIdx.x * 64 + (w & 1) * 32;
+    int ri[2], cj[2];
+    bool vi[2], vj[2];
+    for (int t = 0; t < 2; t++) {
+        ri[t] = i0 + 16 * t + r16;
+        cj[t] = j0 + 16 * t + r16;
+        vi[t] = ri[t] < M;
+        vj[t] = cj[t] < N;
+        ri[t] = vi[t] ? ri[t] : 0;  // clamped rows are read and their values dropped
+        cj[t] = vj[t] ? cj[t] : 0;
+    }
+    mra_f4 acc[2][2] = {};
+    const mra_s4 z = {0, 0, 0, 0};
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        mra_s4 a[2], b[2];
+        for (int t = 0; t < 2; t++) {
+            a[t] = vi[t] ? bf16x4(A + (int64_t)ri[t] * lda + k0 + 4 * k4) : z;
+            b[t] = vj[t] ? bf16x4(B + (int64_t)cj[t] * ldb + k0 + 4 * k4) : z;
+        }
+        for (int ti = 0; ti < 2; ti++)
+            for (int tj = 0; tj < 2; tj++)
+                acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
+    }
+    for (int ti = 0; ti < 2; ti++)
+        for (int tj = 0; tj < 2; tj++)
+            for (int v = 0; v < 4; v++) {
+                const int i = i0 + 16 * ti + 4 * k4 + v, j = j0 + 16 * tj + r16;
+                if (i < M && j < N) C[(int64_t)i * ldc + j] = acc[ti][tj][v];
+            }
 }
 
 __device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
@@ -152,17 +200,30 @@ void gemm(hipStream_t st, int M, int N, int K, const float* A, int lda, const fl
     k_gemm<TA, TB><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
 }
 
+// C = A B^T for row-major A (M x K) and B (N x K): FP32 matrix instructions,
+// or (bf16, operands already rounded to bfloat16) the bfloat16 ones
+void gemm_nt(hipStream_t st, bool bf16, int M, int N, int K, const float* A, int lda, const float* B, int ldb,
+             float* C, int ldc) {
+    if (bf16) {
+        dim3 grid((N + 63) / 64, (M + 63) / 64);
+        k_gemm_bf16<<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
+    } else {
+        gemm<true, true>(st, M, N, K, A, lda, B, ldb, C, ldc);
+    }
+}
+
 // forward pass on the first B rows of m->X; bf16 selects rounded weights (Pb)
+// and activations on the bfloat16 matrix instructions
 void forward(mra_mlp* m, int B, bool bf16) {
     const float* P = bf16 ? m->Pb : m->P;
     hipStream_t st = m->st;
     if (bf16) k_round_bf16<<<nb((int64_t)B * NIN), 256, 0, st>>>((int64_t)B * NIN, m->X, m->X);
     // Z1[b][o] = sum_k X[b][k] W1[o][k]
-    gemm<true, true>(st, B, NH, NIN, m->X, NIN, P + OW1, NIN, m->Z1, NH);
+    gemm_nt(st, bf16, B, NH, NIN, m->X, NIN, P + OW1, NIN, m->Z1, NH);
     k_bias_act<<<nb((int64_t)B * NH), 256, 0, st>>>(B, NH, m->Z1, P + OB1, m->A1, 1, bf16);
-    gemm<true, true>(st, B, NH, NH, m->A1, NH, P + OW2, NH, m->Z2, NH);
+    gemm_nt(st, bf16, B, NH, NH, m->A1, NH, P + OW2, NH, m->Z2, NH);
     k_bias_act<<<nb((int64_t)B * NH), 256, 0, st>>>(B, NH, m->Z2, P + OB2, m->A2, 1, bf16);
-    gemm<true, true>(st, B, NOUT, NH, m->A2, NH, P + OW3, NH, m->Y, NOUT);
+    gemm_nt(st, bf16, B, NOUT, NH, m->A2, NH, P + OW3, NH, m->Y, NOUT);
     k_bias_act<<<nb((int64_t)B * NOUT), 256, 0, st>>>(B, NOUT, m->Y, P + OB3, m->Y, 0, 0);
 }
 

@@ -29,6 +29,13 @@ def test_variogram_features_match_oracle():
     np.testing.assert_allclose(f, fo, atol=2e-5)
 
 
+def bf16_round(x):
+    """Round to the nearest bfloat16 (ties to even), through float32."""
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
 def test_mlp_forward_and_gradient_match_oracle():
     rng = np.random.default_rng(12)
     m = capi.Mlp(seed=5)
@@ -50,6 +57,18 @@ def test_mlp_forward_and_gradient_match_oracle():
         rel = np.abs(Yb - Yo).max() / np.abs(Yo).max()
         print("bf16 rel err", rel, flush=True)
         assert 0 < rel < 3e-2
+        # ... and the bfloat16 matrix instructions against a float64 restatement of
+        # the same roundings (inputs, weights, activations; FP32 accumulation on
+        # the GPU): a misplaced fragment would give O(1) errors, while a rare
+        # activation rounded the other way changes Y by about 1e-4 of its scale
+        w = om.unpack(bf16_round(p))
+        A = bf16_round(X)
+        for k in (1, 2):
+            A = bf16_round(om.gelu(A @ w[f"W{k}"].T + w[f"b{k}"]))
+        Ye = A @ w["W3"].T + w["b3"]
+        err = np.abs(Yb - Ye).max() / np.abs(Ye).max()
+        print("bf16 MFMA vs emulation", err, flush=True)
+        assert err < 2e-3
     finally:
         m.close()
 
