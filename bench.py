@@ -273,6 +273,8 @@ def main():
                          "cov: the covariance kernel alone")
     ap.add_argument("--cov-n", type=int, default=65536)
     ap.add_argument("--c3-fields", type=int, default=256)
+    ap.add_argument("--finest-wave", type=int, default=None, choices=[0, 1],
+                    help="mixed-precision finest conditioning one wave per tile (MRA_OPT_FINEST_WAVE)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--no-cov-kernel", action="store_true", help="skip the standalone covariance-kernel leg")
@@ -305,6 +307,8 @@ def main():
     log(f"setup {time.time() - t0:.1f}s")
     prec = capi.PRECISION_MIXED if args.precision == "mixed" else capi.PRECISION_FP64
     h.set_precision(prec)
+    if args.finest_wave is not None:
+        h.set_option(capi.OPT_FINEST_WAVE, args.finest_wave)
     # work-list shape from the library's autotuner (timed on this workload)
     if args.tuning:
         h.set_tuning(*[int(v) for v in args.tuning.split(",")])
@@ -342,6 +346,7 @@ def main():
                    "n": n, "n_pred": args.npred, "M": args.M, "J": J, "r": args.r, "precision": args.precision,
                    "graph": graph, "max_obs_per_finest_region": int(counts.max()),
                    "worklist_shape": dict(zip(("x32max", "lrmax", "fb16"), tune_cfg)),
+                   "finest_wave": h.get_option(capi.OPT_FINEST_WAVE),
                    "seeds": {"locations": SEED_LOC, "field": SEED_FIELD}},
         "loglik": -0.5 * (D + n * np.log(2 * 3.14159265359)), "D": D,
         "eager_ms_per_step": eager_ms,

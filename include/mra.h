@@ -207,7 +207,12 @@ int mra_use_graph(mra_handle* h, int enable);
  * coarse prior one work group per region.  1 would select the one-wave-per-
  * region form for 64-knot regions (k_prior_wave + k_prior_chol); it is
  * refused with MRA_E_UNSUPPORTED: a GPU run with it stopped making progress
- * and the cause is not established. */
+ * and the cause is not established.
+ * MRA_OPT_FINEST_WAVE: 1 (default) runs the mixed-precision finest
+ * conditioning of 64-knot regions one wave per tile (k_finest_cross_w: the
+ * tile in the registers of the FP32 matrix instructions, no work-group
+ * barrier); 0 one work group per tile (same values up to FP32 rounding).
+ * Persistent launches (MRA_OPT_PERSISTENT) use the work-group form. */
 enum {
     MRA_OPT_FACTORED_POSTERIOR = 1,
     MRA_OPT_GRAM_KERNEL = 2,
@@ -218,7 +223,8 @@ enum {
     MRA_OPT_XCD_MAP = 7,
     MRA_OPT_MIXED_FP64_ABOVE = 8,
     MRA_OPT_PERSISTENT = 9,
-    MRA_OPT_PRIOR_WAVE = 10
+    MRA_OPT_PRIOR_WAVE = 10,
+    MRA_OPT_FINEST_WAVE = 11
 };
 int mra_set_option(mra_handle* h, int option, int value);
 int mra_get_option(mra_handle* h, int option, int* value);

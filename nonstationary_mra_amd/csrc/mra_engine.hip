@@ -65,6 +65,7 @@ struct mra_handle {
     int xcdMap = 1;       // XCD-contiguous item ranges (Tree::xcd)
     int persistent = 0;   // finest-level launches on a resident grid (MRA_OPT_PERSISTENT)
     int priorWave = 0;    // 64-knot coarse prior one wave per region (MRA_OPT_PRIOR_WAVE; refused, see below)
+    int finestWave = 1;   // mixed-precision finest conditioning one wave per tile (MRA_OPT_FINEST_WAVE)
     bool denseFull = false;  // full-size dense messages of level L-1 allocated
     std::vector<uint64_t> hRowS1, hRowE1, hRowS2, hRowE2;  // rows of levels L-1 and L-2
     uint64_t *dRowS1 = nullptr, *dRowE1 = nullptr, *dRowS2 = nullptr, *dRowE2 = nullptr;
@@ -977,6 +978,11 @@ extern "C" int mra_set_option(mra_handle* h, int option, int value) {
         graph_invalidate(h);
         return MRA_OK;
     }
+    if (option == MRA_OPT_FINEST_WAVE) {
+        h->finestWave = value ? 1 : 0;
+        graph_invalidate(h);
+        return MRA_OK;
+    }
     if (option == MRA_OPT_PRIOR_WAVE) {
         // A GPU run of this form stopped making progress in a test that
         // evaluated it in mixed precision and from a captured graph for the
@@ -1039,6 +1045,10 @@ extern "C" int mra_get_option(mra_handle* h, int option, int* value) {
     }
     if (option == MRA_OPT_PRIOR_WAVE) {
         *value = (h->RP == 64 && h->priorWave) ? 1 : 0;
+        return MRA_OK;
+    }
+    if (option == MRA_OPT_FINEST_WAVE) {
+        *value = h->finestWave;
         return MRA_OK;
     }
     if (option == MRA_OPT_MIXED_FP64_ABOVE) {
@@ -1241,6 +1251,35 @@ template <int RP, typename TG>
 static void launch_finest_cross(mra_handle* h, const Tree& T, const CovCtx& cc, const Pts& K, const Pts& O) {
     const Fork fk(h, 1);
     const hipStream_t st = fk.stream(0);
+    if constexpr (RP == 64 && sizeof(TG) == 4) {
+        if (h->finestWave && !h->persistent) {
+            // one wave per tile, four tiles per work group; the float point
+            // data path for the nonstationary chordal and planar kernels
+            const bool pf = !cc.stationary && cc.metric != 1;
+            auto wl = [&](auto k64, auto k32, auto k16) {
+                if (h->nItemsX)
+                    k64<<<(h->nItemsX + 3) / 4, 256, 0, fk.stream(1)>>>(T, cc, K, O, h->dItemsX, h->nItemsX, h->dGoff,
+                                                                        h->dLiT, (float*)h->dGf, (float*)h->dBf,
+                                                                        h->dGc32);
+                if (h->nItemsX32)
+                    k32<<<(h->nItemsX32 + 3) / 4, 256, 0, st>>>(T, cc, K, O, h->dItemsX32, h->nItemsX32, h->dGoff,
+                                                                h->dLiT, (float*)h->dGf, (float*)h->dBf, h->dGc32);
+                if (h->nItemsX16)
+                    k16<<<(h->nItemsX16 + 3) / 4, 256, 0, fk.stream(2)>>>(T, cc, K, O, h->dItemsX16, h->nItemsX16,
+                                                                          h->dGoff, h->dLiT, (float*)h->dGf,
+                                                                          (float*)h->dBf, h->dGc32);
+            };
+            if (pf)
+                wl(&k_finest_cross_w<64, 2, true>, &k_finest_cross_w<32, 4, true>, &k_finest_cross_w<16, 4, true>);
+            else
+                wl(&k_finest_cross_w<64, 2, false>, &k_finest_cross_w<32, 4, false>, &k_finest_cross_w<16, 4, false>);
+            if (h->nItemsXBig && h->dGf64)
+                launch_units(h, &k_finest_cross<RP, 64, double>, &k_finest_cross<RP, 64, double, true>,
+                             h->nItemsXBig, 1, st, T, cc, K, O, h->dItemsXBig, h->nItemsXBig, h->dGoff, h->dGc,
+                             h->dLiT, h->dGf64, h->dBf64, nullptr);
+            return;
+        }
+    }
     if (h->nItemsX)
         launch_units(h, &k_finest_cross<RP, 64, TG>, &k_finest_cross<RP, 64, TG, true>, h->nItemsX, 1, fk.stream(1),
                      

@@ -600,6 +600,9 @@ def test_persistent_finest_launches(case):
     try:
         h.set_covariance(cov)
         h.set_params(**prm)
+        # the work-group form of the mixed conditioning in both (persistent
+        # launches have no one-wave form)
+        h.set_option(capi.OPT_FINEST_WAVE, 0)
         for prec in (capi.PRECISION_FP64, capi.PRECISION_MIXED):
             h.set_precision(prec)
             for pers in (0, 1):
@@ -687,3 +690,66 @@ def test_prior_one_wave_per_region(case):
     assert rel(D, Do) < TOL_D
     np.testing.assert_allclose(mean, mo, rtol=TOL_PRED, atol=1e-10)
     np.testing.assert_allclose(var, vo, rtol=TOL_PRED, atol=1e-10)
+
+
+@pytest.mark.parametrize("case", ["clustered64", "padded40", "chordal_stationary", "cross16_off"])
+def test_finest_one_wave_per_tile(case):
+    """Mixed-precision finest conditioning one wave per tile
+    (MRA_OPT_FINEST_WAVE = 1) against one work group per tile (0): both
+    within the mixed contract (|delta loglik| / n <= 1e-6) of the FP64
+    oracle and within 1e-7 of each other per observation (FP32 rounding in
+    another order; measured 2.5e-9 against 3.6e-9 and 6.1e-9 from the
+    oracle), predictions within 1e-4, graph replay equal to eager bit for bit.
+    The cases cover 64-, 32- and 16-column tiles (and 64/32 only), padded
+    knots (r = 40), the chordal metric and the stationary kernel."""
+    rng = np.random.default_rng(83)
+    if case in ("clustered64", "cross16_off"):
+        x, y = synth.clustered_points(12000, rng)
+        px, py = synth.uniform_points(300, rng)
+        tree = capi.partition_program(x, y, np.zeros(12000), px, py, M=4, J=4, r=64, bounds=synth.UNIT_BOUNDS)
+        cov, prm = synth.linear_trend_cov(), dict(synth.UNIT_PARAMS)
+    elif case == "padded40":
+        tree, cov, prm = cases.synthetic_case(n=8000, npred=200, M=4, J=4, r=40, seed=84, clustered=True,
+                                              partition="program")
+    else:
+        tree, cov, prm = cases.synthetic_case(n=8000, npred=200, M=4, J=4, r=50, seed=85, clustered=True,
+                                              metric="chordal", stationary=True, partition="program")
+    # observations drawn from the model (the mixed contract is stated for data
+    # the model describes; on white noise the rounding error scales with |D|)
+    h = capi.Handle(tree, with_pred=False)
+    h.set_covariance(cov)
+    h.set_params(**prm)
+    tree["obs_val"] = h.sample_prior(seed=8, replace=False)
+    h.close()
+    n = len(tree["obs_lon"])
+    out = {}
+    h = capi.Handle(tree)
+    try:
+        h.set_covariance(cov)
+        h.set_params(**prm)
+        h.set_precision(capi.PRECISION_MIXED)
+        if case == "cross16_off":
+            h.set_option(capi.OPT_CROSS16, 0)
+        for wave in (0, 1):
+            h.set_option(capi.OPT_FINEST_WAVE, wave)
+            assert h.get_option(capi.OPT_FINEST_WAVE) == wave
+            for graph in (0, 1):
+                h.use_graph(graph)
+                out[(wave, graph)] = h.predict()
+                out[(wave, graph, "l")] = h.likelihood()
+    finally:
+        h.close()
+    Do, mo, vo = oracle.evaluate(tree, cov, prm, predict=True)
+    for wave in (0, 1):
+        assert out[(wave, 1)][0] == out[(wave, 0)][0]
+        assert out[(wave, 1, "l")] == out[(wave, 0, "l")]
+        np.testing.assert_array_equal(out[(wave, 1)][1], out[(wave, 0)][1])
+        D, mean, var = out[(wave, 0)]
+        err = abs(D - Do) / (2 * n)
+        print(case, "wave", wave, "D", D, "oracle", Do, "|dl|/n", err, flush=True)
+        assert err <= TOL_MIXED
+        assert abs(out[(wave, 0, "l")] - Do) / (2 * n) <= TOL_MIXED
+        np.testing.assert_allclose(mean, mo, rtol=1e-4, atol=1e-4 * np.abs(mo).max())
+        np.testing.assert_allclose(var, vo, rtol=1e-4, atol=1e-6)
+    # the two orders of FP32 rounding differ by less than either differs from FP64
+    assert abs(out[(1, 0)][0] - out[(0, 0)][0]) / (2 * n) <= 1e-7
