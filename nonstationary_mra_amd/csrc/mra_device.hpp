@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
+
+#include "mra_exp.hpp"
 #include <stdint.h>
 
 #define MRA_WG 256
@@ -59,11 +61,11 @@ __device__ __forceinline__ double pdist(const CovCtx& c, double x1, double y1, d
 // (src/evaluate_covariance.cpp:207-223).
 __device__ __forceinline__ double pcov(const CovCtx& c, const Pts& P, uint64_t i, const Pts& Q, uint64_t j) {
     double d = pdist(c, P.x[i], P.y[i], P.z[i], Q.x[j], Q.y[j], Q.z[j]);
-    if (c.stationary) return c.sill * exp(-d / c.range);
+    if (c.stationary) return c.sill * exp_neg(-d / c.range);
     double b1 = P.b[i], b2 = Q.b[j];
     double bms = (b1 * b1 + b2 * b2) * 0.5;
     double ratio = (b1 * b2) / bms;
-    return (P.s[i] * Q.s[j]) * (ratio * sqrt(ratio)) * exp(-d / sqrt(bms));
+    return (P.s[i] * Q.s[j]) * (ratio * sqrt(ratio)) * exp_neg(-d / sqrt(bms));
 }
 
 // ---------------------------------------------------- covariance tiles
@@ -105,9 +107,9 @@ __device__ __forceinline__ double pcov_pt(const CovCtx& c, const PtV& a, const P
         d = d2 > 0.0 ? d2 * rsqrt_nr(d2) : 0.0;
         if (c.metric == 0) d *= 6371.0;
     }
-    if (c.stationary) return c.sill * exp(-d / c.range);
+    if (c.stationary) return c.sill * exp_neg(-d / c.range);
     const double t = rsqrt_nr((a.b * a.b + b.b * b.b) * 0.5);
-    return (a.q * b.q) * (t * t * t) * exp(-d * t);
+    return (a.q * b.q) * (t * t * t) * exp_neg(-d * t);
 }
 
 // FP32 evaluation of the same form (mixed precision) on point data rounded
@@ -197,6 +199,47 @@ __device__ __forceinline__ void cov_tile_pre(T* C, int LD, const CovCtx& cc, con
                                              int nc, const double* vdiag = nullptr) {
     static_assert(MRA_WG % NC == 0, "tile width must divide the work-group size");
     const int c = threadIdx.x % NC;
+    if (sizeof(T) == 8 && !cc.stationary && cc.metric != 1) {
+        // FP64 nonstationary chordal / planar kernel (the values of pcov_pt):
+        // a straight-line body, validity by selects, four rows at a time so
+        // that their exponential-table loads overlap
+        // (d = d2 / sqrt(d2) with the root taken of max(d2, 1e-300): exactly
+        // 0 for coincident points, the value of pcov_pt otherwise)
+        const double dscale = cc.metric == 0 ? 6371.0 : 1.0;
+        constexpr int STEP = MRA_WG / NC, G = RP / STEP >= 4 ? 4 : RP / STEP;
+        static_assert(G >= 1 && RP % (STEP * G) == 0, "rows in groups of up to four per thread");
+        for (int i0 = threadIdx.x / NC; i0 < RP; i0 += G * STEP) {
+            double cv[G], xg[G], fg[G], tk[G];
+            ExpParts eg[G];
+#pragma unroll
+            for (int g = 0; g < G; g++) {
+                const PtV& a = st[i0 + g * STEP];
+                const double dx = qc.x - a.x, dy = qc.y - a.y, dz = qc.z - a.z;
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                const double d = d2 * rsqrt_nr(fmax(d2, 1e-300)) * dscale;
+                const double t = rsqrt_nr((a.b * a.b + qc.b * qc.b) * 0.5);
+                fg[g] = (a.q * qc.q) * (t * t * t);
+                xg[g] = -d * t;
+                eg[g] = exp_neg_parts(xg[g]);
+            }
+            // the group's table loads in flight together (one wait)
+#pragma unroll
+            for (int g = 0; g < G; g++) tk[g] = Exp2Frac::T[eg[g].k & 63];
+            if constexpr (G == 4)
+                asm volatile("" ::"v"(tk[0]), "v"(tk[1]), "v"(tk[2]), "v"(tk[3]));
+#pragma unroll
+            for (int g = 0; g < G; g++) cv[g] = fg[g] * exp_neg_finish(tk[g], eg[g], xg[g]);
+#pragma unroll
+            for (int g = 0; g < G; g++) {
+                const int i = i0 + g * STEP;
+                C[i * LD + c] = (T)((i < nr && c < nc) ? cv[g] : 0.0);
+            }
+        }
+        if (vdiag)
+            for (int i = threadIdx.x / NC; i < RP; i += MRA_WG / NC)
+                if (i == c) C[i * LD + c] = (T)(i < nr ? vdiag[i] : 1.0);
+        return;
+    }
     for (int i = threadIdx.x / NC; i < RP; i += MRA_WG / NC) {
         T v = T(0);
         if (vdiag && i == c) {
