@@ -209,7 +209,7 @@ def run_cov(args):
     print(json.dumps(out), flush=True)
 
 
-def run_c3(args):
+def c3_measure(n_fields, steps, warmup):
     """Configuration of the uneven batch: independent fields with n drawn
     log-uniform in [5,000, 200,000] (uniform locations), per-field kernel
     parameters (log range in [log 0.02, log 0.2], range and variance slopes
@@ -220,7 +220,7 @@ def run_c3(args):
     setting comes from the batch autotuner.  One step = one evaluation of
     every field."""
     t0 = time.time()
-    specs = workloads.c3_specs(args.c3_fields)
+    specs = workloads.c3_specs(n_fields)
     hs = []
     for i, spec in enumerate(specs):
         hs.append(workloads.c3_field(spec, i))
@@ -230,27 +230,40 @@ def run_c3(args):
     b = capi.Batch(hs)
     streams, graph, tune_ms = b.autotune()
     log(f"c3 batch autotune: streams={streams} graph={graph} {tune_ms:.1f} ms")
-    for _ in range(args.warmup):
+    setup_s = time.time() - t0
+    for _ in range(warmup):
         D, st = b.likelihood()
     t1 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         D, st = b.likelihood()
     el = time.perf_counter() - t1
     # the same fields one after another (each synchronised), for comparison
     t2 = time.perf_counter()
     Ds = np.array([h.likelihood() for h in hs])
     seq_ms = 1e3 * (time.perf_counter() - t2)
-    out = {"metric": "C3 batch loglik evals/sec (256 uneven fields, r=32, mixed)", "value": args.c3_fields * args.steps / el,
-           "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-           "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f32+f64", "data": "synthetic (uniform points, fields drawn from the MRA prior)",
-           "config": {"workload": "256 fields, n log-uniform in [5e3, 2e5], r=32, M per field, mixed precision",
-                      "fields": args.c3_fields, "total_points": n_tot, "streams": streams, "graph": graph},
-           "sequential_ms_per_step": seq_ms, "batch_matches_sequential": bool(np.array_equal(D, Ds)),
-           "not_pd_fields": int((st != 0).sum())}
+    res = {"fields": n_fields, "total_points": n_tot, "ms_per_batch": 1e3 * el / steps,
+           "field_evals_per_s": n_fields * steps / el, "sequential_ms_per_batch": seq_ms,
+           "batch_matches_sequential": bool(np.array_equal(D, Ds)), "not_pd_fields": int((st != 0).sum()),
+           "streams": streams, "graph": graph, "setup_s": setup_s}
     b.close()
     for h in hs:
         h.close()
+    return res
+
+
+def run_c3(args):
+    """--config c3: the batch of independent fields on its own line (see
+    c3_measure)."""
+    r = c3_measure(args.c3_fields, args.steps, args.warmup)
+    out = {"metric": "C3 batch loglik evals/sec (256 uneven fields, r=32, mixed)", "value": r["field_evals_per_s"],
+           "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": r["ms_per_batch"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32+f64", "data": "synthetic (uniform points, fields drawn from the MRA prior)",
+           "config": {"workload": "256 fields, n log-uniform in [5e3, 2e5], r=32, M per field, mixed precision",
+                      "fields": args.c3_fields, "total_points": r["total_points"], "streams": r["streams"],
+                      "graph": r["graph"]},
+           "sequential_ms_per_step": r["sequential_ms_per_batch"],
+           "batch_matches_sequential": r["batch_matches_sequential"], "not_pd_fields": r["not_pd_fields"]}
     print(json.dumps(out), flush=True)
 
 
@@ -276,6 +289,7 @@ def main():
     ap.add_argument("--finest-wave", type=int, default=None, choices=[0, 1],
                     help="mixed-precision finest conditioning one wave per tile (MRA_OPT_FINEST_WAVE)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-c3", action="store_true", help="skip the C3 batch leg of the default run")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--no-cov-kernel", action="store_true", help="skip the standalone covariance-kernel leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -391,6 +405,10 @@ def main():
                                     "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                                     "frac": cbytes / (cms * 1e-3) / PEAK_HBM,
                                     "traffic": load_traffic("cov_f32") if cn == 65536 else None}
+    if not args.no_c3:
+        # the uneven batch of independent fields (C3), three batch evaluations
+        out["c3"] = c3_measure(args.c3_fields, 3, 1)
+        log(f"c3: {out['c3']['ms_per_batch']:.1f} ms per batch of {args.c3_fields} fields")
     if not args.no_cpu_baseline:
         cb = cpu_baseline(None, args.M, J, args.r, args.n)
         cpu_evals = (1.0 / cb["t"]) * (cb["w"] / w_total)
