@@ -55,6 +55,7 @@ OPT_MIXED_FP64_ABOVE = 8
 OPT_PERSISTENT = 9
 OPT_PRIOR_WAVE = 10
 OPT_FINEST_WAVE = 11
+OPT_LOCAL_LL = 12
 
 
 class TreeDesc(C.Structure):

@@ -66,6 +66,7 @@ struct mra_handle {
     int persistent = 0;   // finest-level launches on a resident grid (MRA_OPT_PERSISTENT)
     int priorWave = 0;    // 64-knot coarse prior one wave per region (MRA_OPT_PRIOR_WAVE; refused, see below)
     int finestWave = 1;   // mixed-precision finest conditioning one wave per tile (MRA_OPT_FINEST_WAVE)
+    int localLL = 1;      // finest local step above 64 observations, left-looking (MRA_OPT_LOCAL_LL)
     bool denseFull = false;  // full-size dense messages of level L-1 allocated
     std::vector<uint64_t> hRowS1, hRowE1, hRowS2, hRowE2;  // rows of levels L-1 and L-2
     uint64_t *dRowS1 = nullptr, *dRowE1 = nullptr, *dRowS2 = nullptr, *dRowE2 = nullptr;
@@ -507,7 +508,7 @@ static int build_worklists(mra_handle* h) {
         h->dWsOffBig = nullptr;
         h->dWsOffBig3 = nullptr;
         h->dWsBig = nullptr;
-        if (k) {
+        if (k && !h->localLL) {
             RC_TRY(upload_wl(h, &h->dWsOffBig, wo.data(), std::max<size_t>(b[4].size(), 1)));
             RC_TRY(upload_wl(h, &h->dWsOffBig3, wo.data() + b[4].size(), std::max<size_t>(b[3].size(), 1)));
             RC_TRY(upload_wl(h, &h->dWsBig, (const double*)nullptr, wo.back()));
@@ -983,6 +984,12 @@ extern "C" int mra_set_option(mra_handle* h, int option, int value) {
         graph_invalidate(h);
         return MRA_OK;
     }
+    if (option == MRA_OPT_LOCAL_LL) {
+        h->localLL = value ? 1 : 0;
+        RC_TRY(build_worklists(h));  // the earlier forms need their workspace
+        graph_invalidate(h);
+        return MRA_OK;
+    }
     if (option == MRA_OPT_PRIOR_WAVE) {
         // A GPU run of this form stopped making progress in a test that
         // evaluated it in mixed precision and from a captured graph for the
@@ -1049,6 +1056,10 @@ extern "C" int mra_get_option(mra_handle* h, int option, int* value) {
     }
     if (option == MRA_OPT_FINEST_WAVE) {
         *value = h->finestWave;
+        return MRA_OK;
+    }
+    if (option == MRA_OPT_LOCAL_LL) {
+        *value = h->localLL;
         return MRA_OK;
     }
     if (option == MRA_OPT_MIXED_FP64_ABOVE) {
@@ -1183,6 +1194,7 @@ static void prefill_grids_t() {
     (void)resident_groups((const void*)&k_finest_local<RP, 64, TG, true>);
     (void)resident_groups((const void*)&k_finest_local_big<TG, true>);
     (void)resident_groups((const void*)&k_finest_local_w16<TG, true>);
+    (void)resident_groups((const void*)&k_finest_local_ll<TG, true>);
 }
 
 static void prefill_grids(const mra_handle* h) {
@@ -1204,46 +1216,56 @@ static void launch_finest_local(mra_handle* h, const Tree& T, const CovCtx& cc, 
     // larger buckets first (longest work groups start early); in mixed
     // precision the regions above mixFp64Min observations (buckets 3 and 5)
     // use their FP64 G and B
-    if (h->nItemsFB[5] && sizeof(TG) == 4 && h->nItemsXBig && h->dGf64)
-        launch_units(h, &k_finest_local<RP, 0, double>, &k_finest_local<RP, 0, double, true>, h->nItemsFB[5], 1, s1,
-                     
-            T, cc, O, h->dY, h->dItemsFB[5], h->nItemsFB[5], h->dGf64, h->dBf64, h->dSig, h->dH, h->dU, h->dDterm,
-            h->dBad);
-    else if (h->nItemsFB[5])
-        launch_units(h, &k_finest_local<RP, 0, TG>, &k_finest_local<RP, 0, TG, true>, h->nItemsFB[5], 1, s1,
-                     
-            T, cc, O, h->dY, h->dItemsFB[5], h->nItemsFB[5], Gf, Bf, h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
-    if (h->nItemsFB[3] && sizeof(TG) == 4 && h->nItemsXBig && h->dGf64)
-        launch_units(h, &k_finest_local_big<double>, &k_finest_local_big<double, true>, h->nItemsFB[3], 1, s1,
-                     
-            T, cc, O, h->dY, h->dItemsFB[3], h->nItemsFB[3], h->dWsOffBig3, h->dWsBig, h->dGf64, h->dBf64, h->dSig,
-            h->dH, h->dU, h->dDterm, h->dBad);
-    else if (h->nItemsFB[3])
-        launch_units(h, &k_finest_local_big<TG>, &k_finest_local_big<TG, true>, h->nItemsFB[3], 1, s1,
-                     
-            T, cc, O, h->dY, h->dItemsFB[3], h->nItemsFB[3], h->dWsOffBig3, h->dWsBig, Gf, Bf, h->dSig, h->dH, h->dU,
-            h->dDterm, h->dBad);
-    if (h->nItemsFB[4])
-        launch_units(h, &k_finest_local_big<TG>, &k_finest_local_big<TG, true>, h->nItemsFB[4], 1, s1,
-                     
-            T, cc, O, h->dY, h->dItemsFB[4], h->nItemsFB[4], h->dWsOffBig, h->dWsBig, Gf, Bf, h->dSig, h->dH, h->dU,
-            h->dDterm, h->dBad);
+    if (h->localLL) {
+        // every region above 64 observations: left-looking blocked Cholesky
+        // (FP64 G and B for those above mixFp64Min in mixed precision)
+        const bool g64 = sizeof(TG) == 4 && h->nItemsXBig && h->dGf64;
+        for (int q : {5, 3})
+            if (h->nItemsFB[q] && g64)
+                launch_units(h, &k_finest_local_ll<double>, &k_finest_local_ll<double, true>, h->nItemsFB[q], 1, s1, T,
+                             cc, O, h->dY, h->dItemsFB[q], h->nItemsFB[q], (const double*)h->dGf64,
+                             (const double*)h->dBf64, h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
+            else if (h->nItemsFB[q])
+                launch_units(h, &k_finest_local_ll<TG>, &k_finest_local_ll<TG, true>, h->nItemsFB[q], 1, s1, T, cc, O,
+                             h->dY, h->dItemsFB[q], h->nItemsFB[q], (const TG*)Gf, (const TG*)Bf, h->dSig, h->dH,
+                             h->dU, h->dDterm, h->dBad);
+        if (h->nItemsFB[4])
+            launch_units(h, &k_finest_local_ll<TG>, &k_finest_local_ll<TG, true>, h->nItemsFB[4], 1, s1, T, cc, O,
+                         h->dY, h->dItemsFB[4], h->nItemsFB[4], (const TG*)Gf, (const TG*)Bf, h->dSig, h->dH, h->dU,
+                         h->dDterm, h->dBad);
+    } else {
+        if (h->nItemsFB[5] && sizeof(TG) == 4 && h->nItemsXBig && h->dGf64)
+            launch_units(h, &k_finest_local<RP, 0, double>, &k_finest_local<RP, 0, double, true>, h->nItemsFB[5], 1, s1,
+                T, cc, O, h->dY, h->dItemsFB[5], h->nItemsFB[5], h->dGf64, h->dBf64, h->dSig, h->dH, h->dU, h->dDterm,
+                h->dBad);
+        else if (h->nItemsFB[5])
+            launch_units(h, &k_finest_local<RP, 0, TG>, &k_finest_local<RP, 0, TG, true>, h->nItemsFB[5], 1, s1,
+                T, cc, O, h->dY, h->dItemsFB[5], h->nItemsFB[5], Gf, Bf, h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
+        if (h->nItemsFB[3] && sizeof(TG) == 4 && h->nItemsXBig && h->dGf64)
+            launch_units(h, &k_finest_local_big<double>, &k_finest_local_big<double, true>, h->nItemsFB[3], 1, s1,
+                T, cc, O, h->dY, h->dItemsFB[3], h->nItemsFB[3], h->dWsOffBig3, h->dWsBig, h->dGf64, h->dBf64, h->dSig,
+                h->dH, h->dU, h->dDterm, h->dBad);
+        else if (h->nItemsFB[3])
+            launch_units(h, &k_finest_local_big<TG>, &k_finest_local_big<TG, true>, h->nItemsFB[3], 1, s1,
+                T, cc, O, h->dY, h->dItemsFB[3], h->nItemsFB[3], h->dWsOffBig3, h->dWsBig, Gf, Bf, h->dSig, h->dH, h->dU,
+                h->dDterm, h->dBad);
+        if (h->nItemsFB[4])
+            launch_units(h, &k_finest_local_big<TG>, &k_finest_local_big<TG, true>, h->nItemsFB[4], 1, s1,
+                T, cc, O, h->dY, h->dItemsFB[4], h->nItemsFB[4], h->dWsOffBig, h->dWsBig, Gf, Bf, h->dSig, h->dH, h->dU,
+                h->dDterm, h->dBad);
+    }
     if (h->nItemsFB[2])
         launch_units(h, &k_finest_local<RP, 64, TG>, &k_finest_local<RP, 64, TG, true>, h->nItemsFB[2], 1, s2,
-                     
             T, cc, O, h->dY, h->dItemsFB[2], h->nItemsFB[2], Gf, Bf, h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
     if (h->nItemsFB[1])
         launch_units(h, &k_finest_local<RP, 32, TG>, &k_finest_local<RP, 32, TG, true>, h->nItemsFB[1], 1, s2,
-                     
             T, cc, O, h->dY, h->dItemsFB[1], h->nItemsFB[1], Gf, Bf, h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
     if (h->nItemsFB[0] && h->waveLocal16)
         // one wave per region, four regions per work group
         launch_units(h, &k_finest_local_w16<TG>, &k_finest_local_w16<TG, true>, h->nItemsFB[0], 4, st,
-                     
             T, cc, O, h->dY, h->dItemsFB[0], h->nItemsFB[0], Gf, Bf, h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
     else if (h->nItemsFB[0])
         launch_units(h, &k_finest_local<RP, 16, TG>, &k_finest_local<RP, 16, TG, true>, h->nItemsFB[0], 1, st,
-                     
             T, cc, O, h->dY, h->dItemsFB[0], h->nItemsFB[0], Gf, Bf, h->dSig, h->dH, h->dU, h->dDterm, h->dBad);
 }
 
@@ -1282,19 +1304,15 @@ static void launch_finest_cross(mra_handle* h, const Tree& T, const CovCtx& cc, 
     }
     if (h->nItemsX)
         launch_units(h, &k_finest_cross<RP, 64, TG>, &k_finest_cross<RP, 64, TG, true>, h->nItemsX, 1, fk.stream(1),
-                     
             T, cc, K, O, h->dItemsX, h->nItemsX, h->dGoff, h->dGc, h->dLiT, (TG*)h->dGf, (TG*)h->dBf, h->dGc32);
     if (h->nItemsX32)
         launch_units(h, &k_finest_cross<RP, 32, TG>, &k_finest_cross<RP, 32, TG, true>, h->nItemsX32, 1, st,
-                     
             T, cc, K, O, h->dItemsX32, h->nItemsX32, h->dGoff, h->dGc, h->dLiT, (TG*)h->dGf, (TG*)h->dBf, h->dGc32);
     if (h->nItemsX16)
         launch_units(h, &k_finest_cross<RP, 16, TG>, &k_finest_cross<RP, 16, TG, true>, h->nItemsX16, 1, fk.stream(2),
-                     
             T, cc, K, O, h->dItemsX16, h->nItemsX16, h->dGoff, h->dGc, h->dLiT, (TG*)h->dGf, (TG*)h->dBf, h->dGc32);
     if (sizeof(TG) == 4 && h->nItemsXBig && h->dGf64)
         launch_units(h, &k_finest_cross<RP, 64, double>, &k_finest_cross<RP, 64, double, true>, h->nItemsXBig, 1, st,
-                     
             T, cc, K, O, h->dItemsXBig, h->nItemsXBig, h->dGoff, h->dGc, h->dLiT, h->dGf64, h->dBf64, nullptr);
 }
 

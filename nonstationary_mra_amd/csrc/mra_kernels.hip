@@ -985,6 +985,226 @@ __global__ __launch_bounds__(256) void k_finest_local_w16(Tree T, CovCtx cc, Pts
     }
 }
 
+// The local step for finest regions of more than 64 observations (any size)
+// as a left-looking Cholesky on 32-wide block columns, with no Sigma_F
+// workspace.  Block column c (rows and columns c0 = 32c ..):
+//  (1) panel tiles P_rc = V_rc - G_r^T G_c - L_{r,<c} L_{c,<c}^T (r >= c),
+//      32 x 32 MFMA products whose operands are G_F (segment layout) and the
+//      rows of L_F already final in Sig; P_cc goes to LDS, the others to
+//      their places in Sig; t = y_c - L_{c,<c} u_{<c};
+//  (2) one wave factors P_cc and forms D_c = L_cc^{-1} and u_c = D_c t while
+//      the other three form R_c = B_c - L_{c,<c} H_{<c} for their 32-column
+//      strips of H (stored in H's rows of the block);
+//  (3) L_rc = P_rc D_c^T in place in Sig, H_c = D_c R_c in place in H, and
+//      L_cc with zeros above the diagonal into Sig.
+// Three work-group barriers per block column; d_F = |u|^2 + 2 sum log diag
+// L_F summed by one wave in a fixed order.
+// (…finest-level-in-prior.cpp:165-256)
+template <typename TG, bool PERSIST = false>
+__global__ __launch_bounds__(256) void k_finest_local_ll(Tree T, CovCtx cc, Pts O, const double* yv,
+                                                         const uint32_t* items, uint32_t nitems, const TG* Gf,
+                                                         const TG* Bf, double* Sig, double* H, double* u,
+                                                         double* dterm, int* bad_flag) {
+    constexpr int LDP = 33;
+    __shared__ double Pd[32 * LDP];  // P_cc, factored in place to L_cc
+    __shared__ double Dd[32 * LDP];  // D_c
+    __shared__ double tv[32];        // y_c - L_{c,<c} u_{<c}
+    __shared__ int bad;
+    const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, k4 = lane >> 4;
+    auto item = [&](const uint32_t k) {
+        const uint32_t F = items[k];
+        const uint64_t ob = T.obs_off[F];
+        const int n = (int)(T.obs_off[F + 1] - ob);
+        const int LRP = T.LRP;
+        const int nb = (n + 31) >> 5, nstrip = (LRP + 31) >> 5;
+        double* LF = Sig + T.sig_off[F];
+        const uint64_t so = T.f_seg_os[F];
+        const int sn = (int)T.f_seg_n[F];
+        const TG* GF = Gf + (uint64_t)LRP * so + (ob - so);  // G(m, i) = GF[m*sn + i]
+        const TG* BO = Bf + ob * (uint64_t)LRP;              // B(i, m) = BO[i*LRP + m]
+        double* HF = H + ob * (uint64_t)LRP;
+        double* uF = u + ob;
+        if (threadIdx.x == 0) bad = 0;
+        double part = 0.0;  // wave 0: 2 log L_ii + u_i^2 of its lanes
+        for (int c = 0; c < nb; c++) {
+            const int c0 = 32 * c, cn = min(32, n - c0);
+            // (1) panel tiles r = c + wave + 4q
+            for (int r = c + wave; r < nb; r += 4) {
+                const int r0 = 32 * r, rn = min(32, n - r0);
+                // V_rc (lower part of V_cc; identity padding) into the tile's
+                // place, in the lane layout of the product below (each lane
+                // reads back only what it wrote)
+#pragma unroll 1
+                for (int e = 0; e < 16; e++) {
+                    const int i = 16 * (e >> 3) + k4 + 4 * (e & 3), j = 16 * ((e >> 2) & 1) + r16;
+                    const bool in = i < rn && j < cn;
+                    double val = (r0 + i == c0 + j) ? 1.0 : 0.0;
+                    if (in && (r > c || j <= i)) {
+                        if (r0 + i == c0 + j)
+                            val = O.v[ob + r0 + i];
+                        else if constexpr (sizeof(TG) == 4)
+                            val = (double)pcovf(cc, O, ob + r0 + i, O, ob + c0 + j);
+                        else
+                            val = pcov(cc, O, ob + r0 + i, O, ob + c0 + j);
+                    }
+                    if (r == c)
+                        Pd[i * LDP + j] = val;
+                    else if (in)
+                        LF[(uint64_t)(r0 + i) * n + c0 + j] = val;
+                }
+                mra_d4 acc[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) acc[q] = mra_d4{0, 0, 0, 0};
+                {
+                    // A(k, i) = G(k, r0 + i), B(k, j) = G(k, c0 + j), K = LRP
+                    const BlockOps<false, TG, TG> o(GF + r0, sn, GF + c0, sn, r16, r16 + 16, r16, r16 + 16, r16 < rn,
+                                                    r16 + 16 < rn, r16 < cn, r16 + 16 < cn, k4);
+                    block_kloop<double, true, true, true, 4>(acc, o, 0, LRP, k4);
+                }
+                if (c0 > 0) {
+                    // A(k, i) = L(r0 + i, k), B(k, j) = L(c0 + j, k), K = c0
+                    const BlockOps<true, double, double, true> o(LF + (uint64_t)r0 * n, n, LF + (uint64_t)c0 * n, n, r16,
+                                                                 r16 + 16, r16, r16 + 16, r16 < rn, r16 + 16 < rn,
+                                                                 r16 < cn, r16 + 16 < cn, k4);
+                    block_kloop<double, true, true, true, 4>(acc, o, 0, c0, k4);
+                }
+                // P = V - acc; element v of tile (ti, tj) is row 16 ti + k4 + 4 v, column 16 tj + r16
+#pragma unroll
+                for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+                    for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+                        for (int v = 0; v < 4; v++) {
+                            const int i = 16 * ti + k4 + 4 * v, j = 16 * tj + r16;
+                            if (r == c) {
+                                if (i < rn && j <= i) Pd[i * LDP + j] -= acc[2 * ti + tj][v];
+                            } else if (i < rn) {
+                                LF[(uint64_t)(r0 + i) * n + c0 + j] -= acc[2 * ti + tj][v];
+                            }
+                        }
+            }
+            if (wave == 3) {
+                // t = y_c - L_{c,<c} u_{<c}: row lane & 31, halves of the sum in the two half-waves
+                const int i = lane & 31;
+                double s = 0.0;
+                if (i < cn) {
+                    const double* Lr = LF + (uint64_t)(c0 + i) * n;
+#pragma unroll 8
+                    for (int q = lane >> 5; q < c0; q += 2) s += Lr[q] * uF[q];
+                }
+                s += __shfl_xor(s, 32);
+                if (lane < 32) tv[lane] = i < cn ? yv[ob + c0 + i] - s : 0.0;
+            }
+            __syncthreads();
+            if (wave == 0) {
+                // (2) L_cc, D_c, u_c = D_c t
+                wave_chol32(Pd, Dd, LDP, &bad);
+                if (lane < cn) part += 2.0 * log(Pd[lane * LDP + lane]);
+                if (lane < 32) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int q = 0; q < 32; q++) s += Dd[lane * LDP + q] * tv[q];
+                    if (lane < cn) {
+                        uF[c0 + lane] = s;
+                        part += s * s;
+                    }
+                }
+            } else {
+                // (2) R_c = B_c - L_{c,<c} H_{<c} for strips wave - 1 + 3q, into H's rows c0 ..
+                for (int s = wave - 1; s < nstrip; s += 3) {
+                    const int m0 = 32 * s, mv = min(32, LRP - m0);
+                    mra_d4 acc[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) acc[q] = mra_d4{0, 0, 0, 0};
+                    if (c0 > 0) {
+                        // A(k, i) = L(c0 + i, k), B(k, m) = H(k, m0 + m), K = c0
+                        const BlockOps<true, double, double> o(LF + (uint64_t)c0 * n, n, HF + m0, LRP, r16, r16 + 16,
+                                                               r16, r16 + 16, r16 < cn, r16 + 16 < cn, r16 < mv,
+                                                               r16 + 16 < mv, k4);
+                        block_kloop<double, true, true, true, 4>(acc, o, 0, c0, k4);
+                    }
+#pragma unroll
+                    for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+                        for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+                            for (int v = 0; v < 4; v++) {
+                                const int i = 16 * ti + k4 + 4 * v, m = 16 * tj + r16;
+                                if (i < cn && m < mv) {
+                                    const uint64_t e = (uint64_t)(c0 + i) * LRP + m0 + m;
+                                    HF[e] = (double)BO[e] - acc[2 * ti + tj][v];
+                                }
+                            }
+                }
+            }
+            __syncthreads();
+            // (3) L_rc = P_rc D_c^T (r > c, tiles r = c + 1 + wave + 4q), in place
+            for (int r = c + 1 + wave; r < nb; r += 4) {
+                const int r0 = 32 * r, rn = min(32, n - r0);
+                mra_d4 acc[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) acc[q] = mra_d4{0, 0, 0, 0};
+                // A(k, i) = P(r0 + i, c0 + k), B(k, j) = D(j, k); cn = 32 here (c < nb - 1)
+                const BlockOps<true, double, double, true> o(LF + (uint64_t)r0 * n + c0, n, Dd, LDP, r16, r16 + 16, r16,
+                                                             r16 + 16, r16 < rn, r16 + 16 < rn, true, true, k4);
+                block_kloop<double, true, true, true, 4>(acc, o, 0, 32, k4);
+#pragma unroll
+                for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+                    for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+                        for (int v = 0; v < 4; v++) {
+                            const int i = 16 * ti + k4 + 4 * v, j = 16 * tj + r16;
+                            if (i < rn) LF[(uint64_t)(r0 + i) * n + c0 + j] = acc[2 * ti + tj][v];
+                        }
+            }
+            if (wave > 0) {
+                // (3) H_c = D_c R_c for strips wave - 1 + 3q, in place
+                for (int s = wave - 1; s < nstrip; s += 3) {
+                    const int m0 = 32 * s, mv = min(32, LRP - m0);
+                    mra_d4 acc[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) acc[q] = mra_d4{0, 0, 0, 0};
+                    // A(k, j) = D(j, k), B(k, m) = R(k, m) = H(c0 + k, m0 + m), K = cn
+                    const BlockOps<true, double, double> o(Dd, LDP, HF + (uint64_t)c0 * LRP + m0, LRP, r16, r16 + 16,
+                                                           r16, r16 + 16, true, true, r16 < mv, r16 + 16 < mv, k4);
+                    block_kloop<double, true, true, true, 4>(acc, o, 0, cn, k4);
+#pragma unroll
+                    for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+                        for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+                            for (int v = 0; v < 4; v++) {
+                                const int j = 16 * ti + k4 + 4 * v, m = 16 * tj + r16;
+                                if (j < cn && m < mv) HF[(uint64_t)(c0 + j) * LRP + m0 + m] = acc[2 * ti + tj][v];
+                            }
+                }
+            }
+            // L_cc and the zeros above it (rows 0 .. c0 + cn of block column c)
+            for (int idx = threadIdx.x; idx < (c0 + cn) * cn; idx += MRA_WG) {
+                const int i = idx / cn, j = idx - i * cn;
+                double val = 0.0;
+                if (i >= c0 && j <= i - c0) val = Pd[(i - c0) * LDP + j];
+                LF[(uint64_t)i * n + c0 + j] = val;
+            }
+            __syncthreads();
+        }
+        if (wave == 0) {
+            part = wave_sum(part);
+            if (lane == 0) {
+                dterm[T.nC + F] = part;
+                if (bad) atomicOr(bad_flag, 1);
+            }
+        }
+        __syncthreads();
+    };
+    if constexpr (PERSIST) {
+        for (uint32_t k = blockIdx.x; k < nitems; k += gridDim.x) item(k);
+    } else if (blockIdx.x < nitems) {
+        item(blockIdx.x);
+    }
+}
+
 // -------------------------------------------------------------- posterior
 // One work group per working region R of level l:
 //   A_{jk} = sum over children (from the stacked finest rows H when l = L-1,

@@ -629,6 +629,78 @@ def test_persistent_finest_launches(case):
     assert rel(out[(capi.PRECISION_FP64, 1, 0)][0], Do) < TOL_D
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["buckets", "large", "r16"])
+def test_local_left_looking(case):
+    """Finest local step of the regions above 64 observations as the
+    left-looking blocked Cholesky (MRA_OPT_LOCAL_LL = 1, the default) against
+    the workspace and unblocked forms (0).  The same arithmetic in another
+    order: FP64 D within 1e-12 relative and predictions within TOL_PRED, the
+    mixed-precision D within 1e-10 (both forms run this step in FP64 from the
+    same G); graph replay equal to eager to the last bit; the oracle's D."""
+    rng = np.random.default_rng({"buckets": 91, "large": 56, "r16": 17}[case])
+    if case == "buckets":
+        x, y = synth.clustered_points(12000, rng, n_clusters=4, sigma=0.02)
+        ux, uy = synth.uniform_points(2000, rng)
+        x, y = np.concatenate([x, ux]), np.concatenate([y, uy])
+        px, py = synth.uniform_points(300, rng)
+        M, r = 6, 64
+    else:
+        x, y = synth.clustered_points(3600 if case == "large" else 3000, rng, n_clusters=2, sigma=0.08)
+        px, py = synth.uniform_points(60, rng)
+        M, r = 3, 32 if case == "large" else 16
+    tree = capi.partition_program(x, y, rng.standard_normal(len(x)), px, py, M=M, J=4, r=r,
+                                  bounds=synth.UNIT_BOUNDS)
+    counts = np.diff(tree["obs_offsets"].astype(np.int64))
+    sizes = [int(((counts > lo) & (counts <= hi)).sum()) for lo, hi in ((64, 128), (128, 256), (256, 1 << 30))]
+    print(case, "regions in (64,128], (128,256], >256:", sizes, "max", int(counts.max()), flush=True)
+    assert sizes[0] and sizes[1] and sizes[2]
+    cov, prm = synth.linear_trend_cov(log_range=(np.log(0.2), 0.8, 0.0)), dict(synth.UNIT_PARAMS)
+    h = capi.Handle(tree)
+    out = {}
+    try:
+        h.set_covariance(cov)
+        h.set_params(**prm)
+        assert h.get_option(capi.OPT_LOCAL_LL) == 1
+        for prec in (capi.PRECISION_FP64, capi.PRECISION_MIXED):
+            h.set_precision(prec)
+            for ll in (1, 0):
+                h.set_option(capi.OPT_LOCAL_LL, ll)
+                assert h.get_option(capi.OPT_LOCAL_LL) == ll
+                for graph in (0, 1):
+                    h.use_graph(graph)
+                    out[(prec, ll, graph)] = h.predict()
+                    out[(prec, ll, graph, "l")] = h.likelihood()
+        h.set_option(capi.OPT_LOCAL_LL, 1)
+    finally:
+        h.close()
+    for prec in (capi.PRECISION_FP64, capi.PRECISION_MIXED):
+        for ll in (0, 1):
+            # graph replay equal to eager; in FP64 the prediction's D is the
+            # likelihood's (mixed likelihoods use FP32 Gram products, predictions not)
+            ref = out[(prec, ll, 0)]
+            assert out[(prec, ll, 1, "l")] == out[(prec, ll, 0, "l")]
+            if prec == capi.PRECISION_FP64:
+                assert out[(prec, ll, 0, "l")] == ref[0]
+            assert out[(prec, ll, 1)][0] == ref[0]
+            np.testing.assert_array_equal(out[(prec, ll, 1)][1], ref[1])
+            np.testing.assert_array_equal(out[(prec, ll, 1)][2], ref[2])
+        a, b = out[(prec, 1, 0)], out[(prec, 0, 0)]
+        tol = 1e-12 if prec == capi.PRECISION_FP64 else 1e-10
+        print(case, prec, "D", a[0], b[0], "rel", rel(a[0], b[0]), flush=True)
+        assert rel(a[0], b[0]) < tol
+        la, lb = out[(prec, 1, 0, "l")], out[(prec, 0, 0, "l")]
+        print(case, prec, "likelihood D", la, lb, "rel", rel(la, lb), flush=True)
+        assert rel(la, lb) < tol
+        if prec == capi.PRECISION_FP64:
+            for q in (1, 2):
+                err = np.max(np.abs(a[q] - b[q])) / np.max(np.abs(b[q]))
+                print(case, "prediction output", q, "max rel diff", err, flush=True)
+                assert err < TOL_PRED
+    Do, _, _ = oracle.evaluate(tree, cov, prm, predict=False)
+    assert rel(out[(capi.PRECISION_FP64, 1, 0)][0], Do) < TOL_D
+
+
 def test_prior_one_wave_per_region_refused():
     """The one-wave-per-region coarse prior is switched off: setting the
     option is refused with MRA_E_UNSUPPORTED and the default form stays."""
