@@ -59,13 +59,20 @@ __device__ __forceinline__ double pdist(const CovCtx& c, double x1, double y1, d
 
 // Cross-covariance of points i (set P) and j (set Q), no nugget
 // (src/evaluate_covariance.cpp:207-223).
-__device__ __forceinline__ double pcov(const CovCtx& c, const Pts& P, uint64_t i, const Pts& Q, uint64_t j) {
-    double d = pdist(c, P.x[i], P.y[i], P.z[i], Q.x[j], Q.y[j], Q.z[j]);
+// covariance of two points given by their coordinates, variance field s and
+// range field b (the arithmetic of pcov, which reads them from Pts)
+__device__ __forceinline__ double pcov_v(const CovCtx& c, double x1, double y1, double z1, double s1, double b1,
+                                         double x2, double y2, double z2, double s2, double b2) {
+    double d = pdist(c, x1, y1, z1, x2, y2, z2);
     if (c.stationary) return c.sill * exp_neg(-d / c.range);
-    double b1 = P.b[i], b2 = Q.b[j];
     double bms = (b1 * b1 + b2 * b2) * 0.5;
     double ratio = (b1 * b2) / bms;
-    return (P.s[i] * Q.s[j]) * (ratio * sqrt(ratio)) * exp_neg(-d / sqrt(bms));
+    return (s1 * s2) * (ratio * sqrt(ratio)) * exp_neg(-d / sqrt(bms));
+}
+
+__device__ __forceinline__ double pcov(const CovCtx& c, const Pts& P, uint64_t i, const Pts& Q, uint64_t j) {
+    if (c.stationary) return pcov_v(c, P.x[i], P.y[i], P.z[i], 0.0, 0.0, Q.x[j], Q.y[j], Q.z[j], 0.0, 0.0);
+    return pcov_v(c, P.x[i], P.y[i], P.z[i], P.s[i], P.b[i], Q.x[j], Q.y[j], Q.z[j], Q.s[j], Q.b[j]);
 }
 
 // ---------------------------------------------------- covariance tiles
@@ -283,9 +290,11 @@ __device__ __forceinline__ void wave_cov64(mra_d4 (&acc)[16], const CovCtx& cc, 
     }
 }
 
-__device__ __forceinline__ float pcovf(const CovCtx& c, const Pts& P, uint64_t i, const Pts& Q, uint64_t j) {
-    float x1 = (float)P.x[i], y1 = (float)P.y[i], z1 = (float)P.z[i];
-    float x2 = (float)Q.x[j], y2 = (float)Q.y[j], z2 = (float)Q.z[j];
+// FP32 counterpart of pcov_v (inputs rounded to float)
+__device__ __forceinline__ float pcovf_v(const CovCtx& c, double x1d, double y1d, double z1d, double s1, double b1d,
+                                         double x2d, double y2d, double z2d, double s2, double b2d) {
+    float x1 = (float)x1d, y1 = (float)y1d, z1 = (float)z1d;
+    float x2 = (float)x2d, y2 = (float)y2d, z2 = (float)z2d;
     float d;
     if (c.metric == 1) {
         float ang = x1 * x2 + y1 * y2 * cosf(z1 - z2);
@@ -297,10 +306,15 @@ __device__ __forceinline__ float pcovf(const CovCtx& c, const Pts& P, uint64_t i
         if (c.metric == 0) d *= 6371.0f;
     }
     if (c.stationary) return (float)c.sill * expf(-d / (float)c.range);
-    float b1 = (float)P.b[i], b2 = (float)Q.b[j];
+    float b1 = (float)b1d, b2 = (float)b2d;
     float bms = (b1 * b1 + b2 * b2) * 0.5f;
     float ratio = (b1 * b2) / bms;
-    return ((float)P.s[i] * (float)Q.s[j]) * (ratio * sqrtf(ratio)) * expf(-d / sqrtf(bms));
+    return ((float)s1 * (float)s2) * (ratio * sqrtf(ratio)) * expf(-d / sqrtf(bms));
+}
+
+__device__ __forceinline__ float pcovf(const CovCtx& c, const Pts& P, uint64_t i, const Pts& Q, uint64_t j) {
+    if (c.stationary) return pcovf_v(c, P.x[i], P.y[i], P.z[i], 0.0, 0.0, Q.x[j], Q.y[j], Q.z[j], 0.0, 0.0);
+    return pcovf_v(c, P.x[i], P.y[i], P.z[i], P.s[i], P.b[i], Q.x[j], Q.y[j], Q.z[j], Q.s[j], Q.b[j]);
 }
 
 // ----------------------------------------------------------- MFMA GEMM TN

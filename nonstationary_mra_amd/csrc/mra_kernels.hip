@@ -1001,7 +1001,7 @@ __global__ __launch_bounds__(256) void k_finest_local_w16(Tree T, CovCtx cc, Pts
 // L_F summed by one wave in a fixed order.
 // (…finest-level-in-prior.cpp:165-256)
 template <typename TG, bool PERSIST = false>
-__global__ __launch_bounds__(256) void k_finest_local_ll(Tree T, CovCtx cc, Pts O, const double* yv,
+__global__ __launch_bounds__(256, 2) void k_finest_local_ll(Tree T, CovCtx cc, Pts O, const double* yv,
                                                          const uint32_t* items, uint32_t nitems, const TG* Gf,
                                                          const TG* Bf, double* Sig, double* H, double* u,
                                                          double* dterm, int* bad_flag) {
@@ -1009,6 +1009,7 @@ __global__ __launch_bounds__(256) void k_finest_local_ll(Tree T, CovCtx cc, Pts 
     __shared__ double Pd[32 * LDP];  // P_cc, factored in place to L_cc
     __shared__ double Dd[32 * LDP];  // D_c
     __shared__ double tv[32];        // y_c - L_{c,<c} u_{<c}
+    __shared__ double pw[4][5][64];  // per wave: x, y, z, s, b of the tile's rows (0-31) and columns (32-63)
     __shared__ int bad;
     const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, k4 = lane >> 4;
     auto item = [&](const uint32_t k) {
@@ -1031,6 +1032,21 @@ __global__ __launch_bounds__(256) void k_finest_local_ll(Tree T, CovCtx cc, Pts 
             // (1) panel tiles r = c + wave + 4q
             for (int r = c + wave; r < nb; r += 4) {
                 const int r0 = 32 * r, rn = min(32, n - r0);
+                // the points of the tile's rows and columns, staged in the
+                // wave's LDS (one round of loads for the 16 covariances below)
+                wave_lds_fence();
+                {
+                    const int pi = lane < 32 ? r0 + lane : c0 + lane - 32;
+                    if (pi < n) {
+                        const uint64_t g = ob + pi;
+                        pw[wave][0][lane] = O.x[g];
+                        pw[wave][1][lane] = O.y[g];
+                        pw[wave][2][lane] = O.z[g];
+                        pw[wave][3][lane] = O.s[g];
+                        pw[wave][4][lane] = O.b[g];
+                    }
+                }
+                wave_lds_fence();
                 // V_rc (lower part of V_cc; identity padding) into the tile's
                 // place, in the lane layout of the product below (each lane
                 // reads back only what it wrote)
@@ -1040,12 +1056,15 @@ __global__ __launch_bounds__(256) void k_finest_local_ll(Tree T, CovCtx cc, Pts 
                     const bool in = i < rn && j < cn;
                     double val = (r0 + i == c0 + j) ? 1.0 : 0.0;
                     if (in && (r > c || j <= i)) {
+                        const double(*q)[64] = pw[wave];
                         if (r0 + i == c0 + j)
                             val = O.v[ob + r0 + i];
                         else if constexpr (sizeof(TG) == 4)
-                            val = (double)pcovf(cc, O, ob + r0 + i, O, ob + c0 + j);
+                            val = (double)pcovf_v(cc, q[0][i], q[1][i], q[2][i], q[3][i], q[4][i], q[0][32 + j],
+                                                  q[1][32 + j], q[2][32 + j], q[3][32 + j], q[4][32 + j]);
                         else
-                            val = pcov(cc, O, ob + r0 + i, O, ob + c0 + j);
+                            val = pcov_v(cc, q[0][i], q[1][i], q[2][i], q[3][i], q[4][i], q[0][32 + j], q[1][32 + j],
+                                         q[2][32 + j], q[3][32 + j], q[4][32 + j]);
                     }
                     if (r == c)
                         Pd[i * LDP + j] = val;
