@@ -213,13 +213,16 @@ int mra_use_graph(mra_handle* h, int enable);
  * tile in the registers of the FP32 matrix instructions, no work-group
  * barrier); 0 one work group per tile (same values up to FP32 rounding).
  * Persistent launches (MRA_OPT_PERSISTENT) use the work-group form.
- * MRA_OPT_LOCAL_LL: 1 (default) runs the finest local step of regions of
- * more than 64 observations as a left-looking Cholesky on 32-wide block
- * columns (k_finest_local_ll: panel tiles formed from G_F and the final rows
- * of L_F, no Sigma_F workspace, H and u by forward substitution in the same
- * pass); 0 selects the earlier forms (a padded Sigma_F workspace with 16 x 16
- * blocks up to 256 observations, an unblocked factorisation above).  Same
- * values up to rounding. */
+ * MRA_OPT_LOCAL_LL: the finest local step of regions of more than 64
+ * observations as a left-looking Cholesky on 32-wide block columns
+ * (k_finest_local_ll: panel tiles formed from G_F and the final rows of L_F,
+ * no Sigma_F workspace, H and u by forward substitution in the same pass).
+ * 1 (default): for every such region except those of 65-128 observations
+ * when more than 256 ancestor knot columns enter their Gram product (those
+ * stay on the workspace kernel, faster there); 2: for every such region;
+ * 0: the earlier forms (a padded Sigma_F workspace with 16 x 16 blocks up to
+ * 256 observations, an unblocked factorisation above).  Same values up to
+ * rounding. */
 enum {
     MRA_OPT_FACTORED_POSTERIOR = 1,
     MRA_OPT_GRAM_KERNEL = 2,

@@ -425,6 +425,12 @@ static int build_structure(mra_handle* h, const mra_tree_desc* t) {
     return MRA_OK;
 }
 
+// Regions of 65-128 observations (bucket 4) on the left-looking kernel: with
+// MRA_OPT_LOCAL_LL = 2, or = 1 when at most 256 ancestor knot columns make
+// their Gram product (with more, the workspace kernel's 64 x 64 products are
+// faster for these short regions)
+static bool ll_bucket4(const mra_handle* h) { return h->localLL == 2 || (h->localLL == 1 && h->LRP <= 256); }
+
 // Work lists of one evaluation (item arrays per launch), for the current
 // work-list shape; replaces the previous ones.
 template <typename T>
@@ -508,7 +514,7 @@ static int build_worklists(mra_handle* h) {
         h->dWsOffBig = nullptr;
         h->dWsOffBig3 = nullptr;
         h->dWsBig = nullptr;
-        if (k && !h->localLL) {
+        if (k && (!h->localLL || (!ll_bucket4(h) && !b[4].empty()))) {
             RC_TRY(upload_wl(h, &h->dWsOffBig, wo.data(), std::max<size_t>(b[4].size(), 1)));
             RC_TRY(upload_wl(h, &h->dWsOffBig3, wo.data() + b[4].size(), std::max<size_t>(b[3].size(), 1)));
             RC_TRY(upload_wl(h, &h->dWsBig, (const double*)nullptr, wo.back()));
@@ -985,7 +991,11 @@ extern "C" int mra_set_option(mra_handle* h, int option, int value) {
         return MRA_OK;
     }
     if (option == MRA_OPT_LOCAL_LL) {
-        h->localLL = value ? 1 : 0;
+        if (value < 0 || value > 2) {
+            set_err(h, "MRA_OPT_LOCAL_LL must be 0, 1 or 2");
+            return MRA_E_ARG;
+        }
+        h->localLL = value;
         RC_TRY(build_worklists(h));  // the earlier forms need their workspace
         graph_invalidate(h);
         return MRA_OK;
@@ -1229,9 +1239,13 @@ static void launch_finest_local(mra_handle* h, const Tree& T, const CovCtx& cc, 
                 launch_units(h, &k_finest_local_ll<TG>, &k_finest_local_ll<TG, true>, h->nItemsFB[q], 1, s1, T, cc, O,
                              h->dY, h->dItemsFB[q], h->nItemsFB[q], (const TG*)Gf, (const TG*)Bf, h->dSig, h->dH,
                              h->dU, h->dDterm, h->dBad);
-        if (h->nItemsFB[4])
+        if (h->nItemsFB[4] && ll_bucket4(h))
             launch_units(h, &k_finest_local_ll<TG>, &k_finest_local_ll<TG, true>, h->nItemsFB[4], 1, s1, T, cc, O,
                          h->dY, h->dItemsFB[4], h->nItemsFB[4], (const TG*)Gf, (const TG*)Bf, h->dSig, h->dH, h->dU,
+                         h->dDterm, h->dBad);
+        else if (h->nItemsFB[4])
+            launch_units(h, &k_finest_local_big<TG>, &k_finest_local_big<TG, true>, h->nItemsFB[4], 1, s1, T, cc, O,
+                         h->dY, h->dItemsFB[4], h->nItemsFB[4], h->dWsOffBig, h->dWsBig, Gf, Bf, h->dSig, h->dH, h->dU,
                          h->dDterm, h->dBad);
     } else {
         if (h->nItemsFB[5] && sizeof(TG) == 4 && h->nItemsXBig && h->dGf64)

@@ -633,8 +633,9 @@ def test_persistent_finest_launches(case):
 @pytest.mark.parametrize("case", ["buckets", "large", "r16"])
 def test_local_left_looking(case):
     """Finest local step of the regions above 64 observations as the
-    left-looking blocked Cholesky (MRA_OPT_LOCAL_LL = 1, the default) against
-    the workspace and unblocked forms (0).  The same arithmetic in another
+    left-looking blocked Cholesky (MRA_OPT_LOCAL_LL = 2: every such region;
+    1, the default: all but the 65-128-observation regions with more than 256
+    ancestor knot columns) against the workspace and unblocked forms (0).  The same arithmetic in another
     order: FP64 D within 1e-12 relative and predictions within TOL_PRED, the
     mixed-precision D within 1e-10 (both forms run this step in FP64 from the
     same G); graph replay equal to eager to the last bit; the oracle's D."""
@@ -664,7 +665,7 @@ def test_local_left_looking(case):
         assert h.get_option(capi.OPT_LOCAL_LL) == 1
         for prec in (capi.PRECISION_FP64, capi.PRECISION_MIXED):
             h.set_precision(prec)
-            for ll in (1, 0):
+            for ll in (2, 1, 0):
                 h.set_option(capi.OPT_LOCAL_LL, ll)
                 assert h.get_option(capi.OPT_LOCAL_LL) == ll
                 for graph in (0, 1):
@@ -675,7 +676,7 @@ def test_local_left_looking(case):
     finally:
         h.close()
     for prec in (capi.PRECISION_FP64, capi.PRECISION_MIXED):
-        for ll in (0, 1):
+        for ll in (0, 1, 2):
             # graph replay equal to eager; in FP64 the prediction's D is the
             # likelihood's (mixed likelihoods use FP32 Gram products, predictions not)
             ref = out[(prec, ll, 0)]
@@ -685,18 +686,19 @@ def test_local_left_looking(case):
             assert out[(prec, ll, 1)][0] == ref[0]
             np.testing.assert_array_equal(out[(prec, ll, 1)][1], ref[1])
             np.testing.assert_array_equal(out[(prec, ll, 1)][2], ref[2])
-        a, b = out[(prec, 1, 0)], out[(prec, 0, 0)]
-        tol = 1e-12 if prec == capi.PRECISION_FP64 else 1e-10
-        print(case, prec, "D", a[0], b[0], "rel", rel(a[0], b[0]), flush=True)
-        assert rel(a[0], b[0]) < tol
-        la, lb = out[(prec, 1, 0, "l")], out[(prec, 0, 0, "l")]
-        print(case, prec, "likelihood D", la, lb, "rel", rel(la, lb), flush=True)
-        assert rel(la, lb) < tol
-        if prec == capi.PRECISION_FP64:
-            for q in (1, 2):
-                err = np.max(np.abs(a[q] - b[q])) / np.max(np.abs(b[q]))
-                print(case, "prediction output", q, "max rel diff", err, flush=True)
-                assert err < TOL_PRED
+        for ll in (1, 2):
+            a, b = out[(prec, ll, 0)], out[(prec, 0, 0)]
+            tol = 1e-12 if prec == capi.PRECISION_FP64 else 1e-10
+            print(case, prec, ll, "D", a[0], b[0], "rel", rel(a[0], b[0]), flush=True)
+            assert rel(a[0], b[0]) < tol
+            la, lb = out[(prec, ll, 0, "l")], out[(prec, 0, 0, "l")]
+            print(case, prec, ll, "likelihood D", la, lb, "rel", rel(la, lb), flush=True)
+            assert rel(la, lb) < tol
+            if prec == capi.PRECISION_FP64:
+                for q in (1, 2):
+                    err = np.max(np.abs(a[q] - b[q])) / np.max(np.abs(b[q]))
+                    print(case, ll, "prediction output", q, "max rel diff", err, flush=True)
+                    assert err < TOL_PRED
     Do, _, _ = oracle.evaluate(tree, cov, prm, predict=False)
     assert rel(out[(capi.PRECISION_FP64, 1, 0)][0], Do) < TOL_D
 
