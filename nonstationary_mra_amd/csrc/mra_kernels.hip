@@ -1279,9 +1279,10 @@ struct PostArgs {
     uint64_t start_p;
 };
 
+// kred: LDS for the K-split products of RP <= 32 (3*4*256 doubles)
 template <int RP, bool FROM_OBS>
 __device__ __forceinline__ void post_add_A(const Tree& T, const PostArgs& a, uint64_t R, int j, int k, double* C,
-                                           int ld, bool transpose) {
+                                           int ld, bool transpose, double* kred) {
     // C += A_{jk} (or its transpose)
     if (FROM_OBS) {
         const uint64_t loc = R - a.start_l;
@@ -1290,7 +1291,13 @@ __device__ __forceinline__ void post_add_A(const Tree& T, const PostArgs& a, uin
             // (a region may read no rows: all its children sent dense messages)
             const int ns = (int)(oe - os);
             const double* X = a.X + os * a.ldx;
-            if (!transpose)
+            if constexpr (RP <= 32) {
+                // an RP x RP output is one 2 x 2 tile block: the rows are split
+                // over the 4 waves (each wave's products are then four per
+                // operand load, and its load latency is paid on a quarter of the rows)
+                const int p = transpose ? k : j, q = transpose ? j : k;
+                wg_gemm_tn_ksplit(C, ld, RP, RP, ns, 1.0, X + p * RP, a.ldx, RP, X + q * RP, a.ldx, RP, kred);
+            } else if (!transpose)
                 wg_gemm_tn<false, true, double, double, GEMM_FULL>(C, ld, RP, RP, ns, 1.0, X + j * RP, a.ldx, RP,
                                                                   X + k * RP, a.ldx, RP);
             else
@@ -1341,13 +1348,34 @@ __device__ __forceinline__ void post_add_A(const Tree& T, const PostArgs& a, uin
     }
 }
 
+// Rows part of w_j at a level that reads stacked rows: rw[q] = sum over the
+// region's rows o of X[o][j RP + q] z[o] (q < RP), by RP columns x MRA_WG/RP
+// row phases whose partial sums are added in phase order (red: MRA_WG
+// doubles).  All threads of the work group call it.
+template <int RP>
+__device__ __forceinline__ void wg_rows_w(const PostArgs& a, uint64_t R, int j, double* rw, double* red) {
+    constexpr int PH = MRA_WG / RP;
+    const uint64_t loc = R - a.start_l;
+    const uint64_t os = a.row_s[loc], oe = a.row_e[loc];
+    const int q = (int)threadIdx.x % RP, p = (int)threadIdx.x / RP;
+    double s = 0.0;
+    for (uint64_t o = os + p; o < oe; o += PH) s += a.X[o * a.ldx + j * RP + q] * a.z[o];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if ((int)threadIdx.x < RP) {
+        double t = 0.0;
+        for (int ph = 0; ph < PH; ph++) t += red[ph * RP + threadIdx.x];
+        rw[threadIdx.x] = t;
+    }
+    __syncthreads();
+}
+
+// w_j[q]; at a level that reads stacked rows, rows = the rows part (wg_rows_w)
 template <int RP, bool FROM_OBS>
-__device__ __forceinline__ double post_w(const Tree& T, const PostArgs& a, uint64_t R, int j, int q) {
+__device__ __forceinline__ double post_w(const Tree& T, const PostArgs& a, uint64_t R, int j, int q, double rows) {
     double s = 0.0;
     if (FROM_OBS) {
-        const uint64_t loc = R - a.start_l;
-        const uint64_t os = a.row_s[loc], oe = a.row_e[loc];
-        for (uint64_t o = os; o < oe; o++) s += a.X[o * a.ldx + j * RP + q] * a.z[o];
+        s = rows;
         if (a.cslot) {
             const int lc = a.l + 1;
             for (int c = 0; c < T.J; c++) {
@@ -1378,6 +1406,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     __shared__ double wl[RP];
     __shared__ double red[MRA_WG];
     __shared__ double dbuf[RP >= 48 ? 1 : RP * 16];
+    __shared__ double kred[FROM_OBS && RP <= 32 ? 3 * 4 * 256 : 1];
     __shared__ int bad;
     const int l = a.l;
     const uint64_t R = a.start_l + a.items[blockIdx.x];
@@ -1399,7 +1428,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     wg_gemm_tn<false, true, double, double, GEMM_FULL, false, GEMM_TRI_A | GEMM_TRI_B | GEMM_TRI_C>(
         Km, LD, RP, RP, RP, 1.0, LR, RP, RP, LR, RP, RP);
     __syncthreads();
-    post_add_A<RP, FROM_OBS>(T, a, R, l, l, Km, LD, false);
+    post_add_A<RP, FROM_OBS>(T, a, R, l, l, Km, LD, false, kred);
     __syncthreads();
     double lp = 0.0;
     for (int i = threadIdx.x; i < r; i += MRA_WG) lp += log(LR[i * RP + i]);
@@ -1416,7 +1445,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const int k = idx / RP, i = idx % RP;
         a.LpiT[R * RP * RP + idx] = (i >= k) ? Km[i * LD + k] : 0.0;
     }
-    for (int q = threadIdx.x; q < RP; q += MRA_WG) wl[q] = (q < r) ? post_w<RP, FROM_OBS>(T, a, R, l, q) : 0.0;
+    if constexpr (FROM_OBS) wg_rows_w<RP>(a, R, l, wl, red);
+    for (int q = threadIdx.x; q < RP; q += MRA_WG)
+        wl[q] = (q < r) ? post_w<RP, FROM_OBS>(T, a, R, l, q, FROM_OBS ? wl[q] : 0.0) : 0.0;
     __syncthreads();
     for (int i = threadIdx.x; i < RP; i += MRA_WG) {
         double s = 0.0;
@@ -1439,17 +1470,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         for (int j = 0; j < l; j++) {
             for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) Km[(idx / RP) * LD + idx % RP] = 0.0;
             __syncthreads();
-            post_add_A<RP, FROM_OBS>(T, a, R, j, l, Km, LD, true);
+            post_add_A<RP, FROM_OBS>(T, a, R, j, l, Km, LD, true, kred);
             __syncthreads();
             // P_j stored straight to global memory (row-major RP x RP)
             double* Pj = PR + (uint64_t)j * RP * RP;
             wg_gemm_tn<false, false, double, double, GEMM_FULL, false, GEMM_TRI_A>(Pj, RP, RP, RP, RP, 1.0, LpiTR, RP,
                                                                                   RP, Km, LD, RP);
             __syncthreads();
-            // w~_j = w_j - P_j^T v
+            // w~_j = w_j - P_j^T v (wl, free after v, holds the rows part of w_j)
+            if (FROM_OBS && a.write_msgs) wg_rows_w<RP>(a, R, j, wl, red);
             if (a.write_msgs)
                 for (int q = threadIdx.x; q < RP; q += MRA_WG) {
-                    double wj = (q < r) ? post_w<RP, FROM_OBS>(T, a, R, j, q) : 0.0;
+                    double wj = (q < r) ? post_w<RP, FROM_OBS>(T, a, R, j, q, FROM_OBS ? wl[q] : 0.0) : 0.0;
                     double s = 0.0;
                     for (int m = 0; m < RP; m++) s += Pj[m * RP + q] * vv[m];
                     a.wo[msg * (uint64_t)l * RP + j * RP + q] = wj - s;
@@ -1478,8 +1510,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 for (int k = j; k < l; k++) {
                     const int sl = j * l - j * (j - 1) / 2 + (k - j);
                     double* out = a.Ao + (msg * nsl + sl) * (uint64_t)RP * RP;
-                    wg_gemm2_store(out, RP, RP, RP, ns, 1.0, X + j * RP, a.ldx, X + k * RP, a.ldx, RP, -1.0,
-                                   PR + (uint64_t)j * RP * RP, RP, PR + (uint64_t)k * RP * RP, RP);
+                    if constexpr (RP <= 32) {
+                        // X_j^T X_k with the rows split over the waves, then - P_j^T P_k
+                        for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) Km[(idx / RP) * LD + idx % RP] = 0.0;
+                        __syncthreads();
+                        if (ns > 0)
+                            wg_gemm_tn_ksplit(Km, LD, RP, RP, ns, 1.0, X + j * RP, a.ldx, RP, X + k * RP, a.ldx, RP,
+                                              kred);
+                        wg_gemm_tn(Km, LD, RP, RP, RP, -1.0, PR + (uint64_t)j * RP * RP, RP, RP,
+                                   PR + (uint64_t)k * RP * RP, RP, RP);
+                        __syncthreads();
+                        for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) out[idx] = Km[(idx / RP) * LD + idx % RP];
+                        __syncthreads();
+                    } else {
+                        wg_gemm2_store(out, RP, RP, RP, ns, 1.0, X + j * RP, a.ldx, X + k * RP, a.ldx, RP, -1.0,
+                                       PR + (uint64_t)j * RP * RP, RP, PR + (uint64_t)k * RP * RP, RP);
+                    }
                 }
             if (a.cslot) {
                 __syncthreads();
@@ -1504,7 +1550,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             for (int k = j; k < l; k++) {
                 for (int idx = threadIdx.x; idx < RP * RP; idx += MRA_WG) Km[(idx / RP) * LD + idx % RP] = 0.0;
                 __syncthreads();
-                post_add_A<RP, FROM_OBS>(T, a, R, j, k, Km, LD, false);
+                post_add_A<RP, FROM_OBS>(T, a, R, j, k, Km, LD, false, kred);
                 __syncthreads();
                 wg_gemm_tn(Km, LD, RP, RP, RP, -1.0, PR + (uint64_t)j * RP * RP, RP, RP, PR + (uint64_t)k * RP * RP,
                            RP, RP);
