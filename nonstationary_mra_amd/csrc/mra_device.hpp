@@ -1246,9 +1246,6 @@ __device__ __forceinline__ mra_d4 mfma16(mra_d4 acc, const double* X, int ldx, c
 
 // One wave: factor the 16 x 16 diagonal block P in place (lower triangle)
 // and write its inverse (lower, zeros above) to Dp (row stride ldd).
-// LEAN: the inverse column is solved in place in Dp one row at a time (few
-// registers; each row waits on the previous one) instead of in registers.
-template <bool LEAN = false>
 __device__ __forceinline__ void wave_chol16(double* P, int lda, double* Dp, int ldd, int* bad) {
     const int lane = threadIdx.x & 63, i = lane & 15, cg = lane >> 4;
     // unscaled right-looking elimination of the lower triangle:
@@ -1297,18 +1294,7 @@ __device__ __forceinline__ void wave_chol16(double* P, int lda, double* Dp, int 
     // inverse by forward substitution, one column per lane (lanes 0..15):
     // the column is kept in registers and the factor entries are read as
     // LDS broadcasts that do not wait on earlier results
-    if (LEAN) {
-        if (lane < 16) {
-            const int c = lane;
-#pragma unroll 1
-            for (int ii = 0; ii < 16; ii++) {
-                double sacc = ii == c ? 1.0 : 0.0;
-#pragma unroll 4
-                for (int kk = 0; kk < ii; kk++) sacc -= P[ii * lda + kk] * Dp[kk * ldd + c];
-                Dp[ii * ldd + c] = sacc / P[ii * lda + ii];
-            }
-        }
-    } else if (lane < 16) {
+    if (lane < 16) {
         const int c = lane;
         double x[16];
 #pragma unroll
@@ -1328,14 +1314,13 @@ __device__ __forceinline__ void wave_chol16(double* P, int lda, double* Dp, int 
 // inverse D = L^{-1} (lower, zeros above, row stride ld): the two 16 x 16
 // diagonal blocks by wave_chol16, L_10 = P_10 D_00^T, P_11 -= L_10 L_10^T and
 // D_10 = -D_11 (L_10 D_00).  Wave-synchronous (no work-group barrier).
-template <bool LEAN = true>
 __device__ __forceinline__ void wave_chol32(double* P, double* D, int ld, int* bad) {
     const int lane = threadIdx.x & 63, r16 = lane & 15, k4 = lane >> 4;
     double* P10 = P + 16 * ld;
     double* P11 = P10 + 16;
     double* D10 = D + 16 * ld;
     double* D11 = D10 + 16;
-    wave_chol16<LEAN>(P, ld, D, ld, bad);
+    wave_chol16(P, ld, D, ld, bad);
     mra_d4 acc = {0, 0, 0, 0};
     acc = mfma16<true>(acc, P10, ld, D, ld);
     wave_lds_fence();
@@ -1347,7 +1332,7 @@ __device__ __forceinline__ void wave_chol32(double* P, double* D, int ld, int* b
 #pragma unroll
     for (int v = 0; v < 4; v++) P11[(k4 + 4 * v) * ld + r16] -= acc[v];
     wave_lds_fence();
-    wave_chol16<LEAN>(P11, ld, D11, ld, bad);
+    wave_chol16(P11, ld, D11, ld, bad);
     // T = L_10 D_00 in the D_10 block, then D_10 = -D_11 T
     acc = mra_d4{0, 0, 0, 0};
     acc = mfma16<false>(acc, P10, ld, D, ld);
