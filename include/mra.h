@@ -209,7 +209,7 @@ int mra_use_graph(mra_handle* h, int enable);
  * refused with MRA_E_UNSUPPORTED: a GPU run with it stopped making progress
  * and the cause is not established.
  * MRA_OPT_FINEST_WAVE: 1 (default) runs the mixed-precision finest
- * conditioning of 64-knot regions one wave per tile (k_finest_cross_w: the
+ * conditioning of 32- and 64-knot regions one wave per tile (k_finest_cross_w: the
  * tile in the registers of the FP32 matrix instructions, no work-group
  * barrier); 0 one work group per tile (same values up to FP32 rounding).
  * Persistent launches (MRA_OPT_PERSISTENT) use the work-group form.

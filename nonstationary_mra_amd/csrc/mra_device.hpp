@@ -823,34 +823,36 @@ __device__ __forceinline__ void wave_mm64_f32(mra_f4 (&f)[16], int K, const TI* 
     }
 }
 
-// ---- one wave per finest conditioning tile (FP32, L32 layout): 64 knot
-// rows x NTT column tiles of 16 observations, acc[ti * NTT + tj].
+// ---- one wave per finest conditioning tile (FP32, L32 layout): RP = 16 KT
+// knot rows (KT = 2 or 4 row tiles) x NTT column tiles of 16 observations,
+// acc[ti * NTT + tj].
 
 // acc -= A^T B over K rows (K a multiple of 4 U) on FP32 matrix
-// instructions: A(k, i) = A[k * 64 + i], B(k, c) = B[k * ldb + c] for the
+// instructions: A(k, i) = A[k * 16 KT + i], B(k, c) = B[k * ldb + c] for the
 // columns c = 16 tj + t%16; a column with cv[tj] false is read at column 0
 // and contributes zero.  Operands come in stages of U k steps (4 U rows);
 // the next stage is loaded before the products of the current one are
 // issued.
-template <int NTT, int U = 4>
-__device__ __forceinline__ void wave_cond_f32(mra_f4 (&acc)[4 * NTT], int K, const float* A, const float* B,
+template <int NTT, int U = 4, int KT = 4>
+__device__ __forceinline__ void wave_cond_f32(mra_f4 (&acc)[KT * NTT], int K, const float* A, const float* B,
                                               int64_t ldb, const bool (&cv)[NTT]) {
+    constexpr int RP = 16 * KT;
     const int lane = threadIdx.x & 63;
     const int r16 = lane & 15, k4 = lane >> 4;
-    const uint32_t oa = (uint32_t)((k4 * 64 + r16) * sizeof(float));
+    const uint32_t oa = (uint32_t)((k4 * RP + r16) * sizeof(float));
     uint32_t ob[NTT];
 #pragma unroll
     for (int tj = 0; tj < NTT; tj++) ob[tj] = (uint32_t)((k4 * ldb + (cv[tj] ? 16 * tj + r16 : 0)) * sizeof(float));
     struct Stage {
-        float a[U][4], b[U][NTT];
+        float a[U][KT], b[U][NTT];
     };
     auto load = [&](int k0, Stage& p) {
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const char* Ak = (const char*)(A + (int64_t)(k0 + 4 * u) * 64);
+            const char* Ak = (const char*)(A + (int64_t)(k0 + 4 * u) * RP);
             const char* Bk = (const char*)(B + (int64_t)(k0 + 4 * u) * ldb);
 #pragma unroll
-            for (int t = 0; t < 4; t++) p.a[u][t] = *(const float*)(Ak + oa + 16 * t * sizeof(float));
+            for (int t = 0; t < KT; t++) p.a[u][t] = *(const float*)(Ak + oa + 16 * t * sizeof(float));
 #pragma unroll
             for (int tj = 0; tj < NTT; tj++) p.b[u][tj] = *(const float*)(Bk + ob[tj]);
         }
@@ -862,7 +864,7 @@ __device__ __forceinline__ void wave_cond_f32(mra_f4 (&acc)[4 * NTT], int K, con
             for (int tj = 0; tj < NTT; tj++) {
                 const float b = cv[tj] ? -p.b[u][tj] : 0.0f;
 #pragma unroll
-                for (int ti = 0; ti < 4; ti++)
+                for (int ti = 0; ti < KT; ti++)
                     acc[ti * NTT + tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(p.a[u][ti], b, acc[ti * NTT + tj], 0, 0, 0);
             }
     };
@@ -878,24 +880,25 @@ __device__ __forceinline__ void wave_cond_f32(mra_f4 (&acc)[4 * NTT], int K, con
     mm(cur);
 }
 
-// acc = L acc with L lower triangular, given transposed in FP64
-// (L[i][k] = Lt[k * 64 + i], rounded to float): the accumulators are the B
-// operand, k step (ti, v) of column tile tj being element v of
+// acc = L acc with L (16 KT x 16 KT) lower triangular, given transposed in
+// FP64 (L[i][k] = Lt[k * 16 KT + i], rounded to float): the accumulators are
+// the B operand, k step (ti, v) of column tile tj being element v of
 // acc[ti * NTT + tj] (rows 16 ti + 4 (t/16) + v, so A is read at those k);
 // output row tiles from the last to the first, each overwriting its rows
 // once no later output reads them.
-template <int NTT>
-__device__ __forceinline__ void wave_trmm_l32(mra_f4 (&acc)[4 * NTT], const double* Lt) {
+template <int NTT, int KT = 4>
+__device__ __forceinline__ void wave_trmm_l32(mra_f4 (&acc)[KT * NTT], const double* Lt) {
+    constexpr int RP = 16 * KT;
     const int lane = threadIdx.x & 63;
     const int r16 = lane & 15, k4 = lane >> 4;
-    const double* Ll = Lt + 4 * k4 * 64 + r16;
+    const double* Ll = Lt + 4 * k4 * RP + r16;
 #pragma unroll
-    for (int to = 3; to >= 0; to--) {
-        float a[16];
+    for (int to = KT - 1; to >= 0; to--) {
+        float a[4 * KT];
 #pragma unroll
         for (int ti = 0; ti <= to; ti++)
 #pragma unroll
-            for (int v = 0; v < 4; v++) a[4 * ti + v] = (float)Ll[(16 * ti + v) * 64 + 16 * to];
+            for (int v = 0; v < 4; v++) a[4 * ti + v] = (float)Ll[(16 * ti + v) * RP + 16 * to];
         mra_f4 o[NTT];
 #pragma unroll
         for (int tj = 0; tj < NTT; tj++) o[tj] = mra_f4{0, 0, 0, 0};

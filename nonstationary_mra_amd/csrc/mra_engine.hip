@@ -1287,7 +1287,7 @@ template <int RP, typename TG>
 static void launch_finest_cross(mra_handle* h, const Tree& T, const CovCtx& cc, const Pts& K, const Pts& O) {
     const Fork fk(h, 1);
     const hipStream_t st = fk.stream(0);
-    if constexpr (RP == 64 && sizeof(TG) == 4) {
+    if constexpr ((RP == 64 || RP == 32) && sizeof(TG) == 4) {
         if (h->finestWave && !h->persistent) {
             // one wave per tile, four tiles per work group; the float point
             // data path for the nonstationary chordal and planar kernels
@@ -1306,9 +1306,11 @@ static void launch_finest_cross(mra_handle* h, const Tree& T, const CovCtx& cc, 
                                                                           (float*)h->dBf, h->dGc32);
             };
             if (pf)
-                wl(&k_finest_cross_w<64, 2, true>, &k_finest_cross_w<32, 4, true>, &k_finest_cross_w<16, 4, true>);
+                wl(&k_finest_cross_w<64, 2, true, RP>, &k_finest_cross_w<32, 4, true, RP>,
+                   &k_finest_cross_w<16, 4, true, RP>);
             else
-                wl(&k_finest_cross_w<64, 2, false>, &k_finest_cross_w<32, 4, false>, &k_finest_cross_w<16, 4, false>);
+                wl(&k_finest_cross_w<64, 2, false, RP>, &k_finest_cross_w<32, 4, false, RP>,
+                   &k_finest_cross_w<16, 4, false, RP>);
             if (h->nItemsXBig && h->dGf64)
                 launch_units(h, &k_finest_cross<RP, 64, double>, &k_finest_cross<RP, 64, double, true>,
                              h->nItemsXBig, 1, st, T, cc, K, O, h->dItemsXBig, h->nItemsXBig, h->dGoff, h->dGc,

@@ -538,25 +538,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     }
 }
 
-// The mixed-precision finest conditioning (RP = 64) one wave per tile of NT
-// observation columns, four tiles per work group, no work-group barrier:
+// The mixed-precision finest conditioning (RP = 32 or 64 knots per region)
+// one wave per tile of NT observation columns, four tiles per work group, no
+// work-group barrier:
 // per ancestor level j the FP32 tile lives in the registers of the FP32
 // matrix instruction's output layout (row 16 ti + 4 (t/16) + v, column
 // 16 tj + t%16):
 //   acc = C(Q_{a_j}, S)                      covariance (pcov_ptf, as k_finest_cross)
-//   acc -= Gcat32_{a_j}^T Gcat_F[0:j]         wave_cond_f32, K = 64 j
+//   acc -= Gcat32_{a_j}^T Gcat_F[0:j]         wave_cond_f32, K = RP j
 //   B_{F,j} = acc  -> observation-major rows (through the wave's LDS buffer)
 //   G_{F,j} = L_{a_j}^{-1} acc               wave_trmm_l32
 // (…finest-level-in-prior.cpp:116-163)
 // PF: the nonstationary chordal or planar kernel, evaluated by pcov_ff on
 // point data rounded to float once (the values pcov_ptf gives); otherwise
 // pcov_ptf on the FP64 point data (great circle, stationary).
-template <int NT, int CU, bool PF>
+template <int NT, int CU, bool PF, int RP = 64>
 __global__ __launch_bounds__(256) void k_finest_cross_w(Tree T0, CovCtx cc, Pts K, Pts O, const uint32_t* items,
                                                         uint32_t nitems, const uint64_t* goff, const double* LiT,
                                                         float* Gf, float* Bf, const float* Gc32) {
-    constexpr int RP = 64, NTT = NT / 16, LDT = RP + 1;
-    static_assert(16 * 64 * NTT <= NT * LDT, "the covariance strip fits the transpose buffer");
+    constexpr int NTT = NT / 16, LDT = RP + 1, KT = RP / 16, NQ = 4 * KT;
+    static_assert(RP == 32 || RP == 64, "32 or 64 knots per region");
+    static_assert(NQ * 64 <= NT * LDT, "the covariance strip fits the transpose buffer");
     typedef typename std::conditional<PF, PtF, PtV>::type Pt;
     __shared__ Pt knw[4][RP];            // per wave: the current ancestor's knots
     __shared__ float bufw[4][NT * LDT];  // per wave: covariance strip, then the B transpose
@@ -592,42 +594,44 @@ __global__ __launch_bounds__(256) void k_finest_cross_w(Tree T0, CovCtx cc, Pts 
         cv[tj] = c < n;
         oc[tj] = to_pt(cv[tj] ? load_pt(cc, O, ob + c0 + c) : PtV{0, 0, 0, 0, 1});
     }
-    mra_f4 acc[4 * NTT];
+    mra_f4 acc[KT * NTT];
     for (int j = 0; j < L; j++) {
         const uint64_t a = ancestor_up(T0.s_parent[sg], L - 1, j, T0.J);
-        kw[lane] = to_pt(lane < r ? load_pt(cc, K, a * RP + lane) : PtV{0, 0, 0, 0, 1});
+        if (lane < RP) kw[lane] = to_pt(lane < r ? load_pt(cc, K, a * RP + lane) : PtV{0, 0, 0, 0, 1});
         wave_lds_fence();
         // covariance tile: entry q = 4 ti + v of column tile tj is knot
         // 16 ti + 4 k4 + v; evaluated in a loop through the lane's strip
 #pragma unroll
         for (int tj = 0; tj < NTT; tj++) {
 #pragma unroll 1
-            for (int q = 0; q < 16; q++) {
+            for (int q = 0; q < NQ; q++) {
                 const int i = 16 * (q >> 2) + 4 * k4 + (q & 3);
                 bw[64 * q + lane] = (i < r && cv[tj]) ? cov(kw[i], oc[tj]) : 0.0f;
             }
 #pragma unroll
-            for (int q = 0; q < 16; q++) acc[(q >> 2) * NTT + tj][q & 3] = bw[64 * q + lane];
+            for (int q = 0; q < NQ; q++) acc[(q >> 2) * NTT + tj][q & 3] = bw[64 * q + lane];
         }
         if (j > 0)
-            wave_cond_f32<NTT, CU>(acc, j * RP, Gc32 + goff[j] + (a - lvl_start(j, T0.J)) * (uint64_t)j * RP * RP,
-                               GF + c0, nF, cv);
-        // B_{F,j}: observation-major, one row of 64 knots per observation
+            wave_cond_f32<NTT, CU, KT>(acc, j * RP,
+                                       Gc32 + goff[j] + (a - lvl_start(j, T0.J)) * (uint64_t)j * RP * RP, GF + c0,
+                                       nF, cv);
+        // B_{F,j}: observation-major, one row of RP knots per observation
         wave_lds_fence();
 #pragma unroll
-        for (int ti = 0; ti < 4; ti++)
+        for (int ti = 0; ti < KT; ti++)
 #pragma unroll
             for (int tj = 0; tj < NTT; tj++)
 #pragma unroll
                 for (int v = 0; v < 4; v++) bw[(16 * tj + r16) * LDT + 16 * ti + 4 * k4 + v] = acc[ti * NTT + tj][v];
         wave_lds_fence();
-        for (int c = 0; c < n; c++) BO[(uint64_t)(c0 + c) * LRP + j * RP + lane] = bw[c * LDT + lane];
+        if (lane < RP)
+            for (int c = 0; c < n; c++) BO[(uint64_t)(c0 + c) * LRP + j * RP + lane] = bw[c * LDT + lane];
         wave_lds_fence();
         // G_{F,j} = L_a^{-1} B, stored for the local step and read back by
         // this wave's later products
-        wave_trmm_l32<NTT>(acc, LiT + a * RP * RP);
+        wave_trmm_l32<NTT, KT>(acc, LiT + a * RP * RP);
 #pragma unroll
-        for (int ti = 0; ti < 4; ti++)
+        for (int ti = 0; ti < KT; ti++)
 #pragma unroll
             for (int tj = 0; tj < NTT; tj++)
                 if (cv[tj])

@@ -766,7 +766,8 @@ def test_prior_one_wave_per_region(case):
     np.testing.assert_allclose(var, vo, rtol=TOL_PRED, atol=1e-10)
 
 
-@pytest.mark.parametrize("case", ["clustered64", "padded40", "chordal_stationary", "cross16_off"])
+@pytest.mark.parametrize("case", ["clustered64", "padded40", "chordal_stationary", "cross16_off", "clustered32",
+                                  "padded24_chordal_stationary"])
 def test_finest_one_wave_per_tile(case):
     """Mixed-precision finest conditioning one wave per tile
     (MRA_OPT_FINEST_WAVE = 1) against one work group per tile (0): both
@@ -774,14 +775,19 @@ def test_finest_one_wave_per_tile(case):
     oracle and within 1e-7 of each other per observation (FP32 rounding in
     another order; measured 2.5e-9 against 3.6e-9 and 6.1e-9 from the
     oracle), predictions within 1e-4, graph replay equal to eager bit for bit.
-    The cases cover 64-, 32- and 16-column tiles (and 64/32 only), padded
-    knots (r = 40), the chordal metric and the stationary kernel."""
+    The cases cover 64-, 32- and 16-column tiles (and 64/32 only), 64- and
+    32-knot regions, padded knots (r = 40, 24), the chordal metric and the
+    stationary kernel."""
     rng = np.random.default_rng(83)
-    if case in ("clustered64", "cross16_off"):
+    if case in ("clustered64", "cross16_off", "clustered32"):
         x, y = synth.clustered_points(12000, rng)
         px, py = synth.uniform_points(300, rng)
-        tree = capi.partition_program(x, y, np.zeros(12000), px, py, M=4, J=4, r=64, bounds=synth.UNIT_BOUNDS)
+        tree = capi.partition_program(x, y, np.zeros(12000), px, py, M=4, J=4, r=32 if case == "clustered32" else 64,
+                                      bounds=synth.UNIT_BOUNDS)
         cov, prm = synth.linear_trend_cov(), dict(synth.UNIT_PARAMS)
+    elif case == "padded24_chordal_stationary":
+        tree, cov, prm = cases.synthetic_case(n=8000, npred=200, M=5, J=4, r=24, seed=86, clustered=True,
+                                              metric="chordal", stationary=True, partition="program")
     elif case == "padded40":
         tree, cov, prm = cases.synthetic_case(n=8000, npred=200, M=4, J=4, r=40, seed=84, clustered=True,
                                               partition="program")
