@@ -201,11 +201,34 @@ __device__ __forceinline__ void cov_tile(T* C, int LD, const CovCtx& cc, const P
 // valid when < nc); the caller synchronises before reuse of C.
 // vdiag (optional): rows and columns are the same point set, the diagonal
 // takes vdiag[i] (identity on padding rows).
-template <int RP, int NC, typename T>
+// F32 (nonstationary chordal / planar kernel): the coordinate differences and
+// d^2 in FP64, the rest (root, range factor, exponential) in FP32 on the
+// hardware's reciprocal square root and exponential, widened to T.
+template <int RP, int NC, typename T, bool F32 = false>
 __device__ __forceinline__ void cov_tile_pre(T* C, int LD, const CovCtx& cc, const PtV* st, int nr, const PtV& qc,
                                              int nc, const double* vdiag = nullptr) {
     static_assert(MRA_WG % NC == 0, "tile width must divide the work-group size");
     const int c = threadIdx.x % NC;
+    if (F32 && !cc.stationary && cc.metric != 1) {
+        const float dscale = cc.metric == 0 ? 6371.0f : 1.0f;
+        const float qb = (float)qc.b, qb2 = qb * qb, qq = (float)qc.q;
+        const bool cok = c < nc;
+#pragma unroll 4
+        for (int i = threadIdx.x / NC; i < RP; i += MRA_WG / NC) {
+            const PtV& a = st[i];
+            const double dx = qc.x - a.x, dy = qc.y - a.y, dz = qc.z - a.z;
+            const float d2 = (float)(dx * dx + dy * dy + dz * dz);
+            const float dd = d2 * __builtin_amdgcn_rsqf(fmaxf(d2, 1e-30f)) * dscale;
+            const float ab = (float)a.b;
+            const float t = __builtin_amdgcn_rsqf((ab * ab + qb2) * 0.5f);
+            const float v = ((float)a.q * qq) * (t * t * t) * __expf(-dd * t);
+            C[i * LD + c] = (T)((i < nr && cok) ? v : 0.0f);
+        }
+        if (vdiag)
+            for (int i = threadIdx.x / NC; i < RP; i += MRA_WG / NC)
+                if (i == c) C[i * LD + c] = (T)(i < nr ? vdiag[i] : 1.0);
+        return;
+    }
     if (sizeof(T) == 8 && !cc.stationary && cc.metric != 1) {
         // FP64 nonstationary chordal / planar kernel (the values of pcov_pt):
         // a straight-line body, validity by selects, four rows at a time so

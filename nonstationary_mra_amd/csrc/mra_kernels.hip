@@ -306,6 +306,8 @@ __global__ void k_cross_cov(CovCtx cc, Pts P, uint64_t n1, Pts Q, uint64_t n2, d
 //   B = C(Q_{a_j}, Q_R) - Gcat_{a_j}^T Gcat_R[0:j]      (MFMA, K = j*RP)
 //   G_{R,j} = L_{a_j}^{-1} B                            (MFMA with L^{-1})
 // then K_R = V(Q_R) - Gcat_R^T Gcat_R, L_R = chol(K_R), L_R^{-1}.
+// COPY32 (mixed precision): the knot covariances in FP32 arithmetic (widened;
+// the products, K_R and its factorisation stay FP64) and an FP32 copy of G.
 // (src/class_approximation.cpp:108-159)
 template <int RP, bool COPY32>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_prior_coarse(Tree T, CovCtx cc, Pts K, int l, uint64_t start_l,
@@ -340,7 +342,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             ns_ = K.s[q];
             nb = K.b[q];
         }
-        cov_tile_pre<RP, RP, double>(Cm, LD, cc, kst[j & 1], r, qc, r);
+        cov_tile_pre<RP, RP, double, COPY32>(Cm, LD, cc, kst[j & 1], r, qc, r);
         __syncthreads();
         if (j > 0) {
             const double* Ga = Gc + goff[j] + (a - lvl_start(j, T.J)) * (uint64_t)j * RP * RP;
@@ -358,7 +360,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         __syncthreads();
     }
     // K_R = V(Q_R) - Gcat_R^T Gcat_R: the stage now holds R's own knots
-    cov_tile_pre<RP, RP, double>(Cm, LD, cc, kst[l & 1], r, qc, r, K.v + R * RP);
+    cov_tile_pre<RP, RP, double, COPY32>(Cm, LD, cc, kst[l & 1], r, qc, r, K.v + R * RP);
     __syncthreads();
     if (l > 0) {
         // only the lower part is read by the Cholesky factorisation
