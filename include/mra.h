@@ -108,9 +108,10 @@ int mra_set_params(mra_handle* h, double sill, double range, double nugget,
                    double sill_scale, double range_scale, double nugget_scale);
 
 /* MRA_PRECISION_FP64 (default) or MRA_PRECISION_MIXED: covariance evaluation
- * and the conditioning products of the finest level in FP32 on FP32 matrix
- * cores, knot Cholesky factors, conditioning of knots and log-determinants
- * in FP64.  Finest regions above MRA_OPT_MIXED_FP64_ABOVE observations
+ * (finest observation blocks and the coarse knot covariances) in FP32
+ * arithmetic and the conditioning products of the finest level on FP32
+ * matrix cores; knot Cholesky factors, conditioning of knots and
+ * log-determinants in FP64.  Finest regions above MRA_OPT_MIXED_FP64_ABOVE observations
  * (default 128) are conditioned in FP64 in either mode.  In mra_likelihood
  * (not mra_predict) with 64-knot blocks the row Gram products of the
  * posterior (X_j^T X_k, X_l^T X_j) also run on FP32 matrix cores, so the two
