@@ -25,6 +25,21 @@ import os
 import sys
 import time
 
+# The C3 batch runs its fields on up to 16 streams, which run side by side
+# only on as many hardware queues (HIP's default is 4; --hw-queues N sets
+# another count, at most 32); the setting is read when the process first uses
+# the GPU, so it is made before anything does.
+def _hw_queues(argv):
+    for i, a in enumerate(argv):
+        if a.startswith("--hw-queues="):
+            return a.split("=", 1)[1]
+        if a == "--hw-queues" and i + 1 < len(argv):
+            return argv[i + 1]
+    return "16"
+
+
+os.environ["GPU_MAX_HW_QUEUES"] = _hw_queues(sys.argv)
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -290,6 +305,8 @@ def main():
                     help="mixed-precision finest conditioning one wave per tile (MRA_OPT_FINEST_WAVE)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-c3", action="store_true", help="skip the C3 batch leg of the default run")
+    ap.add_argument("--hw-queues", type=int, default=16, choices=range(1, 33), metavar="N",
+                    help="HIP hardware queues of the process (GPU_MAX_HW_QUEUES, read before the GPU is used)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--no-cov-kernel", action="store_true", help="skip the standalone covariance-kernel leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",

@@ -396,8 +396,11 @@ int mra_autotune(mra_handle* h, int* x32max, int* lrmax, int* fb16, double* best
 /* Many independent fields (one handle each, any sizes) evaluated as one
  * launch sequence (no reference counterpart; the reference runs one data set
  * per MPI job): one kernel scatters every field's parameters, the fields'
- * evaluations run on `streams` (1..8) batch streams (largest first, each to
- * the least loaded stream), one kernel gathers every D and status flag.  With
+ * evaluations run on `streams` (1..16) batch streams (largest first, each to
+ * the least loaded stream), one kernel gathers every D and status flag.
+ * Streams run side by side only on as many hardware queues: HIP's
+ * GPU_MAX_HW_QUEUES (4 by default) must be set before the process first
+ * uses the GPU to let more than four run at once.  With
  * graph = 1 the whole sequence is captured once as one HIP graph and
  * replayed; it is captured again when a handle's work lists, options,
  * covariance model or precision change (or, for a stationary model, its
@@ -413,7 +416,7 @@ int mra_batch_set(mra_batch* b, int streams, int graph);
 int mra_batch_get(mra_batch* b, int* streams, int* graph);
 /* D[i] of handle i; status[i] = MRA_OK or MRA_E_NOT_PD (status may be NULL). */
 int mra_batch_likelihood(mra_batch* b, double* D, int* status);
-/* Times one batch evaluation for streams in {1,2,4,8} x graph in {0,1} and
+/* Times one batch evaluation for streams in {1,2,4,8,16} x graph in {0,1} and
  * keeps the fastest setting (returned; best_ms = its wall time). */
 int mra_batch_autotune(mra_batch* b, int* streams, int* graph, double* best_ms);
 

@@ -2386,7 +2386,7 @@ struct mra_batch {
     std::vector<mra_handle*> h;
     std::vector<int> order;   // issue order: decreasing work
     std::vector<double> work; // work estimate per field
-    static constexpr int MAXS = 8;
+    static constexpr int MAXS = 16;
     hipStream_t st[MAXS] = {};
     hipEvent_t evRoot = nullptr, evDone[MAXS] = {};
     int streams = 4;
@@ -2665,7 +2665,7 @@ extern "C" int mra_batch_likelihood(mra_batch* b, double* D, int* status) {
 extern "C" int mra_batch_autotune(mra_batch* b, int* streams, int* graph, double* best_ms) {
     if (!b) return MRA_E_ARG;
     std::vector<double> D(b->n);
-    const int cand_s[] = {1, 2, 4, 8};
+    const int cand_s[] = {1, 2, 4, 8, 16};
     double best = 1e300;
     int bs = b->streams, bg = b->graph;
     for (int g = 0; g <= 1; g++)
