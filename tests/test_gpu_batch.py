@@ -36,7 +36,7 @@ def test_batch_matches_individual_evaluations():
     ref = np.array([h.likelihood() for h in hs])
     b = capi.Batch(hs)
     try:
-        for streams, graph in [(1, 0), (4, 0), (2, 1), (8, 1), (4, 1)]:
+        for streams, graph in [(1, 0), (4, 0), (2, 1), (8, 1), (4, 1), (16, 0), (16, 1)]:
             b.set(streams, graph)
             D, st = b.likelihood()
             print(streams, graph, np.abs(D - ref).max(), flush=True)
@@ -44,7 +44,9 @@ def test_batch_matches_individual_evaluations():
             np.testing.assert_array_equal(D, ref)
         s, g, ms = b.autotune()
         print("autotune", s, g, ms, flush=True)
-        assert s in (1, 2, 4, 8) and g in (0, 1) and ms > 0
+        assert s in (1, 2, 4, 8, 16) and g in (0, 1) and ms > 0
+        with pytest.raises(capi.MraError):
+            b.set(17, 0)
         assert b.get() == (s, g)
         D, st = b.likelihood()
         np.testing.assert_array_equal(D, ref)
