@@ -703,6 +703,45 @@ def test_local_left_looking(case):
     assert rel(out[(capi.PRECISION_FP64, 1, 0)][0], Do) < TOL_D
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("r", [16, 32, 64])
+@pytest.mark.parametrize("sizes", [(65, 96, 129, 257), (97, 128, 160, 320), (100, 200, 300, 415)])
+def test_local_left_looking_region_sizes(sizes, r):
+    """The left-looking local step at chosen region sizes (one finest region
+    per quadrant: block-column edges at 64, 96, 128, 160, 256, 320 and sizes
+    just past them), 16/32/64 knots per region: FP64 D against the earlier
+    forms (1e-12) and the oracle (TOL_D), mixed D against the earlier forms
+    (1e-10)."""
+    rng = np.random.default_rng(sum(sizes) + r)
+    xs, ys = [], []
+    for k, m in enumerate(sizes):
+        xs.append(0.5 * (k % 2) + 0.02 + 0.46 * rng.random(m))
+        ys.append(0.5 * (k // 2) + 0.02 + 0.46 * rng.random(m))
+    x, y = np.concatenate(xs), np.concatenate(ys)
+    tree = capi.partition_program(x, y, rng.standard_normal(len(x)), None, None, M=2, J=4, r=r,
+                                  bounds=synth.UNIT_BOUNDS)
+    assert tuple(np.diff(tree["obs_offsets"].astype(np.int64))) == tuple(sizes)
+    cov, prm = synth.linear_trend_cov(log_range=(np.log(0.2), 0.8, 0.0)), dict(synth.UNIT_PARAMS)
+    h = capi.Handle(tree, with_pred=False)
+    out = {}
+    try:
+        h.set_covariance(cov)
+        h.set_params(**prm)
+        for prec in (capi.PRECISION_FP64, capi.PRECISION_MIXED):
+            h.set_precision(prec)
+            for ll in (2, 0):
+                h.set_option(capi.OPT_LOCAL_LL, ll)
+                out[(prec, ll)] = h.likelihood()
+        h.set_option(capi.OPT_LOCAL_LL, 1)
+    finally:
+        h.close()
+    Do, _, _ = oracle.evaluate(tree, cov, prm, predict=False)
+    print(sizes, r, out, "oracle", Do, flush=True)
+    assert rel(out[(capi.PRECISION_FP64, 2)], out[(capi.PRECISION_FP64, 0)]) < 1e-12
+    assert rel(out[(capi.PRECISION_FP64, 2)], Do) < TOL_D
+    assert rel(out[(capi.PRECISION_MIXED, 2)], out[(capi.PRECISION_MIXED, 0)]) < 1e-10
+
+
 def test_prior_one_wave_per_region_refused():
     """The one-wave-per-region coarse prior is switched off: setting the
     option is refused with MRA_E_UNSUPPORTED and the default form stays."""
