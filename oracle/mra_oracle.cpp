@@ -285,6 +285,10 @@ struct Engine {
     std::vector<std::vector<double>> BT;            // BTilde per finest: L blocks r x p
     std::vector<double> predMean, predVar;          // finest-region order
     std::vector<double> levelTerm;                  // D partial sums
+    // wall seconds per phase (measurement only): coarse prior per level,
+    // finest level, posterior per level
+    std::vector<double> tPrior, tPost;
+    double tFinest = 0;
     int err = 0;
 
     int level_of(uint64_t id) const {
@@ -376,7 +380,9 @@ struct Engine {
 
     // src/class_approximation.cpp:84-169
     void prior_coarse() {
+        tPrior.assign(L, 0);
         for (int l = 0; l < L; l++) {
+            const double t0 = omp_get_wtime();
             uint64_t n_l = (l + 1 < L ? lvlStart[l + 1] : nC) - lvlStart[l];
 #pragma omp parallel for schedule(dynamic, 1)
             for (uint64_t i = 0; i < n_l; i++) {
@@ -404,11 +410,13 @@ struct Engine {
                     err = -10;
                 }
             }
+            tPrior[l] = omp_get_wtime() - t0;
         }
     }
 
     // src/class_approximation-loop-regions-at-finest-level-in-prior.cpp:15-409
     void prior_finest() {
+        const double t0 = omp_get_wtime();
         std::vector<double> term(nF, 0);
 #pragma omp parallel for schedule(dynamic, 1)
         for (uint64_t f = 0; f < nF; f++) {
@@ -485,6 +493,7 @@ struct Engine {
         double s = 0;
         for (uint64_t f = 0; f < nF; f++) s += term[f];
         levelTerm[0] = s;
+        tFinest = omp_get_wtime() - t0;
     }
 
     // src/class_approximation-loop-regions-at-finest-level-in-prior.cpp:271-392
@@ -555,7 +564,9 @@ struct Engine {
     // src/class_approximation-posterior-inference.cpp:14-246 and
     // src/class_approximation.cpp:266-306
     void posterior() {
+        tPost.assign(L, 0);
         for (int l = L - 1; l >= 0; l--) {
+            const double t0 = omp_get_wtime();
             uint64_t n_l = (l + 1 < L ? lvlStart[l + 1] : nC) - lvlStart[l];
             std::vector<double> term(n_l, 0);
 #pragma omp parallel for schedule(dynamic, 1)
@@ -629,6 +640,7 @@ struct Engine {
             double s = 0;
             for (uint64_t i = 0; i < n_l; i++) s += term[i];
             levelTerm[L - l] = s;
+            tPost[l] = omp_get_wtime() - t0;
         }
     }
 
@@ -730,7 +742,7 @@ int orc_variance_covariance(const orc_cov* cov, const orc_params* p, const doubl
 }
 
 static int run(const orc_tree* t, const orc_cov* cov, const orc_params* p, double* D, double* mean, double* var,
-               double* levels, int nthreads) {
+               double* levels, int nthreads, double* phase_s = nullptr) {
     if (nthreads > 0) omp_set_num_threads(nthreads);
     Engine e;
     int rc = e.setup(t, cov, p);
@@ -748,6 +760,11 @@ static int run(const orc_tree* t, const orc_cov* cov, const orc_params* p, doubl
     if (D) *D = d;
     if (levels)
         for (int i = 0; i < e.M; i++) levels[i] = e.levelTerm[i];
+    if (phase_s) {
+        for (int l = 0; l < e.L; l++) phase_s[l] = e.tPrior[l];
+        phase_s[e.L] = e.tFinest;
+        for (int l = 0; l < e.L; l++) phase_s[e.L + 1 + l] = e.tPost[l];
+    }
     if (e.predicting && mean && var) {
         e.predict();
         std::memcpy(mean, e.predMean.data(), sizeof(double) * t->n_pred);
@@ -759,6 +776,14 @@ static int run(const orc_tree* t, const orc_cov* cov, const orc_params* p, doubl
 int orc_evaluate(const orc_tree* t, const orc_cov* cov, const orc_params* p, double* D, double* mean, double* var,
                  int nthreads) {
     return run(t, cov, p, D, mean, var, nullptr, nthreads);
+}
+
+int orc_evaluate_timed(const orc_tree* t, const orc_cov* cov, const orc_params* p, double* D, double* phase_s,
+                       int nthreads) {
+    orc_tree t2 = *t;
+    t2.pred_offsets = nullptr;
+    t2.n_pred = 0;
+    return run(&t2, cov, p, D, nullptr, nullptr, nullptr, nthreads, phase_s);
 }
 
 int orc_evaluate_levels(const orc_tree* t, const orc_cov* cov, const orc_params* p, double* level_terms,

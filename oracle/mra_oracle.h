@@ -77,6 +77,11 @@ int orc_evaluate(const orc_tree* t, const orc_cov* cov, const orc_params* p,
 
 /* Per-level partial sums of D (finest level first, then levels M-2..0);
  * out has M entries.  Same order of summation as orc_evaluate. */
+/* D of the likelihood evaluation (no prediction) and the wall seconds of
+ * its phases (measurement only): phase_s[l] coarse prior level l (l < L),
+ * phase_s[L] the finest level, phase_s[L + 1 + l] posterior level l; L = M - 1. */
+int orc_evaluate_timed(const orc_tree* t, const orc_cov* cov, const orc_params* p, double* D, double* phase_s,
+                       int nthreads);
 int orc_evaluate_levels(const orc_tree* t, const orc_cov* cov, const orc_params* p,
                         double* level_terms, int nthreads);
 

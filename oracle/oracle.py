@@ -158,6 +158,21 @@ def evaluate(tree, cov, prm, predict=True, nthreads=0):
     return D.value, mean, var
 
 
+def evaluate_timed(tree, cov, prm, nthreads=0):
+    """Returns (D, phase seconds): coarse prior per level, finest level,
+    posterior per level (2 M - 1 values; measurement only)."""
+    keep = _Keep()
+    t = make_tree(tree, keep, with_pred=False)
+    c = make_cov(cov, keep)
+    p = make_params(prm)
+    D = C.c_double(0)
+    ph = np.zeros(2 * tree["M"] - 1)
+    rc = lib().orc_evaluate_timed(C.byref(t), C.byref(c), C.byref(p), C.byref(D), _d(ph), nthreads)
+    if rc != 0:
+        raise RuntimeError(f"oracle evaluate_timed failed rc={rc}")
+    return D.value, ph
+
+
 def evaluate_levels(tree, cov, prm, nthreads=0):
     keep = _Keep()
     t = make_tree(tree, keep, with_pred=False)

@@ -294,3 +294,14 @@ def test_program_partition_drops_observations_on_ancestor_knots():
     assert all(1000 + k not in kept for k in range(len(picks)))
     assert len(tree["obs_lon"]) == 1000
     assert len(tree["pred_lon"]) == len(picks)
+
+
+def test_oracle_phase_times():
+    """The timed evaluation (used only for the CPU baseline) returns the same
+    D as the plain one and one non-negative wall time per phase: coarse prior
+    per level, finest level, posterior per level."""
+    tree, cov, prm = cases.synthetic_case(n=2000, npred=0, M=4, J=4, r=16, seed=3, clustered=True)
+    D, ph = oracle.evaluate_timed(tree, cov, prm)
+    D2, _, _ = oracle.evaluate(tree, cov, prm, predict=False)
+    assert D == D2
+    assert ph.shape == (2 * tree["M"] - 1,) and np.all(ph >= 0) and ph.sum() > 0
