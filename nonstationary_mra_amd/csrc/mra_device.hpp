@@ -1142,6 +1142,119 @@ __device__ __forceinline__ void wg_gemm_tn_f32(TC* C, int ldc, int M, int N, int
                                                                                        lda, Mv, B, ldb, Nv);
 }
 
+// G = L^{-1} B for one 64 x 64 block, work balanced over the triangle: Li
+// holds L^{-1} column-major (Li[k*64 + i] = L^{-1}(i, k), zero for k > i), B
+// is 64 x 64 in LDS (leading dimension ldb).  Wave w computes output columns
+// 16w .. 16w+15 for all four row tiles; row tile ti stops at k < 16(ti+1), so
+// every wave issues the same 40 matrix instructions (a 2 x 2 tile block per
+// wave would give two waves K = 64 and two K = 32).  G (ld 64) is written
+// from the accumulators, and with G32 also its FP32 rounding.
+__device__ __forceinline__ void wg_trmm64_cols(double* G, float* G32, const double* Li, const double* B, int ldb) {
+    const int lane = threadIdx.x & 63, wave = wave_id();
+    const int r16 = lane & 15, k4 = lane >> 4;
+    const int col = 16 * wave + r16;
+    // wave-uniform bases plus 32-bit lane offsets (row k4, column r16 / col)
+    const uint32_t oa = (uint32_t)((k4 * 64 + r16) * sizeof(double));
+    const uint32_t og = (uint32_t)((k4 * 64 + col) * sizeof(double));
+    const double* Bl = B + k4 * ldb + col;
+    mra_d4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) acc[t] = mra_d4{0, 0, 0, 0};
+#pragma unroll
+    for (int kh = 0; kh < 8; kh++) {  // 8 k rows at a time; row tiles kh/2 .. 3
+        const int kb = kh >> 1;
+        double a[2][4], b[2];
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            const int k0 = 8 * kh + 4 * s;
+            const char* Lk = (const char*)(Li + k0 * 64);
+            b[s] = Bl[k0 * ldb];
+#pragma unroll
+            for (int t = kb; t < 4; t++) a[s][t] = *(const double*)(Lk + oa + 16 * t * sizeof(double));
+        }
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+#pragma unroll
+            for (int t = kb; t < 4; t++) acc[t] = Mfma<double>::op(a[s][t], b[s], acc[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            // row 16 t + k4 + 4 v
+            const int rb = 16 * t + 4 * v;
+            *(double*)((char*)(G + rb * 64) + og) = acc[t][v];
+            if (G32) *(float*)((char*)(G32 + rb * 64) + og / 2) = (float)acc[t][v];
+        }
+}
+
+// C -= G^T G on the ten 16 x 16 tiles on and below the diagonal of a 64 x 64
+// C in LDS (ldc); G is K x 64 row-major (ld 64), K a multiple of 16.  Column
+// tiles {0, 3} and {1, 2} hold five lower tiles each; waves 0 / 1 take them
+// over k < K/2, waves 2 / 3 over k >= K/2 (320 matrix instructions per wave
+// at K = 512 where a 2 x 2 block per wave takes 512).  A and B operands of
+// the symmetric product are the same loads.  The first half is subtracted
+// before the second (a work-group barrier between), so the result does not
+// depend on timing.  Called by all 256 threads.
+__device__ __forceinline__ void wg_syrk64_lower(double* C, int ldc, const double* G, int K) {
+    const int lane = threadIdx.x & 63, wave = wave_id();
+    const int r16 = lane & 15, k4 = lane >> 4;
+    const bool outer = (wave & 1) == 0;  // column tiles {0, 3}, else {1, 2}
+    const int kh = K >> 1, kb = (wave >> 1) * kh;
+    // tiles (row tile, column tile): outer (0,0) (1,0) (2,0) (3,0) (3,3);
+    // inner (1,1) (2,1) (3,1) (2,2) (3,2)
+    mra_d4 acc[5];
+#pragma unroll
+    for (int t = 0; t < 5; t++) acc[t] = mra_d4{0, 0, 0, 0};
+    const double* Gl = G + (int64_t)k4 * 64 + r16;
+    if (outer) {
+        for (int k0 = kb; k0 < kb + kh; k0 += 8) {
+            double g[2][4];
+#pragma unroll
+            for (int s = 0; s < 2; s++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) g[s][t] = Gl[(int64_t)(k0 + 4 * s) * 64 + 16 * t];
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                acc[0] = Mfma<double>::op(g[s][0], g[s][0], acc[0]);
+                acc[1] = Mfma<double>::op(g[s][1], g[s][0], acc[1]);
+                acc[2] = Mfma<double>::op(g[s][2], g[s][0], acc[2]);
+                acc[3] = Mfma<double>::op(g[s][3], g[s][0], acc[3]);
+                acc[4] = Mfma<double>::op(g[s][3], g[s][3], acc[4]);
+            }
+        }
+    } else {
+        for (int k0 = kb; k0 < kb + kh; k0 += 8) {
+            double g[2][3];
+#pragma unroll
+            for (int s = 0; s < 2; s++)
+#pragma unroll
+                for (int t = 0; t < 3; t++) g[s][t] = Gl[(int64_t)(k0 + 4 * s) * 64 + 16 * (t + 1)];
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                acc[0] = Mfma<double>::op(g[s][0], g[s][0], acc[0]);
+                acc[1] = Mfma<double>::op(g[s][1], g[s][0], acc[1]);
+                acc[2] = Mfma<double>::op(g[s][2], g[s][0], acc[2]);
+                acc[3] = Mfma<double>::op(g[s][1], g[s][1], acc[3]);
+                acc[4] = Mfma<double>::op(g[s][2], g[s][1], acc[4]);
+            }
+        }
+    }
+    auto sub = [&]() {
+#pragma unroll
+        for (int t = 0; t < 5; t++) {
+            const int ti = outer ? (t < 4 ? t : 3) : (t < 3 ? t + 1 : t - 1);
+            const int tj = outer ? (t < 4 ? 0 : 3) : (t < 3 ? 1 : 2);
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                C[(16 * ti + Mfma<double>::row(k4, v)) * ldc + 16 * tj + r16] -= acc[t][v];
+        }
+    };
+    if (wave < 2) sub();
+    __syncthreads();
+    if (wave >= 2) sub();
+}
+
 // Same product for an output of at most 32 x 32 (one 2x2 tile block), with
 // K split over the 4 waves; the partial tiles are summed in wave order
 // through red (LDS: 3*256 doubles for a 16 x 16 output, else 3*4*256), so

@@ -352,9 +352,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         // G_{R,j} = L_a^{-1} B, stored straight to global memory
         // (mixed precision: also an FP32 copy for the finest-level products)
         const uint64_t o = goff[l] + (uint64_t)bid * l * RP * RP + (uint64_t)j * RP * RP;
-        wg_gemm_tn<false, false, double, double, GEMM_FULL, COPY32, GEMM_TRI_A>(
-            GR + (uint64_t)j * RP * RP, RP, RP, RP, RP, 1.0, LiT + a * RP * RP, RP, RP, Cm, LD, RP,
-            COPY32 ? Gc32 + o : nullptr);
+        if constexpr (RP == 64)
+            wg_trmm64_cols(GR + (uint64_t)j * RP * RP, COPY32 ? Gc32 + o : nullptr, LiT + a * RP * RP, Cm, LD);
+        else
+            wg_gemm_tn<false, false, double, double, GEMM_FULL, COPY32, GEMM_TRI_A>(
+                GR + (uint64_t)j * RP * RP, RP, RP, RP, RP, 1.0, LiT + a * RP * RP, RP, RP, Cm, LD, RP,
+                COPY32 ? Gc32 + o : nullptr);
         if (threadIdx.x < r)
             kst[(j + 1) & 1][threadIdx.x] = PtV{nx, ny, nz, cc.stationary ? 0.0 : ns_ * (nb * sqrt(nb)), nb};
         __syncthreads();
@@ -364,8 +367,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     __syncthreads();
     if (l > 0) {
         // only the lower part is read by the Cholesky factorisation
-        wg_gemm_tn<false, true, double, double, GEMM_FULL, false, GEMM_TRI_C>(Cm, LD, RP, RP, l * RP, -1.0, GR, RP, RP,
-                                                                              GR, RP, RP);
+        if constexpr (RP == 64)
+            wg_syrk64_lower(Cm, LD, GR, l * RP);
+        else
+            wg_gemm_tn<false, true, double, double, GEMM_FULL, false, GEMM_TRI_C>(Cm, LD, RP, RP, l * RP, -1.0, GR, RP,
+                                                                                  RP, GR, RP, RP);
         __syncthreads();
     }
     wg_chol_blk(Cm, LD, RP / 16, dbuf, &bad);
