@@ -1160,22 +1160,32 @@ __device__ __forceinline__ void wg_trmm64_cols(double* G, float* G32, const doub
     mra_d4 acc[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) acc[t] = mra_d4{0, 0, 0, 0};
-#pragma unroll
-    for (int kh = 0; kh < 8; kh++) {  // 8 k rows at a time; row tiles kh/2 .. 3
-        const int kb = kh >> 1;
-        double a[2][4], b[2];
+    // 8 k rows at a time (row tiles kh/2 .. 3), the next 8 rows' operands
+    // loaded before the current products are issued
+    auto load = [&](const int kh, double (&a)[2][4], double (&b)[2]) {
 #pragma unroll
         for (int s = 0; s < 2; s++) {
             const int k0 = 8 * kh + 4 * s;
             const char* Lk = (const char*)(Li + k0 * 64);
             b[s] = Bl[k0 * ldb];
 #pragma unroll
-            for (int t = kb; t < 4; t++) a[s][t] = *(const double*)(Lk + oa + 16 * t * sizeof(double));
+            for (int t = kh >> 1; t < 4; t++) a[s][t] = *(const double*)(Lk + oa + 16 * t * sizeof(double));
         }
+    };
+    auto mm = [&](const int kh, const double (&a)[2][4], const double (&b)[2]) {
 #pragma unroll
         for (int s = 0; s < 2; s++)
 #pragma unroll
-            for (int t = kb; t < 4; t++) acc[t] = Mfma<double>::op(a[s][t], b[s], acc[t]);
+            for (int t = kh >> 1; t < 4; t++) acc[t] = Mfma<double>::op(a[s][t], b[s], acc[t]);
+    };
+    double a0[2][4], b0[2], a1[2][4], b1[2];
+    load(0, a0, b0);
+#pragma unroll
+    for (int kh = 0; kh < 8; kh += 2) {
+        load(kh + 1, a1, b1);
+        mm(kh, a0, b0);
+        if (kh + 2 < 8) load(kh + 2, a0, b0);
+        mm(kh + 1, a1, b1);
     }
 #pragma unroll
     for (int t = 0; t < 4; t++)
@@ -1189,7 +1199,7 @@ __device__ __forceinline__ void wg_trmm64_cols(double* G, float* G32, const doub
 }
 
 // C -= G^T G on the ten 16 x 16 tiles on and below the diagonal of a 64 x 64
-// C in LDS (ldc); G is K x 64 row-major (ld 64), K a multiple of 16.  Column
+// C in LDS (ldc); G is K x 64 row-major (ld 64), K a multiple of 32.  Column
 // tiles {0, 3} and {1, 2} hold five lower tiles each; waves 0 / 1 take them
 // over k < K/2, waves 2 / 3 over k >= K/2 (320 matrix instructions per wave
 // at K = 512 where a 2 x 2 block per wave takes 512).  A and B operands of
@@ -1207,13 +1217,16 @@ __device__ __forceinline__ void wg_syrk64_lower(double* C, int ldc, const double
 #pragma unroll
     for (int t = 0; t < 5; t++) acc[t] = mra_d4{0, 0, 0, 0};
     const double* Gl = G + (int64_t)k4 * 64 + r16;
+    // 8 k rows at a time, the next 8 rows loaded before the current products
+    // are issued (kh is a multiple of 16)
     if (outer) {
-        for (int k0 = kb; k0 < kb + kh; k0 += 8) {
-            double g[2][4];
+        auto ld = [&](int k0, double (&g)[2][4]) {
 #pragma unroll
             for (int s = 0; s < 2; s++)
 #pragma unroll
                 for (int t = 0; t < 4; t++) g[s][t] = Gl[(int64_t)(k0 + 4 * s) * 64 + 16 * t];
+        };
+        auto mm = [&](const double (&g)[2][4]) {
 #pragma unroll
             for (int s = 0; s < 2; s++) {
                 acc[0] = Mfma<double>::op(g[s][0], g[s][0], acc[0]);
@@ -1222,14 +1235,23 @@ __device__ __forceinline__ void wg_syrk64_lower(double* C, int ldc, const double
                 acc[3] = Mfma<double>::op(g[s][3], g[s][0], acc[3]);
                 acc[4] = Mfma<double>::op(g[s][3], g[s][3], acc[4]);
             }
+        };
+        double g0[2][4], g1[2][4];
+        ld(kb, g0);
+        for (int k0 = kb; k0 < kb + kh; k0 += 16) {
+            ld(k0 + 8, g1);
+            mm(g0);
+            if (k0 + 16 < kb + kh) ld(k0 + 16, g0);
+            mm(g1);
         }
     } else {
-        for (int k0 = kb; k0 < kb + kh; k0 += 8) {
-            double g[2][3];
+        auto ld = [&](int k0, double (&g)[2][3]) {
 #pragma unroll
             for (int s = 0; s < 2; s++)
 #pragma unroll
                 for (int t = 0; t < 3; t++) g[s][t] = Gl[(int64_t)(k0 + 4 * s) * 64 + 16 * (t + 1)];
+        };
+        auto mm = [&](const double (&g)[2][3]) {
 #pragma unroll
             for (int s = 0; s < 2; s++) {
                 acc[0] = Mfma<double>::op(g[s][0], g[s][0], acc[0]);
@@ -1238,6 +1260,14 @@ __device__ __forceinline__ void wg_syrk64_lower(double* C, int ldc, const double
                 acc[3] = Mfma<double>::op(g[s][1], g[s][1], acc[3]);
                 acc[4] = Mfma<double>::op(g[s][2], g[s][1], acc[4]);
             }
+        };
+        double g0[2][3], g1[2][3];
+        ld(kb, g0);
+        for (int k0 = kb; k0 < kb + kh; k0 += 16) {
+            ld(k0 + 8, g1);
+            mm(g0);
+            if (k0 + 16 < kb + kh) ld(k0 + 16, g0);
+            mm(g1);
         }
     }
     auto sub = [&]() {
