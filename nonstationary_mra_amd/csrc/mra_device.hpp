@@ -1413,8 +1413,19 @@ __device__ __forceinline__ mra_d4 mfma16(mra_d4 acc, const double* X, int ldx, c
     return acc;
 }
 
+// 1/x for a positive normal x: the hardware reciprocal refined by two Newton
+// steps (r += r (1 - x r)), five dependent instructions where the IEEE
+// division takes about ten; within an ulp or so of the division.
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+    return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+}
+
 // One wave: factor the 16 x 16 diagonal block P in place (lower triangle)
-// and write its inverse (lower, zeros above) to Dp (row stride ldd).
+// and write its inverse (lower, zeros above) to Dp (row stride ldd).  The
+// column chain carries no IEEE division: the pivot reciprocals by rcp_nr,
+// and the inverse multiplies by 1/L_ii (staged on Dp's diagonal).
 __device__ __forceinline__ void wave_chol16(double* P, int lda, double* Dp, int ldd, int* bad) {
     const int lane = threadIdx.x & 63, i = lane & 15, cg = lane >> 4;
     // unscaled right-looking elimination of the lower triangle:
@@ -1423,7 +1434,7 @@ __device__ __forceinline__ void wave_chol16(double* P, int lda, double* Dp, int 
         wave_lds_fence();
         double akk = P[k * lda + k];
         if (!(akk > 0.0)) akk = 1.0;
-        const double inv = 1.0 / akk;
+        const double inv = rcp_nr(akk);
         const double aik = P[i * lda + k];
         double upd[4];
         bool act[4];
@@ -1457,7 +1468,11 @@ __device__ __forceinline__ void wave_chol16(double* P, int lda, double* Dp, int 
             pj = 1.0;
         }
         if (j < i) P[i * lda + j] = val[m] / sqrt(pj);
-        if (j == i) P[i * lda + i] = sqrt(pj);
+        if (j == i) {
+            const double lii = sqrt(pj);
+            P[i * lda + i] = lii;
+            Dp[i * ldd + i] = 1.0 / lii;
+        }
     }
     wave_lds_fence();
     // inverse by forward substitution, one column per lane (lanes 0..15):
@@ -1471,7 +1486,7 @@ __device__ __forceinline__ void wave_chol16(double* P, int lda, double* Dp, int 
             double sacc = ii == c ? 1.0 : 0.0;
 #pragma unroll
             for (int kk = 0; kk < ii; kk++) sacc -= P[ii * lda + kk] * x[kk];
-            x[ii] = sacc / P[ii * lda + ii];
+            x[ii] = sacc * Dp[ii * ldd + ii];
         }
 #pragma unroll
         for (int ii = 0; ii < 16; ii++) Dp[ii * ldd + c] = x[ii];
