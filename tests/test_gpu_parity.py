@@ -379,6 +379,38 @@ def test_binary_tree_padded_knots(gram):
     check(tree, cov, prm, gram=gram)
 
 
+@pytest.mark.parametrize("r", [64, 50])
+def test_deepest_coarse_level(r):
+    """J = 2, M = 10: the coarse prior down to level 8, the depth of the
+    n = 2e6 benchmark tree (eight ancestor levels, K_R = V - G^T G over
+    K = 8 * 64), with r = 64 and r = 50 padded to 64-wide blocks, on a field
+    drawn from the model: FP64 against the oracle, mixed precision within
+    |dloglik|/n <= 1e-6 of FP64 (white-noise data, far from the model, give
+    a larger mixed error, about 4e-7 |D| here: the documented limit of the
+    mixed contract)."""
+    rng = np.random.default_rng(90 + r)
+    n = 6000
+    x, y = synth.clustered_points(n, rng, n_clusters=3, sigma=0.05)
+    px, py = synth.uniform_points(100, rng)
+    tree = capi.partition_program(x, y, np.zeros(n), px, py, M=10, J=2, r=r, bounds=synth.UNIT_BOUNDS)
+    cov, prm = synth.linear_trend_cov(log_range=(np.log(0.1), 0.8, 0.0)), dict(synth.UNIT_PARAMS)
+    h = capi.Handle(tree, with_pred=False)
+    h.set_covariance(cov)
+    h.set_params(**prm)
+    tree["obs_val"] = h.sample_prior(seed=11, replace=False)
+    h.close()
+    check(tree, cov, prm)
+    h = capi.Handle(tree, with_pred=False)
+    h.set_covariance(cov)
+    h.set_params(**prm)
+    D64 = h.likelihood()
+    h.set_precision(capi.PRECISION_MIXED)
+    Dm = h.likelihood()
+    h.close()
+    print(f"mixed {Dm!r} fp64 {D64!r} |dloglik|/n {abs(Dm - D64) / (2 * n):.3e}", flush=True)
+    assert abs(Dm - D64) / (2 * n) <= 1e-6
+
+
 def test_concurrent_streams():
     """Independent launches on forked streams (default) against one stream:
     the same D to the last bit, eager and graph replay, FP64 and mixed, and
